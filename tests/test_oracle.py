@@ -1,0 +1,307 @@
+"""Pins the CPU oracle (oracle/flam.py) against the reference's published
+numbers, closed forms, dense LAPACK and invariants.  CPU only."""
+import json
+import math
+import os
+import warnings
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import flam, problems
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _readme():
+    with open(os.path.join(GOLDEN, "readme_ie_circle.json")) as f:
+        return json.load(f)
+
+
+# ---------------------------------------------------------------- tree -----
+def test_hypoct_readme_block_counts():
+    """README.md:99-106: the nblk column depends on geometry only."""
+    g = _readme()
+    pr = problems.CurveDL(g["params"]["N"], "ie_circle")
+    t = flam.hypoct(pr.x, g["params"]["occ"])
+    assert t.nlvl == 8
+    nonempty = np.array([len(v) > 0 for v in t.xi])
+    pblk = np.concatenate([[0], np.cumsum([nonempty[t.lvp[l]:t.lvp[l + 1]].sum()
+                                           for l in range(t.nlvl)])])
+    nblk = [int(pblk[lv - 1] + t.lvp[lv] - t.lvp[lv - 1]) for lv in range(t.nlvl, 0, -1)]
+    assert nblk == [r[1] for r in g["rows"]]
+
+
+def _brute_adjacent(t, i, j):
+    """Closed boxes touch (share a face, edge or corner) without overlapping."""
+    li, lj = t.l[:, t.level_of(i)], t.l[:, t.level_of(j)]
+    gap = np.abs(t.ctr[i] - t.ctr[j]) - 0.5 * (li + lj)
+    return bool(np.all(gap <= 1e-12 * li.max()))
+
+
+@pytest.mark.parametrize("d,n,occ", [(2, 3000, 20), (3, 2000, 40), (1, 500, 7)])
+def test_hypoct_invariants(d, n, occ):
+    rng = np.random.default_rng(1)
+    x = rng.random((d, n)) ** 2        # non-uniform so leaves land on several levels
+    t = flam.hypoct(x, occ)
+    leaves = [b for b in range(t.nbox) if not t.chld[b]]
+    pts = np.concatenate([t.xi[b] for b in leaves])
+    assert np.array_equal(np.sort(pts), np.arange(n))            # leaves partition
+    for b in leaves:
+        assert len(t.xi[b]) <= occ or t.level_of(b) == t.nlvl - 1
+        lo = t.ctr[b] - 0.5 * t.l[:, t.level_of(b)]
+        hi = t.ctr[b] + 0.5 * t.l[:, t.level_of(b)]
+        xb = x[:, t.xi[b]]
+        assert np.all(xb >= lo[:, None] - 1e-12) and np.all(xb <= hi[:, None] + 1e-12)
+        assert np.all(np.diff(t.xi[b]) > 0)                      # order kept
+    for b in range(t.nbox):
+        if t.chld[b]:
+            assert len(t.xi[b]) == 0
+        lv = t.level_of(b)
+        for j in t.nbor[b]:
+            assert j != b and _brute_adjacent(t, b, j)
+            if t.level_of(j) == lv:
+                assert b in t.nbor[j]                             # same-level symmetry
+            else:
+                assert t.level_of(j) < lv and not t.chld[j]       # coarser leaf
+        # brute force: every adjacent same-level box is listed
+        for j in range(t.lvp[lv], t.lvp[lv + 1]):
+            if j != b and _brute_adjacent(t, b, j):
+                assert j in t.nbor[b]
+
+
+def test_hypoct_errors():
+    with pytest.raises(flam.FlamError) as e:
+        flam.hypoct(np.zeros((2, 3)), -1)
+    assert e.value.ident == "FLAM:hypoct:invalidOcc"
+    with pytest.raises(flam.FlamError) as e:
+        flam.hypoct(np.zeros((2, 3)), 2, lvlmax=0)
+    assert e.value.ident == "FLAM:hypoct:invalidLvlmax"
+
+
+def test_mround_matches_matlab():
+    assert list(flam.mround([0.5, 1.5, 2.5, -0.5, -2.5, 0.49])) == [1, 2, 3, -1, -3, 0]
+
+
+# ------------------------------------------------------------------ id -----
+def _lowrank(m, n, k, rng, decay=1e-3):
+    U = np.linalg.qr(rng.standard_normal((m, k)))[0]
+    V = np.linalg.qr(rng.standard_normal((n, k)))[0]
+    s = decay ** (np.arange(k) / max(k - 1, 1))
+    return (U * s) @ V.T
+
+
+@pytest.mark.parametrize("tol", [1e-3, 1e-6, 1e-10])
+def test_id_rank_matches_textbook_qrcp(tol):
+    """Without refinement the rank is #{|R_jj| > tol |R_11|} of QRCP."""
+    rng = np.random.default_rng(2)
+    A = _lowrank(90, 60, 40, rng, decay=1e-12)
+    sk, rd, T, _ = flam.interp_decomp(A, tol, Tmax=np.inf)
+    R = sla.qr(A, mode="r", pivoting=True)[0]
+    d = np.abs(np.diag(R))
+    assert len(sk) == int(np.sum(d > tol * d[0]))
+    assert sorted(np.concatenate([sk, rd]).tolist()) == list(range(60))
+
+
+@pytest.mark.parametrize("m,n", [(80, 50), (50, 80), (900, 40)])
+def test_id_error_bound_and_T_bound(m, n):
+    rng = np.random.default_rng(3)
+    A = _lowrank(m, n, min(m, n) // 2, rng, decay=1e-9)
+    tol = 1e-7
+    sk, rd, T, _ = flam.interp_decomp(A, tol)
+    err = np.linalg.norm(A[:, rd] - A[:, sk] @ T, 2)
+    smax = np.linalg.norm(A, 2)
+    assert err <= 100 * tol * smax
+    assert np.abs(T).max() <= 2.0 + 1e-12                         # Tmax after RRQR
+    # SVD brute force: a rank below the tol-numerical rank cannot reach tol
+    s = np.linalg.svd(A, compute_uv=False)
+    assert len(sk) >= int(np.sum(s > 1.01 * err)) - 1
+
+
+def _kahan(n, c):
+    """Kahan's matrix: column pivoting keeps the natural order, so plain
+    QRCP gives huge interpolation coefficients."""
+    s = math.sqrt(1 - c * c)
+    K = np.diag(s ** np.arange(n)) @ (np.triu(-c * np.ones((n, n)), 1) + np.eye(n))
+    return K @ np.diag(1 - 1e-10 * np.arange(n))
+
+
+def test_id_refinement_swaps_large_entries():
+    A = _kahan(50, 0.7)
+    sk0, rd0, T0, _ = flam.interp_decomp(A, 1e-6, Tmax=np.inf)
+    assert np.abs(T0).max() > 1e4
+    sk, rd, T, niter = flam.interp_decomp(A, 1e-6, Tmax=2)
+    assert niter >= 1 and len(sk) == len(sk0)
+    assert np.abs(T).max() <= 2 + 1e-10
+    assert np.linalg.norm(A[:, rd] - A[:, sk] @ T, 2) <= 1e-4 * np.linalg.norm(A, 2)
+
+
+def test_id_special_cases():
+    sk, rd, T, _ = flam.interp_decomp(np.zeros((0, 5)), 1e-6)
+    assert len(sk) == 0 and list(rd) == list(range(5)) and T.shape == (0, 5)
+    sk, rd, T, _ = flam.interp_decomp(np.zeros((4, 0)), 1e-6)
+    assert len(sk) == 0 and len(rd) == 0
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((30, 12))
+    sk, rd, T, _ = flam.interp_decomp(A, 5)                          # fixed rank 5
+    assert len(sk) == 5 and T.shape == (5, 7)
+    sk, rd, T, _ = flam.interp_decomp(A, 0)                          # rank 0 => kmax = n
+    assert len(sk) == 12
+    v = rng.standard_normal((7, 1))
+    sk, rd, T, _ = flam.interp_decomp(v @ rng.standard_normal((1, 9)), 1e-12)
+    assert len(sk) == 1
+    with pytest.raises(flam.FlamError) as e:
+        flam.interp_decomp(A, -1)
+    assert e.value.ident == "FLAM:id:invalidRankOrTol"
+    with pytest.raises(flam.FlamError) as e:
+        flam.interp_decomp(A, 1e-3, Tmax=0.5)
+    assert e.value.ident == "FLAM:id:invalidTmax"
+    with pytest.raises(flam.FlamError) as e:
+        flam.interp_decomp(A, 1e-3, rrqr_iter=-1)
+    assert e.value.ident == "FLAM:id:invalidRRQRIter"
+
+
+def test_id_iteration_cap_warns():
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        flam.interp_decomp(_kahan(50, 0.7), 1e-6, Tmax=2, rrqr_iter=1)
+    assert any("FLAM:id:maxIterCount" in str(x.message) for x in w)
+
+
+# -------------------------------------------------------------- rskelf -----
+def test_readme_rank_table():
+    """README.md:99-106 with the demo's own matrix and settings."""
+    g = _readme()
+    p = g["params"]
+    pr = problems.CurveDL(p["N"], "ie_circle", nproxy=p["p"])
+    F = flam.rskelf(pr.afun, pr.x, p["occ"], p["rank_or_tol"], pr.pxyfun,
+                    dict(Tmax=p["Tmax"], symm=p["symm"]))
+    got = flam.rank_table(F)
+    for (lvl, nblk, s, e), (rl, rn, rs, re) in zip(got, g["rows"]):
+        assert (lvl, nblk) == (rl, rn)
+        assert abs(e - re) <= max(2, 0.05 * re)          # BLAS/ordering sensitive
+        assert abs(s - rs) <= max(2, 0.05 * rs)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(p["N"])
+    x = flam.rskelf_sv(F, b)
+    xe = pr.closed_form_inverse_apply(b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 10 * g["errors"]["rskelf_sv"]
+    y = flam.rskelf_mv(F, b)
+    assert np.linalg.norm(y - pr.matvec(b)) / np.linalg.norm(pr.matvec(b)) < 10 * g["errors"]["rskelf_mv"]
+
+
+@pytest.mark.parametrize("order", ["sequential", "level"])
+def test_circle_closed_form(order):
+    """Benchmark circle matrix (alpha I + beta 11^T), Cholesky path."""
+    N = 4096
+    pr = problems.CurveDL(N, "circle")
+    F = flam.rskelf(pr.afun, pr.x, 64, 1e-12, pr.pxyfun, dict(symm="p", order=order))
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(N)
+    x = flam.rskelf_sv(F, b)
+    xe = pr.closed_form_inverse_apply(b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-10
+    assert np.allclose(pr.matvec(xe), b, rtol=0, atol=1e-12 * np.abs(b).max())
+
+
+def test_ellipse_dense_lu():
+    N = 1536
+    pr = problems.CurveDL(N, "ellipse", a=2.0, b=1.0)
+    A = pr.dense()
+    F = flam.rskelf(pr.afun, pr.x, 64, 1e-10, pr.pxyfun, dict(symm="n"))
+    rng = np.random.default_rng(1)
+    B = rng.standard_normal((N, 3))
+    X = flam.rskelf_sv(F, B)
+    Xe = np.linalg.solve(A, B)
+    assert np.linalg.norm(X - Xe) / np.linalg.norm(Xe) < 1e-7
+    Y = flam.rskelf_mv(F, B)
+    assert np.linalg.norm(Y - A @ B) / np.linalg.norm(A @ B) < 1e-7
+
+
+def _factor(symm, cplx=False, N=1024):
+    pr = problems.CurveDL(N, "circle")
+    scale = (1.0 + 0.5j) if cplx else 1.0
+    afun = (lambda I, J: pr.afun(I, J) * scale)
+
+    def pxy(x, slf, nbr, l, ctr):
+        K, nb = pr.pxyfun(x, slf, nbr, l, ctr)
+        return K * scale, nb
+    return flam.rskelf(afun, pr.x, 32, 1e-11, pxy, dict(symm=symm)), pr
+
+
+@pytest.mark.parametrize("symm,cplx", [("n", False), ("p", False), ("h", False),
+                                       ("n", True), ("s", True)])
+def test_sweep_identities(symm, cplx):
+    F, pr = _factor(symm, cplx)
+    N = pr.N
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((N, 2)) + (1j * rng.standard_normal((N, 2)) if cplx else 0)
+    Y = rng.standard_normal((N, 2)) + (1j * rng.standard_normal((N, 2)) if cplx else 0)
+    for tr in ("n", "c", "t"):
+        Z = flam.rskelf_sv(F, flam.rskelf_mv(F, X, tr), tr)
+        assert np.linalg.norm(Z - X) / np.linalg.norm(X) < 1e-12
+    # adjoint: <Y, F X> = <F^H Y, X>;  't' = conj('c' on conj)
+    lhs = np.vdot(Y, flam.rskelf_mv(F, X, "n"))
+    rhs = np.vdot(flam.rskelf_mv(F, Y, "c"), X)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    lhs = np.vdot(Y, flam.rskelf_sv(F, X, "n"))
+    rhs = np.vdot(flam.rskelf_sv(F, Y, "c"), X)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    Zt = flam.rskelf_mv(F, X, "t")
+    assert np.allclose(Zt, np.conj(flam.rskelf_mv(F, np.conj(X), "c")))
+    # F approximates A
+    scale = (1.0 + 0.5j) if cplx else 1.0
+    AX = pr.matvec(X.real) * scale + (pr.matvec(X.imag) * scale * 1j if cplx else 0)
+    assert np.linalg.norm(flam.rskelf_mv(F, X) - AX) / np.linalg.norm(AX) < 1e-9
+
+
+def test_symm_modes_agree_on_spd():
+    Fn, pr = _factor("n")
+    Fp, _ = _factor("p")
+    Fh, _ = _factor("h")
+    b = np.random.default_rng(3).standard_normal(pr.N)
+    xn, xp, xh = (flam.rskelf_sv(F, b) for F in (Fn, Fp, Fh))
+    assert np.linalg.norm(xn - xp) < 1e-10 * np.linalg.norm(xn)
+    assert np.linalg.norm(xn - xh) < 1e-10 * np.linalg.norm(xn)
+
+
+def test_chol_fails_on_indefinite():
+    """rskelf.m:235: symm 'p' on the demo's negative-definite circle matrix."""
+    pr = problems.CurveDL(512, "ie_circle")
+    with pytest.raises(np.linalg.LinAlgError):
+        flam.rskelf(pr.afun, pr.x, 32, 1e-10, pr.pxyfun, dict(symm="p"))
+
+
+def test_chk_errors():
+    with pytest.raises(flam.FlamError) as e:
+        flam.chksymm("q")
+    assert e.value.ident == "FLAM:chksymm:invalidSymm"
+    with pytest.raises(flam.FlamError) as e:
+        flam.chktrans("x")
+    assert e.value.ident == "FLAM:chktrans:invalidTrans"
+    assert flam.chksymm("P") == "p" and flam.chktrans("C") == "c"
+
+
+def test_partial_factorization_schur_system():
+    """opts.stop (rskelf.m:128,172,279-308): sweeps with mode 1, a dense
+    solve of the remaining skeleton system, then mode 2, solve A x = b."""
+    N = 2048
+    pr = problems.CurveDL(N, "circle")
+    F = flam.rskelf(pr.afun, pr.x, 32, 1e-12, pr.pxyfun, dict(symm="n", stop=3))
+    assert "Si" in F and len(F["Si"]) > 0
+    b = np.random.default_rng(4).standard_normal(N)
+    y = flam.rskelf_sv(F, b, mode=1)
+    Si = F["Si"]
+    S = pr.afun(Si, Si) + F["S"].toarray()
+    y[Si] = np.linalg.solve(S, y[Si])
+    x = flam.rskelf_sv(F, y, mode=2)
+    xe = pr.closed_form_inverse_apply(b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-10
+
+
+def test_snorm_matches_svd():
+    rng = np.random.default_rng(6)
+    A = rng.standard_normal((60, 60))
+    s = flam.snorm(60, lambda v: A @ v, lambda v: A.T @ v, tol=1e-10, niter_max=2000)
+    assert abs(s - np.linalg.norm(A, 2)) < 1e-3 * s
