@@ -1,0 +1,733 @@
+// Host side of the rskx library: the C ABI of include/rskx.h, the level
+// loop of the factorization (rskelf/rskelf.m:109-309) and the sweep drivers
+// (rskelf/rskelf_sv.m:13-36, rskelf/rskelf_mv.m:13-36).
+//
+// The tree is built on the host; every per-box numerical step runs on the
+// device.  The host keeps only index bookkeeping: per box, where its current
+// point list (xi) lives in the device pool and how long it is.  After each
+// level's ID it reads back the ranks (one int per box) to lay out the
+// factor storage of that level.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "rskx.h"
+#include "tree.hpp"
+
+namespace rskx {
+
+// ------------------------------------------------------------ launchers --
+void launch_lists(const LvlBox*, const Seg*, const Seg*, int*, int*, int*, const PointsDev&,
+                  const LevelParams&, int, hipStream_t);
+template <typename S>
+void launch_assemble(const LvlBox*, const int*, const int*, const int*, int*, S*,
+                     const PointsDev&, const LevelParams&, int, hipStream_t);
+template <typename S>
+void launch_id(const LvlBox*, const int*, S*, int*, double*, const IdParams&, int*, int*, int,
+               size_t, hipStream_t);
+template <typename S>
+void launch_factor(bool, const LvlBox*, const BoxOut*, const Seg*, int*, const int*,
+                   const double*, const S*, S*, S*, S*, int*, const PointsDev&, int*, int,
+                   hipStream_t);
+template <typename S>
+void launch_sweep(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int,
+                  size_t, hipStream_t);
+template <typename U, typename S>
+void launch_perm(bool, void*, int64_t, S*, int64_t, int, const int*, hipStream_t);
+void launch_matvec(const double*, int64_t, const double*, const double*, const double*, double,
+                   const double*, int64_t, double*, int64_t, int, hipStream_t);
+
+// --------------------------------------------------------------- errors --
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+static int fail(int code, const std::string& msg) { set_error(msg); return code; }
+
+static hipStream_t g_stream = nullptr;
+
+// ------------------------------------------------------- device buffers --
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes, bool keep = false) {
+    if (bytes <= cap) return RSKX_OK;
+    size_t nc = std::max(bytes, cap + cap / 2);
+    void* q = nullptr;
+    if (hipMalloc(&q, nc) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(RSKX_ERR_MEM, "FLAM:rskx:oom: device allocation of " + std::to_string(nc) + " bytes failed");
+    }
+    if (keep && p && cap) {
+      if (hipMemcpyAsync(q, p, cap, hipMemcpyDeviceToDevice, g_stream) != hipSuccess)
+        return fail(RSKX_ERR_HIP, "FLAM:rskx:hip: arena copy failed");
+      hipStreamSynchronize(g_stream);
+    }
+    if (p) hipFree(p);
+    p = q;
+    cap = nc;
+    return RSKX_OK;
+  }
+  void release() { if (p) hipFree(p); p = nullptr; cap = 0; }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// scratch reused across factorizations of this process
+struct Workspace {
+  DevBuf boxes, nsegs, csegs, outs, pool, nbr, mnbr, mrows, kout, iter, K, perm, T, W, M[2], flag;
+  DevBuf pts, xw;
+  int* hflag = nullptr;
+};
+static Workspace g_ws;
+
+#define RSKX_TRY(expr) do { int rc_ = (expr); if (rc_ != RSKX_OK) return rc_; } while (0)
+
+template <typename T>
+static int upload(DevBuf& b, const std::vector<T>& v) {
+  RSKX_TRY(b.ensure(std::max<size_t>(v.size() * sizeof(T), 16)));
+  if (!v.empty()) RSKX_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, g_stream));
+  return RSKX_OK;
+}
+
+// -------------------------------------------------------------- profile --
+struct KStat { double ms = 0; int64_t launches = 0; int64_t bytes = 0; double flops = 0; };
+struct Prof {
+  bool on = false;
+  std::map<std::string, KStat> st;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  void begin(const char* name, hipEvent_t& a) {
+    if (!on) return;
+    hipEventCreate(&a);
+    hipEventRecord(a, g_stream);
+    (void)name;
+  }
+  void end(const char* name, hipEvent_t a, int64_t bytes = 0, double flops = 0) {
+    if (!on) return;
+    hipEvent_t b;
+    hipEventCreate(&b);
+    hipEventRecord(b, g_stream);
+    pending.push_back({name, {a, b}});
+    auto& s = st[name];
+    s.launches++;
+    s.bytes += bytes;
+    s.flops += flops;
+  }
+  void flush() {
+    if (!on) return;
+    for (auto& p : pending) {
+      hipEventSynchronize(p.second.second);
+      float ms = 0;
+      hipEventElapsedTime(&ms, p.second.first, p.second.second);
+      st[p.first].ms += ms;
+      hipEventDestroy(p.second.first);
+      hipEventDestroy(p.second.second);
+    }
+    pending.clear();
+  }
+  void host(const char* name, double ms) {
+    if (!on) return;
+    auto& s = st[name];
+    s.ms += ms;
+    s.launches++;
+  }
+};
+
+}  // namespace rskx
+
+using namespace rskx;
+
+// ---------------------------------------------------------- the factor --
+struct rskx_factor {
+  int64_t N = 0;
+  int d = 0;
+  char symm = 'n';
+  int prec = RSKX_PREC_FP64;
+  int nlvl = 0, nlvl_done = 0;
+  std::vector<int64_t> lvp;               // cumulative factor counts per processed level
+  std::vector<int> lev_kmax, lev_rmax;
+  std::vector<FacRec> recs;               // host copy
+  DevBuf d_recs, fval, fidx, order;       // device data
+  std::vector<int64_t> h_order;           // tree order -> original index
+  std::vector<std::array<int64_t, 4>> table;
+  Prof prof;
+  int64_t fval_used = 0, fidx_used = 0;
+  ~rskx_factor() { d_recs.release(); fval.release(); fidx.release(); order.release(); }
+};
+
+namespace {
+
+// per-box host state during the factorization
+struct BoxState {
+  int64_t cur_off = 0;   // pool offset of the current point list
+  int cur_len = 0;
+  int64_t moff = -1;     // M block offset in the arena of its level (-1: zero)
+};
+
+constexpr int kBlock = 256;
+
+template <typename S>
+int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
+                const rskx_kernel& A, const rskx_proxy& PX, const rskx_opts& o, rskx_factor* F) {
+  using clk = std::chrono::steady_clock;
+  Workspace& ws = g_ws;
+  Prof& prof = F->prof;
+  prof.on = o.profile != 0;
+  auto t0 = clk::now();
+
+  // ---- tree (host)
+  std::vector<double> hx((size_t)d * N);
+  RSKX_HIP(hipMemcpyAsync(hx.data(), x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  RSKX_HIP(hipStreamSynchronize(g_stream));
+  HostTree t = build_tree(hx.data(), d, N, occ, o.lvlmax, o.ext);
+  F->nlvl = t.nlvl;
+  F->h_order = t.order;
+  prof.host("tree", std::chrono::duration<double, std::milli>(clk::now() - t0).count());
+  if (o.verb) {
+    printf("%s\n", std::string(69, '-').c_str());
+    printf("%3s | %6s | %19s | %19s | %10s\n", "lvl", "nblk", "start/end npts", "start/end npts/blk", "time (s)");
+    printf("%s\n", std::string(69, '-').c_str());
+    printf("%3s | %63.2e\n", "t", std::chrono::duration<double>(clk::now() - t0).count());
+  }
+
+  // ---- points in tree order (SoA)
+  std::vector<int> ord32(N);
+  for (int64_t p = 0; p < N; ++p) ord32[p] = (int)t.order[p];
+  RSKX_TRY(upload(F->order, ord32));
+  RSKX_TRY(ws.pts.ensure((size_t)N * (2 * d + 2) * sizeof(double)));
+  double* pts = ws.pts.as<double>();
+  {
+    // gather on the host copy of x; the other arrays are gathered on device
+    std::vector<double> hp((size_t)N * (2 * d + 2));
+    std::vector<double> hnu((size_t)d * N), hw(N), hdg(N);
+    RSKX_HIP(hipMemcpyAsync(hnu.data(), A.nu, hnu.size() * 8, hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipMemcpyAsync(hw.data(), A.w, N * 8, hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipMemcpyAsync(hdg.data(), A.diag, N * 8, hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipStreamSynchronize(g_stream));
+    for (int64_t p = 0; p < N; ++p) {
+      int64_t q = t.order[p];
+      for (int i = 0; i < d; ++i) {
+        hp[(size_t)i * N + p] = hx[(size_t)d * q + i];
+        hp[(size_t)(d + i) * N + p] = hnu[(size_t)d * q + i];
+      }
+      hp[(size_t)2 * d * N + p] = hw[q];
+      hp[(size_t)(2 * d + 1) * N + p] = hdg[q];
+    }
+    RSKX_HIP(hipMemcpyAsync(pts, hp.data(), hp.size() * 8, hipMemcpyHostToDevice, g_stream));
+  }
+  PointsDev P{};
+  for (int i = 0; i < 3; ++i) {
+    P.x[i] = i < d ? pts + (size_t)i * N : nullptr;
+    P.nu[i] = i < d ? pts + (size_t)(d + i) * N : nullptr;
+  }
+  P.w = pts + (size_t)2 * d * N;
+  P.diag = pts + (size_t)(2 * d + 1) * N;
+  P.d = d;
+  P.scale = A.scale;
+
+  // ---- pool of point lists: leaves own their tree-order ranges
+  int nbox = t.nbox();
+  std::vector<BoxState> bs(nbox);
+  for (int b = 0; b < nbox; ++b)
+    if (t.cnt[b] > 0) { bs[b].cur_off = t.beg[b]; bs[b].cur_len = (int)t.cnt[b]; }
+  int64_t pool_cap = N * (int64_t)(2 * t.nlvl + 2) + 64;
+  RSKX_TRY(ws.pool.ensure(pool_cap * sizeof(int)));
+  {
+    std::vector<int> iota(N);
+    for (int64_t p = 0; p < N; ++p) iota[p] = (int)p;
+    RSKX_HIP(hipMemcpyAsync(ws.pool.p, iota.data(), N * sizeof(int), hipMemcpyHostToDevice, g_stream));
+  }
+  int64_t pool_top = N;
+  if (!ws.hflag) RSKX_HIP(hipHostMalloc((void**)&ws.hflag, 64 * sizeof(int), hipHostMallocDefault));
+  RSKX_TRY(ws.flag.ensure(64));
+  RSKX_HIP(hipMemsetAsync(ws.flag.p, 0, 64, g_stream));
+
+  const bool lu = (o.symm == 'n');
+  const bool two = (o.symm == 'n');
+  IdParams prm;
+  prm.tol = std::fmod(rank_or_tol, 1.0);
+  prm.kmax = (int)std::floor(rank_or_tol);
+  prm.Tmax = o.Tmax;
+  prm.rrqr_iter = o.rrqr_iter;
+
+  int64_t nrem = N;
+  int64_t nfac = 0;
+  int mpar = 0;                 // M arena of the level below: ws.M[mpar]
+  std::vector<int64_t> pblk(t.nlvl + 1, 0);
+  for (int lv = 0; lv < t.nlvl; ++lv) {
+    int64_t c = 0;
+    for (int64_t b = t.lvp[lv]; b < t.lvp[lv + 1]; ++b) c += t.cnt[b] > 0;
+    pblk[lv + 1] = pblk[lv] + c;
+  }
+  F->lvp.assign(1, 0);
+
+  std::vector<LvlBox> hb;
+  std::vector<Seg> hn, hc;
+  std::vector<BoxOut> ho;
+  std::vector<int> hk, hit;
+  for (int lv = t.nlvl - 1; lv >= 0; --lv) {
+    auto tl = clk::now();
+    const int lvl_ref = lv + 1;   // the reference's level number
+    const int64_t b0 = t.lvp[lv], nb = t.lvp[lv + 1] - b0;
+    const int64_t nrem1 = nrem;
+    LevelParams L{};
+    for (int i = 0; i < d; ++i) L.l[i] = t.lsize[(size_t)d * lv + i];
+    L.radius = PX.radius;
+    L.pweight = PX.weight;
+    L.pkind = PX.kind;
+    const bool use_proxy = lvl_ref > 2 && PX.kind != RSKX_PROXY_NONE;
+    L.p = use_proxy ? PX.p : 0;
+    L.filter = use_proxy ? 1 : 0;
+    L.two_sided = two ? 1 : 0;
+
+    hb.assign(nb, LvlBox{});
+    hn.clear();
+    hc.clear();
+    int64_t off_slf = 0, off_nbr = 0, off_K = 0, off_perm = 0, off_T = 0, off_W = 0;
+    int nmax = 0, mcapmax = 0;
+    for (int64_t ib = 0; ib < nb; ++ib) {
+      const int b = (int)(b0 + ib);
+      LvlBox& B = hb[ib];
+      B.cs0 = (int)hc.size();
+      if (t.leaf(b)) {
+        hc.push_back(Seg{bs[b].cur_off, -1, bs[b].cur_len, 0});
+      } else {
+        for (int q = t.chld_off[b]; q < t.chld_off[b + 1]; ++q) {
+          int c = t.chld[q];
+          hc.push_back(Seg{bs[c].cur_off, bs[c].moff, bs[c].cur_len, 0});
+        }
+      }
+      B.ncs = (int)hc.size() - B.cs0;
+      int n = 0;
+      for (int q = B.cs0; q < (int)hc.size(); ++q) n += hc[q].len;
+      B.n = n;
+      B.seg0 = (int)hn.size();
+      for (int q = t.nbor_off[b]; q < t.nbor_off[b + 1]; ++q) {
+        int j = t.nbor[q];
+        if (t.level_of(j) == lv && !t.leaf(j)) {
+          for (int q2 = t.chld_off[j]; q2 < t.chld_off[j + 1]; ++q2) {
+            int c = t.chld[q2];
+            if (bs[c].cur_len) hn.push_back(Seg{bs[c].cur_off, -1, bs[c].cur_len, 0});
+          }
+        } else if (bs[j].cur_len) {
+          hn.push_back(Seg{bs[j].cur_off, -1, bs[j].cur_len, 0});
+        }
+      }
+      B.nseg = (int)hn.size() - B.seg0;
+      int mc = 0;
+      for (int q = B.seg0; q < (int)hn.size(); ++q) mc += hn[q].len;
+      B.mc = mc;
+      B.mcap = std::max(1, mc * (two ? 2 : 1) + L.p);
+      B.slf_off = pool_top + off_slf;  off_slf += n;
+      B.nbr_off = off_nbr;             off_nbr += mc;
+      B.K_off = off_K;                 off_K += (int64_t)B.mcap * n;
+      B.perm_off = off_perm;           off_perm += n;
+      B.T_off = off_T;                 off_T += 3 * (int64_t)n * n;
+      B.W_off = off_W;                 off_W += (int64_t)n * n;
+      for (int i = 0; i < d; ++i) B.ctr[i] = t.ctr[(size_t)d * b + i];
+      nmax = std::max(nmax, n);
+      mcapmax = std::max(mcapmax, B.mcap);
+    }
+    if (pool_top + off_slf > pool_cap) return fail(RSKX_ERR_MEM, "FLAM:rskx:pool: point-list pool exhausted");
+    RSKX_TRY(upload(ws.boxes, hb));
+    RSKX_TRY(upload(ws.nsegs, hn));
+    RSKX_TRY(upload(ws.csegs, hc));
+    RSKX_TRY(ws.nbr.ensure(std::max<int64_t>(off_nbr, 1) * sizeof(int)));
+    RSKX_TRY(ws.mnbr.ensure(nb * sizeof(int)));
+    RSKX_TRY(ws.mrows.ensure(nb * sizeof(int)));
+    RSKX_TRY(ws.kout.ensure(nb * sizeof(int)));
+    RSKX_TRY(ws.iter.ensure(nb * sizeof(int)));
+    RSKX_TRY(ws.K.ensure(std::max<int64_t>(off_K, 1) * sizeof(S)));
+    RSKX_TRY(ws.perm.ensure(std::max<int64_t>(off_perm, 1) * sizeof(int)));
+    RSKX_TRY(ws.T.ensure(std::max<int64_t>(off_T, 1) * sizeof(double)));
+    RSKX_TRY(ws.W.ensure(std::max<int64_t>(off_W, 1) * sizeof(S)));
+
+    size_t shm = (size_t)(2 * nmax + std::max(nmax, mcapmax) + 32) * sizeof(double);
+    if (shm > 160 * 1024)
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box too large for the ID kernel (n=" +
+                                       std::to_string(nmax) + ", m=" + std::to_string(mcapmax) + ")");
+    hipEvent_t ev;
+    prof.begin("lists", ev);
+    launch_lists(ws.boxes.as<LvlBox>(), ws.nsegs.as<Seg>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
+                 ws.nbr.as<int>(), ws.mnbr.as<int>(), P, L, (int)nb, g_stream);
+    prof.end("lists", ev);
+    int64_t Kbytes = 0;
+    for (auto& B : hb) Kbytes += (int64_t)B.mcap * B.n;
+    prof.begin("assemble", ev);
+    launch_assemble<S>(ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(), ws.mnbr.as<int>(),
+                       ws.mrows.as<int>(), ws.K.as<S>(), P, L, (int)nb, g_stream);
+    prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
+    prof.begin("id", ev);
+    launch_id<S>(ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(), ws.perm.as<int>(),
+                 ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(), (int)nb, shm, g_stream);
+    prof.end("id", ev);
+    RSKX_HIP(hipGetLastError());
+    hk.resize(nb);
+    hit.resize(nb);
+    std::vector<int> hm(nb);
+    RSKX_HIP(hipMemcpyAsync(hk.data(), ws.kout.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipMemcpyAsync(hit.data(), ws.iter.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipMemcpyAsync(hm.data(), ws.mrows.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipStreamSynchronize(g_stream));
+    for (int64_t ib = 0; ib < nb; ++ib)
+      if (hit[ib] < 0) {
+        fprintf(stderr, "Warning: FLAM:id:maxIterCount: Maximum number of RRQR iterations reached.\n");
+        break;
+      }
+
+    // ---- lay out this level's factors
+    ho.assign(nb, BoxOut{});
+    int64_t offM = 0, off_nl = 0, need_v = 0, need_i = 0;
+    int kmx = 0, rmx = 0;
+    double id_flops = 0, el_flops = 0;
+    for (int64_t ib = 0; ib < nb; ++ib) {
+      const LvlBox& B = hb[ib];
+      BoxOut& O = ho[ib];
+      O.k = hk[ib];
+      O.r = B.n - O.k;
+      const int64_t k = O.k, r = O.r, m = hm[ib], n = B.n;
+      id_flops += 4.0 * m * n * k - 2.0 * k * k * (m + n) + 4.0 / 3.0 * k * k * k + (double)k * k * r;
+      if (r > 0) {
+        O.fac = (int)(nfac++);
+        O.T = F->fval_used + need_v; need_v += k * r;
+        O.LU = F->fval_used + need_v; need_v += r * r;
+        O.E = F->fval_used + need_v; need_v += k * r;
+        O.G = F->fval_used + need_v; if (lu) need_v += r * k;
+        O.sk = F->fidx_used + need_i; need_i += k;
+        O.rd = F->fidx_used + need_i; need_i += r;
+        O.piv = F->fidx_used + need_i; if (lu) need_i += r;
+        O.M_off = offM; offM += k * k;
+        O.nl_off = pool_top + off_slf + off_nl; off_nl += k;
+        kmx = std::max<int>(kmx, (int)k);
+        rmx = std::max<int>(rmx, (int)r);
+        el_flops += 4.0 * n * k * r + (lu ? 2.0 / 3.0 : 1.0 / 3.0) * r * r * r +
+                    (lu ? 2.0 : 1.0) * k * r * r + 2.0 * k * k * r;
+      } else {
+        O.fac = -1;
+        O.M_off = offM; offM += n * n;
+        O.nl_off = -1;
+      }
+    }
+    prof.st["id"].flops += id_flops;
+    RSKX_TRY(F->fval.ensure((F->fval_used + need_v + 1) * sizeof(S), true));
+    RSKX_TRY(F->fidx.ensure((F->fidx_used + need_i + 1) * sizeof(int), true));
+    const int mcur = 1 - mpar;
+    RSKX_TRY(ws.M[mcur].ensure(std::max<int64_t>(offM, 1) * sizeof(S)));
+    RSKX_TRY(ws.M[mpar].ensure(sizeof(S)));
+    RSKX_TRY(upload(ws.outs, ho));
+    prof.begin("factor", ev);
+    launch_factor<S>(lu, ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(),
+                     ws.pool.as<int>(), ws.perm.as<int>(), ws.T.as<double>(), ws.M[mpar].as<S>(),
+                     ws.M[mcur].as<S>(), ws.W.as<S>(), F->fval.as<S>(), F->fidx.as<int>(), P,
+                     ws.flag.as<int>(), (int)nb, g_stream);
+    prof.end("factor", ev, 0, el_flops);
+    RSKX_HIP(hipGetLastError());
+    RSKX_HIP(hipMemcpyAsync(ws.hflag, ws.flag.p, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipStreamSynchronize(g_stream));
+    if (ws.hflag[0] & 1)
+      return fail(RSKX_ERR_NUM, "FLAM:rskx:chol: matrix is not positive definite (symm 'p')");
+    if (ws.hflag[0] & 2)
+      return fail(RSKX_ERR_NUM, "FLAM:rskx:lu: exactly singular redundant block");
+    prof.flush();
+
+    // ---- commit the level
+    int64_t elim = 0;
+    for (int64_t ib = 0; ib < nb; ++ib) {
+      const int b = (int)(b0 + ib);
+      const LvlBox& B = hb[ib];
+      const BoxOut& O = ho[ib];
+      if (O.r > 0) {
+        bs[b].cur_off = O.nl_off;
+        bs[b].cur_len = O.k;
+        FacRec fr;
+        fr.k = O.k; fr.r = O.r;
+        fr.T = O.T; fr.LU = O.LU; fr.E = O.E; fr.G = O.G;
+        fr.sk = O.sk; fr.rd = O.rd; fr.piv = O.piv;
+        F->recs.push_back(fr);
+        elim += O.r;
+      } else {
+        bs[b].cur_off = B.slf_off;
+        bs[b].cur_len = B.n;
+      }
+      bs[b].moff = O.M_off;
+    }
+    F->fval_used += need_v;
+    F->fidx_used += need_i;
+    pool_top += off_slf + off_nl;
+    mpar = mcur;
+    nrem -= elim;
+    F->lvp.push_back((int64_t)F->recs.size());
+    F->lev_kmax.push_back(kmx);
+    F->lev_rmax.push_back(rmx);
+    const int64_t nblk = pblk[lv] + nb;
+    F->table.push_back({lvl_ref, nblk, nrem1, nrem});
+    F->nlvl_done++;
+    if (o.verb)
+      printf("%3d | %6lld | %8lld | %8lld | %8.2f | %8.2f | %10.2e\n", lvl_ref, (long long)nblk,
+             (long long)nrem1, (long long)nrem, (double)nrem1 / nblk, (double)nrem / nblk,
+             std::chrono::duration<double>(clk::now() - tl).count());
+  }
+  if (o.verb) printf("%s\n", std::string(69, '-').c_str());
+  RSKX_TRY(upload(F->d_recs, F->recs));
+  RSKX_HIP(hipStreamSynchronize(g_stream));
+  prof.flush();
+  return RSKX_OK;
+}
+
+template <typename S, typename U>
+int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans, bool solve) {
+  rskx_factor* F = const_cast<rskx_factor*>(Fc);
+  const int64_t N = F->N;
+  if (nrhs <= 0 || N == 0) return RSKX_OK;
+  RSKX_TRY(g_ws.xw.ensure((size_t)N * nrhs * sizeof(S)));
+  S* Y = g_ws.xw.as<S>();
+  hipEvent_t ev;
+  Prof& prof = F->prof;
+  prof.begin("permute", ev);
+  launch_perm<U, S>(true, X, ldx, Y, N, nrhs, F->order.as<int>(), g_stream);
+  prof.end("permute", ev, (int64_t)N * nrhs * (sizeof(S) + sizeof(U)) + N * 4);
+  int prog;
+  if (F->symm == 'p') prog = solve ? 2 : 5;
+  else prog = solve ? (trans == 'n' ? 0 : 1) : (trans == 'n' ? 3 : 4);
+  const int nl = (int)F->lvp.size() - 1;
+  // rskelf_sv.m:25-35 / sv/*.m: mode 3 = upward then downward sweep
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool up = pass == 0;
+    for (int q = 0; q < nl; ++q) {
+      const int lv = up ? q : nl - 1 - q;
+      const int64_t f0 = F->lvp[lv], f1 = F->lvp[lv + 1];
+      if (f1 == f0) continue;
+      const int kmx = F->lev_kmax[lv], rmx = F->lev_rmax[lv];
+      size_t per = (size_t)(kmx + 2 * rmx) * sizeof(S);
+      int cchunk = (int)std::min<size_t>(nrhs, std::max<size_t>(1, (64 * 1024) / std::max<size_t>(per, 1)));
+      size_t shm = per * cchunk;
+      if (shm > 160 * 1024) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: factor too large for the sweep kernel");
+      int64_t bytes = 0;
+      for (int64_t f = f0; f < f1; ++f) {
+        const FacRec& fr = F->recs[f];
+        int64_t kr = (int64_t)fr.k * fr.r, rr = (int64_t)fr.r * fr.r;
+        bytes += (int64_t)sizeof(S) * (2 * kr + (rr + fr.r) / 2 + 2 * (int64_t)(fr.k + fr.r) * nrhs) +
+                 4 * (fr.k + fr.r);
+      }
+      prof.begin("sweep", ev);
+      launch_sweep<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->fval.as<S>(),
+                      F->fidx.as<int>(), Y, N, nrhs, cchunk, shm, g_stream);
+      prof.end("sweep", ev, bytes);
+    }
+  }
+  prof.begin("permute", ev);
+  launch_perm<U, S>(false, X, ldx, Y, N, nrhs, F->order.as<int>(), g_stream);
+  prof.end("permute", ev, (int64_t)N * nrhs * (sizeof(S) + sizeof(U)) + N * 4);
+  RSKX_HIP(hipGetLastError());
+  RSKX_HIP(hipStreamSynchronize(g_stream));
+  prof.flush();
+  return RSKX_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI ==
+extern "C" {
+
+int rskx_opts_default(rskx_opts* o) {
+  if (!o) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: opts is NULL");
+  o->lvlmax = 0;
+  o->ext = nullptr;
+  o->Tmax = 2.0;
+  o->rrqr_iter = INFINITY;
+  o->symm = 'n';
+  o->verb = 0;
+  o->precision = RSKX_PREC_FP64;
+  o->profile = 0;
+  return RSKX_OK;
+}
+
+const char* rskx_last_error(void) { return g_err.c_str(); }
+
+int rskx_set_stream(void* s) {
+  g_stream = (hipStream_t)s;
+  return RSKX_OK;
+}
+
+int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
+                const rskx_kernel* A, const rskx_proxy* pxy, const rskx_opts* opts,
+                rskx_factor** out) {
+  if (!out) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: output handle is NULL");
+  *out = nullptr;
+  rskx_opts o;
+  rskx_opts_default(&o);
+  if (opts) o = *opts;
+  char s = (char)tolower(o.symm);
+  if (s != 'n' && s != 's' && s != 'h' && s != 'p')
+    return fail(RSKX_ERR_ARG, "FLAM:chksymm:invalidSymm: Symmetry parameter must be one of 'N', 'S', 'H', or 'P'.");
+  if (s == 's' || s == 'h')
+    return fail(RSKX_ERR_UNSUP, std::string("FLAM:rskx:symm: symm '") + s + "' is not built on the device (use 'n' or 'p')");
+  o.symm = s;
+  if (occ < 0) return fail(RSKX_ERR_ARG, "FLAM:hypoct:invalidOcc: Leaf occupancy must be nonnegative.");
+  if (o.lvlmax < 0) return fail(RSKX_ERR_ARG, "FLAM:hypoct:invalidLvlmax: Maximum tree depth must be at least 1.");
+  if (!(rank_or_tol >= 0)) return fail(RSKX_ERR_ARG, "FLAM:id:invalidRankOrTol: Rank or tolerance must be nonnegative.");
+  if (!(o.Tmax >= 1)) return fail(RSKX_ERR_ARG, "FLAM:id:invalidTmax: Interpolation matrix entry bound must be >= 1.");
+  if (!(o.rrqr_iter >= 0)) return fail(RSKX_ERR_ARG, "FLAM:id:invalidRRQRIter: Maximum number of RRQR iterations must be nonnegative.");
+  if (rank_or_tol >= 1 && std::isfinite(o.Tmax) && o.rrqr_iter > 0)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:rank: fixed-rank ID with RRQR refinement is not built (pass a tolerance < 1, or Tmax = Inf)");
+  if (!A || A->kind != RSKX_KERNEL_LAPLACE2D_DL || d != 2)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: only the 2D Laplace double-layer kernel is built");
+  if (!x || !A->nu || !A->w || !A->diag) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: point or kernel array is NULL");
+  if (N < 1) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: N must be positive");
+  if (N > (int64_t)1 << 30) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: N above 2^30");
+  rskx_proxy px{RSKX_PROXY_NONE, 0, 1.5, 0.0};
+  if (pxy) px = *pxy;
+  if (px.kind != RSKX_PROXY_RING_SL && px.kind != RSKX_PROXY_RING_DL)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: naive global compression (no pxyfun) is not built; pass a proxy ring");
+  if (px.p < 1 || !(px.radius > 0)) return fail(RSKX_ERR_ARG, "FLAM:rskx:proxy: invalid proxy ring");
+  if (o.precision != RSKX_PREC_FP64 && o.precision != RSKX_PREC_FP32)
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:precision: unknown precision");
+
+  rskx_factor* F = new rskx_factor();
+  F->N = N;
+  F->d = d;
+  F->symm = o.symm;
+  F->prec = o.precision;
+  int rc = o.precision == RSKX_PREC_FP64
+               ? factor_impl<double>(x, d, N, occ, rank_or_tol, *A, px, o, F)
+               : factor_impl<float>(x, d, N, occ, rank_or_tol, *A, px, o, F);
+  if (rc != RSKX_OK) { delete F; return rc; }
+  *out = F;
+  return RSKX_OK;
+}
+
+static int sweep_entry(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
+                       bool solve) {
+  if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
+  char t = (char)tolower(trans);
+  if (t != 'n' && t != 't' && t != 'c')
+    return fail(RSKX_ERR_ARG, "FLAM:chktrans:invalidTrans: Transpose parameter must be one of 'N', 'T', or 'C'.");
+  if (t == 't') t = 'c';   // real data: transpose = conjugate transpose (rskelf_sv.m:22)
+  if (nrhs < 0 || ldx < F->N) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad nrhs or leading dimension");
+  if (nrhs > 0 && !X) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: X is NULL");
+  if (dtype != RSKX_F64 && dtype != RSKX_F32) return fail(RSKX_ERR_ARG, "FLAM:rskx:dtype: unknown element type");
+  if (F->prec == RSKX_PREC_FP64)
+    return dtype == RSKX_F64 ? sweep_impl<double, double>(F, X, ldx, nrhs, t, solve)
+                             : sweep_impl<double, float>(F, X, ldx, nrhs, t, solve);
+  return dtype == RSKX_F64 ? sweep_impl<float, double>(F, X, ldx, nrhs, t, solve)
+                           : sweep_impl<float, float>(F, X, ldx, nrhs, t, solve);
+}
+
+int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype) {
+  return sweep_entry(F, X, ldx, nrhs, trans, dtype, true);
+}
+
+int rskx_rskelf_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype) {
+  return sweep_entry(F, X, ldx, nrhs, trans, dtype, false);
+}
+
+void rskx_free(rskx_factor* F) { delete F; }
+
+int rskx_factor_info(const rskx_factor* F, rskx_info* info) {
+  if (!F || !info) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  info->N = F->N;
+  info->d = F->d;
+  info->nlvl = F->nlvl;
+  info->nlvl_done = F->nlvl_done;
+  info->nfactor = (int64_t)F->recs.size();
+  int es = F->prec == RSKX_PREC_FP64 ? 8 : 4;
+  info->bytes = F->fval_used * es + F->fidx_used * 4 + (int64_t)F->recs.size() * sizeof(FacRec) + F->N * 4;
+  info->precision = F->prec;
+  info->symm = F->symm;
+  return RSKX_OK;
+}
+
+int rskx_level_table(const rskx_factor* F, int64_t* rows, int maxrows) {
+  if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
+  int n = (int)F->table.size();
+  for (int i = 0; i < n && i < maxrows && rows; ++i)
+    for (int j = 0; j < 4; ++j) rows[4 * i + j] = F->table[i][j];
+  return n;
+}
+
+int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
+  if (!F || !v) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  if (i < 0 || i >= (int64_t)F->recs.size()) return fail(RSKX_ERR_ARG, "FLAM:rskx:index: factor index out of range");
+  const FacRec& f = F->recs[i];
+  v->k = f.k;
+  v->r = f.r;
+  if (!v->sk) return RSKX_OK;
+  const int64_t k = f.k, r = f.r;
+  const bool lu = F->symm == 'n';
+  const int es = F->prec == RSKX_PREC_FP64 ? 8 : 4;
+  auto getv = [&](int64_t off, int64_t cnt, std::vector<double>& dst) -> int {
+    dst.resize(cnt);
+    if (!cnt) return RSKX_OK;
+    std::vector<char> raw(cnt * es);
+    RSKX_HIP(hipMemcpy(raw.data(), F->fval.as<char>() + off * es, cnt * es, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < cnt; ++q)
+      dst[q] = es == 8 ? reinterpret_cast<double*>(raw.data())[q] : reinterpret_cast<float*>(raw.data())[q];
+    return RSKX_OK;
+  };
+  auto geti = [&](int64_t off, int64_t cnt, std::vector<int>& dst) -> int {
+    dst.resize(cnt);
+    if (cnt) RSKX_HIP(hipMemcpy(dst.data(), F->fidx.as<int>() + off, cnt * 4, hipMemcpyDeviceToHost));
+    return RSKX_OK;
+  };
+  std::vector<double> T, LU, E, G;
+  std::vector<int> sk, rd, pv;
+  RSKX_TRY(getv(f.T, k * r, T));
+  RSKX_TRY(getv(f.LU, r * r, LU));
+  RSKX_TRY(getv(f.E, k * r, E));
+  if (lu) RSKX_TRY(getv(f.G, r * k, G));
+  RSKX_TRY(geti(f.sk, k, sk));
+  RSKX_TRY(geti(f.rd, r, rd));
+  if (lu) RSKX_TRY(geti(f.piv, r, pv));
+  for (int64_t q = 0; q < k; ++q) v->sk[q] = F->h_order[sk[q]];
+  for (int64_t q = 0; q < r; ++q) v->rd[q] = F->h_order[rd[q]];
+  if (v->T) std::copy(T.begin(), T.end(), v->T);
+  if (v->E) std::copy(E.begin(), E.end(), v->E);
+  if (v->L)
+    for (int64_t c = 0; c < r; ++c)
+      for (int64_t a = 0; a < r; ++a)
+        v->L[a + c * r] = a > c ? LU[a + c * r] : (a == c ? (lu ? 1.0 : LU[a + c * r]) : 0.0);
+  if (lu) {
+    if (v->U)
+      for (int64_t c = 0; c < r; ++c)
+        for (int64_t a = 0; a < r; ++a) v->U[a + c * r] = a <= c ? LU[a + c * r] : 0.0;
+    if (v->F) std::copy(G.begin(), G.end(), v->F);
+    if (v->p) for (int64_t q = 0; q < r; ++q) v->p[q] = pv[q];
+  }
+  return RSKX_OK;
+}
+
+int rskx_kernel_times(const rskx_factor* F, const char** names, double* ms, int64_t* launches,
+                      int64_t* bytes, double* flops, int maxn) {
+  if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
+  int i = 0;
+  for (auto& kv : F->prof.st) {
+    if (i < maxn) {
+      if (names) names[i] = kv.first.c_str();
+      if (ms) ms[i] = kv.second.ms;
+      if (launches) launches[i] = kv.second.launches;
+      if (bytes) bytes[i] = kv.second.bytes;
+      if (flops) flops[i] = kv.second.flops;
+    }
+    ++i;
+  }
+  return i;
+}
+
+int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, const double* X,
+                       int64_t ldx, double* Y, int64_t ldy, int nrhs) {
+  if (!A || A->kind != RSKX_KERNEL_LAPLACE2D_DL || d != 2)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: only the 2D Laplace double-layer kernel is built");
+  if (!x || !X || !Y) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL array");
+  if (ldx < N || ldy < N || nrhs < 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad leading dimension");
+  launch_matvec(x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, g_stream);
+  RSKX_HIP(hipGetLastError());
+  RSKX_HIP(hipStreamSynchronize(g_stream));
+  return RSKX_OK;
+}
+
+}  // extern "C"

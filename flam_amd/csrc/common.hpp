@@ -1,0 +1,96 @@
+// Shared host/device definitions of the rskx library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace rskx {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const std::string& msg);
+#define RSKX_HIP(call)                                                         \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::rskx::set_error(std::string("FLAM:rskx:hip: ") + #call + ": " +        \
+                        hipGetErrorString(e_));                                \
+      return RSKX_ERR_HIP;                                                     \
+    }                                                                          \
+  } while (0)
+
+// ------------------------------------------------- point data (tree order) --
+// Structure-of-arrays copy of the points and the matrix description,
+// permuted into tree order so a leaf's points are contiguous.
+struct PointsDev {
+  const double* x[3];     // coordinates
+  const double* nu[3];    // unit normals
+  const double* w;        // quadrature weights
+  const double* diag;     // diagonal entries
+  int d;
+  double scale;           // off-diagonal multiplier
+};
+
+// proxy / filter parameters of one level
+struct LevelParams {
+  double l[3];            // box size of the level
+  double radius;          // proxy ring radius = neighbour filter radius (box units)
+  double pweight;         // single-layer proxy multiplier
+  int p;                  // proxy points (0 at levels <= 2)
+  int pkind;              // RSKX_PROXY_*
+  int filter;             // apply the neighbour filter (levels > 2)
+  int two_sided;          // symm 'n': stack A(slf,nbr)' under A(nbr,slf)
+};
+
+// one box of the level being processed
+struct LvlBox {
+  int n;                  // |slf|
+  int mc;                 // neighbour candidates
+  int mcap;               // K row capacity (leading dimension of K)
+  int nseg, seg0;         // neighbour candidate segments
+  int ncs, cs0;           // child segments forming slf (own points for a leaf)
+  int pad0;
+  int64_t slf_off;        // pool offset of slf
+  int64_t nbr_off;        // offset in the neighbour list arena
+  int64_t K_off;          // offset in the K arena
+  int64_t perm_off;       // offset in the ID permutation arena (n)
+  int64_t T_off;          // offset in the double T/R scratch (2 n^2)
+  int64_t W_off;          // offset in the factor scratch (n^2)
+  double ctr[3];
+};
+
+// list segment: `len` point indices at pool[off]; for child segments moff
+// is the child's M block offset in the previous level's M arena (-1: none)
+struct Seg {
+  int64_t off;
+  int64_t moff;
+  int len;
+  int pad;
+};
+
+// outputs of one box's elimination, filled by the host after the ID
+struct BoxOut {
+  int k, r;
+  int fac;                // factor index (global), -1 if no compression
+  int pad;
+  int64_t M_off;          // M block output (k x k, or n x n if r == 0)
+  int64_t nl_off;         // pool offset of the new current list (k)
+  int64_t T, LU, E, G;    // offsets into the factor value arena
+  int64_t sk, rd, piv;    // offsets into the factor index arena
+};
+
+// per-factor record used by the sweeps (values in the factor arena)
+struct FacRec {
+  int k, r;
+  int64_t T, LU, E, G;    // T k x r, LU r x r packed, E k x r, G r x k
+  int64_t sk, rd, piv;    // index arena offsets (tree order point indices)
+};
+
+struct IdParams {
+  double tol;             // relative tolerance
+  int kmax;               // rank cap (0: none)
+  double Tmax;            // interpolation bound (inf: no refinement)
+  double rrqr_iter;       // max refinement iterations
+};
+
+}  // namespace rskx

@@ -1,0 +1,652 @@
+// Per-level factorization kernels (one workgroup per box of the level).
+//
+//   k_lists     slf = own points or children's skeletons (rskelf.m:179-187),
+//               neighbour rows filtered by the proxy ring (ie_circle.m:114)
+//   k_assemble  K = [A(nbr,slf); A(slf,nbr)' ('n'); Kpxy] (rskelf.m:201-212)
+//   k_id        column ID of K by QRCP truncated at the tolerance rank,
+//               T = R11\R12 and Gu-Eisenstat refinement (core/id.m:44-248)
+//   k_factor    sparsify, LU / Cholesky of the redundant block, E, G and
+//               the skeleton Schur complement (rskelf.m:218-261)
+#include "common.hpp"
+#include "devutil.hpp"
+#include "rskx.h"
+
+namespace rskx {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// ------------------------------------------------------- matrix entries --
+// A_ij of RSKX_KERNEL_LAPLACE2D_DL, i != j (points in tree order)
+__device__ inline double entry_offdiag(const PointsDev& P, int i, int j) {
+  double dx = P.x[0][i] - P.x[0][j];
+  double dy = P.x[1][i] - P.x[1][j];
+  double rdotn = dx * P.nu[0][j] + dy * P.nu[1][j];
+  return P.scale * P.w[j] * rdotn / (kTwoPi * (dx * dx + dy * dy));
+}
+
+__device__ inline double entry(const PointsDev& P, int i, int j) {
+  return i == j ? P.diag[j] : entry_offdiag(P, i, j);
+}
+
+// proxy row q against column point j (ie_circle.m:106-111, ie_ellipse.m:126-128)
+__device__ inline double proxy_entry(const PointsDev& P, const LevelParams& L, const double* ctr,
+                                     int q, int j) {
+  double th = (double)(q + 1) * kTwoPi / (double)L.p;
+  double px = L.radius * cos(th) * L.l[0] + ctr[0];
+  double py = L.radius * sin(th) * L.l[1] + ctr[1];
+  double dx = px - P.x[0][j];
+  double dy = py - P.x[1][j];
+  double r2 = dx * dx + dy * dy;
+  if (L.pkind == RSKX_PROXY_RING_SL) return -0.5 * log(r2) / kTwoPi * L.pweight;
+  return P.w[j] * (dx * P.nu[0][j] + dy * P.nu[1][j]) / (kTwoPi * r2);
+}
+
+__device__ inline bool in_ring(const PointsDev& P, const LevelParams& L, const double* ctr, int pt) {
+  double s = 0;
+  for (int q = 0; q < P.d; ++q) {
+    double t = (P.x[q][pt] - ctr[q]) / L.l[q];
+    s += t * t;
+  }
+  return s < L.radius * L.radius;
+}
+
+// ------------------------------------------------------------- k_lists --
+__global__ void __launch_bounds__(256) k_lists(const LvlBox* __restrict__ boxes,
+                                               const Seg* __restrict__ nsegs,
+                                               const Seg* __restrict__ csegs, int* pool,
+                                               int* __restrict__ nbrar, int* __restrict__ mnbr,
+                                               PointsDev P, LevelParams L) {
+  const LvlBox B = boxes[blockIdx.x];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & (kWave - 1), wid = tid / kWave, nw = nt / kWave;
+  int* slf = pool + B.slf_off;
+  int pos = 0;
+  for (int s = 0; s < B.ncs; ++s) {
+    Seg g = csegs[B.cs0 + s];
+    for (int i = tid; i < g.len; i += nt) slf[pos + i] = pool[g.off + i];
+    pos += g.len;
+  }
+  __shared__ int wtot[16];
+  int* nbr = nbrar + B.nbr_off;
+  int cnt = 0;
+  for (int s = 0; s < B.nseg; ++s) {
+    Seg g = nsegs[B.seg0 + s];
+    for (int b0 = 0; b0 < g.len; b0 += nt) {
+      int i = b0 + tid;
+      bool keep = false;
+      int pt = 0;
+      if (i < g.len) {
+        pt = pool[g.off + i];
+        keep = !L.filter || in_ring(P, L, B.ctr, pt);
+      }
+      uint64_t mask = __ballot(keep);
+      int pre = __popcll(mask & ((1ull << lane) - 1ull));
+      if (lane == 0) wtot[wid] = __popcll(mask);
+      __syncthreads();
+      int off = 0, tot = 0;
+      for (int w = 0; w < nw; ++w) {
+        if (w < wid) off += wtot[w];
+        tot += wtot[w];
+      }
+      if (keep) nbr[cnt + off + pre] = pt;
+      cnt += tot;
+      __syncthreads();
+    }
+  }
+  if (tid == 0) mnbr[blockIdx.x] = cnt;
+}
+
+// ----------------------------------------------------------- k_assemble --
+template <typename S>
+__global__ void __launch_bounds__(256) k_assemble(const LvlBox* __restrict__ boxes,
+                                                  const int* __restrict__ pool,
+                                                  const int* __restrict__ nbrar,
+                                                  const int* __restrict__ mnbr,
+                                                  int* __restrict__ mrows, S* __restrict__ Kar,
+                                                  PointsDev P, LevelParams L) {
+  const LvlBox B = boxes[blockIdx.x];
+  const int n = B.n, mn = mnbr[blockIdx.x];
+  const int mt = L.two_sided ? 2 * mn : mn;
+  const int m = mt + L.p;
+  if (threadIdx.x == 0) mrows[blockIdx.x] = m;
+  const int* slf = pool + B.slf_off;
+  const int* nbr = nbrar + B.nbr_off;
+  S* K = Kar + B.K_off;
+  const int ld = B.mcap;
+  for (int j = 0; j < n; ++j) {
+    const int col = slf[j];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      double v;
+      if (i < mn) v = entry_offdiag(P, nbr[i], col);
+      else if (i < mt) v = entry_offdiag(P, col, nbr[i - mn]);
+      else v = proxy_entry(P, L, B.ctr, i - mt, col);
+      K[i + (int64_t)j * ld] = (S)v;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- k_id --
+// Householder QR of the k x k matrix R (leading dimension ldr) in place;
+// only the triangular factor is kept.  All threads of the block take part.
+__device__ void small_qr(double* R, int k, int ldr, double* v, double* red) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int s = 0; s < k - 1; ++s) {
+    double part = 0;
+    for (int i = s + 1 + tid; i < k; i += nt) part += R[i + s * ldr] * R[i + s * ldr];
+    double sigma = block_sum(part, red);
+    if (sigma == 0) continue;
+    double alpha = R[s + s * ldr];
+    double nrm = sqrt(alpha * alpha + sigma);
+    double beta = alpha >= 0 ? -nrm : nrm;
+    double tau = (beta - alpha) / beta, scal = 1.0 / (alpha - beta);
+    for (int i = s + tid; i < k; i += nt) v[i] = i == s ? 1.0 : R[i + s * ldr] * scal;
+    __syncthreads();
+    for (int j = s + 1 + tid; j < k; j += nt) {
+      double dot = 0;
+      for (int i = s; i < k; ++i) dot += v[i] * R[i + j * ldr];
+      dot *= tau;
+      for (int i = s; i < k; ++i) R[i + j * ldr] -= dot * v[i];
+    }
+    __syncthreads();
+    if (tid == 0) R[s + s * ldr] = beta;
+    for (int i = s + 1 + tid; i < k; i += nt) R[i + s * ldr] = 0;
+    __syncthreads();
+  }
+}
+
+// squared row norms of inv(R) for upper-triangular k x k R; X is k x k scratch
+__device__ void inv_row_norms(const double* R, int k, int ldr, double* X, int ldx, double* r2) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int c = tid; c < k; c += nt) {
+    for (int l = k - 1; l >= 0; --l) {
+      if (l > c) { X[l + c * ldx] = 0; continue; }
+      double s = (l == c) ? 1.0 : 0.0;
+      for (int q = l + 1; q <= c; ++q) s -= R[l + q * ldr] * X[q + c * ldx];
+      X[l + c * ldx] = s / R[l + l * ldr];
+    }
+  }
+  __syncthreads();
+  for (int l = tid; l < k; l += nt) {
+    double s = 0;
+    for (int c = l; c < k; ++c) s += X[l + c * ldx] * X[l + c * ldx];
+    r2[l] = s;
+  }
+  __syncthreads();
+}
+
+// swap skeleton positions a and b: perm entries, T rows, R columns
+__device__ void swap_skel(int* perm, double* T, int ldt, int r, double* R, int ldr, int k, int a,
+                          int b) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) { int t = perm[a]; perm[a] = perm[b]; perm[b] = t; }
+  for (int c = tid; c < r; c += nt) {
+    double t = T[a + c * ldt]; T[a + c * ldt] = T[b + c * ldt]; T[b + c * ldt] = t;
+  }
+  for (int i = tid; i < k; i += nt) {
+    double t = R[i + a * ldr]; R[i + a * ldr] = R[i + b * ldr]; R[i + b * ldr] = t;
+  }
+  __syncthreads();
+}
+
+// Shared memory: vn1[n], vn2[n], v[max(m, n)], red[32]
+template <typename S>
+__global__ void __launch_bounds__(256) k_id(const LvlBox* __restrict__ boxes,
+                                            const int* __restrict__ mrows, S* __restrict__ Kar,
+                                            int* __restrict__ permar, double* __restrict__ Tar,
+                                            IdParams prm, int* __restrict__ kout,
+                                            int* __restrict__ iterout) {
+  extern __shared__ double sh[];
+  const int b = blockIdx.x;
+  const LvlBox B = boxes[b];
+  const int n = B.n, m = mrows[b], ld = B.mcap;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & (kWave - 1), wid = tid / kWave, nw = nt / kWave;
+  S* K = Kar + B.K_off;
+  int* perm = permar + B.perm_off;
+  double* vn1 = sh;
+  double* vn2 = sh + n;
+  double* v = sh + 2 * n;
+  double* red = v + (m > n ? m : n);
+  __shared__ int s_piv;
+
+  for (int j = tid; j < n; j += nt) perm[j] = j;
+  if (m == 0 || n == 0) {           // id.m:64-68: nothing to compress against
+    if (tid == 0) { kout[b] = 0; iterout[b] = 0; }
+    return;
+  }
+  int kmax = prm.kmax;
+  if (kmax <= 0 || kmax > n) kmax = n;
+
+  for (int j = wid; j < n; j += nw) {
+    double s = 0;
+    for (int i = lane; i < m; i += kWave) { double a = (double)K[i + (int64_t)j * ld]; s += a * a; }
+    s = wave_sum(s);
+    if (lane == 0) { vn1[j] = sqrt(s); vn2[j] = vn1[j]; }
+  }
+  __syncthreads();
+
+  const double tol3z = sqrt(Eps<S>::v);
+  const int smax = m < n ? m : n;
+  int k = 0;
+  double tol_abs = 0;
+  for (int s = 0; s < smax; ++s) {
+    double bv = -1.0;
+    int bi = n;
+    for (int j = s + tid; j < n; j += nt) argmax_merge(bv, bi, vn1[j], j);
+    block_argmax(bv, bi, red);
+    const int piv = bi;
+    if (piv != s) {
+      for (int i = tid; i < m; i += nt) {
+        S t = K[i + (int64_t)s * ld];
+        K[i + (int64_t)s * ld] = K[i + (int64_t)piv * ld];
+        K[i + (int64_t)piv * ld] = t;
+      }
+      if (tid == 0) {
+        double t = vn1[s]; vn1[s] = vn1[piv]; vn1[piv] = t;
+        t = vn2[s]; vn2[s] = vn2[piv]; vn2[piv] = t;
+        int q = perm[s]; perm[s] = perm[piv]; perm[piv] = q;
+      }
+      __syncthreads();
+    }
+    double part = 0;
+    for (int i = s + 1 + tid; i < m; i += nt) { double a = (double)K[i + (int64_t)s * ld]; part += a * a; }
+    const double sigma = block_sum(part, red);
+    const double alpha = (double)K[s + (int64_t)s * ld];
+    const double nrm = sqrt(alpha * alpha + sigma);
+    if (s == 0) tol_abs = nrm * prm.tol;      // id.m:106-107: relative to |R11|
+    if (!(nrm > tol_abs)) break;               // id.m:108-111
+    k = s + 1;
+    double beta, tau, scal;
+    if (sigma == 0) { beta = alpha; tau = 0; scal = 0; }
+    else {
+      beta = alpha >= 0 ? -nrm : nrm;
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    for (int i = s + tid; i < m; i += nt) v[i] = i == s ? 1.0 : (double)K[i + (int64_t)s * ld] * scal;
+    __syncthreads();
+    if (tid == 0) K[s + (int64_t)s * ld] = (S)beta;
+    const bool last = (k == kmax) || (s + 1 == smax);
+    for (int j = s + 1 + wid; j < n; j += nw) {
+      S* col = K + (int64_t)j * ld;
+      double dot = 0;
+      for (int i = s + lane; i < m; i += kWave) dot += v[i] * (double)col[i];
+      dot = wave_sum(dot) * tau;
+      if (last) {
+        if (lane == 0) col[s] = (S)((double)col[s] - dot);
+        continue;
+      }
+      double rsj = 0, tail = 0;
+      for (int i = s + lane; i < m; i += kWave) {
+        S nvS = (S)((double)col[i] - dot * v[i]);
+        col[i] = nvS;
+        double nv = (double)nvS;
+        if (i == s) rsj = nv; else tail += nv * nv;
+      }
+      rsj = __shfl(rsj, 0, kWave);
+      tail = wave_sum(tail);
+      if (lane == 0) {
+        double nv1 = vn1[j];
+        if (nv1 != 0) {
+          double t = fabs(rsj) / nv1;
+          t = fmax(0.0, (1.0 + t) * (1.0 - t));
+          double q = nv1 / vn2[j];
+          if (t * q * q <= tol3z) { vn1[j] = sqrt(tail); vn2[j] = vn1[j]; }
+          else vn1[j] = nv1 * sqrt(t);
+        }
+      }
+    }
+    __syncthreads();
+    if (last) break;
+  }
+
+  // T = R11 \ R12 in double, leading dimension n (id.m:114)
+  double* T = Tar + B.T_off;
+  double* R = T + (int64_t)n * n;
+  double* X = R + (int64_t)n * n;
+  const int r0 = n - k;
+  for (int c = tid; c < r0; c += nt) {
+    for (int l = k - 1; l >= 0; --l) {
+      double s = (double)K[l + (int64_t)(k + c) * ld];
+      for (int q = l + 1; q < k; ++q) s -= (double)K[l + (int64_t)q * ld] * T[q + (int64_t)c * n];
+      T[l + (int64_t)c * n] = s / (double)K[l + (int64_t)l * ld];
+    }
+  }
+  int iter = 0;
+  bool conv = true;
+  if (isfinite(prm.Tmax) && prm.rrqr_iter > 0 && k > 0 && k < n) {
+    // Gu-Eisenstat refinement in the fixed-precision form the reference
+    // takes when R has no trailing block (id.m:116-236, c2 = 0).
+    for (int e = tid; e < k * k; e += nt) {
+      int i = e % k, j = e / k;
+      R[i + (int64_t)j * n] = i <= j ? (double)K[i + (int64_t)j * ld] : 0.0;
+    }
+    __syncthreads();
+    const double f2 = prm.Tmax * prm.Tmax;
+    conv = false;
+    double* t = vn1;        // reuse shared scratch: t[k], u[k]
+    double* u = vn2;
+    // Every exchange multiplies |det R11| by more than Tmax, so the loop
+    // ends in exact arithmetic; the device also caps it at 4n + 64 swaps so
+    // that rounding can never keep a workgroup spinning.
+    const double cap = fmin(prm.rrqr_iter, 4.0 * n + 64.0);
+    while ((double)iter < cap) {
+      const int r = n - k;
+      double bv = -1.0;
+      int bi = k * r;
+      for (int e = tid; e < k * r; e += nt) {
+        int i = e % k, c = e / k;
+        double a = T[i + (int64_t)c * n];
+        argmax_merge(bv, bi, a * a, e);
+      }
+      block_argmax(bv, bi, red);
+      if (bv <= f2) { conv = true; break; }
+      ++iter;
+      const int i = bi % k, j = bi / k;
+      if (i < k - 1) swap_skel(perm, T, n, r, R, n, k, i, k - 1);
+      if (j > 0) {
+        if (tid == 0) { int q = perm[k]; perm[k] = perm[k + j]; perm[k + j] = q; }
+        for (int l = tid; l < k; l += nt) {
+          double a = T[l]; T[l] = T[l + (int64_t)j * n]; T[l + (int64_t)j * n] = a;
+        }
+        __syncthreads();
+      }
+      for (int l = tid; l < k; l += nt) t[l] = T[l];
+      __syncthreads();
+      // new last skeleton column in R-space: R * t
+      for (int l = tid; l < k; l += nt) {
+        double s = 0;
+        for (int q = 0; q < k; ++q) s += R[l + (int64_t)q * n] * t[q];
+        u[l] = s;
+      }
+      __syncthreads();
+      // rank-one update of T for the exchange of skeleton k-1 and redundant 0
+      const double g = 1.0 / t[k - 1];
+      for (int c = tid; c < r; c += nt) {
+        double* Tc = T + (int64_t)c * n;
+        if (c == 0) {
+          for (int l = 0; l < k - 1; ++l) Tc[l] = -t[l] * g;
+          Tc[k - 1] = g;
+        } else {
+          double rho = Tc[k - 1] * g;
+          for (int l = 0; l < k - 1; ++l) Tc[l] -= rho * t[l];
+          Tc[k - 1] = rho;
+        }
+      }
+      for (int l = tid; l < k; l += nt) R[l + (int64_t)(k - 1) * n] = u[l];
+      if (tid == 0) { int q = perm[k - 1]; perm[k - 1] = perm[k]; perm[k] = q; }
+      __syncthreads();
+      small_qr(R, k, n, v, red);
+      // id.m:210-229: drop a skeleton within tol of the others' span
+      double* r2 = u;
+      inv_row_norms(R, k, n, X, n, r2);
+      double best = -1.0;
+      int ib = k;
+      for (int l = tid; l < k; l += nt) argmax_merge(best, ib, r2[l], l);
+      block_argmax(best, ib, red);
+      if (1.0 / sqrt(best) > tol_abs) continue;
+      if (ib < k - 1) {
+        swap_skel(perm, T, n, r, R, n, k, ib, k - 1);
+        small_qr(R, k, n, v, red);
+      }
+      // skeleton k-1 becomes the first redundant column
+      const int k1 = k - 1;
+      for (int l = tid; l < k1; l += nt) t[l] = R[l + (int64_t)k1 * n];
+      __syncthreads();
+      if (tid == 0) {
+        for (int l = k1 - 1; l >= 0; --l) {
+          double s = t[l];
+          for (int q = l + 1; q < k1; ++q) s -= R[l + (int64_t)q * n] * t[q];
+          t[l] = s / R[l + (int64_t)l * n];
+        }
+      }
+      __syncthreads();
+      for (int e = tid; e < k1 * r; e += nt) {        // shift columns right by one
+        int l = e % k1, c = e / k1;
+        X[l + (int64_t)(c + 1) * n] = T[l + (int64_t)c * n] + t[l] * T[k1 + (int64_t)c * n];
+      }
+      for (int l = tid; l < k1; l += nt) X[l] = t[l];
+      __syncthreads();
+      for (int e = tid; e < k1 * (r + 1); e += nt) {
+        int l = e % k1, c = e / k1;
+        T[l + (int64_t)c * n] = X[l + (int64_t)c * n];
+      }
+      __syncthreads();
+      k = k1;
+      if (k == 0) break;
+    }
+  }
+  if (tid == 0) { kout[b] = k; iterout[b] = conv ? iter : -1 - iter; }
+}
+
+// ------------------------------------------------------------- k_factor --
+// child block lookup for the modified diagonal M (rskelf.m:190-199)
+struct ChildMap {
+  int nc;
+  int start[9];
+  int64_t moff[8];
+};
+
+template <typename S>
+__device__ inline S mval(const ChildMap& cm, const S* Mprev, int a, int b) {
+  for (int c = 0; c < cm.nc; ++c) {
+    int s0 = cm.start[c], s1 = cm.start[c + 1];
+    if (a >= s0 && a < s1) {
+      if (b < s0 || b >= s1 || cm.moff[c] < 0) return (S)0;
+      int len = s1 - s0;
+      return Mprev[cm.moff[c] + (a - s0) + (int64_t)(b - s0) * len];
+    }
+  }
+  return (S)0;
+}
+
+template <typename S, bool LU>
+__global__ void __launch_bounds__(256) k_factor(const LvlBox* __restrict__ boxes,
+                                                const BoxOut* __restrict__ outs,
+                                                const Seg* __restrict__ csegs, int* pool,
+                                                const int* __restrict__ permar,
+                                                const double* __restrict__ Tar,
+                                                const S* __restrict__ Mprev, S* __restrict__ Mcur,
+                                                S* __restrict__ War, S* __restrict__ fval,
+                                                int* __restrict__ fidx, PointsDev P,
+                                                int* __restrict__ fail) {
+  const LvlBox B = boxes[blockIdx.x];
+  const BoxOut O = outs[blockIdx.x];
+  const int n = B.n, k = O.k, r = O.r;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ ChildMap cm;
+  __shared__ double red[32];
+  if (tid == 0) {
+    int nc = B.ncs > 8 ? 8 : B.ncs;
+    cm.nc = nc;
+    int acc = 0;
+    for (int c = 0; c < nc; ++c) {
+      Seg g = csegs[B.cs0 + c];
+      cm.start[c] = acc;
+      cm.moff[c] = g.moff;
+      acc += g.len;
+    }
+    cm.start[nc] = acc;
+  }
+  __syncthreads();
+  const int* slf = pool + B.slf_off;
+  if (r == 0) {                      // rskelf.m:216: no compression, keep M
+    S* Mo = Mcur + O.M_off;
+    for (int e = tid; e < n * n; e += nt) Mo[e] = mval(cm, Mprev, e % n, e / n);
+    return;
+  }
+  const int* q = permar + B.perm_off;
+  const double* T = Tar + B.T_off;   // k x r, leading dimension n
+  S* W = War + B.W_off;              // n x n in skeleton-first order
+  for (int e = tid; e < n * n; e += nt) {
+    int a = e % n, c = e / n;
+    int qa = q[a], qc = q[c];
+    W[e] = (S)entry(P, slf[qa], slf[qc]) + mval(cm, Mprev, qa, qc);
+  }
+  __syncthreads();
+  // sparsify: K(rd,:) -= T' K(sk,:), then K(:,rd) -= K(:,sk) T (rskelf.m:220-223)
+  for (int e = tid; e < r * n; e += nt) {
+    int a = e % r, c = e / r;
+    const double* Ta = T + (int64_t)a * n;
+    S s = 0;
+    for (int l = 0; l < k; ++l) s += (S)Ta[l] * W[l + (int64_t)c * n];
+    W[(k + a) + (int64_t)c * n] -= s;
+  }
+  __syncthreads();
+  for (int e = tid; e < n * r; e += nt) {
+    int a = e % n, c = e / n;
+    const double* Tc = T + (int64_t)c * n;
+    S s = 0;
+    for (int l = 0; l < k; ++l) s += W[a + (int64_t)l * n] * (S)Tc[l];
+    W[a + (int64_t)(k + c) * n] -= s;
+  }
+  __syncthreads();
+  S* Krr = W + k + (int64_t)k * n;   // r x r, leading dimension n
+  int* piv = fidx + O.piv;
+  if (LU) {
+    for (int i = tid; i < r; i += nt) piv[i] = i;
+    __syncthreads();
+    for (int s = 0; s < r; ++s) {     // right-looking LU, partial pivoting
+      double bv = -1.0;
+      int bi = r;
+      for (int i = s + tid; i < r; i += nt) argmax_merge(bv, bi, fabs((double)Krr[i + (int64_t)s * n]), i);
+      block_argmax(bv, bi, red);
+      if (!(bv > 0)) { if (tid == 0) atomicOr(fail, 2); }
+      if (bi != s && bi < r) {
+        for (int c = tid; c < n; c += nt) {    // full rows: also permutes Krs
+          S a = W[(k + s) + (int64_t)c * n];
+          W[(k + s) + (int64_t)c * n] = W[(k + bi) + (int64_t)c * n];
+          W[(k + bi) + (int64_t)c * n] = a;
+        }
+        if (tid == 0) { int t = piv[s]; piv[s] = piv[bi]; piv[bi] = t; }
+        __syncthreads();
+      }
+      const S d = Krr[s + (int64_t)s * n];
+      for (int i = s + 1 + tid; i < r; i += nt) Krr[i + (int64_t)s * n] /= d;
+      __syncthreads();
+      const int w = r - s - 1;
+      for (int e = tid; e < w * w; e += nt) {
+        int i = s + 1 + e % w, j = s + 1 + e / w;
+        Krr[i + (int64_t)j * n] -= Krr[i + (int64_t)s * n] * Krr[s + (int64_t)j * n];
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int s = 0; s < r; ++s) {     // right-looking Cholesky, lower
+      S d = Krr[s + (int64_t)s * n];
+      if (!(d > (S)0)) { if (tid == 0) atomicOr(fail, 1); d = (S)1; }
+      d = sqrt(d);
+      __syncthreads();
+      if (tid == 0) Krr[s + (int64_t)s * n] = d;
+      for (int i = s + 1 + tid; i < r; i += nt) Krr[i + (int64_t)s * n] /= d;
+      __syncthreads();
+      const int w = r - s - 1;
+      for (int e = tid; e < w * w; e += nt) {
+        int i = s + 1 + e % w, j = s + 1 + e / w;
+        if (j <= i) Krr[i + (int64_t)j * n] -= Krr[i + (int64_t)s * n] * Krr[j + (int64_t)s * n];
+      }
+      __syncthreads();
+    }
+  }
+  // E = K(sk,rd)/U ('n') or K(sk,rd)/L' ('p'): row-wise forward sweeps
+  S* Ksr = W + (int64_t)k * n;       // k x r, leading dimension n
+  for (int a = tid; a < k; a += nt) {
+    for (int j = 0; j < r; ++j) {
+      S s = Ksr[a + (int64_t)j * n];
+      if (LU) for (int l = 0; l < j; ++l) s -= Ksr[a + (int64_t)l * n] * Krr[l + (int64_t)j * n];
+      else    for (int l = 0; l < j; ++l) s -= Ksr[a + (int64_t)l * n] * Krr[j + (int64_t)l * n];
+      Ksr[a + (int64_t)j * n] = s / Krr[j + (int64_t)j * n];
+    }
+  }
+  // G = L \ K(rd(p),sk) ('n'): column-wise forward sweeps with unit L
+  S* Krs = W + k;                    // r x k, leading dimension n
+  if (LU) {
+    for (int c = tid; c < k; c += nt) {
+      for (int i = 0; i < r; ++i) {
+        S s = Krs[i + (int64_t)c * n];
+        for (int l = 0; l < i; ++l) s -= Krr[i + (int64_t)l * n] * Krs[l + (int64_t)c * n];
+        Krs[i + (int64_t)c * n] = s;
+      }
+    }
+  }
+  __syncthreads();
+  // M = M(sk,sk) - E G  or  - E E' (rskelf.m:240-246)
+  S* Mo = Mcur + O.M_off;
+  for (int e = tid; e < k * k; e += nt) {
+    int a = e % k, c = e / k;
+    S s = 0;
+    if (LU) for (int l = 0; l < r; ++l) s += Ksr[a + (int64_t)l * n] * Krs[l + (int64_t)c * n];
+    else    for (int l = 0; l < r; ++l) s += Ksr[a + (int64_t)l * n] * Ksr[c + (int64_t)l * n];
+    Mo[e] = mval(cm, Mprev, q[a], q[c]) - s;
+  }
+  // store the factor (rskelf.m:249-257) and the new skeleton list
+  S* fT = fval + O.T;
+  for (int e = tid; e < k * r; e += nt) fT[e] = (S)T[(e % k) + (int64_t)(e / k) * n];
+  S* fLU = fval + O.LU;
+  for (int e = tid; e < r * r; e += nt) fLU[e] = Krr[(e % r) + (int64_t)(e / r) * n];
+  S* fE = fval + O.E;
+  for (int e = tid; e < k * r; e += nt) fE[e] = Ksr[(e % k) + (int64_t)(e / k) * n];
+  if (LU) {
+    S* fG = fval + O.G;
+    for (int e = tid; e < r * k; e += nt) fG[e] = Krs[(e % r) + (int64_t)(e / r) * n];
+  }
+  int* fsk = fidx + O.sk;
+  int* frd = fidx + O.rd;
+  for (int a = tid; a < k; a += nt) {
+    int pt = slf[q[a]];
+    fsk[a] = pt;
+    pool[O.nl_off + a] = pt;
+  }
+  for (int i = tid; i < r; i += nt) frd[i] = slf[q[k + i]];
+}
+
+// ------------------------------------------------------------ launchers --
+void launch_lists(const LvlBox* boxes, const Seg* nsegs, const Seg* csegs, int* pool, int* nbrar,
+                  int* mnbr, const PointsDev& P, const LevelParams& L, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(k_lists, dim3(nb), dim3(256), 0, st, boxes, nsegs, csegs, pool, nbrar, mnbr,
+                     P, L);
+}
+
+template <typename S>
+void launch_assemble(const LvlBox* boxes, const int* pool, const int* nbrar, const int* mnbr,
+                     int* mrows, S* Kar, const PointsDev& P, const LevelParams& L, int nb,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_assemble<S>, dim3(nb), dim3(256), 0, st, boxes, pool, nbrar, mnbr, mrows,
+                     Kar, P, L);
+}
+
+template <typename S>
+void launch_id(const LvlBox* boxes, const int* mrows, S* Kar, int* permar, double* Tar,
+               const IdParams& prm, int* kout, int* iterout, int nb, size_t shmem,
+               hipStream_t st) {
+  if (shmem > 65536)
+    (void)hipFuncSetAttribute((const void*)k_id<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)shmem);
+  hipLaunchKernelGGL(k_id<S>, dim3(nb), dim3(256), shmem, st, boxes, mrows, Kar, permar, Tar, prm,
+                     kout, iterout);
+}
+
+template <typename S>
+void launch_factor(bool lu, const LvlBox* boxes, const BoxOut* outs, const Seg* csegs, int* pool,
+                   const int* permar, const double* Tar, const S* Mprev, S* Mcur, S* War,
+                   S* fval, int* fidx, const PointsDev& P, int* fail, int nb, hipStream_t st) {
+  if (lu)
+    hipLaunchKernelGGL((k_factor<S, true>), dim3(nb), dim3(256), 0, st, boxes, outs, csegs, pool,
+                       permar, Tar, Mprev, Mcur, War, fval, fidx, P, fail);
+  else
+    hipLaunchKernelGGL((k_factor<S, false>), dim3(nb), dim3(256), 0, st, boxes, outs, csegs, pool,
+                       permar, Tar, Mprev, Mcur, War, fval, fidx, P, fail);
+}
+
+#define RSKX_INST(S)                                                                          \
+  template void launch_assemble<S>(const LvlBox*, const int*, const int*, const int*, int*, S*, \
+                                   const PointsDev&, const LevelParams&, int, hipStream_t);     \
+  template void launch_id<S>(const LvlBox*, const int*, S*, int*, double*, const IdParams&,     \
+                             int*, int*, int, size_t, hipStream_t);                            \
+  template void launch_factor<S>(bool, const LvlBox*, const BoxOut*, const Seg*, int*,          \
+                                 const int*, const double*, const S*, S*, S*, S*, int*,         \
+                                 const PointsDev&, int*, int, hipStream_t);
+RSKX_INST(double)
+RSKX_INST(float)
+
+}  // namespace rskx

@@ -1,0 +1,40 @@
+// Adaptive 2^d-tree over the points (host side).
+//
+// Follows core/hypoct.m:51-171: long sides (>= max side / sqrt 2) are halved
+// each level, a box holding more than occ points is split, a point goes to
+// the upper half of a divided dimension iff x > ctr strictly, and each box
+// lists as neighbours its siblings, adjacent coarser leaves and adjacent
+// children of its parent's neighbours.
+//
+// Points are kept in one index array that is stably partitioned as boxes
+// split, so every leaf owns a contiguous range of it, in ascending original
+// index order (the order hypoct's xi(idx == i) keeps).  That array is the
+// "tree order" the device works in.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace rskx {
+
+struct HostTree {
+  int d = 0;
+  int nlvl = 0;
+  std::vector<int64_t> lvp;       // box offsets per level, nlvl+1 entries
+  std::vector<double> lsize;      // d x nlvl box sizes
+  std::vector<double> ctr;        // d x nbox centres
+  std::vector<int> prnt;          // parent box, -1 for the root
+  std::vector<int> chld_off, chld;  // CSR children
+  std::vector<int> nbor_off, nbor;  // CSR neighbours
+  std::vector<int64_t> beg, cnt;  // leaf point range in `order` (cnt 0: not a leaf)
+  std::vector<int64_t> order;     // tree order -> original point index
+
+  int nbox() const { return (int)prnt.size(); }
+  int level_of(int b) const;      // 0-based level (root = 0)
+  bool leaf(int b) const { return chld_off[b + 1] == chld_off[b]; }
+};
+
+// x: host d x N column-major.  lvlmax <= 0 means no limit.  ext: d x 2
+// column-major (lower bounds then upper bounds) or nullptr.
+HostTree build_tree(const double* x, int d, int64_t N, int occ, int lvlmax, const double* ext);
+
+}  // namespace rskx

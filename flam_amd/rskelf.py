@@ -1,0 +1,244 @@
+"""Python mirror of FLAM's rskelf interface over the HIP library.
+
+    F = rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None)   # rskelf/rskelf.m:109
+    Y = rskelf_sv(F, X, trans='n')                                # rskelf/rskelf_sv.m:13
+    Y = rskelf_mv(F, X, trans='n')                                # rskelf/rskelf_mv.m:13
+
+Differences forced by the device: A is a kernel description (LaplaceDL2D)
+rather than a MATLAB function handle, pxyfun a ProxyRing rather than a
+callback, and all arrays are torch tensors on the GPU.  Argument errors
+raise FlamError carrying the reference's identifiers (FLAM:chksymm:...,
+FLAM:id:..., FLAM:hypoct:...).  Like MATLAB, rskelf_sv / rskelf_mv return
+a new array and leave X untouched.
+"""
+import ctypes
+import math
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib as L
+
+
+class FlamError(ValueError):
+    """Error raised by the library; `ident` is the FLAM:* identifier."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+        self.ident = msg.split(": ")[0] if ": " in msg else msg
+
+
+def _check(rc):
+    if rc != L.RSKX_OK:
+        raise FlamError(rc, L.last_error())
+
+
+def _dev(t, dtype=torch.float64):
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(t, dtype=dtype)
+    if not t.is_cuda:
+        t = t.to("cuda")
+    return t.to(dtype).contiguous()
+
+
+@dataclass
+class LaplaceDL2D:
+    """A_ij = scale w_j ((x_i - x_j).nu_j) / (2 pi |x_i - x_j|^2), A_ii = diag_i.
+
+    nu: (2, N) unit normals, w: (N,) weights, diag: (N,) diagonal.  This is
+    the Afun of rskelf/test/ie_circle.m:85-103 (scale 1) and
+    rskelf/test/ie_ellipse.m:103-123, written as data."""
+    nu: torch.Tensor
+    w: torch.Tensor
+    diag: torch.Tensor
+    scale: float = 1.0
+
+    def _struct(self):
+        self.nu = _dev(self.nu).reshape(2, -1)
+        self.w = _dev(self.w).reshape(-1)
+        self.diag = _dev(self.diag).reshape(-1)
+        # column-major d x N = row-major N x d
+        self._nu_cm = self.nu.t().contiguous()
+        return L.Kernel(L.RSKX_KERNEL_LAPLACE2D_DL, self._nu_cm.data_ptr(), self.w.data_ptr(),
+                        self.diag.data_ptr(), float(self.scale))
+
+
+@dataclass
+class ProxyRing:
+    """pxyfun of the demos: p points on a ring of radius `radius` box sizes
+    around the box centre; neighbours inside the ring are kept explicitly.
+    kind 'sl': single layer times `weight` (ie_circle.m:106-115);
+    kind 'dl': double layer of the box points (ie_ellipse.m:126-132)."""
+    p: int = 64
+    radius: float = 1.5
+    kind: str = "sl"
+    weight: float = 0.0
+
+    def _struct(self):
+        kind = {"sl": L.RSKX_PROXY_RING_SL, "dl": L.RSKX_PROXY_RING_DL}[self.kind]
+        return L.Proxy(kind, int(self.p), float(self.radius), float(self.weight))
+
+
+_PREC = {"fp64": L.RSKX_PREC_FP64, "fp32": L.RSKX_PREC_FP32}
+
+
+class Factor:
+    """Device-resident factorization (opaque handle of the C ABI)."""
+
+    def __init__(self, handle, N, keep):
+        self._h = handle
+        self.N = N
+        self._keep = keep
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                L.lib().rskx_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        inf = L.Info()
+        _check(L.lib().rskx_factor_info(self._h, ctypes.byref(inf)))
+        return dict(N=inf.N, d=inf.d, nlvl=inf.nlvl, nlvl_done=inf.nlvl_done,
+                    nfactor=inf.nfactor, bytes=inf.bytes,
+                    precision="fp64" if inf.precision == L.RSKX_PREC_FP64 else "fp32",
+                    symm=inf.symm.decode())
+
+    def level_table(self):
+        """(lvl, nblk, start npts, end npts) per level, as rskelf.m:264-272 prints."""
+        n = L.lib().rskx_level_table(self._h, None, 0)
+        rows = (ctypes.c_int64 * (4 * max(n, 1)))()
+        L.lib().rskx_level_table(self._h, rows, n)
+        return [tuple(rows[4 * i: 4 * i + 4]) for i in range(n)]
+
+    def export(self, i):
+        """Host copy of F.factors(i) (0-based): dict sk, rd, T, L, U, p, E, F
+        as numpy arrays (indices 0-based)."""
+        import numpy as np
+        v = L.FactorView()
+        _check(L.lib().rskx_export_factor(self._h, i, ctypes.byref(v)))
+        k, r = v.k, v.r
+        lu = self.info()["symm"] == "n"
+        arr = dict(sk=np.zeros(k, np.int64), rd=np.zeros(r, np.int64),
+                   T=np.zeros((r, k)), L=np.zeros((r, r)), U=np.zeros((r, r)),
+                   p=np.zeros(r, np.int64), E=np.zeros((r, k)), F=np.zeros((k, r)))
+
+        def ptr(a, t):
+            return a.ctypes.data_as(ctypes.POINTER(t))
+        v.sk, v.rd, v.p = ptr(arr["sk"], ctypes.c_int64), ptr(arr["rd"], ctypes.c_int64), ptr(arr["p"], ctypes.c_int64)
+        for key in ("T", "L", "U", "E", "F"):
+            setattr(v, key, ptr(arr[key], ctypes.c_double))
+        _check(L.lib().rskx_export_factor(self._h, i, ctypes.byref(v)))
+        out = dict(sk=arr["sk"], rd=arr["rd"], T=arr["T"].T.copy(), L=arr["L"].T.copy(),
+                   E=arr["E"].T.copy(), U=arr["U"].T.copy() if lu else None,
+                   p=arr["p"] if lu else None, F=arr["F"].T.copy() if lu else None)
+        return out
+
+    def kernel_times(self):
+        n = L.lib().rskx_kernel_times(self._h, None, None, None, None, None, 0)
+        names = (ctypes.c_char_p * max(n, 1))()
+        ms = (ctypes.c_double * max(n, 1))()
+        cnt = (ctypes.c_int64 * max(n, 1))()
+        byt = (ctypes.c_int64 * max(n, 1))()
+        fl = (ctypes.c_double * max(n, 1))()
+        L.lib().rskx_kernel_times(self._h, names, ms, cnt, byt, fl, n)
+        return {names[i].decode(): dict(ms=ms[i], launches=cnt[i], bytes=byt[i], flops=fl[i])
+                for i in range(n)}
+
+
+def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
+    """F = rskelf(A, x, occ, rank_or_tol, pxyfun, opts) on the GPU.
+
+    opts keys (rskelf.m:112-120): lvlmax (inf), ext (None), Tmax (2),
+    rrqr_iter (inf), symm ('n'), verb (0); plus precision ('fp64' or
+    'fp32') and profile (0)."""
+    opts = dict(opts or {})
+    x = _dev(x)
+    if x.dim() != 2:
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: x must be d x N")
+    d, N = x.shape
+    xc = x.t().contiguous()                # column-major d x N
+    o = L.Opts()
+    L.lib().rskx_opts_default(ctypes.byref(o))
+    lvlmax = opts.get("lvlmax", math.inf)
+    o.lvlmax = 0 if lvlmax == math.inf else int(lvlmax)
+    ext = opts.get("ext")
+    ext_buf = None
+    if ext is not None:
+        import numpy as np
+        e = np.asarray(ext, dtype=np.float64).reshape(d, 2)
+        ext_buf = (ctypes.c_double * (2 * d))(*e.T.ravel())   # lower bounds, then upper
+        o.ext = ext_buf
+    o.Tmax = float(opts.get("Tmax", 2.0))
+    o.rrqr_iter = float(opts.get("rrqr_iter", math.inf))
+    o.symm = str(opts.get("symm", "n")).encode()[:1]
+    o.verb = int(opts.get("verb", 0))
+    prec = opts.get("precision", "fp64")
+    if prec not in _PREC:
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:precision: precision must be 'fp64' or 'fp32'")
+    o.precision = _PREC[prec]
+    o.profile = int(opts.get("profile", 0))
+    if "stop" in opts and opts["stop"] not in (None, math.inf):
+        raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:stop: partial factorization is not built on the device")
+    k = A._struct()
+    p = pxyfun._struct() if pxyfun is not None else L.Proxy(L.RSKX_PROXY_NONE, 0, 1.5, 0.0)
+    h = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    _check(L.lib().rskx_rskelf(xc.data_ptr(), d, N, int(occ), float(rank_or_tol),
+                               ctypes.byref(k), ctypes.byref(p), ctypes.byref(o), ctypes.byref(h)))
+    return Factor(h.value, N, keep=(A, xc))
+
+
+def _sweep(fn, F, X, trans):
+    X = X if isinstance(X, torch.Tensor) else torch.as_tensor(X)
+    if not X.is_cuda:
+        X = X.to("cuda")
+    if X.dtype not in (torch.float64, torch.float32):
+        X = X.to(torch.float64)
+    vec = X.dim() == 1
+    X2 = X.reshape(-1, 1) if vec else X
+    if X2.shape[0] != F.N:
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: X must have N rows")
+    Xc = X2.t().contiguous()               # column-major N x nrhs, a copy
+    dt = L.RSKX_F64 if Xc.dtype == torch.float64 else L.RSKX_F32
+    t = str(trans).encode()[:1] if trans else b"n"
+    torch.cuda.synchronize()
+    _check(fn(F.handle, Xc.data_ptr(), F.N, Xc.shape[0], t, dt))
+    Y = Xc.t()
+    return Y.reshape(-1) if vec else Y
+
+
+def rskelf_sv(F, X, trans="n"):
+    """Apply F^{-1} (rskelf/rskelf_sv.m:13-36)."""
+    return _sweep(L.lib().rskx_rskelf_sv, F, X, trans)
+
+
+def rskelf_mv(F, X, trans="n"):
+    """Apply F (rskelf/rskelf_mv.m:13-36)."""
+    return _sweep(L.lib().rskx_rskelf_mv, F, X, trans)
+
+
+def kernel_matvec(A, x, X):
+    """FP64 Y = A X with the kernel evaluated on the fly (no compression)."""
+    x = _dev(x)
+    d, N = x.shape
+    xc = x.t().contiguous()
+    X = _dev(X)
+    vec = X.dim() == 1
+    X2 = X.reshape(-1, 1) if vec else X
+    Xc = X2.t().contiguous()
+    Yc = torch.empty_like(Xc)
+    k = A._struct()
+    torch.cuda.synchronize()
+    _check(L.lib().rskx_kernel_matvec(xc.data_ptr(), d, N, ctypes.byref(k), Xc.data_ptr(), N,
+                                      Yc.data_ptr(), N, Xc.shape[0]))
+    Y = Yc.t()
+    return Y.reshape(-1) if vec else Y

@@ -1,0 +1,157 @@
+/*
+ * rskx.h -- C ABI of the MI355X recursive skeletonization factorization.
+ *
+ * Drop-in boundary for FLAM's rskelf path.  The reference is MATLAB; the
+ * entry points below are what a MEX gateway (or Python ctypes) binds in
+ * place of the MATLAB functions cited next to each one.  Conventions:
+ *
+ *   - Every array argument is a DEVICE pointer (hipMalloc'd or a torch
+ *     tensor's data_ptr) unless stated otherwise; the caller keeps
+ *     ownership.  Matrices are column-major, like MATLAB arrays.
+ *   - Indices handed back to the host (rskx_export_factor) are 0-based;
+ *     the gateway adds 1 for MATLAB.
+ *   - Every call is synchronous on the library's stream (rskx_set_stream,
+ *     default: the null stream) and returns RSKX_OK (0) or a negative
+ *     code; rskx_last_error() then gives a message that starts with the
+ *     reference's error identifier ("FLAM:chksymm:invalidSymm: ...").
+ *   - A factor is device-resident behind an opaque handle; one handle must
+ *     not be used from two threads at once.
+ */
+#ifndef RSKX_H
+#define RSKX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSKX_OK 0
+#define RSKX_ERR_ARG (-1)     /* invalid argument (FLAM:* identifier)       */
+#define RSKX_ERR_HIP (-2)     /* HIP runtime failure                        */
+#define RSKX_ERR_NUM (-3)     /* numerical failure (e.g. chol not SPD)      */
+#define RSKX_ERR_UNSUP (-4)   /* option outside what this library builds   */
+#define RSKX_ERR_MEM (-5)     /* allocation failure                         */
+
+/* precision of the factorization (rskx_opts.precision) */
+#define RSKX_PREC_FP64 0      /* everything in FP64                         */
+#define RSKX_PREC_FP32 1      /* FP32 storage/assembly/LU/Schur/sweeps, FP64 ID
+                                 norms and pivots, FP64 root factorization    */
+
+/* element type of sweep right-hand sides */
+#define RSKX_F64 0
+#define RSKX_F32 1
+
+/* built-in matrices (the reference passes a MATLAB function handle A(I,J);
+ * a handle cannot run on the GPU, so the kernel is described by data) */
+#define RSKX_KERNEL_LAPLACE2D_DL 1
+/*   A_ij = scale * w_j * ((x_i - x_j) . nu_j) / (2 pi |x_i - x_j|^2), i != j
+ *   A_ii = diag_i
+ * rskelf/test/ie_circle.m:85-103 (scale 1, diag -1/2(1+1/N)),
+ * rskelf/test/ie_ellipse.m:103-123 (scale 1, diag -1/2(1 + h kappa/2pi)),
+ * and the benchmark circle (scale -1, diag 1/2 - w/4pi). */
+typedef struct {
+  int kind;             /* RSKX_KERNEL_*                                    */
+  const double* nu;     /* d x N unit normals (device)                      */
+  const double* w;      /* N quadrature weights (device)                    */
+  const double* diag;   /* N diagonal entries (device)                      */
+  double scale;         /* off-diagonal multiplier                          */
+} rskx_kernel;
+
+/* proxy surfaces (the reference passes pxyfun(x,slf,nbr,l,ctr)) */
+#define RSKX_PROXY_NONE 0     /* naive global compression (rskelf.m:204)     */
+#define RSKX_PROXY_RING_SL 1  /* ring, single layer -w/2pi log r (ie_circle.m:106-115) */
+#define RSKX_PROXY_RING_DL 2  /* ring, double layer of slf (ie_ellipse.m:126-132) */
+typedef struct {
+  int kind;             /* RSKX_PROXY_*                                     */
+  int p;                /* number of proxy points (64 in the demos)         */
+  double radius;        /* ring radius and neighbour filter radius, in units
+                           of the box size (1.5 in ie_circle.m:35,114)      */
+  double weight;        /* RING_SL multiplier (2 pi / N in ie_circle.m:111) */
+} rskx_proxy;
+
+/* options: rskelf.m:112-128 */
+typedef struct {
+  int lvlmax;           /* maximum tree depth, <= 0 means Inf               */
+  const double* ext;    /* HOST d x 2 root extent, or NULL (from the data)  */
+  double Tmax;          /* ID interpolation bound, default 2                */
+  double rrqr_iter;     /* max RRQR iterations, default Inf                 */
+  char symm;            /* 'n' (LU) or 'p' (Cholesky); 's','h' -> UNSUP     */
+  int verb;             /* 1: print the per-level table to stdout           */
+  int precision;        /* RSKX_PREC_*                                      */
+  int profile;          /* 1: time each kernel class with HIP events        */
+} rskx_opts;
+
+typedef struct rskx_factor rskx_factor;
+
+typedef struct {
+  int64_t N;            /* matrix order                                     */
+  int d;                /* point dimension                                  */
+  int nlvl;             /* tree levels                                      */
+  int nlvl_done;        /* levels factored                                  */
+  int64_t nfactor;      /* number of per-box factors (F.lvp(end))           */
+  int64_t bytes;        /* device bytes held by the factor                  */
+  int precision;        /* RSKX_PREC_*                                      */
+  char symm;
+} rskx_info;
+
+/* host copy of one factor, F.factors(i) of rskelf.m:249-257.  Call with
+ * NULL arrays to get the sizes k = |sk|, r = |rd|, then again with
+ * buffers of k, r, k*r, r*r, r*r, r, k*r, r*k elements.  Matrices are
+ * column-major doubles; U and F are written only for symm 'n', p (0-based
+ * pivot vector, K(rd(p),rd) = L U) only for 'n'. */
+typedef struct {
+  int64_t k, r;
+  int64_t* sk; int64_t* rd;
+  double* T; double* L; double* U; int64_t* p; double* E; double* F;
+} rskx_factor_view;
+
+/* defaults of rskelf.m:112-120 */
+int rskx_opts_default(rskx_opts* opts);
+
+/* F = rskelf(A,x,occ,rank_or_tol,pxyfun,opts)   -- rskelf/rskelf.m:109
+ * x: device d x N coordinates.  d must be 2 for RSKX_KERNEL_LAPLACE2D_DL. */
+int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
+                const rskx_kernel* A, const rskx_proxy* pxy, const rskx_opts* opts,
+                rskx_factor** out);
+
+/* Y = rskelf_sv(F,X,trans)                      -- rskelf/rskelf_sv.m:13
+ * In place on device X (N x nrhs, leading dimension ldx, type dtype).
+ * trans 'n', 't' or 'c' (core/chktrans.m:8). */
+int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
+                   int dtype);
+
+/* Y = rskelf_mv(F,X,trans)                      -- rskelf/rskelf_mv.m:13 */
+int rskx_rskelf_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
+                   int dtype);
+
+void rskx_free(rskx_factor* F);
+const char* rskx_last_error(void);
+int rskx_set_stream(void* hip_stream);
+
+int rskx_factor_info(const rskx_factor* F, rskx_info* info);
+
+/* The verbose table of rskelf.m:264-272: for each factored level, rows of
+ * (lvl, nblk, start npts, end npts) written to host rows[4*i..4*i+3].
+ * Returns the number of rows. */
+int rskx_level_table(const rskx_factor* F, int64_t* rows, int maxrows);
+
+/* Host copy of F.factors(i) (0-based i), indices in the caller's order. */
+int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* view);
+
+/* Accumulated HIP-event time (ms) and launch count per kernel class of the
+ * last rskx_rskelf / sweep calls when opts.profile was set.  names[i] is a
+ * static string.  Returns the number of classes. */
+int rskx_kernel_times(const rskx_factor* F, const char** names, double* ms, int64_t* launches,
+                      int64_t* bytes, double* flops, int maxn);
+
+/* FP64 product Y = A X with the built-in matrix, evaluated on the fly
+ * (refinement residuals and tests; the reference's exact matvecs,
+ * e.g. rskelf/test/ie_ellipse.m:56).  All pointers device. */
+int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A,
+                       const double* X, int64_t ldx, double* Y, int64_t ldy, int nrhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSKX_H */

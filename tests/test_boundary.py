@@ -1,0 +1,102 @@
+"""The C-ABI library: it loads, exports every entry point of include/rskx.h,
+and rejects bad arguments with the reference's error identifiers before it
+touches a GPU.  CPU only (no compute calls)."""
+import ctypes
+import math
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _lib():
+    from flam_amd import _lib as L
+    if not os.path.exists(L.LIB_PATH):
+        from flam_amd import build
+        build.build()
+    return L
+
+
+def _header_functions():
+    src = open(os.path.join(ROOT, "include", "rskx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rskx_[a-z_]+)\s*\(", src)))
+
+
+def test_header_declares_expected_surface():
+    names = _header_functions()
+    for want in ("rskx_rskelf", "rskx_rskelf_sv", "rskx_rskelf_mv", "rskx_free",
+                 "rskx_last_error", "rskx_export_factor", "rskx_level_table"):
+        assert want in names
+
+
+def test_library_exports_every_header_symbol():
+    L = _lib()
+    lib = ctypes.CDLL(L.LIB_PATH)
+    for name in _header_functions():
+        assert hasattr(lib, name), name
+    assert set(_header_functions()) == set(L.SIGNATURES)
+
+
+def test_opts_defaults_match_reference():
+    L = _lib()
+    o = L.Opts()
+    assert L.lib().rskx_opts_default(ctypes.byref(o)) == 0
+    # rskelf.m:112-120
+    assert o.lvlmax == 0 and not o.ext and o.Tmax == 2.0 and math.isinf(o.rrqr_iter)
+    assert o.symm == b"n" and o.verb == 0 and o.precision == L.RSKX_PREC_FP64
+
+
+def _call(L, *, symm=b"n", occ=64, tol=1e-6, Tmax=2.0, rrqr=math.inf, lvlmax=0, d=2,
+          kind=None, pkind=None):
+    o = L.Opts()
+    L.lib().rskx_opts_default(ctypes.byref(o))
+    o.symm, o.Tmax, o.rrqr_iter, o.lvlmax = symm, Tmax, rrqr, lvlmax
+    dummy = 16  # never dereferenced: validation fails first
+    k = L.Kernel(kind or L.RSKX_KERNEL_LAPLACE2D_DL, dummy, dummy, dummy, 1.0)
+    p = L.Proxy(pkind if pkind is not None else L.RSKX_PROXY_RING_SL, 64, 1.5, 1.0)
+    h = ctypes.c_void_p()
+    rc = L.lib().rskx_rskelf(dummy, d, 100, occ, tol, ctypes.byref(k), ctypes.byref(p),
+                             ctypes.byref(o), ctypes.byref(h))
+    return rc, L.last_error()
+
+
+@pytest.mark.parametrize("kw,ident", [
+    (dict(symm=b"q"), "FLAM:chksymm:invalidSymm"),
+    (dict(occ=-1), "FLAM:hypoct:invalidOcc"),
+    (dict(lvlmax=-3), "FLAM:hypoct:invalidLvlmax"),
+    (dict(tol=-1.0), "FLAM:id:invalidRankOrTol"),
+    (dict(Tmax=0.5), "FLAM:id:invalidTmax"),
+    (dict(rrqr=-1.0), "FLAM:id:invalidRRQRIter"),
+])
+def test_argument_errors_carry_reference_ids(kw, ident):
+    L = _lib()
+    rc, msg = _call(L, **kw)
+    assert rc == L.RSKX_ERR_ARG
+    assert msg.startswith(ident)
+
+
+@pytest.mark.parametrize("kw", [dict(symm=b"h"), dict(symm=b"s"), dict(d=3),
+                                dict(pkind=0), dict(tol=5.0)])
+def test_unbuilt_options_are_refused(kw):
+    L = _lib()
+    rc, msg = _call(L, **kw)
+    assert rc == L.RSKX_ERR_UNSUP
+    assert msg.startswith("FLAM:rskx:")
+
+
+def test_bad_trans_is_refused():
+    L = _lib()
+    rc = L.lib().rskx_rskelf_sv(None, None, 1, 1, b"x", L.RSKX_F64)
+    assert rc == L.RSKX_ERR_ARG       # NULL factor checked first
+    assert L.last_error().startswith("FLAM:rskx:null")
+
+
+def test_python_mirror_rejects_without_gpu():
+    """The Python layer raises FlamError (not a silent CPU path)."""
+    import flam_amd
+    from flam_amd.rskelf import FlamError
+    assert issubclass(FlamError, ValueError)
+    assert hasattr(flam_amd, "rskelf") and hasattr(flam_amd, "rskelf_sv")
