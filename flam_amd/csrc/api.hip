@@ -219,6 +219,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
       hp[(size_t)(2 * d + 1) * N + p] = hdg[q];
     }
     RSKX_HIP(hipMemcpyAsync(pts, hp.data(), hp.size() * 8, hipMemcpyHostToDevice, g_stream));
+    RSKX_HIP(hipStreamSynchronize(g_stream));   // hp is freed at the end of this block
   }
   PointsDev P{};
   for (int i = 0; i < 3; ++i) {
@@ -241,6 +242,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     std::vector<int> iota(N);
     for (int64_t p = 0; p < N; ++p) iota[p] = (int)p;
     RSKX_HIP(hipMemcpyAsync(ws.pool.p, iota.data(), N * sizeof(int), hipMemcpyHostToDevice, g_stream));
+    RSKX_HIP(hipStreamSynchronize(g_stream));
   }
   int64_t pool_top = N;
   if (!ws.hflag) RSKX_HIP(hipHostMalloc((void**)&ws.hflag, 64 * sizeof(int), hipHostMallocDefault));

@@ -207,7 +207,9 @@ def _sweep(fn, F, X, trans):
     X2 = X.reshape(-1, 1) if vec else X
     if X2.shape[0] != F.N:
         raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: X must have N rows")
-    Xc = X2.t().contiguous()               # column-major N x nrhs, a copy
+    # column-major N x nrhs; always a fresh copy (a (1, N) view of a vector
+    # is already "contiguous" and would alias the caller's storage)
+    Xc = X2.t().clone(memory_format=torch.contiguous_format)
     dt = L.RSKX_F64 if Xc.dtype == torch.float64 else L.RSKX_F32
     t = str(trans).encode()[:1] if trans else b"n"
     torch.cuda.synchronize()

@@ -208,3 +208,19 @@ def test_readme_configuration_rank_table():
     for (lvl, nblk, s, e), (rl, rn, rs, re) in zip(F.level_table(), gold["rows"]):
         assert (lvl, nblk) == (rl, rn)
         assert abs(e - re) <= max(2, 0.06 * re)
+
+
+def test_sweeps_leave_input_untouched():
+    """rskelf_sv / rskelf_mv return new arrays (MATLAB value semantics);
+    a 1-D CUDA input must not be overwritten by the in-place device sweep."""
+    flam_amd, geometry = _dev()
+    g = geometry.circle(1024)
+    F = flam_amd.rskelf(g.kernel, g.x, 32, 1e-10, g.proxy, dict(symm="p"))
+    for shape in ((1024,), (1024, 1), (1024, 3)):
+        b = torch.randn(*shape, dtype=torch.float64, device="cuda")
+        keep = b.clone()
+        x = flam_amd.rskelf_sv(F, b)
+        y = flam_amd.rskelf_mv(F, b)
+        assert torch.equal(b, keep)
+        assert x.shape == b.shape and y.shape == b.shape
+        assert not torch.equal(x, b)
