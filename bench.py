@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-N", type=int, default=1 << 17)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a known-size "
+                         "reference dispatch for FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -137,6 +140,14 @@ def main():
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
     N = args.N
     log(f"workload: circle N={N} occ={args.occ} tol={args.tol:g} nrhs={args.nrhs} "
         f"target {args.target:g}")
