@@ -29,14 +29,15 @@ void launch_lists(const LvlBox*, const Seg*, const Seg*, int*, int*, int*, const
                   const LevelParams&, int, hipStream_t);
 template <typename S>
 void launch_assemble(const LvlBox*, const int*, const int*, const int*, int*, S*,
-                     const PointsDev&, const LevelParams&, int, hipStream_t);
+                     const PointsDev&, const LevelParams&, int, size_t, hipStream_t);
 template <typename S>
 void launch_id(const LvlBox*, const int*, S*, int*, double*, const IdParams&, int*, int*, int,
                size_t, hipStream_t);
+size_t elim_elems(bool lu, int k, int r);
 template <typename S>
-void launch_factor(bool, const LvlBox*, const BoxOut*, const Seg*, int*, const int*,
-                   const double*, const S*, S*, S*, S*, int*, const PointsDev&, int*, int,
-                   hipStream_t);
+void launch_elim(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*,
+                 int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
+                 int*, hipStream_t);
 template <typename S>
 void launch_sweep(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int,
                   size_t, hipStream_t);
@@ -44,6 +45,8 @@ template <typename U, typename S>
 void launch_perm(bool, void*, int64_t, S*, int64_t, int, const int*, hipStream_t);
 void launch_matvec(const double*, int64_t, const double*, const double*, const double*, double,
                    const double*, int64_t, double*, int64_t, int, hipStream_t);
+void launch_gather_points(const double*, const double*, const double*, const double*, const int*,
+                          int64_t, int, double*, hipStream_t);
 
 // --------------------------------------------------------------- errors --
 static thread_local std::string g_err;
@@ -80,7 +83,7 @@ struct DevBuf {
 
 // scratch reused across factorizations of this process
 struct Workspace {
-  DevBuf boxes, nsegs, csegs, outs, pool, nbr, mnbr, mrows, kout, iter, K, perm, T, W, M[2], flag;
+  DevBuf boxes, nsegs, csegs, outs, ids, pool, nbr, mnbr, mrows, kout, iter, K, perm, T, W, M[2], flag;
   DevBuf pts, xw;
   int* hflag = nullptr;
 };
@@ -201,26 +204,9 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   RSKX_TRY(upload(F->order, ord32));
   RSKX_TRY(ws.pts.ensure((size_t)N * (2 * d + 2) * sizeof(double)));
   double* pts = ws.pts.as<double>();
-  {
-    // gather on the host copy of x; the other arrays are gathered on device
-    std::vector<double> hp((size_t)N * (2 * d + 2));
-    std::vector<double> hnu((size_t)d * N), hw(N), hdg(N);
-    RSKX_HIP(hipMemcpyAsync(hnu.data(), A.nu, hnu.size() * 8, hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipMemcpyAsync(hw.data(), A.w, N * 8, hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipMemcpyAsync(hdg.data(), A.diag, N * 8, hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipStreamSynchronize(g_stream));
-    for (int64_t p = 0; p < N; ++p) {
-      int64_t q = t.order[p];
-      for (int i = 0; i < d; ++i) {
-        hp[(size_t)i * N + p] = hx[(size_t)d * q + i];
-        hp[(size_t)(d + i) * N + p] = hnu[(size_t)d * q + i];
-      }
-      hp[(size_t)2 * d * N + p] = hw[q];
-      hp[(size_t)(2 * d + 1) * N + p] = hdg[q];
-    }
-    RSKX_HIP(hipMemcpyAsync(pts, hp.data(), hp.size() * 8, hipMemcpyHostToDevice, g_stream));
-    RSKX_HIP(hipStreamSynchronize(g_stream));   // hp is freed at the end of this block
-  }
+  // gather points, normals, weights and diagonal into tree order on the device
+  launch_gather_points(x, A.nu, A.w, A.diag, F->order.as<int>(), N, d, pts, g_stream);
+  RSKX_HIP(hipGetLastError());
   PointsDev P{};
   for (int i = 0; i < 3; ++i) {
     P.x[i] = i < d ? pts + (size_t)i * N : nullptr;
@@ -291,7 +277,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     hn.clear();
     hc.clear();
     int64_t off_slf = 0, off_nbr = 0, off_K = 0, off_perm = 0, off_T = 0, off_W = 0;
-    int nmax = 0, mcapmax = 0;
+    int nmax = 0, mcapmax = 0, mcmax = 0;
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int b = (int)(b0 + ib);
       LvlBox& B = hb[ib];
@@ -330,10 +316,11 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
       B.K_off = off_K;                 off_K += (int64_t)B.mcap * n;
       B.perm_off = off_perm;           off_perm += n;
       B.T_off = off_T;                 off_T += 3 * (int64_t)n * n;
-      B.W_off = off_W;                 off_W += (int64_t)n * n;
+      B.W_off = off_W;                 off_W += 2 * (int64_t)n * n;
       for (int i = 0; i < d; ++i) B.ctr[i] = t.ctr[(size_t)d * b + i];
       nmax = std::max(nmax, n);
       mcapmax = std::max(mcapmax, B.mcap);
+      mcmax = std::max(mcmax, mc);
     }
     if (pool_top + off_slf > pool_cap) return fail(RSKX_ERR_MEM, "FLAM:rskx:pool: point-list pool exhausted");
     RSKX_TRY(upload(ws.boxes, hb));
@@ -350,6 +337,8 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     RSKX_TRY(ws.W.ensure(std::max<int64_t>(off_W, 1) * sizeof(S)));
 
     size_t shm = (size_t)(2 * nmax + std::max(nmax, mcapmax) + 32) * sizeof(double);
+    if ((size_t)(nmax + mcmax) * 5 * sizeof(double) > 160 * 1024)
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box neighbourhood too large for the assembly kernel");
     if (shm > 160 * 1024)
       return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box too large for the ID kernel (n=" +
                                        std::to_string(nmax) + ", m=" + std::to_string(mcapmax) + ")");
@@ -362,7 +351,8 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     for (auto& B : hb) Kbytes += (int64_t)B.mcap * B.n;
     prof.begin("assemble", ev);
     launch_assemble<S>(ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(), ws.mnbr.as<int>(),
-                       ws.mrows.as<int>(), ws.K.as<S>(), P, L, (int)nb, g_stream);
+                       ws.mrows.as<int>(), ws.K.as<S>(), P, L, (int)nb,
+                       (size_t)(nmax + mcmax) * 5 * sizeof(double), g_stream);
     prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
     prof.begin("id", ev);
     launch_id<S>(ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(), ws.perm.as<int>(),
@@ -422,11 +412,35 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     RSKX_TRY(ws.M[mcur].ensure(std::max<int64_t>(offM, 1) * sizeof(S)));
     RSKX_TRY(ws.M[mpar].ensure(sizeof(S)));
     RSKX_TRY(upload(ws.outs, ho));
+    // elimination: boxes whose blocks fit two per CU in shared memory, one
+    // per CU, or neither (global scratch; also the no-compression copies)
+    const size_t kHalf = 78 * 1024, kFull = 158 * 1024;   // 1024-thread workgroups
+    std::vector<int> cls[3];
+    size_t shm_cls[3] = {0, 0, 0};
+    for (int64_t ib = 0; ib < nb; ++ib) {
+      const BoxOut& O = ho[ib];
+      const size_t cache = 6 * sizeof(double) * (size_t)hb[ib].n;
+      size_t by = O.r > 0 ? elim_elems(lu, O.k, O.r) * sizeof(S) + cache : 0;
+      int c = (O.r > 0 && by <= kHalf) ? 0 : (O.r > 0 && by <= kFull) ? 1 : 2;
+      cls[c].push_back((int)ib);
+      shm_cls[c] = std::max(shm_cls[c], c < 2 ? by : cache);
+    }
+    std::vector<int> allids;
+    for (int c = 2; c >= 0; --c) allids.insert(allids.end(), cls[c].begin(), cls[c].end());
+    RSKX_TRY(upload(ws.ids, allids));
     prof.begin("factor", ev);
-    launch_factor<S>(lu, ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(),
-                     ws.pool.as<int>(), ws.perm.as<int>(), ws.T.as<double>(), ws.M[mpar].as<S>(),
-                     ws.M[mcur].as<S>(), ws.W.as<S>(), F->fval.as<S>(), F->fidx.as<int>(), P,
-                     ws.flag.as<int>(), (int)nb, g_stream);
+    {
+      int off = 0;
+      for (int c = 2; c >= 0; --c) {
+        if (cls[c].empty()) continue;
+        launch_elim<S>(lu, c < 2 ? 1 : 0, shm_cls[c], ws.ids.as<int>() + off,
+                       (int)cls[c].size(), ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(),
+                       ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(), ws.T.as<double>(),
+                       ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), ws.W.as<S>(), F->fval.as<S>(),
+                       F->fidx.as<int>(), P, ws.flag.as<int>(), g_stream);
+        off += (int)cls[c].size();
+      }
+    }
     prof.end("factor", ev, 0, el_flops);
     RSKX_HIP(hipGetLastError());
     RSKX_HIP(hipMemcpyAsync(ws.hflag, ws.flag.p, sizeof(int), hipMemcpyDeviceToHost, g_stream));
@@ -506,7 +520,7 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
       if (f1 == f0) continue;
       const int kmx = F->lev_kmax[lv], rmx = F->lev_rmax[lv];
       size_t per = (size_t)(kmx + 2 * rmx) * sizeof(S);
-      int cchunk = (int)std::min<size_t>(nrhs, std::max<size_t>(1, (64 * 1024) / std::max<size_t>(per, 1)));
+      int cchunk = (int)std::min<size_t>(std::min(nrhs, 8), std::max<size_t>(1, (64 * 1024) / std::max<size_t>(per, 1)));
       size_t shm = per * cchunk;
       if (shm > 160 * 1024) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: factor too large for the sweep kernel");
       int64_t bytes = 0;

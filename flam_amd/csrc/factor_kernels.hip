@@ -104,6 +104,7 @@ __global__ void __launch_bounds__(256) k_assemble(const LvlBox* __restrict__ box
                                                   const int* __restrict__ mnbr,
                                                   int* __restrict__ mrows, S* __restrict__ Kar,
                                                   PointsDev P, LevelParams L) {
+  extern __shared__ double sp[];
   const LvlBox B = boxes[blockIdx.x];
   const int n = B.n, mn = mnbr[blockIdx.x];
   const int mt = L.two_sided ? 2 * mn : mn;
@@ -111,16 +112,52 @@ __global__ void __launch_bounds__(256) k_assemble(const LvlBox* __restrict__ box
   if (threadIdx.x == 0) mrows[blockIdx.x] = m;
   const int* slf = pool + B.slf_off;
   const int* nbr = nbrar + B.nbr_off;
+  // stage column (slf) and row (nbr) point data: x, y, nu_x, nu_y, w
+  double* sx = sp;
+  double* sy = sx + n;
+  double* snx = sy + n;
+  double* sny = snx + n;
+  double* sw = sny + n;
+  double* rx = sw + n;
+  double* ry = rx + mn;
+  double* rnx = ry + mn;
+  double* rny = rnx + mn;
+  double* rw = rny + mn;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    int pt = slf[j];
+    sx[j] = P.x[0][pt]; sy[j] = P.x[1][pt]; snx[j] = P.nu[0][pt]; sny[j] = P.nu[1][pt];
+    sw[j] = P.w[pt];
+  }
+  for (int i = threadIdx.x; i < mn; i += blockDim.x) {
+    int pt = nbr[i];
+    rx[i] = P.x[0][pt]; ry[i] = P.x[1][pt]; rnx[i] = P.nu[0][pt]; rny[i] = P.nu[1][pt];
+    rw[i] = P.w[pt];
+  }
+  __syncthreads();
   S* K = Kar + B.K_off;
   const int ld = B.mcap;
+  const double c0 = P.scale / kTwoPi;
   for (int j = 0; j < n; ++j) {
-    const int col = slf[j];
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      double v;
-      if (i < mn) v = entry_offdiag(P, nbr[i], col);
-      else if (i < mt) v = entry_offdiag(P, col, nbr[i - mn]);
-      else v = proxy_entry(P, L, B.ctr, i - mt, col);
-      K[i + (int64_t)j * ld] = (S)v;
+    const double xj = sx[j], yj = sy[j], nxj = snx[j], nyj = sny[j], wj = sw[j];
+    S* Kj = K + (int64_t)j * ld;
+    for (int i = threadIdx.x; i < mn; i += blockDim.x) {    // A(nbr, slf)
+      double dx = rx[i] - xj, dy = ry[i] - yj;
+      Kj[i] = (S)(c0 * wj * (dx * nxj + dy * nyj) / (dx * dx + dy * dy));
+    }
+    if (L.two_sided)
+      for (int i = threadIdx.x; i < mn; i += blockDim.x) {  // A(slf, nbr)'
+        double dx = xj - rx[i], dy = yj - ry[i];
+        Kj[mn + i] = (S)(c0 * rw[i] * (dx * rnx[i] + dy * rny[i]) / (dx * dx + dy * dy));
+      }
+    for (int i = threadIdx.x; i < L.p; i += blockDim.x) {   // proxy rows
+      double th = (double)(i + 1) * kTwoPi / (double)L.p;
+      double px = L.radius * cos(th) * L.l[0] + B.ctr[0];
+      double py = L.radius * sin(th) * L.l[1] + B.ctr[1];
+      double dx = px - xj, dy = py - yj;
+      double r2 = dx * dx + dy * dy;
+      double v = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * L.pweight
+                                               : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
+      Kj[mt + i] = (S)v;
     }
   }
 }
@@ -440,22 +477,38 @@ __device__ inline S mval(const ChildMap& cm, const S* Mprev, int a, int b) {
   return (S)0;
 }
 
-template <typename S, bool LU>
-__global__ void __launch_bounds__(256) k_factor(const LvlBox* __restrict__ boxes,
-                                                const BoxOut* __restrict__ outs,
-                                                const Seg* __restrict__ csegs, int* pool,
-                                                const int* __restrict__ permar,
-                                                const double* __restrict__ Tar,
-                                                const S* __restrict__ Mprev, S* __restrict__ Mcur,
-                                                S* __restrict__ War, S* __restrict__ fval,
-                                                int* __restrict__ fidx, PointsDev P,
-                                                int* __restrict__ fail) {
-  const LvlBox B = boxes[blockIdx.x];
-  const BoxOut O = outs[blockIdx.x];
+// Elimination of one box's redundant points (rskelf.m:218-261).
+//
+// With the skeleton-first order [sk rd] and K = A(slf,slf) + M, the two
+// sparsification updates K(rd,:) -= T' K(sk,:) and K(:,rd) -= K(:,sk) T
+// leave
+//     Ksr' = Ksr - Kss T,   Krs' = Krs - T' Kss,
+//     Krr' = Krr - T' Ksr' - Krs T,
+// so only these blocks are formed, never the full n x n matrix.  Then
+// Krr' = L U (partial pivoting, 'n') or L L' ('p'), E = Ksr'/U or Ksr'/L',
+// G = L \ Krs'(p,:), and M(sk,sk) - E G (or - E E') is the new skeleton
+// block.  The blocks live in shared memory when SH (one or two boxes per
+// CU), else in the box's global scratch.
+template <typename S, bool LU, bool SH>
+__global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
+                                              const LvlBox* __restrict__ boxes,
+                                              const BoxOut* __restrict__ outs,
+                                              const Seg* __restrict__ csegs, int* pool,
+                                              const int* __restrict__ permar,
+                                              const double* __restrict__ Tar,
+                                              const S* __restrict__ Mprev, S* __restrict__ Mcur,
+                                              S* __restrict__ War, S* __restrict__ fval,
+                                              int* __restrict__ fidx, PointsDev P,
+                                              int* __restrict__ fail) {
+  extern __shared__ double smem_d[];
+  const int bidx = ids[blockIdx.x];
+  const LvlBox B = boxes[bidx];
+  const BoxOut O = outs[bidx];
   const int n = B.n, k = O.k, r = O.r;
   const int tid = threadIdx.x, nt = blockDim.x;
   __shared__ ChildMap cm;
   __shared__ double red[32];
+  __shared__ S s_d;
   if (tid == 0) {
     int nc = B.ncs > 8 ? 8 : B.ncs;
     cm.nc = nc;
@@ -472,123 +525,194 @@ __global__ void __launch_bounds__(256) k_factor(const LvlBox* __restrict__ boxes
   const int* slf = pool + B.slf_off;
   if (r == 0) {                      // rskelf.m:216: no compression, keep M
     S* Mo = Mcur + O.M_off;
-    for (int e = tid; e < n * n; e += nt) Mo[e] = mval(cm, Mprev, e % n, e / n);
+    for (int c = 0; c < n; ++c)
+      for (int a = tid; a < n; a += nt) Mo[a + c * n] = mval(cm, Mprev, a, c);
     return;
   }
   const int* q = permar + B.perm_off;
-  const double* T = Tar + B.T_off;   // k x r, leading dimension n
-  S* W = War + B.W_off;              // n x n in skeleton-first order
-  for (int e = tid; e < n * n; e += nt) {
-    int a = e % n, c = e / n;
-    int qa = q[a], qc = q[c];
-    W[e] = (S)entry(P, slf[qa], slf[qc]) + mval(cm, Mprev, qa, qc);
+  const double* Tg = Tar + B.T_off;  // k x r, leading dimension n
+  // point data of slf in skeleton-first order, cached in shared memory
+  double* cx = smem_d;
+  double* cy = cx + n;
+  double* cnx = cy + n;
+  double* cny = cnx + n;
+  double* cw = cny + n;
+  double* cdg = cw + n;
+  for (int a = tid; a < n; a += nt) {
+    int pt = slf[q[a]];
+    cx[a] = P.x[0][pt]; cy[a] = P.x[1][pt];
+    cnx[a] = P.nu[0][pt]; cny[a] = P.nu[1][pt];
+    cw[a] = P.w[pt]; cdg[a] = P.diag[pt];
   }
   __syncthreads();
-  // sparsify: K(rd,:) -= T' K(sk,:), then K(:,rd) -= K(:,sk) T (rskelf.m:220-223)
-  for (int e = tid; e < r * n; e += nt) {
-    int a = e % r, c = e / r;
-    const double* Ta = T + (int64_t)a * n;
-    S s = 0;
-    for (int l = 0; l < k; ++l) s += (S)Ta[l] * W[l + (int64_t)c * n];
-    W[(k + a) + (int64_t)c * n] -= s;
+  S* base = SH ? reinterpret_cast<S*>(smem_d + 6 * n) : War + B.W_off;
+  S* Krr = base;                     // r x r
+  S* Ksr = Krr + r * r;              // k x r  (becomes E)
+  S* Krs = Ksr + k * r;              // r x k  (becomes G; LU only)
+  S* Kss = Krs + (LU ? r * k : 0);   // k x k
+  S* Ts = Kss + k * k;               // k x r
+  const int lane = tid & (kWave - 1), wid = tid / kWave, nw = nt / kWave;
+  // rows over lanes, columns over waves: no integer division in the loops
+#define FOR_COLS_ROWS(i, i0, i1, j, j0, j1) \
+  for (int j = (j0) + wid; j < (j1); j += nw) for (int i = (i0) + lane; i < (i1); i += kWave)
+  auto kval = [&](int a, int c) -> S {
+    double v;
+    if (a == c) v = cdg[c];
+    else {
+      double dx = cx[a] - cx[c], dy = cy[a] - cy[c];
+      v = P.scale * cw[c] * (dx * cnx[c] + dy * cny[c]) / (kTwoPi * (dx * dx + dy * dy));
+    }
+    return (S)v + (cm.moff[0] < 0 && cm.nc == 1 ? (S)0 : mval(cm, Mprev, q[a], q[c]));
+  };
+  FOR_COLS_ROWS(a, 0, k, c, 0, r) Ts[a + c * k] = (S)Tg[a + (int64_t)c * n];
+  FOR_COLS_ROWS(a, 0, k, c, 0, k) Kss[a + c * k] = kval(a, c);
+  // Krs (r x k): own region for LU; for Cholesky it passes through Ksr's
+  S* KrsTmp = LU ? Krs : Ksr;
+  FOR_COLS_ROWS(i, 0, r, c, 0, k) KrsTmp[i + c * r] = kval(k + i, c);
+  __syncthreads();
+  // C(i,j) (+)= sum_l A(i,l) B(l,j) over an r x r block, rows i in groups
+  // of four per lane (independent accumulators), columns over waves;
+  // A(i,l) = A[i*ai + l*al], B(l,j) = B[l*bl + j*bj]
+  auto rankk = [&](S* C, const S* A, int ai, int al, const S* Bm, int bl, int bj, S sgn,
+                   bool add) {
+    for (int j = wid; j < r; j += nw) {
+      const S* bcol = Bm + j * bj;
+      for (int i0 = lane; i0 < r; i0 += 4 * kWave) {
+        S acc[4] = {0, 0, 0, 0};
+        const int i1 = i0 + kWave, i2 = i0 + 2 * kWave, i3 = i0 + 3 * kWave;
+        const bool v1 = i1 < r, v2 = i2 < r, v3 = i3 < r;
+        for (int l = 0; l < k; ++l) {
+          const S b = bcol[l * bl];
+          const S* a = A + l * al;
+          acc[0] += a[i0 * ai] * b;
+          if (v1) acc[1] += a[i1 * ai] * b;
+          if (v2) acc[2] += a[i2 * ai] * b;
+          if (v3) acc[3] += a[i3 * ai] * b;
+        }
+        S* c = C + j * r;
+        if (add) {
+          c[i0] += sgn * acc[0];
+          if (v1) c[i1] += sgn * acc[1];
+          if (v2) c[i2] += sgn * acc[2];
+          if (v3) c[i3] += sgn * acc[3];
+        } else {
+          c[i0] = sgn * acc[0];
+          if (v1) c[i1] = sgn * acc[1];
+          if (v2) c[i2] = sgn * acc[2];
+          if (v3) c[i3] = sgn * acc[3];
+        }
+      }
+    }
+  };
+  if (!LU) {
+    // Krr = -Krs T while Krs sits in Ksr's region
+    rankk(Krr, KrsTmp, 1, r, Ts, 1, k, (S)-1, false);
+    __syncthreads();
+  }
+  FOR_COLS_ROWS(a, 0, k, c, 0, r) Ksr[a + c * k] = kval(a, k + c);
+  __syncthreads();
+  FOR_COLS_ROWS(a, 0, k, c, 0, r) {                 // Ksr' = Ksr - Kss T
+    S acc = 0;
+    for (int l = 0; l < k; ++l) acc += Kss[a + l * k] * Ts[l + c * k];
+    Ksr[a + c * k] -= acc;
   }
   __syncthreads();
-  for (int e = tid; e < n * r; e += nt) {
-    int a = e % n, c = e / n;
-    const double* Tc = T + (int64_t)c * n;
-    S s = 0;
-    for (int l = 0; l < k; ++l) s += W[a + (int64_t)l * n] * (S)Tc[l];
-    W[a + (int64_t)(k + c) * n] -= s;
+  // Krr' = Krr - T' Ksr' - Krs T
+  if (LU) {
+    FOR_COLS_ROWS(i, 0, r, j, 0, r) Krr[i + j * r] = kval(k + i, k + j);
+    __syncthreads();
+    rankk(Krr, Krs, 1, r, Ts, 1, k, (S)-1, true);
+  } else {
+    FOR_COLS_ROWS(i, 0, r, j, 0, r) Krr[i + j * r] += kval(k + i, k + j);
   }
   __syncthreads();
-  S* Krr = W + k + (int64_t)k * n;   // r x r, leading dimension n
+  rankk(Krr, Ts, k, 1, Ksr, 1, k, (S)-1, true);
+  if (LU) {                                         // Krs' = Krs - T' Kss
+    __syncthreads();
+    FOR_COLS_ROWS(i, 0, r, c, 0, k) {
+      S acc = 0;
+      for (int l = 0; l < k; ++l) acc += Ts[l + i * k] * Kss[l + c * k];
+      Krs[i + c * r] -= acc;
+    }
+  }
+  __syncthreads();
+
   int* piv = fidx + O.piv;
   if (LU) {
     for (int i = tid; i < r; i += nt) piv[i] = i;
-    __syncthreads();
-    for (int s = 0; s < r; ++s) {     // right-looking LU, partial pivoting
+    for (int s = 0; s < r; ++s) {    // right-looking LU with partial pivoting
       double bv = -1.0;
       int bi = r;
-      for (int i = s + tid; i < r; i += nt) argmax_merge(bv, bi, fabs((double)Krr[i + (int64_t)s * n]), i);
+      for (int i = s + tid; i < r; i += nt) argmax_merge(bv, bi, fabs((double)Krr[i + s * r]), i);
       block_argmax(bv, bi, red);
       if (!(bv > 0)) { if (tid == 0) atomicOr(fail, 2); }
-      if (bi != s && bi < r) {
-        for (int c = tid; c < n; c += nt) {    // full rows: also permutes Krs
-          S a = W[(k + s) + (int64_t)c * n];
-          W[(k + s) + (int64_t)c * n] = W[(k + bi) + (int64_t)c * n];
-          W[(k + bi) + (int64_t)c * n] = a;
+      if (bi != s && bi < r) {      // swap rows s and bi of Krr and Krs
+        for (int c = tid; c < r + k; c += nt) {
+          S* M = c < r ? Krr + c * r : Krs + (c - r) * r;
+          S t = M[s]; M[s] = M[bi]; M[bi] = t;
         }
         if (tid == 0) { int t = piv[s]; piv[s] = piv[bi]; piv[bi] = t; }
         __syncthreads();
       }
-      const S d = Krr[s + (int64_t)s * n];
-      for (int i = s + 1 + tid; i < r; i += nt) Krr[i + (int64_t)s * n] /= d;
+      const S dinv = (S)1 / Krr[s + s * r];
+      FOR_COLS_ROWS(i, s + 1, r, j, s + 1, r)
+        Krr[i + j * r] -= Krr[i + s * r] * dinv * Krr[s + j * r];
       __syncthreads();
-      const int w = r - s - 1;
-      for (int e = tid; e < w * w; e += nt) {
-        int i = s + 1 + e % w, j = s + 1 + e / w;
-        Krr[i + (int64_t)j * n] -= Krr[i + (int64_t)s * n] * Krr[s + (int64_t)j * n];
-      }
+      for (int i = s + 1 + tid; i < r; i += nt) Krr[i + s * r] *= dinv;
       __syncthreads();
     }
   } else {
-    for (int s = 0; s < r; ++s) {     // right-looking Cholesky, lower
-      S d = Krr[s + (int64_t)s * n];
-      if (!(d > (S)0)) { if (tid == 0) atomicOr(fail, 1); d = (S)1; }
-      d = sqrt(d);
-      __syncthreads();
-      if (tid == 0) Krr[s + (int64_t)s * n] = d;
-      for (int i = s + 1 + tid; i < r; i += nt) Krr[i + (int64_t)s * n] /= d;
-      __syncthreads();
-      const int w = r - s - 1;
-      for (int e = tid; e < w * w; e += nt) {
-        int i = s + 1 + e % w, j = s + 1 + e / w;
-        if (j <= i) Krr[i + (int64_t)j * n] -= Krr[i + (int64_t)s * n] * Krr[j + (int64_t)s * n];
+    for (int s = 0; s < r; ++s) {    // right-looking Cholesky, lower
+      if (tid == 0) {
+        S d = Krr[s + s * r];
+        if (!(d > (S)0)) { atomicOr(fail, 1); d = (S)1; }
+        d = sqrt(d);
+        Krr[s + s * r] = d;
+        s_d = (S)1 / d;
       }
       __syncthreads();
-    }
-  }
-  // E = K(sk,rd)/U ('n') or K(sk,rd)/L' ('p'): row-wise forward sweeps
-  S* Ksr = W + (int64_t)k * n;       // k x r, leading dimension n
-  for (int a = tid; a < k; a += nt) {
-    for (int j = 0; j < r; ++j) {
-      S s = Ksr[a + (int64_t)j * n];
-      if (LU) for (int l = 0; l < j; ++l) s -= Ksr[a + (int64_t)l * n] * Krr[l + (int64_t)j * n];
-      else    for (int l = 0; l < j; ++l) s -= Ksr[a + (int64_t)l * n] * Krr[j + (int64_t)l * n];
-      Ksr[a + (int64_t)j * n] = s / Krr[j + (int64_t)j * n];
-    }
-  }
-  // G = L \ K(rd(p),sk) ('n'): column-wise forward sweeps with unit L
-  S* Krs = W + k;                    // r x k, leading dimension n
-  if (LU) {
-    for (int c = tid; c < k; c += nt) {
-      for (int i = 0; i < r; ++i) {
-        S s = Krs[i + (int64_t)c * n];
-        for (int l = 0; l < i; ++l) s -= Krr[i + (int64_t)l * n] * Krs[l + (int64_t)c * n];
-        Krs[i + (int64_t)c * n] = s;
+      const S d2 = s_d * s_d;
+      for (int j = s + 1 + wid; j < r; j += nw) {
+        const S ljs = Krr[j + s * r] * d2;
+        for (int i = j + lane; i < r; i += kWave) Krr[i + j * r] -= Krr[i + s * r] * ljs;
       }
+      __syncthreads();
+      for (int i = s + 1 + tid; i < r; i += nt) Krr[i + s * r] *= s_d;
+      __syncthreads();
     }
   }
-  __syncthreads();
+  // E = Ksr'/U ('n') or Ksr'/L' ('p'); G = L \ Krs'(p,:) ('n'); column sweeps
+  for (int j = 0; j < r; ++j) {
+    const S inv = (S)1 / Krr[j + j * r];
+    for (int a = tid; a < k; a += nt) Ksr[a + j * k] *= inv;
+    __syncthreads();
+    FOR_COLS_ROWS(a, 0, k, c, j + 1, r) {
+      S u = LU ? Krr[j + c * r] : Krr[c + j * r];
+      Ksr[a + c * k] -= Ksr[a + j * k] * u;
+    }
+    if (LU)
+      FOR_COLS_ROWS(i, j + 1, r, c, 0, k) Krs[i + c * r] -= Krr[i + j * r] * Krs[j + c * r];
+    __syncthreads();
+  }
   // M = M(sk,sk) - E G  or  - E E' (rskelf.m:240-246)
   S* Mo = Mcur + O.M_off;
-  for (int e = tid; e < k * k; e += nt) {
-    int a = e % k, c = e / k;
-    S s = 0;
-    if (LU) for (int l = 0; l < r; ++l) s += Ksr[a + (int64_t)l * n] * Krs[l + (int64_t)c * n];
-    else    for (int l = 0; l < r; ++l) s += Ksr[a + (int64_t)l * n] * Ksr[c + (int64_t)l * n];
-    Mo[e] = mval(cm, Mprev, q[a], q[c]) - s;
+  FOR_COLS_ROWS(a, 0, k, c, 0, k) {
+    S acc = 0;
+    if (LU) for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * Krs[l + c * r];
+    else    for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * Ksr[c + l * k];
+    Mo[a + c * k] = mval(cm, Mprev, q[a], q[c]) - acc;
   }
+#undef FOR_COLS_ROWS
   // store the factor (rskelf.m:249-257) and the new skeleton list
   S* fT = fval + O.T;
-  for (int e = tid; e < k * r; e += nt) fT[e] = (S)T[(e % k) + (int64_t)(e / k) * n];
+  for (int e = tid; e < k * r; e += nt) fT[e] = Ts[e];
   S* fLU = fval + O.LU;
-  for (int e = tid; e < r * r; e += nt) fLU[e] = Krr[(e % r) + (int64_t)(e / r) * n];
+  for (int e = tid; e < r * r; e += nt) fLU[e] = Krr[e];
   S* fE = fval + O.E;
-  for (int e = tid; e < k * r; e += nt) fE[e] = Ksr[(e % k) + (int64_t)(e / k) * n];
+  for (int e = tid; e < k * r; e += nt) fE[e] = Ksr[e];
   if (LU) {
     S* fG = fval + O.G;
-    for (int e = tid; e < r * k; e += nt) fG[e] = Krs[(e % r) + (int64_t)(e / r) * n];
+    for (int e = tid; e < r * k; e += nt) fG[e] = Krs[e];
   }
   int* fsk = fidx + O.sk;
   int* frd = fidx + O.rd;
@@ -610,9 +734,12 @@ void launch_lists(const LvlBox* boxes, const Seg* nsegs, const Seg* csegs, int* 
 template <typename S>
 void launch_assemble(const LvlBox* boxes, const int* pool, const int* nbrar, const int* mnbr,
                      int* mrows, S* Kar, const PointsDev& P, const LevelParams& L, int nb,
-                     hipStream_t st) {
-  hipLaunchKernelGGL(k_assemble<S>, dim3(nb), dim3(256), 0, st, boxes, pool, nbrar, mnbr, mrows,
-                     Kar, P, L);
+                     size_t shmem, hipStream_t st) {
+  if (shmem > 65536)
+    (void)hipFuncSetAttribute((const void*)k_assemble<S>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  hipLaunchKernelGGL(k_assemble<S>, dim3(nb), dim3(256), shmem, st, boxes, pool, nbrar, mnbr,
+                     mrows, Kar, P, L);
 }
 
 template <typename S>
@@ -626,26 +753,40 @@ void launch_id(const LvlBox* boxes, const int* mrows, S* Kar, int* permar, doubl
                      kout, iterout);
 }
 
+// shared-memory elements of one box's elimination blocks
+size_t elim_elems(bool lu, int k, int r) {
+  return (size_t)r * r + (size_t)k * r * (lu ? 3 : 2) + (size_t)k * k;
+}
+
 template <typename S>
-void launch_factor(bool lu, const LvlBox* boxes, const BoxOut* outs, const Seg* csegs, int* pool,
-                   const int* permar, const double* Tar, const S* Mprev, S* Mcur, S* War,
-                   S* fval, int* fidx, const PointsDev& P, int* fail, int nb, hipStream_t st) {
-  if (lu)
-    hipLaunchKernelGGL((k_factor<S, true>), dim3(nb), dim3(256), 0, st, boxes, outs, csegs, pool,
-                       permar, Tar, Mprev, Mcur, War, fval, fidx, P, fail);
-  else
-    hipLaunchKernelGGL((k_factor<S, false>), dim3(nb), dim3(256), 0, st, boxes, outs, csegs, pool,
-                       permar, Tar, Mprev, Mcur, War, fval, fidx, P, fail);
+void launch_elim(bool lu, int mode, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
+                 const BoxOut* outs, const Seg* csegs, int* pool, const int* permar,
+                 const double* Tar, const S* Mprev, S* Mcur, S* War, S* fval, int* fidx,
+                 const PointsDev& P, int* fail, hipStream_t st) {
+  // mode 1: blocks in shared memory, mode 0: blocks in global scratch
+#define RSKX_ELIM(LUV, SHV)                                                                   \
+  {                                                                                           \
+    auto kern = k_elim<S, LUV, SHV>;                                                          \
+    if (shmem > 65536)                                                                        \
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)shmem);                                                  \
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(1024), shmem, st, ids, boxes, outs, csegs, pool,  \
+                       permar, Tar, Mprev, Mcur, War, fval, fidx, P, fail);                   \
+  }
+  if (lu) { if (mode) RSKX_ELIM(true, true) else RSKX_ELIM(true, false) }
+  else    { if (mode) RSKX_ELIM(false, true) else RSKX_ELIM(false, false) }
+#undef RSKX_ELIM
 }
 
 #define RSKX_INST(S)                                                                          \
   template void launch_assemble<S>(const LvlBox*, const int*, const int*, const int*, int*, S*, \
-                                   const PointsDev&, const LevelParams&, int, hipStream_t);     \
+                                   const PointsDev&, const LevelParams&, int, size_t,           \
+                                   hipStream_t);                                                \
   template void launch_id<S>(const LvlBox*, const int*, S*, int*, double*, const IdParams&,     \
                              int*, int*, int, size_t, hipStream_t);                            \
-  template void launch_factor<S>(bool, const LvlBox*, const BoxOut*, const Seg*, int*,          \
-                                 const int*, const double*, const S*, S*, S*, S*, int*,         \
-                                 const PointsDev&, int*, int, hipStream_t);
+  template void launch_elim<S>(bool, int, size_t, const int*, int, const LvlBox*,             \
+                               const BoxOut*, const Seg*, int*, const int*, const double*,     \
+                               const S*, S*, S*, S*, int*, const PointsDev&, int*, hipStream_t);
 RSKX_INST(double)
 RSKX_INST(float)
 

@@ -4,6 +4,8 @@
 // one launch with a workgroup per factor: X(sk,:) and X(rd,:) are staged in
 // shared memory, the three steps of the factor are applied there and the
 // rows are written back.  X is in tree order and in the factor's precision.
+#include <algorithm>
+
 #include "common.hpp"
 #include "devutil.hpp"
 #include "rskx.h"
@@ -14,13 +16,21 @@ namespace rskx {
 template <typename S>
 __device__ void upd(S* Y, int ny, const S* Z, int nz, const S* __restrict__ A, int lda, bool trA,
                     S sgn, int cc) {
-  for (int e = threadIdx.x; e < ny * cc; e += blockDim.x) {
-    int i = e % ny, c = e / ny;
-    const S* z = Z + c * nz;
-    S s = 0;
-    if (trA) { const S* a = A + (int64_t)i * lda; for (int l = 0; l < nz; ++l) s += a[l] * z[l]; }
-    else     { for (int l = 0; l < nz; ++l) s += A[i + (int64_t)l * lda] * z[l]; }
-    Y[e] += sgn * s;
+  for (int i = threadIdx.x; i < ny; i += blockDim.x) {
+    S acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (trA) {
+      const S* a = A + (int64_t)i * lda;
+      for (int l = 0; l < nz; ++l) {
+        S al = a[l];
+        for (int c = 0; c < cc; ++c) acc[c] += al * Z[l + c * nz];
+      }
+    } else {
+      for (int l = 0; l < nz; ++l) {
+        S al = A[i + (int64_t)l * lda];
+        for (int c = 0; c < cc; ++c) acc[c] += al * Z[l + c * nz];
+      }
+    }
+    for (int c = 0; c < cc; ++c) Y[i + c * ny] += sgn * acc[c];
   }
   __syncthreads();
 }
@@ -39,11 +49,9 @@ __device__ void trsolve(S* Y, int r, const S* __restrict__ A, bool lower, bool u
     }
     // rows still to be solved: below j (fwd) or above j (bwd)
     const int lo = fwd ? j + 1 : 0, hi = fwd ? r : j;
-    const int w = hi - lo;
-    for (int e = threadIdx.x; e < w * cc; e += blockDim.x) {
-      int i = lo + e % w, c = e / w;
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
       S a = tr ? A[j + (int64_t)i * r] : A[i + (int64_t)j * r];
-      Y[i + c * r] -= a * Y[j + c * r];
+      for (int c = 0; c < cc; ++c) Y[i + c * r] -= a * Y[j + c * r];
     }
     __syncthreads();
   }
@@ -53,35 +61,37 @@ __device__ void trsolve(S* Y, int r, const S* __restrict__ A, bool lower, bool u
 template <typename S>
 __device__ void trmul(S* Y, S* tmp, int r, const S* __restrict__ A, bool lower, bool unit, bool tr,
                       int cc) {
-  for (int e = threadIdx.x; e < r * cc; e += blockDim.x) {
-    int i = e % r, c = e / r;
-    const S* y = Y + c * r;
-    S s = unit ? y[i] : (S)0;
-    // op(A)(i,l) nonzero for l in the triangle
-    bool low = lower != tr;   // effective lower-triangular operator
+  const bool low = lower != tr;   // effective lower-triangular operator
+  for (int i = threadIdx.x; i < r; i += blockDim.x) {
+    S acc[8];
+    for (int c = 0; c < cc; ++c) acc[c] = unit ? Y[i + c * r] : (S)0;
     int l0 = low ? 0 : (unit ? i + 1 : i);
     int l1 = low ? (unit ? i : i + 1) : r;
     for (int l = l0; l < l1; ++l) {
       S a = tr ? A[l + (int64_t)i * r] : A[i + (int64_t)l * r];
-      s += a * y[l];
+      for (int c = 0; c < cc; ++c) acc[c] += a * Y[l + c * r];
     }
-    tmp[e] = s;
+    for (int c = 0; c < cc; ++c) tmp[i + c * r] = acc[c];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < r * cc; e += blockDim.x) Y[e] = tmp[e];
+  for (int c = 0; c < cc; ++c)
+    for (int i = threadIdx.x; i < r; i += blockDim.x) Y[i + c * r] = tmp[i + c * r];
   __syncthreads();
 }
 
 // gather (tmp(i) = Y(p(i))) or scatter (tmp(p(i)) = Y(i)), then copy back
 template <typename S>
 __device__ void permrows(S* Y, S* tmp, int r, const int* __restrict__ p, bool gather, int cc) {
-  for (int e = threadIdx.x; e < r * cc; e += blockDim.x) {
-    int i = e % r, c = e / r;
-    if (gather) tmp[e] = Y[p[i] + c * r];
-    else tmp[p[i] + c * r] = Y[e];
+  for (int i = threadIdx.x; i < r; i += blockDim.x) {
+    int pi = p[i];
+    for (int c = 0; c < cc; ++c) {
+      if (gather) tmp[i + c * r] = Y[pi + c * r];
+      else tmp[pi + c * r] = Y[i + c * r];
+    }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < r * cc; e += blockDim.x) Y[e] = tmp[e];
+  for (int c = 0; c < cc; ++c)
+    for (int i = threadIdx.x; i < r; i += blockDim.x) Y[i + c * r] = tmp[i + c * r];
   __syncthreads();
 }
 
@@ -108,10 +118,10 @@ __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
     S* xs = sm;
     S* xr = xs + k * cc;
     S* tmp = xr + r * cc;
-    for (int e = threadIdx.x; e < k * cc; e += blockDim.x)
-      xs[e] = X[sk[e % k] + (c0 + e / k) * ldx];
-    for (int e = threadIdx.x; e < r * cc; e += blockDim.x)
-      xr[e] = X[rd[e % r] + (c0 + e / r) * ldx];
+    for (int c = 0; c < cc; ++c) {
+      for (int a = threadIdx.x; a < k; a += blockDim.x) xs[a + c * k] = X[sk[a] + (c0 + c) * ldx];
+      for (int i = threadIdx.x; i < r; i += blockDim.x) xr[i + c * r] = X[rd[i] + (c0 + c) * ldx];
+    }
     __syncthreads();
     if (PROG == 0) {            // sv_nn.m
       if (UP) {
@@ -178,10 +188,10 @@ __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
         upd(xr, r, xs, k, T, k, true, one, cc);
       }
     }
-    for (int e = threadIdx.x; e < k * cc; e += blockDim.x)
-      X[sk[e % k] + (c0 + e / k) * ldx] = xs[e];
-    for (int e = threadIdx.x; e < r * cc; e += blockDim.x)
-      X[rd[e % r] + (c0 + e / r) * ldx] = xr[e];
+    for (int c = 0; c < cc; ++c) {
+      for (int a = threadIdx.x; a < k; a += blockDim.x) X[sk[a] + (c0 + c) * ldx] = xs[a + c * k];
+      for (int i = threadIdx.x; i < r; i += blockDim.x) X[rd[i] + (c0 + c) * ldx] = xr[i + c * r];
+    }
     __syncthreads();
   }
 }
@@ -306,6 +316,30 @@ void launch_matvec(const double* x, int64_t N, const double* nu, const double* w
   int grid = (int)((N + 255) / 256);
   hipLaunchKernelGGL(k_matvec, dim3(grid), dim3(256), 0, st, x, N, nu, w, diag, scale, X, ldx, Y,
                      ldy, nrhs);
+}
+
+// points, normals, weights and diagonal into tree-order structure of arrays
+__global__ void k_gather_points(const double* __restrict__ x, const double* __restrict__ nu,
+                                const double* __restrict__ w, const double* __restrict__ diag,
+                                const int* __restrict__ order, int64_t N, int d,
+                                double* __restrict__ pts) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t q = order[p];
+    for (int i = 0; i < d; ++i) {
+      pts[i * N + p] = x[d * q + i];
+      pts[(d + i) * N + p] = nu[d * q + i];
+    }
+    pts[2 * d * N + p] = w[q];
+    pts[(2 * d + 1) * N + p] = diag[q];
+  }
+}
+
+void launch_gather_points(const double* x, const double* nu, const double* w, const double* diag,
+                          const int* order, int64_t N, int d, double* pts, hipStream_t st) {
+  int grid = (int)std::min<int64_t>((N + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_gather_points, dim3(grid < 1 ? 1 : grid), dim3(256), 0, st, x, nu, w, diag,
+                     order, N, d, pts);
 }
 
 #define RSKX_SWEEP_INST(S)                                                                    \
