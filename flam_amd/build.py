@@ -4,7 +4,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["api.hip", "factor_kernels.hip", "sweep_kernels.hip", "tree.cpp"]
+SOURCES = ["api.hip", "factor_kernels.hip", "sweep_kernels.hip", "tree_kernels.hip", "tree.cpp"]
 LIB = os.path.join(HERE, "librskx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

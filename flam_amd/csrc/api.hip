@@ -47,6 +47,9 @@ void launch_matvec(const double*, int64_t, const double*, const double*, const d
                    const double*, int64_t, double*, int64_t, int, hipStream_t);
 void launch_gather_points(const double*, const double*, const double*, const double*, const int*,
                           int64_t, int, double*, hipStream_t);
+void launch_tree_init(const double*, int, int64_t, double*, int*, double*, int, hipStream_t);
+void launch_tree_split(bool, const TreeChunk*, int, int, int, int64_t, double*, int*, double*, int*,
+                       int*, hipStream_t);
 
 // --------------------------------------------------------------- errors --
 static thread_local std::string g_err;
@@ -84,7 +87,7 @@ struct DevBuf {
 // scratch reused across factorizations of this process
 struct Workspace {
   DevBuf boxes, nsegs, csegs, outs, ids, pool, nbr, mnbr, mrows, kout, iter, K, perm, T, W, M[2], flag;
-  DevBuf pts, xw;
+  DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs;
   int* hflag = nullptr;
 };
 static Workspace g_ws;
@@ -156,7 +159,7 @@ struct rskx_factor {
   std::vector<int> lev_kmax, lev_rmax;
   std::vector<FacRec> recs;               // host copy
   DevBuf d_recs, fval, fidx, order;       // device data
-  std::vector<int64_t> h_order;           // tree order -> original index
+  mutable std::vector<int64_t> h_order;   // tree order -> original index (filled on export)
   std::vector<std::array<int64_t, 4>> table;
   Prof prof;
   int64_t fval_used = 0, fidx_used = 0;
@@ -164,6 +167,136 @@ struct rskx_factor {
 };
 
 namespace {
+
+// Tree on the device: the same boxes, centres and point order as
+// build_tree (core/hypoct.m:51-171), with the point partition done by
+// k_tree_split.  The tree order (tree position -> original index) is left
+// in `order` on the device.
+int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const double* ext,
+                   HostTree& t, DevBuf& order) {
+  Workspace& ws = g_ws;
+  constexpr int kChunk = 8192;
+  constexpr int kMaxLevels = 64;
+  RSKX_TRY(ws.xs.ensure((size_t)d * N * sizeof(double)));
+  RSKX_TRY(ws.txs.ensure((size_t)d * N * sizeof(double)));
+  RSKX_TRY(order.ensure((size_t)N * sizeof(int)));
+  RSKX_TRY(ws.tord.ensure((size_t)N * sizeof(int)));
+  const int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (N + 255) / 256));
+  RSKX_TRY(ws.tpart.ensure((size_t)nblk * 6 * sizeof(double)));
+  double* xs = ws.xs.as<double>();
+  int* ord = order.as<int>();
+  launch_tree_init(x, d, N, xs, ord, ws.tpart.as<double>(), nblk, g_stream);
+  std::vector<double> part((size_t)nblk * 6);
+  RSKX_HIP(hipMemcpyAsync(part.data(), ws.tpart.p, part.size() * 8, hipMemcpyDeviceToHost, g_stream));
+  RSKX_HIP(hipStreamSynchronize(g_stream));
+  std::vector<double> lo(d), hi(d), l(d);
+  if (ext) {
+    for (int i = 0; i < d; ++i) { lo[i] = ext[i]; hi[i] = ext[d + i]; }
+  } else {
+    for (int i = 0; i < d; ++i) {
+      lo[i] = INFINITY; hi[i] = -INFINITY;
+      for (int b = 0; b < nblk; ++b) {
+        lo[i] = std::min(lo[i], part[b * 6 + i]);
+        hi[i] = std::max(hi[i], part[b * 6 + 3 + i]);
+      }
+    }
+  }
+  t = HostTree();
+  t.d = d;
+  for (int i = 0; i < d; ++i) {
+    l[i] = hi[i] - lo[i];
+    t.ctr.push_back(0.5 * (lo[i] + hi[i]));
+  }
+  t.beg.push_back(0);
+  t.cnt.push_back(N);
+  t.prnt.push_back(-1);
+  std::vector<std::vector<int>> chld(1);
+  t.lvp = {0, 1};
+  t.lsize = l;
+  int nlvl = 1;
+  const int ncode = 1 << d;
+  std::vector<TreeChunk> chunks;
+  std::vector<int> counts;
+  while (lvlmax <= 0 || nlvl < lvlmax) {
+    if (nlvl >= kMaxLevels)
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:tree: more than 64 levels (occ too small for the data)");
+    double lmax = *std::max_element(l.begin(), l.end());
+    int divmask = 0;
+    for (int i = 0; i < d; ++i)
+      if (l[i] >= lmax / std::sqrt(2.0)) { divmask |= 1 << i; l[i] *= 0.5; }
+    chunks.clear();
+    std::vector<int> split;   // boxes of this level being split
+    for (int64_t p = t.lvp[nlvl - 1]; p < t.lvp[nlvl]; ++p) {
+      if (t.cnt[p] <= occ) continue;
+      const int sb = (int)split.size();
+      split.push_back((int)p);
+      for (int64_t s0 = 0; s0 < t.cnt[p]; s0 += kChunk) {
+        TreeChunk c{};
+        c.start = t.beg[p] + s0;
+        c.len = (int)std::min<int64_t>(kChunk, t.cnt[p] - s0);
+        c.box = sb;
+        for (int i = 0; i < d; ++i) c.ctr[i] = t.ctr[(size_t)d * p + i];
+        chunks.push_back(c);
+      }
+    }
+    if (split.empty()) break;
+    const int nch = (int)chunks.size();
+    RSKX_TRY(upload(ws.tchunks, chunks));
+    RSKX_TRY(ws.tcounts.ensure((size_t)nch * 8 * sizeof(int)));
+    launch_tree_split(false, ws.tchunks.as<TreeChunk>(), nch, d, divmask, N, xs, ord,
+                      ws.txs.as<double>(), ws.tord.as<int>(), ws.tcounts.as<int>(), g_stream);
+    counts.resize((size_t)nch * 8);
+    RSKX_HIP(hipMemcpyAsync(counts.data(), ws.tcounts.p, counts.size() * sizeof(int),
+                            hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipStreamSynchronize(g_stream));
+    // child boxes and each chunk's destinations (stable within a child)
+    int c0 = 0;
+    for (int sb = 0; sb < (int)split.size(); ++sb) {
+      const int p = split[sb];
+      int c1 = c0;
+      while (c1 < nch && chunks[c1].box == sb) ++c1;
+      int64_t tot[8] = {0};
+      for (int c = c0; c < c1; ++c)
+        for (int u = 0; u < ncode; ++u) tot[u] += counts[(size_t)c * 8 + u];
+      int64_t start[8];
+      int64_t acc = t.beg[p];
+      for (int u = 0; u < ncode; ++u) { start[u] = acc; acc += tot[u]; }
+      for (int c = c0; c < c1; ++c)
+        for (int u = 0; u < ncode; ++u) {
+          chunks[c].off[u] = start[u];
+          start[u] += counts[(size_t)c * 8 + u];
+        }
+      const double pc[3] = {t.ctr[(size_t)d * p], d > 1 ? t.ctr[(size_t)d * p + 1] : 0,
+                            d > 2 ? t.ctr[(size_t)d * p + 2] : 0};
+      int64_t cb = t.beg[p];
+      for (int u = 0; u < ncode; ++u) {
+        if (tot[u] == 0) continue;
+        for (int i = 0; i < d; ++i) {
+          double bit = (double)((u >> i) & 1);
+          t.ctr.push_back(pc[i] + (((divmask >> i) & 1) ? l[i] : 0.0) * (bit - 0.5));
+        }
+        t.beg.push_back(cb);
+        t.cnt.push_back(tot[u]);
+        cb += tot[u];
+        t.prnt.push_back(p);
+        chld.emplace_back();
+        chld[p].push_back(t.nbox() - 1);
+      }
+      t.cnt[p] = 0;
+      c0 = c1;
+    }
+    RSKX_TRY(upload(ws.tchunks, chunks));
+    launch_tree_split(true, ws.tchunks.as<TreeChunk>(), nch, d, divmask, N, xs, ord,
+                      ws.txs.as<double>(), ws.tord.as<int>(), ws.tcounts.as<int>(), g_stream);
+    ++nlvl;
+    t.lvp.push_back(t.nbox());
+    t.lsize.insert(t.lsize.end(), l.begin(), l.end());
+  }
+  RSKX_HIP(hipGetLastError());
+  t.nlvl = nlvl;
+  finish_tree(t, chld);
+  return RSKX_OK;
+}
 
 // per-box host state during the factorization
 struct BoxState {
@@ -183,13 +316,10 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   prof.on = o.profile != 0;
   auto t0 = clk::now();
 
-  // ---- tree (host)
-  std::vector<double> hx((size_t)d * N);
-  RSKX_HIP(hipMemcpyAsync(hx.data(), x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-  RSKX_HIP(hipStreamSynchronize(g_stream));
-  HostTree t = build_tree(hx.data(), d, N, occ, o.lvlmax, o.ext);
+  // ---- tree (device partition, host box bookkeeping)
+  HostTree t;
+  RSKX_TRY(build_tree_dev(x, d, N, occ, o.lvlmax, o.ext, t, F->order));
   F->nlvl = t.nlvl;
-  F->h_order = t.order;
   prof.host("tree", std::chrono::duration<double, std::milli>(clk::now() - t0).count());
   if (o.verb) {
     printf("%s\n", std::string(69, '-').c_str());
@@ -199,9 +329,6 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   }
 
   // ---- points in tree order (SoA)
-  std::vector<int> ord32(N);
-  for (int64_t p = 0; p < N; ++p) ord32[p] = (int)t.order[p];
-  RSKX_TRY(upload(F->order, ord32));
   RSKX_TRY(ws.pts.ensure((size_t)N * (2 * d + 2) * sizeof(double)));
   double* pts = ws.pts.as<double>();
   // gather points, normals, weights and diagonal into tree order on the device
@@ -699,6 +826,11 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
   RSKX_TRY(geti(f.sk, k, sk));
   RSKX_TRY(geti(f.rd, r, rd));
   if (lu) RSKX_TRY(geti(f.piv, r, pv));
+  if ((int64_t)F->h_order.size() != F->N) {
+    std::vector<int> o32(F->N);
+    RSKX_HIP(hipMemcpy(o32.data(), F->order.p, F->N * sizeof(int), hipMemcpyDeviceToHost));
+    F->h_order.assign(o32.begin(), o32.end());
+  }
   for (int64_t q = 0; q < k; ++q) v->sk[q] = F->h_order[sk[q]];
   for (int64_t q = 0; q < r; ++q) v->rd[q] = F->h_order[rd[q]];
   if (v->T) std::copy(T.begin(), T.end(), v->T);

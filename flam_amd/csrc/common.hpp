@@ -93,4 +93,13 @@ struct IdParams {
   double rrqr_iter;       // max refinement iterations
 };
 
+// one chunk of a splitting box's point range (device tree build)
+struct TreeChunk {
+  int64_t start;          // first point of the chunk (current order)
+  int len;                // points in the chunk
+  int box;                // splitting box index within this level's list
+  double ctr[3];          // that box's centre
+  int64_t off[8];         // scatter: first destination of each child code
+};
+
 }  // namespace rskx

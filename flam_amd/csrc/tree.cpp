@@ -102,6 +102,12 @@ HostTree build_tree(const double* x, int d, int64_t N, int occ, int lvlmax, cons
     t.lsize.insert(t.lsize.end(), l.begin(), l.end());
   }
   t.nlvl = nlvl;
+  finish_tree(t, chld);
+  return t;
+}
+
+void finish_tree(HostTree& t, const std::vector<std::vector<int>>& chld) {
+  const int d = t.d, nlvl = t.nlvl;
   int nbox = t.nbox();
   t.chld_off.assign(nbox + 1, 0);
   for (int b = 0; b < nbox; ++b) t.chld_off[b + 1] = t.chld_off[b] + (int)chld[b].size();
@@ -138,7 +144,6 @@ HostTree build_tree(const double* x, int d, int64_t N, int occ, int lvlmax, cons
   t.nbor_off.assign(nbox + 1, 0);
   for (int b = 0; b < nbox; ++b) t.nbor_off[b + 1] = t.nbor_off[b] + (int)nb[b].size();
   for (int b = 0; b < nbox; ++b) t.nbor.insert(t.nbor.end(), nb[b].begin(), nb[b].end());
-  return t;
 }
 
 }  // namespace rskx

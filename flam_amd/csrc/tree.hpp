@@ -37,4 +37,8 @@ struct HostTree {
 // column-major (lower bounds then upper bounds) or nullptr.
 HostTree build_tree(const double* x, int d, int64_t N, int occ, int lvlmax, const double* ext);
 
+// Children in CSR form and the neighbour lists (hypoct.m:141-170), given
+// the boxes (ctr, prnt, beg/cnt, lvp, lsize, nlvl) and each box's children.
+void finish_tree(HostTree& t, const std::vector<std::vector<int>>& chld);
+
 }  // namespace rskx
