@@ -39,10 +39,10 @@ void launch_elim(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut
                  int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
                  int*, hipStream_t);
 template <typename S>
-void launch_sweep(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int,
-                  size_t, hipStream_t);
+void launch_sweep(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int, int,
+                  int, size_t, hipStream_t);
 template <typename U, typename S>
-void launch_perm(bool, void*, int64_t, S*, int64_t, int, const int*, hipStream_t);
+void launch_perm(bool, void*, int64_t, S*, int64_t, int, int, const int*, hipStream_t);
 void launch_matvec(const double*, int64_t, const double*, const double*, const double*, double,
                    const double*, int64_t, double*, int64_t, int, hipStream_t);
 void launch_gather_points(const double*, const double*, const double*, const double*, const int*,
@@ -517,6 +517,9 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
         O.LU = F->fval_used + need_v; need_v += r * r;
         O.E = F->fval_used + need_v; need_v += k * r;
         O.G = F->fval_used + need_v; if (lu) need_v += r * k;
+        const int64_t nbk = (r + kDiagBlock - 1) / kDiagBlock;
+        O.Li = F->fval_used + need_v; need_v += nbk * kDiagBlock * kDiagBlock;
+        O.Ui = F->fval_used + need_v; if (lu) need_v += nbk * kDiagBlock * kDiagBlock;
         O.sk = F->fidx_used + need_i; need_i += k;
         O.rd = F->fidx_used + need_i; need_i += r;
         O.piv = F->fidx_used + need_i; if (lu) need_i += r;
@@ -589,7 +592,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
         bs[b].cur_len = O.k;
         FacRec fr;
         fr.k = O.k; fr.r = O.r;
-        fr.T = O.T; fr.LU = O.LU; fr.E = O.E; fr.G = O.G;
+        fr.T = O.T; fr.LU = O.LU; fr.E = O.E; fr.G = O.G; fr.Li = O.Li; fr.Ui = O.Ui;
         fr.sk = O.sk; fr.rd = O.rd; fr.piv = O.piv;
         F->recs.push_back(fr);
         elim += O.r;
@@ -627,13 +630,16 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
   rskx_factor* F = const_cast<rskx_factor*>(Fc);
   const int64_t N = F->N;
   if (nrhs <= 0 || N == 0) return RSKX_OK;
-  RSKX_TRY(g_ws.xw.ensure((size_t)N * nrhs * sizeof(S)));
+  // work copy in tree order, padded with zero columns to a multiple of 8
+  constexpr int kRhs = 8;
+  const int ncol = (nrhs + kRhs - 1) / kRhs * kRhs;
+  RSKX_TRY(g_ws.xw.ensure((size_t)N * ncol * sizeof(S)));
   S* Y = g_ws.xw.as<S>();
   hipEvent_t ev;
   Prof& prof = F->prof;
   prof.begin("permute", ev);
-  launch_perm<U, S>(true, X, ldx, Y, N, nrhs, F->order.as<int>(), g_stream);
-  prof.end("permute", ev, (int64_t)N * nrhs * (sizeof(S) + sizeof(U)) + N * 4);
+  launch_perm<U, S>(true, X, ldx, Y, N, nrhs, ncol, F->order.as<int>(), g_stream);
+  prof.end("permute", ev, (int64_t)N * nrhs * sizeof(U) + N * ncol * sizeof(S) + N * 4);
   int prog;
   if (F->symm == 'p') prog = solve ? 2 : 5;
   else prog = solve ? (trans == 'n' ? 0 : 1) : (trans == 'n' ? 3 : 4);
@@ -646,9 +652,15 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
       const int64_t f0 = F->lvp[lv], f1 = F->lvp[lv + 1];
       if (f1 == f0) continue;
       const int kmx = F->lev_kmax[lv], rmx = F->lev_rmax[lv];
-      size_t per = (size_t)(kmx + 2 * rmx) * sizeof(S);
-      int cchunk = (int)std::min<size_t>(std::min(nrhs, 8), std::max<size_t>(1, (64 * 1024) / std::max<size_t>(per, 1)));
-      size_t shm = per * cchunk;
+      // xs, xr, tmp, red, then a stage buffer for one factor block: the
+      // level's largest block if the total stays under 76 KB (two
+      // workgroups per CU), else what is left; larger blocks are read from
+      // global memory directly
+      const size_t xb = ((size_t)(kmx + 2 * rmx) * kRhs + (size_t)kRhs * 256) * sizeof(S);
+      const size_t want = (size_t)std::max(rmx * rmx, kmx * rmx) * sizeof(S);
+      const size_t room = xb < 76 * 1024 ? 76 * 1024 - xb : 0;
+      const int bcap = (int)(std::min(want, room) / sizeof(S));
+      size_t shm = xb + (size_t)bcap * sizeof(S);
       if (shm > 160 * 1024) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: factor too large for the sweep kernel");
       int64_t bytes = 0;
       for (int64_t f = f0; f < f1; ++f) {
@@ -659,12 +671,12 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
       }
       prof.begin("sweep", ev);
       launch_sweep<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->fval.as<S>(),
-                      F->fidx.as<int>(), Y, N, nrhs, cchunk, shm, g_stream);
+                      F->fidx.as<int>(), Y, N, ncol, kmx, rmx, bcap, shm, g_stream);
       prof.end("sweep", ev, bytes);
     }
   }
   prof.begin("permute", ev);
-  launch_perm<U, S>(false, X, ldx, Y, N, nrhs, F->order.as<int>(), g_stream);
+  launch_perm<U, S>(false, X, ldx, Y, N, nrhs, ncol, F->order.as<int>(), g_stream);
   prof.end("permute", ev, (int64_t)N * nrhs * (sizeof(S) + sizeof(U)) + N * 4);
   RSKX_HIP(hipGetLastError());
   RSKX_HIP(hipStreamSynchronize(g_stream));

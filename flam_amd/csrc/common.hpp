@@ -76,6 +76,7 @@ struct BoxOut {
   int64_t M_off;          // M block output (k x k, or n x n if r == 0)
   int64_t nl_off;         // pool offset of the new current list (k)
   int64_t T, LU, E, G;    // offsets into the factor value arena
+  int64_t Li, Ui;         // inverted 32 x 32 diagonal blocks of L and U
   int64_t sk, rd, piv;    // offsets into the factor index arena
 };
 
@@ -83,8 +84,12 @@ struct BoxOut {
 struct FacRec {
   int k, r;
   int64_t T, LU, E, G;    // T k x r, LU r x r packed, E k x r, G r x k
+  int64_t Li, Ui;         // ceil(r/32) inverted diagonal blocks of L (and U), 32 x 32 each
   int64_t sk, rd, piv;    // index arena offsets (tree order point indices)
 };
+
+// diagonal block size of the blocked triangular solves in the sweeps
+constexpr int kDiagBlock = 32;
 
 struct IdParams {
   double tol;             // relative tolerance

@@ -681,6 +681,50 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
       __syncthreads();
     }
   }
+  // inverted 32 x 32 diagonal blocks of L (unit for LU) and U, one wave
+  // per block, lane c computing column c of the inverse; the sweeps apply
+  // them as small dense products instead of a 32-step substitution
+  {
+    const int nbk = (r + kDiagBlock - 1) / kDiagBlock;
+    for (int bb = wid; bb < 2 * nbk; bb += nw) {
+      const bool upper = bb >= nbk;
+      if (upper && !LU) continue;
+      const int b = upper ? bb - nbk : bb;
+      const int j0 = b * kDiagBlock;
+      const int bs = r - j0 < kDiagBlock ? r - j0 : kDiagBlock;
+      const int c = lane;
+      S x[kDiagBlock];
+#pragma unroll
+      for (int t = 0; t < kDiagBlock; ++t) x[t] = (S)0;
+      if (!upper) {
+#pragma unroll
+        for (int t = 0; t < kDiagBlock; ++t) {
+          if (t < bs) {
+            S v = (t == c) ? (S)1 : (S)0;
+#pragma unroll
+            for (int u = 0; u < t; ++u) v -= Krr[(j0 + t) + (j0 + u) * r] * x[u];
+            x[t] = LU ? v : v / Krr[(j0 + t) + (j0 + t) * r];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int tt = kDiagBlock - 1; tt >= 0; --tt) {
+          if (tt < bs) {
+            S v = (tt == c) ? (S)1 : (S)0;
+#pragma unroll
+            for (int u = tt + 1; u < kDiagBlock; ++u)
+              if (u < bs) v -= Krr[(j0 + tt) + (j0 + u) * r] * x[u];
+            x[tt] = v / Krr[(j0 + tt) + (j0 + tt) * r];
+          }
+        }
+      }
+      if (c < kDiagBlock) {
+        S* dst = fval + (upper ? O.Ui : O.Li) + (int64_t)b * kDiagBlock * kDiagBlock;
+#pragma unroll
+        for (int t = 0; t < kDiagBlock; ++t) dst[t + c * kDiagBlock] = (t < bs && c < bs) ? x[t] : (S)0;
+      }
+    }
+  }
   // E = Ksr'/U ('n') or Ksr'/L' ('p'); G = L \ Krs'(p,:) ('n'); column sweeps
   for (int j = 0; j < r; ++j) {
     const S inv = (S)1 / Krr[j + j * r];

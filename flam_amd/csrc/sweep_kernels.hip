@@ -12,95 +12,187 @@
 
 namespace rskx {
 
-// Y (ny x cc) += sgn * op(A) Z,  op(A)(i,l) = A[i + l lda] or A[l + i lda]
+// Right-hand sides are processed in blocks of kRhs columns (the host pads
+// the work copy of X with zero columns), so every loop over them is
+// unrolled at compile time.
+constexpr int kRhs = 8;
+
+// Y (ny x kRhs) += sgn * op(A) Z, op(A)(i,l) = A[i + l lda] or A[l + i lda].
+// When there are fewer rows than threads, each row's sum over l is split
+// between P threads and combined through `red` (P * ny * kRhs elements).
 template <typename S>
 __device__ void upd(S* Y, int ny, const S* Z, int nz, const S* __restrict__ A, int lda, bool trA,
-                    S sgn, int cc) {
-  for (int i = threadIdx.x; i < ny; i += blockDim.x) {
-    S acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (trA) {
-      const S* a = A + (int64_t)i * lda;
+                    S sgn, S* red) {
+  const int nt = blockDim.x, tid = threadIdx.x;
+  if (ny == 0 || nz == 0) return;
+  int P = nt / ny;
+  P = P < 1 ? 1 : (P > 16 ? 16 : P);
+  if (P == 1) {
+    for (int i = tid; i < ny; i += nt) {
+      S acc[kRhs] = {};
       for (int l = 0; l < nz; ++l) {
-        S al = a[l];
-        for (int c = 0; c < cc; ++c) acc[c] += al * Z[l + c * nz];
+        const S a = trA ? A[l + (int64_t)i * lda] : A[i + (int64_t)l * lda];
+#pragma unroll
+        for (int c = 0; c < kRhs; ++c) acc[c] += a * Z[l + c * nz];
       }
-    } else {
-      for (int l = 0; l < nz; ++l) {
-        S al = A[i + (int64_t)l * lda];
-        for (int c = 0; c < cc; ++c) acc[c] += al * Z[l + c * nz];
-      }
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) Y[i + c * ny] += sgn * acc[c];
     }
-    for (int c = 0; c < cc; ++c) Y[i + c * ny] += sgn * acc[c];
+    __syncthreads();
+    return;
+  }
+  if (tid < ny * P) {
+    const int i = tid % ny, part = tid / ny;
+    S acc[kRhs] = {};
+    for (int l = part; l < nz; l += P) {
+      const S a = trA ? A[l + (int64_t)i * lda] : A[i + (int64_t)l * lda];
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) acc[c] += a * Z[l + c * nz];
+    }
+#pragma unroll
+    for (int c = 0; c < kRhs; ++c) red[(part * kRhs + c) * ny + i] = acc[c];
+  }
+  __syncthreads();
+  for (int o = tid; o < ny * kRhs; o += nt) {
+    S sum = 0;
+    for (int part = 0; part < P; ++part) sum += red[part * kRhs * ny + o];
+    Y[o] += sgn * sum;
   }
   __syncthreads();
 }
 
-// in-place triangular solve with the r x r triangle of A (leading dim r)
+// In-place blocked triangular solve with the r x r triangle of A (leading
+// dimension r) and its inverted 32 x 32 diagonal blocks Ainv:
 //   lower/trans false: L y = b ; lower/trans true: L' y = b
 //   upper/trans false: U y = b ; upper/trans true: U' y = b
+// Per block: y_b = inv(A_bb) y_b as a 32 x kRhs product (one element per
+// thread), then the block's contribution is subtracted from the rows still
+// to be solved.
 template <typename S>
-__device__ void trsolve(S* Y, int r, const S* __restrict__ A, bool lower, bool unit, bool tr, int cc) {
+__device__ void trsolve(S* Y, int r, const S* __restrict__ A, const S* __restrict__ Ainv,
+                        bool lower, bool tr) {
   const bool fwd = (lower && !tr) || (!lower && tr);
-  for (int t = 0; t < r; ++t) {
-    const int j = fwd ? t : r - 1 - t;
-    if (!unit) {
-      for (int c = threadIdx.x; c < cc; c += blockDim.x) Y[j + c * r] /= A[j + (int64_t)j * r];
-      __syncthreads();
+  auto coef = [&](int i, int t) -> S { return tr ? A[t + (int64_t)i * r] : A[i + (int64_t)t * r]; };
+  const int nbk = (r + kDiagBlock - 1) / kDiagBlock;
+  const int tid = threadIdx.x;
+  const int bi = tid & (kDiagBlock - 1), bc = tid / kDiagBlock;   // block row, rhs column
+  for (int bk = 0; bk < nbk; ++bk) {
+    const int b = fwd ? bk : nbk - 1 - bk;
+    const int j0 = b * kDiagBlock, bs = r - j0 < kDiagBlock ? r - j0 : kDiagBlock;
+    const S* Ib = Ainv + (int64_t)b * kDiagBlock * kDiagBlock;
+    S acc = 0;
+    const bool act = bc < kRhs && bi < bs;
+    if (act) {
+      S iv[kDiagBlock];
+#pragma unroll
+      for (int u = 0; u < kDiagBlock; ++u)
+        iv[u] = u < bs ? (tr ? Ib[u + bi * kDiagBlock] : Ib[bi + u * kDiagBlock]) : (S)0;
+#pragma unroll
+      for (int u = 0; u < kDiagBlock; ++u)
+        if (u < bs) acc += iv[u] * Y[j0 + u + bc * r];
     }
-    // rows still to be solved: below j (fwd) or above j (bwd)
-    const int lo = fwd ? j + 1 : 0, hi = fwd ? r : j;
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      S a = tr ? A[j + (int64_t)i * r] : A[i + (int64_t)j * r];
-      for (int c = 0; c < cc; ++c) Y[i + c * r] -= a * Y[j + c * r];
+    __syncthreads();
+    if (act) Y[j0 + bi + bc * r] = acc;
+    __syncthreads();
+    const int lo = fwd ? j0 + bs : 0, hi = fwd ? r : j0;
+    for (int i = lo + tid; i < hi; i += blockDim.x) {
+      S at[kDiagBlock];
+#pragma unroll
+      for (int t = 0; t < kDiagBlock; ++t) at[t] = t < bs ? coef(i, j0 + t) : (S)0;
+      S a2[kRhs] = {};
+#pragma unroll
+      for (int t = 0; t < kDiagBlock; ++t)
+        if (t < bs) {
+#pragma unroll
+          for (int c = 0; c < kRhs; ++c) a2[c] += at[t] * Y[j0 + t + c * r];
+        }
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) Y[i + c * r] -= a2[c];
     }
     __syncthreads();
   }
 }
 
-// Y = op(tri(A)) Y using tmp (r x cc)
+// Y = op(tri(A)) Y, the sum over l split between threads as in upd
 template <typename S>
 __device__ void trmul(S* Y, S* tmp, int r, const S* __restrict__ A, bool lower, bool unit, bool tr,
-                      int cc) {
+                      S* red) {
   const bool low = lower != tr;   // effective lower-triangular operator
-  for (int i = threadIdx.x; i < r; i += blockDim.x) {
-    S acc[8];
-    for (int c = 0; c < cc; ++c) acc[c] = unit ? Y[i + c * r] : (S)0;
-    int l0 = low ? 0 : (unit ? i + 1 : i);
-    int l1 = low ? (unit ? i : i + 1) : r;
-    for (int l = l0; l < l1; ++l) {
-      S a = tr ? A[l + (int64_t)i * r] : A[i + (int64_t)l * r];
-      for (int c = 0; c < cc; ++c) acc[c] += a * Y[l + c * r];
+  const int nt = blockDim.x, tid = threadIdx.x;
+  int P = nt / (r > 0 ? r : 1);
+  P = P < 1 ? 1 : (P > 16 ? 16 : P);
+  for (int t0 = tid; t0 < r * P; t0 += nt) {
+    const int i = t0 % r, part = t0 / r;
+    const int l0 = low ? 0 : (unit ? i + 1 : i);
+    const int l1 = low ? (unit ? i : i + 1) : r;
+    S acc[kRhs] = {};
+    for (int l = l0 + part; l < l1; l += P) {
+      const S a = tr ? A[l + (int64_t)i * r] : A[i + (int64_t)l * r];
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) acc[c] += a * Y[l + c * r];
     }
-    for (int c = 0; c < cc; ++c) tmp[i + c * r] = acc[c];
+    if (unit && part == 0) {
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) acc[c] += Y[i + c * r];
+    }
+    // one part per row (r >= threads): straight into tmp; else partials
+    S* dst = P == 1 ? tmp : red + part * kRhs * r;
+#pragma unroll
+    for (int c = 0; c < kRhs; ++c) dst[c * r + i] = acc[c];
   }
   __syncthreads();
-  for (int c = 0; c < cc; ++c)
-    for (int i = threadIdx.x; i < r; i += blockDim.x) Y[i + c * r] = tmp[i + c * r];
+  if (P > 1) {
+    for (int o = tid; o < r * kRhs; o += nt) {
+      S sum = 0;
+      for (int part = 0; part < P; ++part) sum += red[part * kRhs * r + o];
+      tmp[o] = sum;
+    }
+    __syncthreads();
+  }
+  for (int o = tid; o < r * kRhs; o += nt) Y[o] = tmp[o];
   __syncthreads();
 }
 
 // gather (tmp(i) = Y(p(i))) or scatter (tmp(p(i)) = Y(i)), then copy back
 template <typename S>
-__device__ void permrows(S* Y, S* tmp, int r, const int* __restrict__ p, bool gather, int cc) {
+__device__ void permrows(S* Y, S* tmp, int r, const int* __restrict__ p, bool gather) {
   for (int i = threadIdx.x; i < r; i += blockDim.x) {
-    int pi = p[i];
-    for (int c = 0; c < cc; ++c) {
+    const int pi = p[i];
+#pragma unroll
+    for (int c = 0; c < kRhs; ++c) {
       if (gather) tmp[i + c * r] = Y[pi + c * r];
       else tmp[pi + c * r] = Y[i + c * r];
     }
   }
   __syncthreads();
-  for (int c = 0; c < cc; ++c)
-    for (int i = threadIdx.x; i < r; i += blockDim.x) Y[i + c * r] = tmp[i + c * r];
+  for (int o = threadIdx.x; o < r * kRhs; o += blockDim.x) Y[o] = tmp[o];
   __syncthreads();
 }
 
+// Copy n contiguous elements global -> shared with eight independent
+// loads in flight per thread.
+template <typename S>
+__device__ void bulk_copy(S* __restrict__ dst, const S* __restrict__ src, int n) {
+  const int nt = blockDim.x;
+  int e = threadIdx.x;
+  for (; e + 7 * nt < n; e += 8 * nt) {
+    S v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[e + u * nt];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) dst[e + u * nt] = v[u];
+  }
+  for (; e < n; e += nt) dst[e] = src[e];
+}
+
 // PROG: 0 sv 'n'/'n', 1 sv 'n'/'c', 2 sv 'p', 3 mv 'n'/'n', 4 mv 'n'/'c', 5 mv 'p'
+// Shared memory: xs (k x 8), xr (r x 8), tmp (r x 8), red (8 x 256), then
+// the stage buffer (bcap elements).  X has nrhs columns, a multiple of 8.
 template <typename S, int PROG, bool UP>
 __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
                                                const S* __restrict__ fval,
                                                const int* __restrict__ fidx, S* X, int64_t ldx,
-                                               int nrhs, int cchunk) {
+                                               int nrhs, int kmax, int rmax, int bcap) {
   extern __shared__ char smem_raw[];
   S* sm = reinterpret_cast<S*>(smem_raw);
   const FacRec f = recs[blockIdx.x];
@@ -108,89 +200,116 @@ __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
   const int* sk = fidx + f.sk;
   const int* rd = fidx + f.rd;
   const int* piv = fidx + f.piv;
-  const S* T = fval + f.T;
-  const S* LU = fval + f.LU;
-  const S* E = fval + f.E;
-  const S* G = fval + f.G;
+  const S* Tg = fval + f.T;
+  const S* LUg = fval + f.LU;
+  const S* Eg = fval + f.E;
+  const S* Gg = fval + f.G;
+  const S* Li = fval + f.Li;
+  const S* Ui = fval + f.Ui;
+  S* xs = sm;
+  S* xr = xs + (size_t)kmax * kRhs;
+  S* tmp = xr + (size_t)rmax * kRhs;
+  S* red = tmp + (size_t)rmax * kRhs;
+  S* buf = red + (size_t)kRhs * 256;
+  // a factor block is staged in shared memory (one bulk round trip) when it
+  // fits in the bcap-element buffer
+  auto stage = [&](const S* g, int cnt) -> const S* {
+    if (cnt > bcap) return g;
+    __syncthreads();
+    bulk_copy(buf, g, cnt);
+    __syncthreads();
+    return buf;
+  };
+  const int kr = k * r, rr = r * r;
   const S one = 1, mone = -1;
-  for (int c0 = 0; c0 < nrhs; c0 += cchunk) {
-    const int cc = nrhs - c0 < cchunk ? nrhs - c0 : cchunk;
-    S* xs = sm;
-    S* xr = xs + k * cc;
-    S* tmp = xr + r * cc;
-    for (int c = 0; c < cc; ++c) {
-      for (int a = threadIdx.x; a < k; a += blockDim.x) xs[a + c * k] = X[sk[a] + (c0 + c) * ldx];
-      for (int i = threadIdx.x; i < r; i += blockDim.x) xr[i + c * r] = X[rd[i] + (c0 + c) * ldx];
+  for (int c0 = 0; c0 < nrhs; c0 += kRhs) {
+    // gather X(sk,:), X(rd,:): one index load, then all columns in flight
+    for (int a = threadIdx.x; a < k + r; a += blockDim.x) {
+      const bool isk = a < k;
+      const int64_t row = isk ? sk[a] : rd[a - k];
+      const S* src = X + row + c0 * ldx;
+      S v[kRhs];
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) v[c] = src[c * ldx];
+      S* dst = isk ? xs + a : xr + (a - k);
+      const int ld = isk ? k : r;
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) dst[c * ld] = v[c];
     }
     __syncthreads();
     if (PROG == 0) {            // sv_nn.m
       if (UP) {
-        upd(xr, r, xs, k, T, k, true, mone, cc);
-        permrows(xr, tmp, r, piv, true, cc);
-        trsolve(xr, r, LU, true, true, false, cc);
-        upd(xs, k, xr, r, E, k, false, mone, cc);
+        upd(xr, r, xs, k, stage(Tg, kr), k, true, mone, red);
+        permrows(xr, tmp, r, piv, true);
+        trsolve(xr, r, stage(LUg, rr), Li, true, false);
+        upd(xs, k, xr, r, stage(Eg, kr), k, false, mone, red);
       } else {
-        upd(xr, r, xs, k, G, r, false, mone, cc);
-        trsolve(xr, r, LU, false, false, false, cc);
-        upd(xs, k, xr, r, T, k, false, mone, cc);
+        upd(xr, r, xs, k, stage(Gg, kr), r, false, mone, red);
+        trsolve(xr, r, stage(LUg, rr), Ui, false, false);
+        upd(xs, k, xr, r, stage(Tg, kr), k, false, mone, red);
       }
     } else if (PROG == 1) {     // sv_nc.m
       if (UP) {
-        upd(xr, r, xs, k, T, k, true, mone, cc);
-        trsolve(xr, r, LU, false, false, true, cc);
-        upd(xs, k, xr, r, G, r, true, mone, cc);
+        upd(xr, r, xs, k, stage(Tg, kr), k, true, mone, red);
+        trsolve(xr, r, stage(LUg, rr), Ui, false, true);
+        upd(xs, k, xr, r, stage(Gg, kr), r, true, mone, red);
       } else {
-        upd(xr, r, xs, k, E, k, true, mone, cc);
-        trsolve(xr, r, LU, true, true, true, cc);
-        permrows(xr, tmp, r, piv, false, cc);
-        upd(xs, k, xr, r, T, k, false, mone, cc);
+        upd(xr, r, xs, k, stage(Eg, kr), k, true, mone, red);
+        trsolve(xr, r, stage(LUg, rr), Li, true, true);
+        permrows(xr, tmp, r, piv, false);
+        upd(xs, k, xr, r, stage(Tg, kr), k, false, mone, red);
       }
     } else if (PROG == 2) {     // sv_p.m
       if (UP) {
-        upd(xr, r, xs, k, T, k, true, mone, cc);
-        trsolve(xr, r, LU, true, false, false, cc);
-        upd(xs, k, xr, r, E, k, false, mone, cc);
+        upd(xr, r, xs, k, stage(Tg, kr), k, true, mone, red);
+        trsolve(xr, r, stage(LUg, rr), Li, true, false);
+        upd(xs, k, xr, r, stage(Eg, kr), k, false, mone, red);
       } else {
-        upd(xr, r, xs, k, E, k, true, mone, cc);
-        trsolve(xr, r, LU, true, false, true, cc);
-        upd(xs, k, xr, r, T, k, false, mone, cc);
+        upd(xr, r, xs, k, stage(Eg, kr), k, true, mone, red);
+        trsolve(xr, r, stage(LUg, rr), Li, true, true);
+        upd(xs, k, xr, r, stage(Tg, kr), k, false, mone, red);
       }
     } else if (PROG == 3) {     // mv_nn.m
       if (UP) {
-        upd(xs, k, xr, r, T, k, false, one, cc);
-        trmul(xr, tmp, r, LU, false, false, false, cc);
-        upd(xr, r, xs, k, G, r, false, one, cc);
+        upd(xs, k, xr, r, stage(Tg, kr), k, false, one, red);
+        trmul(xr, tmp, r, stage(LUg, rr), false, false, false, red);
+        upd(xr, r, xs, k, stage(Gg, kr), r, false, one, red);
       } else {
-        upd(xs, k, xr, r, E, k, false, one, cc);
-        trmul(xr, tmp, r, LU, true, true, false, cc);
-        permrows(xr, tmp, r, piv, false, cc);
-        upd(xr, r, xs, k, T, k, true, one, cc);
+        upd(xs, k, xr, r, stage(Eg, kr), k, false, one, red);
+        trmul(xr, tmp, r, stage(LUg, rr), true, true, false, red);
+        permrows(xr, tmp, r, piv, false);
+        upd(xr, r, xs, k, stage(Tg, kr), k, true, one, red);
       }
     } else if (PROG == 4) {     // mv_nc.m
       if (UP) {
-        upd(xs, k, xr, r, T, k, false, one, cc);
-        permrows(xr, tmp, r, piv, true, cc);
-        trmul(xr, tmp, r, LU, true, true, true, cc);
-        upd(xr, r, xs, k, E, k, true, one, cc);
+        upd(xs, k, xr, r, stage(Tg, kr), k, false, one, red);
+        permrows(xr, tmp, r, piv, true);
+        trmul(xr, tmp, r, stage(LUg, rr), true, true, true, red);
+        upd(xr, r, xs, k, stage(Eg, kr), k, true, one, red);
       } else {
-        upd(xs, k, xr, r, G, r, true, one, cc);
-        trmul(xr, tmp, r, LU, false, false, true, cc);
-        upd(xr, r, xs, k, T, k, true, one, cc);
+        upd(xs, k, xr, r, stage(Gg, kr), r, true, one, red);
+        trmul(xr, tmp, r, stage(LUg, rr), false, false, true, red);
+        upd(xr, r, xs, k, stage(Tg, kr), k, true, one, red);
       }
     } else {                    // mv_p.m
       if (UP) {
-        upd(xs, k, xr, r, T, k, false, one, cc);
-        trmul(xr, tmp, r, LU, true, false, true, cc);
-        upd(xr, r, xs, k, E, k, true, one, cc);
+        upd(xs, k, xr, r, stage(Tg, kr), k, false, one, red);
+        trmul(xr, tmp, r, stage(LUg, rr), true, false, true, red);
+        upd(xr, r, xs, k, stage(Eg, kr), k, true, one, red);
       } else {
-        upd(xs, k, xr, r, E, k, false, one, cc);
-        trmul(xr, tmp, r, LU, true, false, false, cc);
-        upd(xr, r, xs, k, T, k, true, one, cc);
+        upd(xs, k, xr, r, stage(Eg, kr), k, false, one, red);
+        trmul(xr, tmp, r, stage(LUg, rr), true, false, false, red);
+        upd(xr, r, xs, k, stage(Tg, kr), k, true, one, red);
       }
     }
-    for (int c = 0; c < cc; ++c) {
-      for (int a = threadIdx.x; a < k; a += blockDim.x) X[sk[a] + (c0 + c) * ldx] = xs[a + c * k];
-      for (int i = threadIdx.x; i < r; i += blockDim.x) X[rd[i] + (c0 + c) * ldx] = xr[i + c * r];
+    for (int a = threadIdx.x; a < k + r; a += blockDim.x) {
+      const bool isk = a < k;
+      const int64_t row = isk ? sk[a] : rd[a - k];
+      S* dstg = X + row + c0 * ldx;
+      const S* srcl = isk ? xs + a : xr + (a - k);
+      const int ld = isk ? k : r;
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) dstg[c * ldx] = srcl[c * ld];
     }
     __syncthreads();
   }
@@ -198,7 +317,8 @@ __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
 
 template <typename S, int PROG>
 static void launch_prog(bool up, const FacRec* recs, int nrec, const S* fval, const int* fidx, S* X,
-                        int64_t ldx, int nrhs, int cchunk, size_t shmem, hipStream_t st) {
+                        int64_t ldx, int nrhs, int kmax, int rmax, int bcap, size_t shmem,
+                        hipStream_t st) {
   if (shmem > 65536) {
     (void)hipFuncSetAttribute((const void*)k_sweep<S, PROG, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
@@ -207,58 +327,57 @@ static void launch_prog(bool up, const FacRec* recs, int nrec, const S* fval, co
   }
   if (up)
     hipLaunchKernelGGL((k_sweep<S, PROG, true>), dim3(nrec), dim3(256), shmem, st, recs, fval, fidx,
-                       X, ldx, nrhs, cchunk);
+                       X, ldx, nrhs, kmax, rmax, bcap);
   else
     hipLaunchKernelGGL((k_sweep<S, PROG, false>), dim3(nrec), dim3(256), shmem, st, recs, fval,
-                       fidx, X, ldx, nrhs, cchunk);
+                       fidx, X, ldx, nrhs, kmax, rmax, bcap);
 }
 
 template <typename S>
 void launch_sweep(int prog, bool up, const FacRec* recs, int nrec, const S* fval, const int* fidx,
-                  S* X, int64_t ldx, int nrhs, int cchunk, size_t shmem, hipStream_t st) {
+                  S* X, int64_t ldx, int nrhs, int kmax, int rmax, int bcap, size_t shmem,
+                  hipStream_t st) {
   switch (prog) {
-    case 0: launch_prog<S, 0>(up, recs, nrec, fval, fidx, X, ldx, nrhs, cchunk, shmem, st); break;
-    case 1: launch_prog<S, 1>(up, recs, nrec, fval, fidx, X, ldx, nrhs, cchunk, shmem, st); break;
-    case 2: launch_prog<S, 2>(up, recs, nrec, fval, fidx, X, ldx, nrhs, cchunk, shmem, st); break;
-    case 3: launch_prog<S, 3>(up, recs, nrec, fval, fidx, X, ldx, nrhs, cchunk, shmem, st); break;
-    case 4: launch_prog<S, 4>(up, recs, nrec, fval, fidx, X, ldx, nrhs, cchunk, shmem, st); break;
-    default: launch_prog<S, 5>(up, recs, nrec, fval, fidx, X, ldx, nrhs, cchunk, shmem, st); break;
+    case 0: launch_prog<S, 0>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
+    case 1: launch_prog<S, 1>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
+    case 2: launch_prog<S, 2>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
+    case 3: launch_prog<S, 3>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
+    case 4: launch_prog<S, 4>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
+    default: launch_prog<S, 5>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
   }
 }
 
 // ------------------------------------------------ X to / from tree order --
+// Y has ncol >= nrhs columns; the extra ones are zero
 template <typename U, typename S>
 __global__ void k_to_tree(const U* __restrict__ X, int64_t ldx, S* __restrict__ Y, int64_t N,
-                          int nrhs, const int* __restrict__ order) {
-  int64_t tot = N * nrhs;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t p = e % N, c = e / N;
-    Y[e] = (S)X[order[p] + c * ldx];
+                          int nrhs, int ncol, const int* __restrict__ order) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = order[p];
+    for (int c = 0; c < ncol; ++c) Y[p + c * N] = c < nrhs ? (S)X[q + c * ldx] : (S)0;
   }
 }
 
 template <typename U, typename S>
 __global__ void k_from_tree(const S* __restrict__ Y, U* __restrict__ X, int64_t ldx, int64_t N,
                             int nrhs, const int* __restrict__ order) {
-  int64_t tot = N * nrhs;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t p = e % N, c = e / N;
-    X[order[p] + c * ldx] = (U)Y[e];
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = order[p];
+    for (int c = 0; c < nrhs; ++c) X[q + c * ldx] = (U)Y[p + c * N];
   }
 }
 
 template <typename U, typename S>
-void launch_perm(bool to, void* X, int64_t ldx, S* Y, int64_t N, int nrhs, const int* order,
-                 hipStream_t st) {
-  int64_t tot = N * nrhs;
-  int grid = (int)((tot + 255) / 256);
+void launch_perm(bool to, void* X, int64_t ldx, S* Y, int64_t N, int nrhs, int ncol,
+                 const int* order, hipStream_t st) {
+  int grid = (int)((N + 255) / 256);
   if (grid > 65536) grid = 65536;
   if (grid < 1) grid = 1;
   if (to)
     hipLaunchKernelGGL((k_to_tree<U, S>), dim3(grid), dim3(256), 0, st, (const U*)X, ldx, Y, N,
-                       nrhs, order);
+                       nrhs, ncol, order);
   else
     hipLaunchKernelGGL((k_from_tree<U, S>), dim3(grid), dim3(256), 0, st, (const S*)Y, (U*)X, ldx,
                        N, nrhs, order);
@@ -302,11 +421,13 @@ __global__ void __launch_bounds__(256) k_matvec(const double* __restrict__ x, in
           double dx = xi - sx[t], dy = yi - sy[t];
           a = scale * sw[t] * (dx * snx[t] + dy * sny[t]) / (kTwoPiM * (dx * dx + dy * dy));
         }
-        for (int c = 0; c < cc; ++c) acc[c] += a * sX[c][t];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) if (c < cc) acc[c] += a * sX[c][t];
       }
     }
     if (act)
-      for (int c = 0; c < cc; ++c) Y[i + (c0 + c) * ldy] = acc[c];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) if (c < cc) Y[i + (c0 + c) * ldy] = acc[c];
   }
 }
 
@@ -344,11 +465,11 @@ void launch_gather_points(const double* x, const double* nu, const double* w, co
 
 #define RSKX_SWEEP_INST(S)                                                                    \
   template void launch_sweep<S>(int, bool, const FacRec*, int, const S*, const int*, S*,      \
-                                int64_t, int, int, size_t, hipStream_t);                      \
-  template void launch_perm<double, S>(bool, void*, int64_t, S*, int64_t, int, const int*,    \
-                                       hipStream_t);                                          \
-  template void launch_perm<float, S>(bool, void*, int64_t, S*, int64_t, int, const int*,     \
-                                      hipStream_t);
+                                int64_t, int, int, int, int, size_t, hipStream_t);            \
+  template void launch_perm<double, S>(bool, void*, int64_t, S*, int64_t, int, int,         \
+                                       const int*, hipStream_t);                              \
+  template void launch_perm<float, S>(bool, void*, int64_t, S*, int64_t, int, int,          \
+                                      const int*, hipStream_t);
 RSKX_SWEEP_INST(double)
 RSKX_SWEEP_INST(float)
 
