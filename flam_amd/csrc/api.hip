@@ -34,6 +34,12 @@ template <typename S>
 void launch_id(const LvlBox*, const int*, S*, int*, double*, const IdParams&, int*, int*, int,
                size_t, hipStream_t);
 size_t elim_elems(bool lu, int k, int r);
+size_t elim_reg_shmem(int n, int k, int r, size_t es);
+int elim_reg_nmax(size_t es);
+template <typename S>
+void launch_elim_reg(bool, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
+                     const int*, const double*, const S*, S*, S*, int*, const PointsDev&, int*,
+                     hipStream_t);
 template <typename S>
 void launch_elim(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*,
                  int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
@@ -545,22 +551,36 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     // elimination: boxes whose blocks fit two per CU in shared memory, one
     // per CU, or neither (global scratch; also the no-compression copies)
     const size_t kHalf = 78 * 1024, kFull = 158 * 1024;   // 1024-thread workgroups
-    std::vector<int> cls[3];
-    size_t shm_cls[3] = {0, 0, 0};
+    std::vector<int> cls[3], creg;
+    size_t shm_cls[3] = {0, 0, 0}, shm_reg = 0;
+    const int nreg = elim_reg_nmax(sizeof(S));
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
+      if (O.r > 0 && hb[ib].n <= nreg) {     // augmented matrix in registers
+        creg.push_back((int)ib);
+        shm_reg = std::max(shm_reg, elim_reg_shmem(hb[ib].n, O.k, O.r, sizeof(S)));
+        continue;
+      }
       const size_t cache = 6 * sizeof(double) * (size_t)hb[ib].n;
       size_t by = O.r > 0 ? elim_elems(lu, O.k, O.r) * sizeof(S) + cache : 0;
       int c = (O.r > 0 && by <= kHalf) ? 0 : (O.r > 0 && by <= kFull) ? 1 : 2;
       cls[c].push_back((int)ib);
       shm_cls[c] = std::max(shm_cls[c], c < 2 ? by : cache);
     }
-    std::vector<int> allids;
+    std::vector<int> allids(creg);
     for (int c = 2; c >= 0; --c) allids.insert(allids.end(), cls[c].begin(), cls[c].end());
     RSKX_TRY(upload(ws.ids, allids));
     prof.begin("factor", ev);
     {
       int off = 0;
+      if (!creg.empty()) {
+        launch_elim_reg<S>(lu, shm_reg, ws.ids.as<int>(), (int)creg.size(), ws.boxes.as<LvlBox>(),
+                           ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
+                           ws.perm.as<int>(), ws.T.as<double>(), ws.M[mpar].as<S>(),
+                           ws.M[mcur].as<S>(), F->fval.as<S>(), F->fidx.as<int>(), P,
+                           ws.flag.as<int>(), g_stream);
+        off += (int)creg.size();
+      }
       for (int c = 2; c >= 0; --c) {
         if (cls[c].empty()) continue;
         launch_elim<S>(lu, c < 2 ? 1 : 0, shm_cls[c], ws.ids.as<int>() + off,

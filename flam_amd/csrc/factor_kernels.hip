@@ -797,6 +797,405 @@ void launch_id(const LvlBox* boxes, const int* mrows, S* Kar, int* permar, doubl
                      kout, iterout);
 }
 
+// Elimination with the augmented matrix held in registers.
+//
+// In the order [rd; sk] the box's sparsified matrix with the skeleton
+// block replaced by M(sk,sk) is
+//     Z = [ Krr'  Krs' ]        Krr' = Krr - T' Ksr' - Krs T
+//         [ Ksr'  Mss  ]        Ksr' = Ksr - Kss T,  Krs' = Krs - T' Kss
+// and r steps of right-looking elimination on Z -- pivots chosen only
+// among the first r rows for LU, none for Cholesky -- leave L and U
+// (rskelf.m:225/235) in the leading block, G = L \ Krs'(p,:) and
+// E = Ksr' / U (or Ksr' / L') beside and below it (rskelf.m:226-227,236),
+// and Mss - E G (or Mss - E E') in the trailing block (rskelf.m:240-246).
+//
+// Thread (wave w, lane l) holds Z(l + 64 a, w + 16 b), a < TA, b < TB, so
+// n <= min(64 TA, 16 TB).  Pivoting is lazy: rows are never moved; each
+// step's pivot row is found among the rows not yet used, ties going to the
+// row LAPACK's swapped order would reach first.
+template <typename S, bool LU, int TA, int TB>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) k_elim_reg(const int* __restrict__ ids,
+                                                   const LvlBox* __restrict__ boxes,
+                                                   const BoxOut* __restrict__ outs,
+                                                   const Seg* __restrict__ csegs, int* pool,
+                                                   const int* __restrict__ permar,
+                                                   const double* __restrict__ Tar,
+                                                   const S* __restrict__ Mprev, S* __restrict__ Mcur,
+                                                   S* __restrict__ fval, int* __restrict__ fidx,
+                                                   PointsDev P, int* __restrict__ fail) {
+  extern __shared__ double smem_d[];
+  const int bidx = ids[blockIdx.x];
+  const LvlBox B = boxes[bidx];
+  const BoxOut O = outs[bidx];
+  const int n = B.n, k = O.k, r = O.r;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  __shared__ ChildMap cm;
+  __shared__ int s_piv[2];
+  __shared__ S s_pv[2];
+  if (tid == 0) {
+    int nc = B.ncs > 8 ? 8 : B.ncs;
+    cm.nc = nc;
+    int acc = 0;
+    for (int c = 0; c < nc; ++c) {
+      Seg g = csegs[B.cs0 + c];
+      cm.start[c] = acc;
+      cm.moff[c] = g.moff;
+      acc += g.len;
+    }
+    cm.start[nc] = acc;
+  }
+  const int* slf = pool + B.slf_off;
+  const int* q = permar + B.perm_off;           // q[0..k) skeletons, q[k..n) redundant
+  const double* Tg = Tar + B.T_off;              // k x r, leading dimension n
+  // shared layout
+  double* cx = smem_d;
+  double* cy = cx + n;
+  double* cnx = cy + n;
+  double* cny = cnx + n;
+  double* cw = cny + n;
+  double* cdg = cw + n;
+  int* qp = reinterpret_cast<int*>(cdg + n);     // slf position of Z row/col p
+  int* lpos = qp + n;                            // LU: logical position of physical row
+  int* prow = lpos + n;                          // LU: physical row at logical position,
+                                                 // then 2 x TB panel pivot rows
+  S* colb = reinterpret_cast<S*>(prow + n + 2 * TB + ((n + 2 * TB) & 1));   // 2 x n column buffers
+  S* Ts = colb + 2 * TB * n;                     // k x r
+  S* Kss = Ts + k * r;                           // k x k
+  S* KX = Kss + k * k;                           // k x r, then r x k
+  for (int p = tid; p < n; p += 1024) {
+    const int sp = p < r ? q[k + p] : q[p - r];
+    qp[p] = sp;
+    const int pt = slf[sp];
+    cx[p] = P.x[0][pt]; cy[p] = P.x[1][pt];
+    cnx[p] = P.nu[0][pt]; cny[p] = P.nu[1][pt];
+    cw[p] = P.w[pt]; cdg[p] = P.diag[pt];
+    if (LU && p < r) { lpos[p] = p; prow[p] = p; }
+  }
+  __syncthreads();
+  const bool leafbox = cm.nc == 1 && cm.moff[0] < 0;
+  auto aval = [&](int p1, int p2) -> double {    // A entry between Z positions
+    if (p1 == p2) return cdg[p2];
+    double dx = cx[p1] - cx[p2], dy = cy[p1] - cy[p2];
+    return P.scale * cw[p2] * (dx * cnx[p2] + dy * cny[p2]) / (kTwoPi * (dx * dx + dy * dy));
+  };
+  auto mv = [&](int p1, int p2) -> S { return leafbox ? (S)0 : mval(cm, Mprev, qp[p1], qp[p2]); };
+  auto kv = [&](int p1, int p2) -> S { return (S)aval(p1, p2) + mv(p1, p2); };
+  // T, Kss, Ksr (positions: rd j -> j, sk a -> r + a)
+  for (int e = tid; e < k * r; e += 1024) {
+    const int a = e % k, j = e / k;
+    Ts[e] = (S)Tg[a + (int64_t)j * n];
+    KX[e] = kv(r + a, j);
+  }
+  for (int e = tid; e < k * k; e += 1024) Kss[e] = kv(r + e % k, r + e / k);
+  __syncthreads();
+  for (int e = tid; e < k * r; e += 1024) {      // Ksr' = Ksr - Kss T
+    const int a = e % k, j = e / k;
+    S acc = 0;
+    for (int l = 0; l < k; ++l) acc += Kss[a + l * k] * Ts[l + j * k];
+    KX[e] -= acc;
+  }
+  __syncthreads();
+  S Z[TA][TB];
+#pragma unroll
+  for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+      const int i = lane + 64 * ta, j = wid + 16 * tb;
+      S z = 0;
+      if (i < n && j < n) {
+        if (i < r && j < r) z = kv(i, j);                    // Krr
+        else if (i >= r && j < r) z = KX[(i - r) + j * k];   // Ksr'
+        else if (i >= r && j >= r) z = mv(i, j);             // M(sk,sk)
+      }
+      Z[ta][tb] = z;
+    }
+  // Krr -= T' Ksr': per l, four T values per lane and sixteen Ksr' values
+  // per wave feed 64 multiply-adds
+  for (int l = 0; l < k; ++l) {
+    S tv[TA];
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta) {
+      const int i = lane + 64 * ta;
+      tv[ta] = i < r ? Ts[l + i * k] : (S)0;
+    }
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+      const int j = wid + 16 * tb;
+      const S kx = j < r ? KX[l + j * k] : (S)0;
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta) Z[ta][tb] -= tv[ta] * kx;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < r * k; e += 1024) KX[e] = kv(e % r, r + e / r);   // Krs, r x k
+  __syncthreads();
+  for (int l = 0; l < k; ++l) {                  // Krr -= Krs T
+    S kr[TA];
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta) {
+      const int i = lane + 64 * ta;
+      kr[ta] = i < r ? KX[i + l * r] : (S)0;
+    }
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+      const int j = wid + 16 * tb;
+      const S tt = j < r ? Ts[l + j * k] : (S)0;
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta) Z[ta][tb] -= kr[ta] * tt;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < r * k; e += 1024) {      // Krs' = Krs - T' Kss
+    const int i = e % r, c = e / r;
+    S acc = 0;
+    for (int l = 0; l < k; ++l) acc += Ts[l + i * k] * Kss[l + c * k];
+    KX[e] -= acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+      const int i = lane + 64 * ta, j = wid + 16 * tb;
+      if (i < r && j >= r && j < n) Z[ta][tb] = KX[i + (j - r) * r];
+    }
+
+  // ---- r elimination steps, one barrier each; column s is held by wave
+  // s mod 16, slot s / 16
+  bool done[TA];                                 // LU: row already used as a pivot
+#pragma unroll
+  for (int ta = 0; ta < TA; ++ta) done[ta] = false;
+  for (int s = 0; s < r; ++s) {
+    S* cb = colb + (s & 1) * n;
+    const int ws = s & 15, bs = s >> 4;
+    if (wid == ws) {
+      S colv[TA];
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta) {
+        S v = 0;
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) if (tb == bs) v = Z[ta][tb];
+        colv[ta] = v;
+      }
+      int pr;
+      S pv;
+      if (LU) {
+        double best = -1.0;
+        int bl = 1 << 30, bp = -1;
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) {
+          const int i = lane + 64 * ta;
+          if (i < r && !done[ta]) {
+            const double av = fabs((double)colv[ta]);
+            const int lp = lpos[i];
+            if (av > best || (av == best && lp < bl)) { best = av; bl = lp; bp = i; }
+          }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          const double b2 = __shfl_xor(best, o, 64);
+          const int l2 = __shfl_xor(bl, o, 64), p2 = __shfl_xor(bp, o, 64);
+          if (b2 > best || (b2 == best && l2 < bl)) { best = b2; bl = l2; bp = p2; }
+        }
+        // no candidate compared (all NaN): take the row at position s
+        pr = bp >= 0 ? bp : prow[s];
+        S pval = 0;
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) if (ta == (pr >> 6)) pval = colv[ta];
+        pv = __shfl(pval, pr & 63, 64);
+        if (!(best > 0)) { if (lane == 0) atomicOr(fail, 2); pv = (S)1; }
+      } else {
+        pr = s;
+        S dval = 0;
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) if (ta == (s >> 6)) dval = colv[ta];
+        dval = __shfl(dval, s & 63, 64);
+        if (!(dval > (S)0)) { if (lane == 0) atomicOr(fail, 1); dval = (S)1; }
+        pv = sqrt(dval);
+      }
+      const S inv = (S)1 / pv;
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta) {
+        const int i = lane + 64 * ta;
+        bool below;
+        if (LU) below = i < n && i != pr && !done[ta];
+        else below = i > s && i < n;
+        if (below) colv[ta] *= inv;
+        else if (i == pr) colv[ta] = pv;
+        if (below) cb[i] = colv[ta];
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) if (tb == bs && i < n) Z[ta][tb] = colv[ta];
+      }
+      if (lane == 0) { s_piv[s & 1] = pr; s_pv[s & 1] = pv; }
+      if (LU && lane == 0) {
+        // the rows at logical positions s and lpos[pr] trade places
+        const int ls = lpos[pr], ps = prow[s];
+        prow[ls] = ps; lpos[ps] = ls;
+        prow[s] = pr; lpos[pr] = s;
+      }
+    }
+    __syncthreads();
+    const int pr = s_piv[s & 1];
+    S li[TA];
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta) {
+      const int i = lane + 64 * ta;
+      bool live;
+      if (LU) {
+        live = i < n && i != pr && !done[ta];
+        if (i == pr) done[ta] = true;
+      } else {
+        live = i > s && i < n;
+      }
+      li[ta] = live ? cb[i] : (S)0;
+    }
+    const int pa = pr >> 6, pl = pr & 63;
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+      const int j = wid + 16 * tb;
+      if (j <= s || j >= n) continue;
+      S u;
+      if (LU) {
+        S v = 0;
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) if (ta == pa) v = Z[ta][tb];
+        u = __shfl(v, pl, 64);
+      } else {
+        u = cb[j];
+      }
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta) {
+        const int i = lane + 64 * ta;
+        if (LU || j <= i || (i >= r && j >= r)) Z[ta][tb] -= li[ta] * u;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- outputs (rskelf.m:249-257)
+  S* fLU = fval + O.LU;
+  S* fE = fval + O.E;
+  S* fG = fval + O.G;
+  S* Mo = Mcur + O.M_off;
+#pragma unroll
+  for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+      const int i = lane + 64 * ta, j = wid + 16 * tb;
+      if (i >= n || j >= n) continue;
+      const S z = Z[ta][tb];
+      if (i < r) {
+        const int li = LU ? lpos[i] : i;
+        if (j < r) fLU[li + j * r] = z;
+        else if (LU) fG[li + (j - r) * r] = z;
+      } else {
+        if (j < r) fE[(i - r) + j * k] = z;
+        else Mo[(i - r) + (j - r) * k] = z;
+      }
+    }
+  int* piv = fidx + O.piv;
+  if (LU)
+    for (int p = tid; p < r; p += 1024) piv[p] = prow[p];
+  S* fT = fval + O.T;
+  for (int e = tid; e < k * r; e += 1024) fT[e] = Ts[e];
+  int* fsk = fidx + O.sk;
+  int* frd = fidx + O.rd;
+  for (int a = tid; a < k; a += 1024) {
+    const int pt = slf[q[a]];
+    fsk[a] = pt;
+    pool[O.nl_off + a] = pt;
+  }
+  for (int i = tid; i < r; i += 1024) frd[i] = slf[q[k + i]];
+  __syncthreads();
+  // inverted 32 x 32 diagonal blocks of L and U, from copies of the
+  // diagonal blocks staged in the (now unused) panel/T/K scratch when they
+  // fit, else read back from fLU
+  {
+    const int nbk = (r + kDiagBlock - 1) / kDiagBlock;
+    constexpr int kBB = kDiagBlock * kDiagBlock;
+    const bool staged = (size_t)nbk * kBB <= (size_t)2 * TB * n + (size_t)2 * k * r + (size_t)k * k;
+    S* stg = colb;
+    if (staged) {
+      for (int e2 = tid; e2 < nbk * kBB; e2 += 1024) {
+        const int b = e2 / kBB, t = (e2 % kBB) % kDiagBlock, u = (e2 % kBB) / kDiagBlock;
+        const int gi = b * kDiagBlock + t, gj = b * kDiagBlock + u;
+        stg[e2] = (gi < r && gj < r) ? fLU[gi + (int64_t)gj * r] : (S)0;
+      }
+    }
+    __syncthreads();
+    for (int bb = wid; bb < 2 * nbk; bb += 16) {
+      const bool upper = bb >= nbk;
+      if (upper && !LU) continue;
+      const int b = upper ? bb - nbk : bb;
+      const int j0 = b * kDiagBlock;
+      const int bs = r - j0 < kDiagBlock ? r - j0 : kDiagBlock;
+      const int c = lane;
+      S x[kDiagBlock];
+#pragma unroll
+      for (int t = 0; t < kDiagBlock; ++t) x[t] = (S)0;
+      const S* Lb = staged ? stg + (size_t)b * kBB : fLU + j0 + (int64_t)j0 * r;
+      const int ldb = staged ? kDiagBlock : r;
+      if (!upper) {
+#pragma unroll
+        for (int t = 0; t < kDiagBlock; ++t) {
+          if (t < bs) {
+            S v = (t == c) ? (S)1 : (S)0;
+#pragma unroll
+            for (int u = 0; u < t; ++u) v -= Lb[t + u * ldb] * x[u];
+            x[t] = LU ? v : v / Lb[t + t * ldb];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int tt = kDiagBlock - 1; tt >= 0; --tt) {
+          if (tt < bs) {
+            S v = (tt == c) ? (S)1 : (S)0;
+#pragma unroll
+            for (int u = tt + 1; u < kDiagBlock; ++u)
+              if (u < bs) v -= Lb[tt + u * ldb] * x[u];
+            x[tt] = v / Lb[tt + tt * ldb];
+          }
+        }
+      }
+      if (c < kDiagBlock) {
+        S* dst = fval + (upper ? O.Ui : O.Li) + (int64_t)b * kDiagBlock * kDiagBlock;
+#pragma unroll
+        for (int t = 0; t < kDiagBlock; ++t) dst[t + c * kDiagBlock] = (t < bs && c < bs) ? x[t] : (S)0;
+      }
+    }
+  }
+}
+
+// shared bytes of k_elim_reg for a box
+size_t elim_reg_shmem(int n, int k, int r, size_t es) {
+  const int TB = es == 4 ? 16 : 12;
+  size_t b = (size_t)6 * n * sizeof(double) + (size_t)(3 * n + 2 * TB + 2) * sizeof(int);
+  b += ((size_t)2 * TB * n + (size_t)k * r + (size_t)k * k + (size_t)k * r) * es + 16;
+  return b;
+}
+
+template <typename S>
+void launch_elim_reg(bool lu, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
+                     const BoxOut* outs, const Seg* csegs, int* pool, const int* permar,
+                     const double* Tar, const S* Mprev, S* Mcur, S* fval, int* fidx,
+                     const PointsDev& P, int* fail, hipStream_t st) {
+  constexpr int TA = 4, TB = sizeof(S) == 4 ? 16 : 12;
+#define RSKX_ELIMR(LUV)                                                                         \
+  {                                                                                             \
+    auto kern = k_elim_reg<S, LUV, TA, TB>;                                                     \
+    if (shmem > 65536)                                                                          \
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)shmem);                                                    \
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(1024), shmem, st, ids, boxes, outs, csegs, pool,    \
+                       permar, Tar, Mprev, Mcur, fval, fidx, P, fail);                          \
+  }
+  if (lu) RSKX_ELIMR(true) else RSKX_ELIMR(false)
+#undef RSKX_ELIMR
+}
+
+// largest box the register elimination takes
+int elim_reg_nmax(size_t es) { return es == 4 ? 256 : 192; }
+
 // shared-memory elements of one box's elimination blocks
 size_t elim_elems(bool lu, int k, int r) {
   return (size_t)r * r + (size_t)k * r * (lu ? 3 : 2) + (size_t)k * k;
@@ -828,6 +1227,9 @@ void launch_elim(bool lu, int mode, size_t shmem, const int* ids, int nb, const 
                                    hipStream_t);                                                \
   template void launch_id<S>(const LvlBox*, const int*, S*, int*, double*, const IdParams&,     \
                              int*, int*, int, size_t, hipStream_t);                            \
+  template void launch_elim_reg<S>(bool, size_t, const int*, int, const LvlBox*, const BoxOut*, \
+                                   const Seg*, int*, const int*, const double*, const S*, S*, S*, \
+                                   int*, const PointsDev&, int*, hipStream_t);                   \
   template void launch_elim<S>(bool, int, size_t, const int*, int, const LvlBox*,             \
                                const BoxOut*, const Seg*, int*, const int*, const double*,     \
                                const S*, S*, S*, S*, int*, const PointsDev&, int*, hipStream_t);
