@@ -28,11 +28,21 @@ namespace rskx {
 void launch_lists(const LvlBox*, const Seg*, const Seg*, int*, int*, int*, const PointsDev&,
                   const LevelParams&, int, hipStream_t);
 template <typename S>
-void launch_assemble(const LvlBox*, const int*, const int*, const int*, int*, S*,
+void launch_assemble(const int*, const LvlBox*, const int*, const int*, const int*, int*, S*,
                      const PointsDev&, const LevelParams&, int, size_t, hipStream_t);
 template <typename S>
-void launch_id(const LvlBox*, const int*, S*, int*, double*, const IdParams&, int*, int*, int,
-               size_t, hipStream_t);
+void launch_id(int, const int*, const LvlBox*, const int*, S*, int*, double*, const IdParams&,
+               int*, int*, int, size_t, hipStream_t);
+void launch_gram_id(int, const int*, const LvlBox*, const int*, const int*, const int*, int*,
+                    double*, int*, const PointsDev&, const LevelParams&, const IdParams&, int*, int,
+                    size_t, hipStream_t);
+int gram_id_nmax(int cls);
+size_t gram_id_shmem(int cls, int n, int mn, int m, int p);
+template <typename S>
+void launch_id_reg(int, const int*, const LvlBox*, const int*, const int*, const int*, int*, S*,
+                   int*, const PointsDev&, const LevelParams&, const IdParams&, int*, int, size_t,
+                   hipStream_t);
+void id_reg_limits(size_t es, int cls, int* mlim, int* nlim);
 size_t elim_elems(bool lu, int k, int r);
 size_t elim_reg_shmem(int n, int k, int r, size_t es);
 int elim_reg_nmax(size_t es);
@@ -469,27 +479,110 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     RSKX_TRY(ws.T.ensure(std::max<int64_t>(off_T, 1) * sizeof(double)));
     RSKX_TRY(ws.W.ensure(std::max<int64_t>(off_W, 1) * sizeof(S)));
 
-    size_t shm = (size_t)(2 * nmax + std::max(nmax, mcapmax) + 32) * sizeof(double);
-    if ((size_t)(nmax + mcmax) * 5 * sizeof(double) > 160 * 1024)
-      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box neighbourhood too large for the assembly kernel");
-    if (shm > 160 * 1024)
-      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box too large for the ID kernel (n=" +
-                                       std::to_string(nmax) + ", m=" + std::to_string(mcapmax) + ")");
     hipEvent_t ev;
     prof.begin("lists", ev);
     launch_lists(ws.boxes.as<LvlBox>(), ws.nsegs.as<Seg>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
                  ws.nbr.as<int>(), ws.mnbr.as<int>(), P, L, (int)nb, g_stream);
     prof.end("lists", ev);
+    // ID: boxes whose K fits a register tile assemble and factor it in
+    // registers (k_id_reg); the rest assemble K in global memory first
+    std::vector<int> hmn(nb);
+    RSKX_HIP(hipMemcpyAsync(hmn.data(), ws.mnbr.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    RSKX_HIP(hipStreamSynchronize(g_stream));
+    // FP32 factorizations at tol >= 1e-7 take the Gram route (k_gram_id)
+    const bool gram = sizeof(S) == 4 && prm.tol >= 1e-7;
+    std::vector<int> idg, idr[2], idq[2], idall;
+    size_t shm_r[2] = {0, 0}, shm_q[2] = {0, 0};
+    int nmax_g = 0, mcmax_g = 0, mcapmax_g = 0;
+    for (int64_t ib = 0; ib < nb; ++ib) {
+      const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
+      if (gram && n > 0 && mb > 0 && n <= gram_id_nmax(1)) {
+        const int c = n <= gram_id_nmax(0) ? 0 : 1;
+        idq[c].push_back((int)ib);
+        shm_q[c] = std::max(shm_q[c], gram_id_shmem(c, n, hmn[ib], mb, L.p));
+        continue;
+      }
+      int cls = -1;
+      for (int c = 1; c >= 0 && cls < 0; --c) {
+        int ml, nl;
+        id_reg_limits(sizeof(S), c, &ml, &nl);
+        if (mb <= ml && n <= nl) cls = c;
+      }
+      if (cls >= 0 && n > 0 && mb > 0) {
+        idr[cls].push_back((int)ib);
+        // staging (8 x 1024 S), proxy points, point data, norms, v, red, maps
+        const size_t by = (size_t)8 * 1024 * sizeof(S) +
+                          (size_t)(2 * L.p + 7 * n + 5 * hmn[ib] + mb + 32) * sizeof(double) +
+                          (size_t)2 * n * sizeof(int) + 64;
+        shm_r[cls] = std::max(shm_r[cls], by);
+      } else {
+        idg.push_back((int)ib);
+        nmax_g = std::max(nmax_g, n);
+        mcmax_g = std::max(mcmax_g, hb[ib].mc);
+        mcapmax_g = std::max(mcapmax_g, hb[ib].mcap);
+      }
+    }
+    idall = idg;
+    idall.insert(idall.end(), idr[0].begin(), idr[0].end());
+    idall.insert(idall.end(), idr[1].begin(), idr[1].end());
+    idall.insert(idall.end(), idq[0].begin(), idq[0].end());
+    idall.insert(idall.end(), idq[1].begin(), idq[1].end());
+    RSKX_TRY(upload(ws.ids, idall));
+    const int* dids = ws.ids.as<int>();
+    const int ng = (int)idg.size(), nr0 = (int)idr[0].size(), nr1 = (int)idr[1].size();
+    const int nq0 = (int)idq[0].size(), nq1 = (int)idq[1].size();
+    if (shm_r[0] > 160 * 1024 || shm_r[1] > 160 * 1024 || shm_q[0] > 160 * 1024 ||
+        shm_q[1] > 160 * 1024)
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box neighbourhood too large for the ID kernels");
+    if ((size_t)(nmax_g + mcmax_g) * 5 * sizeof(double) > 160 * 1024)
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box neighbourhood too large for the assembly kernel");
     int64_t Kbytes = 0;
-    for (auto& B : hb) Kbytes += (int64_t)B.mcap * B.n;
+    for (int ib : idg) Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
     prof.begin("assemble", ev);
-    launch_assemble<S>(ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(), ws.mnbr.as<int>(),
-                       ws.mrows.as<int>(), ws.K.as<S>(), P, L, (int)nb,
-                       (size_t)(nmax + mcmax) * 5 * sizeof(double), g_stream);
+    if (ng)
+      launch_assemble<S>(dids, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
+                         ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng,
+                         (size_t)(nmax_g + mcmax_g) * 5 * sizeof(double), g_stream);
     prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
     prof.begin("id", ev);
-    launch_id<S>(ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(), ws.perm.as<int>(),
-                 ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(), (int)nb, shm, g_stream);
+    if (ng) {
+      size_t shg = (size_t)(2 * nmax_g + std::max(nmax_g, mcapmax_g) + 32) * sizeof(double);
+      if (shg > 160 * 1024)
+        return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box too large for the ID kernel (n=" +
+                                         std::to_string(nmax_g) + ", m=" + std::to_string(mcapmax_g) + ")");
+      launch_id<S>(0, dids, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
+                   ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
+                   ng, shg, g_stream);
+    }
+    for (int c = 0; c < 2; ++c) {
+      const int cnt = c == 0 ? nr0 : nr1;
+      if (!cnt) continue;
+      launch_id_reg<S>(c, dids + ng + (c == 0 ? 0 : nr0), ws.boxes.as<LvlBox>(), ws.pool.as<int>(),
+                       ws.nbr.as<int>(), ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(),
+                       ws.perm.as<int>(), P, L, prm, ws.kout.as<int>(), cnt, shm_r[c], g_stream);
+    }
+    if (nr0 + nr1) {
+      int nmx = 0;
+      for (int c = 0; c < 2; ++c) for (int ib : idr[c]) nmx = std::max(nmx, hb[ib].n);
+      launch_id<S>(1, dids + ng, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
+                   ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
+                   nr0 + nr1, (size_t)(3 * nmx + 32) * sizeof(double), g_stream);
+    }
+    const int* qids = dids + ng + nr0 + nr1;
+    for (int c = 0; c < 2; ++c) {
+      const int cnt = c == 0 ? nq0 : nq1;
+      if (!cnt) continue;
+      launch_gram_id(c, qids + (c == 0 ? 0 : nq0), ws.boxes.as<LvlBox>(), ws.pool.as<int>(),
+                     ws.nbr.as<int>(), ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.T.as<double>(),
+                     ws.perm.as<int>(), P, L, prm, ws.kout.as<int>(), cnt, shm_q[c], g_stream);
+    }
+    if (nq0 + nq1) {
+      int nmx = 0;
+      for (int c = 0; c < 2; ++c) for (int ib : idq[c]) nmx = std::max(nmx, hb[ib].n);
+      launch_id<S>(2, qids, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
+                   ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
+                   nq0 + nq1, (size_t)(3 * nmx + 32) * sizeof(double), g_stream);
+    }
     prof.end("id", ev);
     RSKX_HIP(hipGetLastError());
     hk.resize(nb);
@@ -516,7 +609,10 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
       O.k = hk[ib];
       O.r = B.n - O.k;
       const int64_t k = O.k, r = O.r, m = hm[ib], n = B.n;
-      id_flops += 4.0 * m * n * k - 2.0 * k * k * (m + n) + 4.0 / 3.0 * k * k * k + (double)k * k * r;
+      if (gram && n > 0 && m > 0 && n <= gram_id_nmax(1))   // G = K'K, pivoted Cholesky, T
+        id_flops += (double)m * n * (n + 1) + 2.0 * (double)n * n * k + (double)k * k * r;
+      else                                                  // truncated QRCP, T
+        id_flops += 4.0 * m * n * k - 2.0 * k * k * (m + n) + 4.0 / 3.0 * k * k * k + (double)k * k * r;
       if (r > 0) {
         O.fac = (int)(nfac++);
         O.T = F->fval_used + need_v; need_v += k * r;

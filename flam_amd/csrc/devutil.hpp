@@ -9,10 +9,39 @@ namespace rskx {
 
 constexpr int kWave = 64;
 
+// DPP lane moves of a 64-bit value (two 32-bit halves).  Lanes whose source
+// is out of range read 0 (row_bcast into row 0).
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const uint64_t u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+template <int CTRL>
+__device__ inline int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+__device__ inline double read_lane(double v, int l) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Wave64 sum with every lane getting the total: butterflies inside each
+// quad (quad_perm 1032, 2301), rotations inside each 16-lane row (row_ror
+// 4, 8), then row_bcast 15 / 31 accumulate the four row sums into lane 63,
+// which is read back as a uniform value.  DPP moves stay in the VALU;
+// shuffles would go through the LDS crossbar twice per double.
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+  v += dpp_move<0xb1>(v);
+  v += dpp_move<0x4e>(v);
+  v += dpp_move<0x124>(v);
+  v += dpp_move<0x128>(v);
+  v += dpp_move<0x142>(v);
+  v += dpp_move<0x143>(v);
+  return read_lane(v, 63);
 }
 
 // (value, index) maximum; ties go to the smaller index (LAPACK's i*amax
@@ -21,13 +50,14 @@ __device__ inline void argmax_merge(double& v, int& i, double v2, int i2) {
   if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
 }
 
+// Same DPP pattern as wave_sum.  Lanes reading out of range get (0, 0),
+// which only happens in row 0 of the broadcasts, whose result is unused.
 __device__ inline void wave_argmax(double& v, int& i) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    double v2 = __shfl_xor(v, o, kWave);
-    int i2 = __shfl_xor(i, o, kWave);
-    argmax_merge(v, i, v2, i2);
-  }
+#define RSKX_AMX(C) { const double v2 = dpp_move<C>(v); const int i2 = dpp_move<C>(i); argmax_merge(v, i, v2, i2); }
+  RSKX_AMX(0xb1) RSKX_AMX(0x4e) RSKX_AMX(0x124) RSKX_AMX(0x128) RSKX_AMX(0x142) RSKX_AMX(0x143)
+#undef RSKX_AMX
+  v = read_lane(v, 63);
+  i = __builtin_amdgcn_readlane(i, 63);
 }
 
 // Workgroup reductions; every thread gets the result.  `red` must hold

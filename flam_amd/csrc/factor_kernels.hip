@@ -98,18 +98,20 @@ __global__ void __launch_bounds__(256) k_lists(const LvlBox* __restrict__ boxes,
 
 // ----------------------------------------------------------- k_assemble --
 template <typename S>
-__global__ void __launch_bounds__(256) k_assemble(const LvlBox* __restrict__ boxes,
+__global__ void __launch_bounds__(256) k_assemble(const int* __restrict__ ids,
+                                                  const LvlBox* __restrict__ boxes,
                                                   const int* __restrict__ pool,
                                                   const int* __restrict__ nbrar,
                                                   const int* __restrict__ mnbr,
                                                   int* __restrict__ mrows, S* __restrict__ Kar,
                                                   PointsDev P, LevelParams L) {
   extern __shared__ double sp[];
-  const LvlBox B = boxes[blockIdx.x];
-  const int n = B.n, mn = mnbr[blockIdx.x];
+  const int bidx = ids[blockIdx.x];
+  const LvlBox B = boxes[bidx];
+  const int n = B.n, mn = mnbr[bidx];
   const int mt = L.two_sided ? 2 * mn : mn;
   const int m = mt + L.p;
-  if (threadIdx.x == 0) mrows[blockIdx.x] = m;
+  if (threadIdx.x == 0) mrows[bidx] = m;
   const int* slf = pool + B.slf_off;
   const int* nbr = nbrar + B.nbr_off;
   // stage column (slf) and row (nbr) point data: x, y, nu_x, nu_y, w
@@ -225,15 +227,21 @@ __device__ void swap_skel(int* perm, double* T, int ldt, int r, double* R, int l
   __syncthreads();
 }
 
-// Shared memory: vn1[n], vn2[n], v[max(m, n)], red[32]
-template <typename S>
-__global__ void __launch_bounds__(256) k_id(const LvlBox* __restrict__ boxes,
+// Shared memory: vn1[n], vn2[n], v[max(m, n)], red[32].
+// MODE 0: run the column-pivoted QR on K first.  Otherwise the leading k
+// rows of R are already there by pivot position, perm holds the pivot order
+// and kout the rank, and only T and the refinement remain; R is in K
+// (MODE 1, from k_id_reg) or, in FP64 with leading dimension n, in the
+// third n x n slice of the box's T scratch (MODE 2, from k_gram_id).
+template <typename S, int MODE>
+__global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
+                                            const LvlBox* __restrict__ boxes,
                                             const int* __restrict__ mrows, S* __restrict__ Kar,
                                             int* __restrict__ permar, double* __restrict__ Tar,
                                             IdParams prm, int* __restrict__ kout,
                                             int* __restrict__ iterout) {
   extern __shared__ double sh[];
-  const int b = blockIdx.x;
+  const int b = ids[blockIdx.x];
   const LvlBox B = boxes[b];
   const int n = B.n, m = mrows[b], ld = B.mcap;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -246,13 +254,18 @@ __global__ void __launch_bounds__(256) k_id(const LvlBox* __restrict__ boxes,
   double* red = v + (m > n ? m : n);
   __shared__ int s_piv;
 
-  for (int j = tid; j < n; j += nt) perm[j] = j;
+  constexpr bool QR = MODE == 0;
+  if (QR)
+    for (int j = tid; j < n; j += nt) perm[j] = j;
   if (m == 0 || n == 0) {           // id.m:64-68: nothing to compress against
     if (tid == 0) { kout[b] = 0; iterout[b] = 0; }
     return;
   }
   int kmax = prm.kmax;
   if (kmax <= 0 || kmax > n) kmax = n;
+  int k = 0;
+  double tol_abs = 0;
+  if (QR) {
 
   for (int j = wid; j < n; j += nw) {
     double s = 0;
@@ -264,8 +277,6 @@ __global__ void __launch_bounds__(256) k_id(const LvlBox* __restrict__ boxes,
 
   const double tol3z = sqrt(Eps<S>::v);
   const int smax = m < n ? m : n;
-  int k = 0;
-  double tol_abs = 0;
   for (int s = 0; s < smax; ++s) {
     double bv = -1.0;
     int bi = n;
@@ -336,17 +347,26 @@ __global__ void __launch_bounds__(256) k_id(const LvlBox* __restrict__ boxes,
     __syncthreads();
     if (last) break;
   }
+  }
 
   // T = R11 \ R12 in double, leading dimension n (id.m:114)
   double* T = Tar + B.T_off;
   double* R = T + (int64_t)n * n;
   double* X = R + (int64_t)n * n;
+  // leading rows of R by position
+  auto rv = [&](int i, int j) -> double {
+    return MODE == 2 ? X[i + (int64_t)j * n] : (double)K[i + (int64_t)j * ld];
+  };
+  if (!QR) {
+    k = kout[b];
+    tol_abs = k > 0 ? fabs(rv(0, 0)) * prm.tol : 0.0;
+  }
   const int r0 = n - k;
   for (int c = tid; c < r0; c += nt) {
     for (int l = k - 1; l >= 0; --l) {
-      double s = (double)K[l + (int64_t)(k + c) * ld];
-      for (int q = l + 1; q < k; ++q) s -= (double)K[l + (int64_t)q * ld] * T[q + (int64_t)c * n];
-      T[l + (int64_t)c * n] = s / (double)K[l + (int64_t)l * ld];
+      double s = rv(l, k + c);
+      for (int q = l + 1; q < k; ++q) s -= rv(l, q) * T[q + (int64_t)c * n];
+      T[l + (int64_t)c * n] = s / rv(l, l);
     }
   }
   int iter = 0;
@@ -354,9 +374,10 @@ __global__ void __launch_bounds__(256) k_id(const LvlBox* __restrict__ boxes,
   if (isfinite(prm.Tmax) && prm.rrqr_iter > 0 && k > 0 && k < n) {
     // Gu-Eisenstat refinement in the fixed-precision form the reference
     // takes when R has no trailing block (id.m:116-236, c2 = 0).
+    __syncthreads();
     for (int e = tid; e < k * k; e += nt) {
       int i = e % k, j = e / k;
-      R[i + (int64_t)j * n] = i <= j ? (double)K[i + (int64_t)j * ld] : 0.0;
+      R[i + (int64_t)j * n] = i <= j ? rv(i, j) : 0.0;
     }
     __syncthreads();
     const double f2 = prm.Tmax * prm.Tmax;
@@ -768,6 +789,514 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
   for (int i = tid; i < r; i += nt) frd[i] = slf[q[k + i]];
 }
 
+// ------------------------------------------------------------- k_id_reg --
+// Column-pivoted Householder QR of one box's K with K assembled straight
+// into registers: thread (wave w, lane l) holds K(l + 64 a, w + 16 c),
+// a < RA, c < CB, so m <= 64 RA and n <= 16 CB.  Columns never move: a
+// position -> column map plays the role of the column swaps, and the pivot
+// at each step is the first position with the largest remaining norm, as
+// in k_id / LAPACK.  Arithmetic is FP64 on values stored as S.  On exit K's
+// leading k rows hold R by pivot position, perm the pivot order and kout
+// the rank; k_id<S, 1> then forms T and refines the ID.
+template <typename S, int RA, int CB>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
+         const int* __restrict__ pool, const int* __restrict__ nbrar,
+         const int* __restrict__ mnbr, int* __restrict__ mrows, S* __restrict__ Kar,
+         int* __restrict__ permar, PointsDev P, LevelParams L, IdParams prm,
+         int* __restrict__ kout) {
+  extern __shared__ double sh[];
+  const int b = ids[blockIdx.x];
+  const LvlBox B = boxes[b];
+  const int n = B.n, mn = mnbr[b];
+  const int mt = L.two_sided ? 2 * mn : mn;
+  const int m = mt + L.p;
+  const int ld = B.mcap;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  S* K = Kar + B.K_off;
+  int* perm = permar + B.perm_off;
+  const int* slf = pool + B.slf_off;
+  const int* nbr = nbrar + B.nbr_off;
+  S* stg = reinterpret_cast<S*>(sh);   // per-thread assembly staging, RA x 1024
+  double* ppx = sh + (RA * 1024 * sizeof(S) + 7) / 8;   // proxy points
+  double* ppy = ppx + L.p;
+  double* sx = ppy + L.p;          // column points
+  double* sy = sx + n;
+  double* snx = sy + n;
+  double* sny = snx + n;
+  double* sw = sny + n;
+  double* rx = sw + n;             // neighbour (row) points
+  double* ry = rx + mn;
+  double* rnx = ry + mn;
+  double* rny = rnx + mn;
+  double* rw = rny + mn;
+  double* vn1 = rw + mn;
+  double* vn2 = vn1 + n;
+  double* v = vn2 + n;             // Householder vector, m
+  double* red = v + m;             // 32
+  int* p2c = reinterpret_cast<int*>(red + 32);   // position -> column
+  __shared__ double s_tau, s_tol;
+  __shared__ int s_stop;
+  for (int j = tid; j < n; j += 1024) {
+    const int pt = slf[j];
+    sx[j] = P.x[0][pt]; sy[j] = P.x[1][pt]; snx[j] = P.nu[0][pt]; sny[j] = P.nu[1][pt];
+    sw[j] = P.w[pt];
+    p2c[j] = j;
+  }
+  for (int i = tid; i < mn; i += 1024) {
+    const int pt = nbr[i];
+    rx[i] = P.x[0][pt]; ry[i] = P.x[1][pt]; rnx[i] = P.nu[0][pt]; rny[i] = P.nu[1][pt];
+    rw[i] = P.w[pt];
+  }
+  for (int q = tid; q < L.p; q += 1024) {    // as proxy_entry
+    const double th = (double)(q + 1) * kTwoPi / (double)L.p;
+    ppx[q] = L.radius * cos(th) * L.l[0] + B.ctr[0];
+    ppy[q] = L.radius * sin(th) * L.l[1] + B.ctr[1];
+  }
+  if (tid == 0) mrows[b] = m;
+  __syncthreads();
+  if (m == 0 || n == 0) {
+    for (int j = tid; j < n; j += 1024) perm[j] = j;
+    if (tid == 0) kout[b] = 0;
+    return;
+  }
+  // K = [A(nbr,slf); A(slf,nbr)'; Kpxy] (rskelf.m:201-212), as k_assemble.
+  // Each column block is evaluated by a rolled loop into this thread's own
+  // LDS slots and then moved to registers, which keeps the entry
+  // evaluation from inflating register pressure.
+  const double c0 = P.scale / kTwoPi;
+  S Kr[RA][CB];
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb) {
+    const int j = wid + 16 * cb;
+    const bool cj = j < n;
+    const double xj = cj ? sx[j] : 0, yj = cj ? sy[j] : 0, nxj = cj ? snx[j] : 0,
+                 nyj = cj ? sny[j] : 0, wj = cj ? sw[j] : 0;
+#pragma unroll 1
+    for (int ra = 0; ra < RA; ++ra) {
+      const int i = lane + 64 * ra;
+      double val = 0;
+      if (cj && i < m) {
+        if (i < mn) {
+          const double dx = rx[i] - xj, dy = ry[i] - yj;
+          val = c0 * wj * (dx * nxj + dy * nyj) / (dx * dx + dy * dy);
+        } else if (i < mt) {
+          const int q = i - mn;
+          const double dx = xj - rx[q], dy = yj - ry[q];
+          val = c0 * rw[q] * (dx * rnx[q] + dy * rny[q]) / (dx * dx + dy * dy);
+        } else {
+          const int q = i - mt;
+          const double dx = ppx[q] - xj, dy = ppy[q] - yj;
+          const double r2 = dx * dx + dy * dy;
+          val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * L.pweight
+                                              : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
+        }
+      }
+      stg[ra * 1024 + tid] = (S)val;
+    }
+#pragma unroll
+    for (int ra = 0; ra < RA; ++ra) Kr[ra][cb] = stg[ra * 1024 + tid];
+  }
+  // column norms
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb) {
+    double sq = 0;
+#pragma unroll
+    for (int ra = 0; ra < RA; ++ra) { const double a = (double)Kr[ra][cb]; sq += a * a; }
+    sq = wave_sum(sq);
+    const int j = wid + 16 * cb;
+    if (lane == 0 && j < n) { vn1[j] = sqrt(sq); vn2[j] = vn1[j]; }
+  }
+  __syncthreads();
+  int kmax = prm.kmax;
+  if (kmax <= 0 || kmax > n) kmax = n;
+  const double tol3z = sqrt(Eps<S>::v);
+  const int smax = m < n ? m : n;
+  unsigned donemask = 0;            // columns of this wave already pivoted (bit cb)
+  int k = 0;
+  for (int s = 0; s < smax; ++s) {
+    // pivot: largest remaining norm over positions s..n-1 (id.m:105)
+    double bv = -1.0;
+    int bi = n;
+    for (int pos = s + tid; pos < n; pos += 1024) argmax_merge(bv, bi, vn1[p2c[pos]], pos);
+    block_argmax(bv, bi, red);
+    if (tid == 0 && bi != s) { const int t = p2c[s]; p2c[s] = p2c[bi]; p2c[bi] = t; }
+    __syncthreads();
+    const int pc = p2c[s], pw = pc & 15, pcb = pc >> 4;
+    const int sa = s >> 6, sl = s & 63;
+    if (wid == pw) {
+      double colv[RA];
+#pragma unroll
+      for (int ra = 0; ra < RA; ++ra) {
+        double val = 0;
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) if (cb == pcb) val = (double)Kr[ra][cb];
+        colv[ra] = val;
+      }
+      double part = 0, alpha_l = 0;
+#pragma unroll
+      for (int ra = 0; ra < RA; ++ra) {
+        const int i = lane + 64 * ra;
+        if (i > s && i < m) part += colv[ra] * colv[ra];
+        if (ra == sa) alpha_l = colv[ra];
+      }
+      const double sigma = wave_sum(part);
+      const double alpha = __shfl(alpha_l, sl, 64);
+      const double nrm = sqrt(alpha * alpha + sigma);
+      if (s == 0 && lane == 0) s_tol = nrm * prm.tol;   // id.m:106-107
+      const double tol_abs = s == 0 ? nrm * prm.tol : s_tol;
+      const bool stop = !(nrm > tol_abs);                // id.m:108-111
+      double beta, tau, scal;
+      if (sigma == 0) { beta = alpha; tau = 0; scal = 0; }
+      else { beta = alpha >= 0 ? -nrm : nrm; tau = (beta - alpha) / beta; scal = 1.0 / (alpha - beta); }
+      if (!stop) {
+#pragma unroll
+        for (int ra = 0; ra < RA; ++ra) {
+          const int i = lane + 64 * ra;
+          if (i >= s && i < m) v[i] = i == s ? 1.0 : colv[ra] * scal;
+        }
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+          if (cb == pcb)
+#pragma unroll
+            for (int ra = 0; ra < RA; ++ra) if (ra == sa && lane == sl) Kr[ra][cb] = (S)beta;
+      }
+      if (lane == 0) { s_tau = tau; s_stop = stop ? 1 : 0; }
+    }
+    if (wid == pw) donemask |= 1u << pcb;
+    __syncthreads();
+    if (s_stop) break;
+    k = s + 1;
+    const bool last = (k == kmax) || (s + 1 == smax);
+    const double tau = s_tau;
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      const int j = wid + 16 * cb;
+      if (j >= n || ((donemask >> cb) & 1u)) continue;
+      double dot = 0;
+#pragma unroll
+      for (int ra = 0; ra < RA; ++ra) {
+        const int i = lane + 64 * ra;
+        if (i >= s && i < m) dot += v[i] * (double)Kr[ra][cb];
+      }
+      dot = wave_sum(dot) * tau;
+      double rsj_l = 0, tail = 0;
+#pragma unroll
+      for (int ra = 0; ra < RA; ++ra) {
+        const int i = lane + 64 * ra;
+        if (i >= s && i < m && (!last || i == s)) {
+          const S nv = (S)((double)Kr[ra][cb] - dot * v[i]);
+          Kr[ra][cb] = nv;
+          if (i == s) rsj_l = (double)nv;
+          else tail += (double)nv * (double)nv;
+        }
+      }
+      if (last) continue;
+      const double rsj = __shfl(rsj_l, sl, 64);
+      tail = wave_sum(tail);
+      if (lane == 0) {
+        const double nv1 = vn1[j];
+        if (nv1 != 0) {
+          double t = fabs(rsj) / nv1;
+          t = fmax(0.0, (1.0 + t) * (1.0 - t));
+          const double q = nv1 / vn2[j];
+          if (t * q * q <= tol3z) { vn1[j] = sqrt(tail); vn2[j] = vn1[j]; }
+          else vn1[j] = nv1 * sqrt(t);
+        }
+      }
+    }
+    __syncthreads();
+    if (last) break;
+  }
+  // R's leading k rows by pivot position into K (leading dimension ld)
+  int* c2p = p2c + n;                // column -> position
+  for (int pos = tid; pos < n; pos += 1024) { c2p[p2c[pos]] = pos; perm[pos] = p2c[pos]; }
+  __syncthreads();
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb) {
+    const int j = wid + 16 * cb;
+    if (j >= n) continue;
+    const int pos = c2p[j];
+#pragma unroll
+    for (int ra = 0; ra < RA; ++ra) {
+      const int i = lane + 64 * ra;
+      if (i < k) K[i + (int64_t)pos * ld] = Kr[ra][cb];
+    }
+  }
+  if (tid == 0) kout[b] = k;
+}
+
+// ------------------------------------------------------------ k_gram_id --
+// Column ID through the Gram matrix, for FP32 factorizations at tolerances
+// of 1e-7 and above.  In exact arithmetic column-pivoted QR of K and the
+// diagonally pivoted Cholesky factorization of G = K'K are the same
+// process: the column of largest remaining norm is the one of largest
+// remaining diagonal of G, and R'R = G(p, p) with R QRCP's triangular factor
+// up to row signs (which T = R11 \ R12 does not see).  K's entries are
+// evaluated in FP64 and never stored; G is accumulated in FP64, each thread
+// holding TPT 4 x 4 tiles of its lower triangle.  The remaining diagonal at
+// the stopping threshold, tol^2 G_11 >= 1e-14 G_11, is then resolved to
+// better than 1e-2 relative -- closer to the FP64 QRCP than a QRCP on
+// FP32-rounded K, whose rounding alone moves |R_jj| by ~1e-7 |R_11|.
+// On exit the third n x n slice of the box's T scratch holds R's leading k
+// rows by pivot position (leading dimension n), perm the pivot order and
+// kout the rank; k_id<S, 2> then forms T and refines the ID.
+template <int NT, int TPT>
+__global__ void __launch_bounds__(NT)
+k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
+          const int* __restrict__ pool, const int* __restrict__ nbrar,
+          const int* __restrict__ mnbr, int* __restrict__ mrows, double* __restrict__ Tar,
+          int* __restrict__ permar, PointsDev P, LevelParams L, IdParams prm,
+          int* __restrict__ kout) {
+  constexpr int RC = 16;             // rows of K per chunk
+  constexpr int NW = NT / 64;
+  extern __shared__ double sh[];
+  const int b = ids[blockIdx.x];
+  const LvlBox B = boxes[b];
+  const int n = B.n, mn = mnbr[b];
+  const int mt = L.two_sided ? 2 * mn : mn;
+  const int m = mt + L.p;
+  const int tid = threadIdx.x;
+  const int n4 = (n + 3) >> 2, np = 4 * n4;
+  int* perm = permar + B.perm_off;
+  const int* slf = pool + B.slf_off;
+  const int* nbr = nbrar + B.nbr_off;
+  double* chunk = sh;                // RC x np, row-major (32-byte rows)
+  double* dg = chunk + RC * np;      // remaining diagonal of G (np)
+  double* prow = dg + np;            // pivot row of the updated G (np)
+  double* red = prow + np;           // 2 NW
+  double* ppx = red + 2 * NW;        // proxy points
+  double* ppy = ppx + L.p;
+  double* sx = ppy + L.p;            // column points
+  double* sy = sx + n;
+  double* snx = sy + n;
+  double* sny = snx + n;
+  double* sw = sny + n;
+  double* rx = sw + n;               // neighbour (row) points
+  double* ry = rx + mn;
+  double* rnx = ry + mn;
+  double* rny = rnx + mn;
+  double* rw = rny + mn;
+  int* p2c = reinterpret_cast<int*>(rw + mn);        // position -> column (n)
+  double* Rc = Tar + B.T_off + (int64_t)n * n;       // R rows by column
+  double* Rp = Rc + (int64_t)n * n;                  // R rows by position
+  for (int j = tid; j < n; j += NT) {
+    const int pt = slf[j];
+    sx[j] = P.x[0][pt]; sy[j] = P.x[1][pt]; snx[j] = P.nu[0][pt]; sny[j] = P.nu[1][pt];
+    sw[j] = P.w[pt];
+    p2c[j] = j;
+  }
+  for (int i = tid; i < mn; i += NT) {
+    const int pt = nbr[i];
+    rx[i] = P.x[0][pt]; ry[i] = P.x[1][pt]; rnx[i] = P.nu[0][pt]; rny[i] = P.nu[1][pt];
+    rw[i] = P.w[pt];
+  }
+  for (int q = tid; q < L.p; q += NT) {    // as proxy_entry
+    const double th = (double)(q + 1) * kTwoPi / (double)L.p;
+    ppx[q] = L.radius * cos(th) * L.l[0] + B.ctr[0];
+    ppy[q] = L.radius * sin(th) * L.l[1] + B.ctr[1];
+  }
+  for (int j = tid; j < np; j += NT) prow[j] = 0;
+  if (tid == 0) mrows[b] = m;
+  __syncthreads();
+  if (m == 0 || n == 0) {
+    for (int j = tid; j < n; j += NT) perm[j] = j;
+    if (tid == 0) kout[b] = 0;
+    return;
+  }
+  // this thread's tiles of the lower triangle, row-major tile order
+  const int ntile = n4 * (n4 + 1) / 2;
+  int ti[TPT], tj[TPT];
+#pragma unroll
+  for (int q = 0; q < TPT; ++q) {
+    const int t = tid + q * NT;
+    ti[q] = -1; tj[q] = 0;
+    if (t < ntile) {
+      int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+      while (i * (i + 1) / 2 > t) --i;
+      while ((i + 1) * (i + 2) / 2 <= t) ++i;
+      ti[q] = i; tj[q] = t - i * (i + 1) / 2;
+    }
+  }
+  double acc[TPT][4][4];
+#pragma unroll
+  for (int q = 0; q < TPT; ++q)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[q][v][u] = 0;
+  // G = K'K over chunks of RC rows of K = [A(nbr,slf); A(slf,nbr)'; Kpxy]
+  // (rskelf.m:201-212), entries as in k_assemble
+  const double c0 = P.scale / kTwoPi;
+  for (int r0 = 0; r0 < m; r0 += RC) {
+    __syncthreads();
+    for (int e = tid; e < RC * np; e += NT) {
+      const int rr = e / np, j = e - rr * np, i = r0 + rr;
+      double val = 0;
+      if (i < m && j < n) {
+        const double xj = sx[j], yj = sy[j];
+        if (i < mn) {
+          const double dx = rx[i] - xj, dy = ry[i] - yj;
+          val = c0 * sw[j] * (dx * snx[j] + dy * sny[j]) / (dx * dx + dy * dy);
+        } else if (i < mt) {
+          const int q = i - mn;
+          const double dx = xj - rx[q], dy = yj - ry[q];
+          val = c0 * rw[q] * (dx * rnx[q] + dy * rny[q]) / (dx * dx + dy * dy);
+        } else {
+          const int q = i - mt;
+          const double dx = ppx[q] - xj, dy = ppy[q] - yj;
+          const double r2 = dx * dx + dy * dy;
+          val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * L.pweight
+                                              : sw[j] * (dx * snx[j] + dy * sny[j]) / (kTwoPi * r2);
+        }
+      }
+      chunk[rr * np + j] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+      if (ti[q] < 0) continue;
+      const double* ca = chunk + 4 * ti[q];
+      const double* cb = chunk + 4 * tj[q];
+#pragma unroll 4
+      for (int rr = 0; rr < RC; ++rr) {
+        const double4 a = *reinterpret_cast<const double4*>(ca + rr * np);
+        const double4 c = *reinterpret_cast<const double4*>(cb + rr * np);
+        const double av[4] = {a.x, a.y, a.z, a.w}, cv[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[q][v][u] += av[v] * cv[u];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < TPT; ++q)
+    if (ti[q] >= 0 && ti[q] == tj[q])
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dg[4 * ti[q] + v] = acc[q][v][v];
+  __syncthreads();
+  // diagonally pivoted Cholesky, right-looking on the register tiles,
+  // stopped as id.m:105-111 stops the QR
+  int kmax = prm.kmax;
+  if (kmax <= 0 || kmax > n) kmax = n;
+  const int smax = m < n ? m : n;
+  double tol_abs = 0;
+  int k = 0;
+  for (int s = 0; s < smax; ++s) {
+    double bv = -1.0;
+    int bi = n;
+    for (int pos = s + tid; pos < n; pos += NT) argmax_merge(bv, bi, dg[p2c[pos]], pos);
+    block_argmax(bv, bi, red);
+    if (bi >= n) break;                // no finite candidate
+    const int pc = p2c[bi];
+    const double nrm = sqrt(fmax(bv, 0.0));
+    if (s == 0) tol_abs = nrm * prm.tol;
+    if (!(nrm > tol_abs)) break;
+    k = s + 1;
+    const int pt = pc >> 2, pl = pc & 3;
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+      if (ti[q] < 0) continue;
+      if (ti[q] == pt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (v == pl)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) prow[4 * tj[q] + u] = acc[q][v][u];
+      if (tj[q] == pt)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (u == pl)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) prow[4 * ti[q] + v] = acc[q][v][u];
+    }
+    __syncthreads();
+    if (tid == 0 && bi != s) { p2c[bi] = p2c[s]; p2c[s] = pc; }
+    const double rinv = 1.0 / nrm;
+    for (int j = tid; j < n; j += NT) {
+      const double r = prow[j] * rinv;
+      Rc[s + (int64_t)j * n] = r;
+      dg[j] -= r * r;
+    }
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+      if (ti[q] < 0) continue;
+      const double4 a = *reinterpret_cast<const double4*>(prow + 4 * ti[q]);
+      const double4 c = *reinterpret_cast<const double4*>(prow + 4 * tj[q]);
+      const double r2 = rinv * rinv;
+      const double av[4] = {a.x * r2, a.y * r2, a.z * r2, a.w * r2}, cv[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[q][v][u] -= av[v] * cv[u];
+    }
+    __syncthreads();
+    if (k == kmax) break;
+  }
+  // R's leading k rows by pivot position
+  for (int e = tid; e < k * n; e += NT) {
+    const int i = e % k, pos = e / k;
+    Rp[i + (int64_t)pos * n] = Rc[i + (int64_t)p2c[pos] * n];
+  }
+  for (int pos = tid; pos < n; pos += NT) perm[pos] = p2c[pos];
+  if (tid == 0) kout[b] = k;
+}
+
+void launch_gram_id(int cls, const int* ids, const LvlBox* boxes, const int* pool,
+                    const int* nbrar, const int* mnbr, int* mrows, double* Tar, int* permar,
+                    const PointsDev& P, const LevelParams& L, const IdParams& prm, int* kout,
+                    int nb, size_t shmem, hipStream_t st) {
+#define RSKX_GID(NTV, TPTV)                                                                      \
+  {                                                                                              \
+    auto kern = k_gram_id<NTV, TPTV>;                                                            \
+    if (shmem > 65536)                                                                           \
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                                (int)shmem);                                                     \
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(NTV), shmem, st, ids, boxes, pool, nbrar, mnbr,      \
+                       mrows, Tar, permar, P, L, prm, kout);                                     \
+  }
+  if (cls == 0) RSKX_GID(256, 2) else RSKX_GID(512, 5)
+#undef RSKX_GID
+}
+
+// k_gram_id classes: {threads, 4 x 4 tiles per thread}; the largest n each
+// holds (n4 (n4 + 1) / 2 <= threads x tiles, n4 = ceil(n / 4))
+int gram_id_nmax(int cls) { return cls == 0 ? 124 : 284; }
+size_t gram_id_shmem(int cls, int n, int mn, int m, int p) {
+  const int np = (n + 3) / 4 * 4, nw = (cls == 0 ? 256 : 512) / 64;
+  return (size_t)(16 * np + 2 * p + 5 * n + 5 * mn + 2 * np + 2 * nw) * sizeof(double) +
+         (size_t)n * sizeof(int) + 64;
+}
+
+template <typename S>
+void launch_id_reg(int cls, const int* ids, const LvlBox* boxes, const int* pool, const int* nbrar,
+                   const int* mnbr, int* mrows, S* Kar, int* permar, const PointsDev& P,
+                   const LevelParams& L, const IdParams& prm, int* kout, int nb, size_t shmem,
+                   hipStream_t st) {
+#define RSKX_IDR(RAV, CBV)                                                                       \
+  {                                                                                              \
+    auto kern = k_id_reg<S, RAV, CBV>;                                                           \
+    if (shmem > 65536)                                                                           \
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                                (int)shmem);                                                     \
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(1024), shmem, st, ids, boxes, pool, nbrar, mnbr,     \
+                       mrows, Kar, permar, P, L, prm, kout);                                     \
+  }
+  if constexpr (sizeof(S) == 4) {
+    if (cls == 0) RSKX_IDR(8, 8) else RSKX_IDR(4, 16)
+  } else {
+    if (cls == 0) RSKX_IDR(8, 4) else RSKX_IDR(4, 8)
+  }
+#undef RSKX_IDR
+}
+
+// register classes of k_id_reg: {rows, cols} limits
+void id_reg_limits(size_t es, int cls, int* mlim, int* nlim) {
+  if (es == 4) { *mlim = cls == 0 ? 512 : 256; *nlim = cls == 0 ? 128 : 256; }
+  else { *mlim = cls == 0 ? 512 : 256; *nlim = cls == 0 ? 64 : 128; }
+}
+
 // ------------------------------------------------------------ launchers --
 void launch_lists(const LvlBox* boxes, const Seg* nsegs, const Seg* csegs, int* pool, int* nbrar,
                   int* mnbr, const PointsDev& P, const LevelParams& L, int nb, hipStream_t st) {
@@ -776,24 +1305,25 @@ void launch_lists(const LvlBox* boxes, const Seg* nsegs, const Seg* csegs, int* 
 }
 
 template <typename S>
-void launch_assemble(const LvlBox* boxes, const int* pool, const int* nbrar, const int* mnbr,
-                     int* mrows, S* Kar, const PointsDev& P, const LevelParams& L, int nb,
-                     size_t shmem, hipStream_t st) {
+void launch_assemble(const int* ids, const LvlBox* boxes, const int* pool, const int* nbrar,
+                     const int* mnbr, int* mrows, S* Kar, const PointsDev& P, const LevelParams& L,
+                     int nb, size_t shmem, hipStream_t st) {
   if (shmem > 65536)
     (void)hipFuncSetAttribute((const void*)k_assemble<S>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  hipLaunchKernelGGL(k_assemble<S>, dim3(nb), dim3(256), shmem, st, boxes, pool, nbrar, mnbr,
-                     mrows, Kar, P, L);
+  hipLaunchKernelGGL(k_assemble<S>, dim3(nb), dim3(256), shmem, st, ids, boxes, pool, nbrar,
+                     mnbr, mrows, Kar, P, L);
 }
 
 template <typename S>
-void launch_id(const LvlBox* boxes, const int* mrows, S* Kar, int* permar, double* Tar,
-               const IdParams& prm, int* kout, int* iterout, int nb, size_t shmem,
-               hipStream_t st) {
+void launch_id(int mode, const int* ids, const LvlBox* boxes, const int* mrows, S* Kar,
+               int* permar, double* Tar, const IdParams& prm, int* kout, int* iterout, int nb,
+               size_t shmem, hipStream_t st) {
+  auto kern = mode == 0 ? k_id<S, 0> : mode == 1 ? k_id<S, 1> : k_id<S, 2>;
   if (shmem > 65536)
-    (void)hipFuncSetAttribute((const void*)k_id<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)shmem);
-  hipLaunchKernelGGL(k_id<S>, dim3(nb), dim3(256), shmem, st, boxes, mrows, Kar, permar, Tar, prm,
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), shmem, st, ids, boxes, mrows, Kar, permar, Tar, prm,
                      kout, iterout);
 }
 
@@ -1222,11 +1752,14 @@ void launch_elim(bool lu, int mode, size_t shmem, const int* ids, int nb, const 
 }
 
 #define RSKX_INST(S)                                                                          \
-  template void launch_assemble<S>(const LvlBox*, const int*, const int*, const int*, int*, S*, \
-                                   const PointsDev&, const LevelParams&, int, size_t,           \
-                                   hipStream_t);                                                \
-  template void launch_id<S>(const LvlBox*, const int*, S*, int*, double*, const IdParams&,     \
-                             int*, int*, int, size_t, hipStream_t);                            \
+  template void launch_assemble<S>(const int*, const LvlBox*, const int*, const int*,          \
+                                   const int*, int*, S*, const PointsDev&, const LevelParams&,  \
+                                   int, size_t, hipStream_t);                                   \
+  template void launch_id<S>(int, const int*, const LvlBox*, const int*, S*, int*, double*,    \
+                             const IdParams&, int*, int*, int, size_t, hipStream_t);           \
+  template void launch_id_reg<S>(int, const int*, const LvlBox*, const int*, const int*,        \
+                                 const int*, int*, S*, int*, const PointsDev&, const LevelParams&, \
+                                 const IdParams&, int*, int, size_t, hipStream_t);             \
   template void launch_elim_reg<S>(bool, size_t, const int*, int, const LvlBox*, const BoxOut*, \
                                    const Seg*, int*, const int*, const double*, const S*, S*, S*, \
                                    int*, const PointsDev&, int*, hipStream_t);                   \
