@@ -205,6 +205,21 @@ def test_circle_closed_form(order):
     assert np.allclose(pr.matvec(xe), b, rtol=0, atol=1e-12 * np.abs(b).max())
 
 
+def test_level_order_matches_sequential_ranks():
+    """The snapshot-per-level order the device uses and the reference's
+    sequential order differ only in which neighbour lists a box sees;
+    per-level ranks agree to a few percent."""
+    N = 8192
+    pr = problems.CurveDL(N, "circle")
+    tabs = {}
+    for order in ("sequential", "level"):
+        F = flam.rskelf(pr.afun, pr.x, 64, 1e-9, pr.pxyfun, dict(symm="p", order=order))
+        tabs[order] = flam.rank_table(F)
+    for a, b in zip(tabs["sequential"], tabs["level"]):
+        assert a[0] == b[0] and a[1] == b[1]
+        assert abs(a[3] - b[3]) <= max(4, 0.05 * a[3])
+
+
 def test_ellipse_dense_lu():
     N = 1536
     pr = problems.CurveDL(N, "ellipse", a=2.0, b=1.0)
