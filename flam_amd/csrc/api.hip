@@ -59,6 +59,12 @@ void launch_sweep(int, bool, const FacRec*, int, const S*, const int*, S*, int64
                   int, size_t, hipStream_t);
 template <typename U, typename S>
 void launch_perm(bool, void*, int64_t, S*, int64_t, int, int, const int*, hipStream_t);
+template <typename S>
+void launch_pack(bool, const FacRec*, int, S*, hipStream_t);
+template <typename S>
+void launch_solve(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int, int,
+                  size_t, hipStream_t);
+size_t solve_shmem(int prog, int kmax, int rmax, size_t es);
 void launch_matvec(const double*, int64_t, const double*, const double*, const double*, double,
                    const double*, int64_t, double*, int64_t, int, hipStream_t);
 void launch_gather_points(const double*, const double*, const double*, const double*, const int*,
@@ -622,6 +628,12 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
         const int64_t nbk = (r + kDiagBlock - 1) / kDiagBlock;
         O.Li = F->fval_used + need_v; need_v += nbk * kDiagBlock * kDiagBlock;
         O.Ui = F->fval_used + need_v; if (lu) need_v += nbk * kDiagBlock * kDiagBlock;
+        O.Tt = F->fval_used + need_v; need_v += k * r;
+        O.Et = F->fval_used + need_v; need_v += k * r;
+        O.Gt = F->fval_used + need_v; if (lu) need_v += r * k;
+        O.LUt = F->fval_used + need_v; need_v += r * r;
+        O.Lti = F->fval_used + need_v; need_v += nbk * kDiagBlock * kDiagBlock;
+        O.Uti = F->fval_used + need_v; if (lu) need_v += nbk * kDiagBlock * kDiagBlock;
         O.sk = F->fidx_used + need_i; need_i += k;
         O.rd = F->fidx_used + need_i; need_i += r;
         O.piv = F->fidx_used + need_i; if (lu) need_i += r;
@@ -710,6 +722,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
         fr.k = O.k; fr.r = O.r;
         fr.T = O.T; fr.LU = O.LU; fr.E = O.E; fr.G = O.G; fr.Li = O.Li; fr.Ui = O.Ui;
         fr.sk = O.sk; fr.rd = O.rd; fr.piv = O.piv;
+        fr.Tt = O.Tt; fr.Et = O.Et; fr.Gt = O.Gt; fr.LUt = O.LUt; fr.Lti = O.Lti; fr.Uti = O.Uti;
         F->recs.push_back(fr);
         elim += O.r;
       } else {
@@ -736,6 +749,19 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   }
   if (o.verb) printf("%s\n", std::string(69, '-').c_str());
   RSKX_TRY(upload(F->d_recs, F->recs));
+  {
+    hipEvent_t ev;
+    int64_t bytes = 0;
+    for (const FacRec& fr : F->recs) {
+      const int64_t kr = (int64_t)fr.k * fr.r, nbk = (fr.r + kDiagBlock - 1) / kDiagBlock;
+      bytes += 2 * (int64_t)sizeof(S) * ((lu ? 3 : 2) * kr + (int64_t)fr.r * fr.r +
+                                         (lu ? 2 : 1) * nbk * kDiagBlock * kDiagBlock);
+    }
+    prof.begin("pack", ev);
+    launch_pack<S>(lu, F->d_recs.as<FacRec>(), (int)F->recs.size(), F->fval.as<S>(), g_stream);
+    prof.end("pack", ev, bytes);
+  }
+  RSKX_HIP(hipGetLastError());
   RSKX_HIP(hipStreamSynchronize(g_stream));
   prof.flush();
   return RSKX_OK;
@@ -772,6 +798,22 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
       // level's largest block if the total stays under 76 KB (two
       // workgroups per CU), else what is left; larger blocks are read from
       // global memory directly
+      if (solve) {
+        const size_t shs = solve_shmem(prog, kmx, rmx, sizeof(S));
+        if (shs > 160 * 1024) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: factor too large for the solve kernel");
+        int64_t bytes = 0;
+        for (int64_t f = f0; f < f1; ++f) {
+          const FacRec& fr = F->recs[f];
+          int64_t kr = (int64_t)fr.k * fr.r, rr = (int64_t)fr.r * fr.r;
+          bytes += (int64_t)sizeof(S) * (2 * kr + (rr + fr.r) / 2 + 2 * (int64_t)(fr.k + fr.r) * nrhs) +
+                   4 * (fr.k + fr.r);
+        }
+        prof.begin("sweep", ev);
+        launch_solve<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->fval.as<S>(),
+                        F->fidx.as<int>(), Y, N, ncol, kmx, rmx, shs, g_stream);
+        prof.end("sweep", ev, bytes);
+        continue;
+      }
       const size_t xb = ((size_t)(kmx + 2 * rmx) * kRhs + (size_t)kRhs * 256) * sizeof(S);
       const size_t want = (size_t)std::max(rmx * rmx, kmx * rmx) * sizeof(S);
       const size_t room = xb < 76 * 1024 ? 76 * 1024 - xb : 0;

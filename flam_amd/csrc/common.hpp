@@ -78,6 +78,7 @@ struct BoxOut {
   int64_t T, LU, E, G;    // offsets into the factor value arena
   int64_t Li, Ui;         // inverted 32 x 32 diagonal blocks of L and U
   int64_t sk, rd, piv;    // offsets into the factor index arena
+  int64_t Tt, Et, Gt, LUt, Lti, Uti;   // transposed copies (FacRec)
 };
 
 // per-factor record used by the sweeps (values in the factor arena)
@@ -86,6 +87,11 @@ struct FacRec {
   int64_t T, LU, E, G;    // T k x r, LU r x r packed, E k x r, G r x k
   int64_t Li, Ui;         // ceil(r/32) inverted diagonal blocks of L (and U), 32 x 32 each
   int64_t sk, rd, piv;    // index arena offsets (tree order point indices)
+  // transposed copies written by k_pack so that every product of the
+  // solve sweeps reads its matrix with consecutive rows on consecutive
+  // lanes: T' (r x k), E' (r x k), G' (k x r, 'n'), LU' (r x r), and the
+  // diagonal-block inverses transposed block by block
+  int64_t Tt, Et, Gt, LUt, Lti, Uti;
 };
 
 // diagonal block size of the blocked triangular solves in the sweeps

@@ -347,6 +347,283 @@ void launch_sweep(int prog, bool up, const FacRec* recs, int nrec, const S* fval
   }
 }
 
+// ------------------------------------------------------- k_pack --------
+// Transposed copies of each factor's blocks (FacRec::Tt ...), written once
+// after the factorization through 32 x 33 LDS tiles: reads and writes are
+// both along columns.
+template <typename S>
+__device__ void transpose_into(S* __restrict__ dst, const S* __restrict__ src, int R, int C,
+                               S (*tile)[33]) {
+  // src R x C (leading dimension R) -> dst C x R (leading dimension C)
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5, ny = blockDim.x >> 5;
+  for (int i0 = 0; i0 < R; i0 += 32)
+    for (int j0 = 0; j0 < C; j0 += 32) {
+      for (int q = ty; q < 32; q += ny) {
+        const int i = i0 + tx, j = j0 + q;
+        tile[q][tx] = (i < R && j < C) ? src[i + (int64_t)j * R] : (S)0;
+      }
+      __syncthreads();
+      for (int q = ty; q < 32; q += ny) {
+        const int j = j0 + tx, i = i0 + q;
+        if (i < R && j < C) dst[j + (int64_t)i * C] = tile[tx][q];
+      }
+      __syncthreads();
+    }
+}
+
+template <typename S, bool LU>
+__global__ void __launch_bounds__(256) k_pack(const FacRec* __restrict__ recs, S* fval) {
+  __shared__ S tile[32][33];
+  const FacRec f = recs[blockIdx.x];
+  const int k = f.k, r = f.r, nbk = (r + kDiagBlock - 1) / kDiagBlock;
+  transpose_into(fval + f.Tt, fval + f.T, k, r, tile);
+  transpose_into(fval + f.Et, fval + f.E, k, r, tile);
+  if (LU) transpose_into(fval + f.Gt, fval + f.G, r, k, tile);
+  transpose_into(fval + f.LUt, fval + f.LU, r, r, tile);
+  for (int b = 0; b < nbk; ++b) {
+    const int64_t o = (int64_t)b * kDiagBlock * kDiagBlock;
+    transpose_into(fval + f.Lti + o, fval + f.Li + o, kDiagBlock, kDiagBlock, tile);
+    if (LU) transpose_into(fval + f.Uti + o, fval + f.Ui + o, kDiagBlock, kDiagBlock, tile);
+  }
+}
+
+template <typename S>
+void launch_pack(bool lu, const FacRec* recs, int nrec, S* fval, hipStream_t st) {
+  if (nrec <= 0) return;
+  if (lu) hipLaunchKernelGGL((k_pack<S, true>), dim3(nrec), dim3(256), 0, st, recs, fval);
+  else hipLaunchKernelGGL((k_pack<S, false>), dim3(nrec), dim3(256), 0, st, recs, fval);
+}
+
+// ------------------------------------------------------- k_solve -------
+// rskelf_sv sweeps (sv_nn.m, sv_nc.m, sv_p.m).  One workgroup per factor;
+// only the factor's rows of X live in LDS, row-major with the kRhs columns
+// of a row together (one broadcast read per row), and every matrix is read
+// from HBM along its columns: consecutive rows on consecutive lanes.  The
+// transposed operands come from k_pack's copies.
+constexpr int kSolveThreads = 256;
+
+// Y(i,:) -= sum_l M[i + l ldm] Z(l,:), Y ny x kRhs, Z nz x kRhs (row-major);
+// when there are fewer rows than threads each row's sum is split between
+// P threads and combined through red (P ny kRhs <= kSolveThreads kRhs).
+template <typename S>
+__device__ void solve_prod(S* Y, int ny, const S* Z, int nz, const S* __restrict__ M, int ldm,
+                           S* red) {
+  const int tid = threadIdx.x;
+  if (ny == 0 || nz == 0) return;
+  int P = kSolveThreads / ny;
+  P = P < 1 ? 1 : (P > 16 ? 16 : P);
+  if (P > nz) P = nz;
+  if (P == 1) {
+    for (int i = tid; i < ny; i += kSolveThreads) {
+      S acc[kRhs] = {};
+#pragma unroll 4
+      for (int l = 0; l < nz; ++l) {
+        const S a = M[i + (int64_t)l * ldm];
+#pragma unroll
+        for (int c = 0; c < kRhs; ++c) acc[c] += a * Z[l * kRhs + c];
+      }
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) Y[i * kRhs + c] -= acc[c];
+    }
+    __syncthreads();
+    return;
+  }
+  if (tid < ny * P) {
+    const int i = tid % ny, part = tid / ny;
+    S acc[kRhs] = {};
+#pragma unroll 4
+    for (int l = part; l < nz; l += P) {
+      const S a = M[i + (int64_t)l * ldm];
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) acc[c] += a * Z[l * kRhs + c];
+    }
+#pragma unroll
+    for (int c = 0; c < kRhs; ++c) red[(part * ny + i) * kRhs + c] = acc[c];
+  }
+  __syncthreads();
+  for (int o = tid; o < ny * kRhs; o += kSolveThreads) {
+    S sum = 0;
+    for (int part = 0; part < P; ++part) sum += red[part * ny * kRhs + o];
+    Y[o] -= sum;
+  }
+  __syncthreads();
+}
+
+// Y = Op^{-1} Y for a triangular r x r operator Op(i, j) = M[i + j r] whose
+// 32 x 32 diagonal blocks have inverses Minv (block b at b * 1024, element
+// (i, j) at i + 32 j).  fwd: Op lower (forward substitution by blocks);
+// otherwise upper (backward).  Only Op's off-diagonal blocks are read.
+template <typename S>
+__device__ void solve_tri(S* Y, int r, const S* __restrict__ M, const S* __restrict__ Minv,
+                          bool fwd, S* red) {
+  const int tid = threadIdx.x;
+  const int nbk = (r + kDiagBlock - 1) / kDiagBlock;
+  const int bi = tid & (kDiagBlock - 1), bc = tid / kDiagBlock;   // kSolveThreads = 32 kRhs
+  for (int bk = 0; bk < nbk; ++bk) {
+    const int b = fwd ? bk : nbk - 1 - bk;
+    const int j0 = b * kDiagBlock, bs = r - j0 < kDiagBlock ? r - j0 : kDiagBlock;
+    const S* Ib = Minv + (int64_t)b * kDiagBlock * kDiagBlock;
+    S acc = 0;
+    if (bi < bs) {
+#pragma unroll 8
+      for (int u = 0; u < bs; ++u) acc += Ib[bi + u * kDiagBlock] * Y[(j0 + u) * kRhs + bc];
+    }
+    __syncthreads();
+    if (bi < bs) Y[(j0 + bi) * kRhs + bc] = acc;
+    __syncthreads();
+    const int lo = fwd ? j0 + bs : 0, hi = fwd ? r : j0, nrow = hi - lo;
+    if (nrow <= 0) continue;
+    int P = kSolveThreads / nrow;
+    P = P < 1 ? 1 : (P > 8 ? 8 : P);
+    if (P == 1) {
+      for (int i = lo + tid; i < hi; i += kSolveThreads) {
+        S a2[kRhs] = {};
+#pragma unroll 4
+        for (int t = 0; t < bs; ++t) {
+          const S a = M[i + (int64_t)(j0 + t) * r];
+#pragma unroll
+          for (int c = 0; c < kRhs; ++c) a2[c] += a * Y[(j0 + t) * kRhs + c];
+        }
+#pragma unroll
+        for (int c = 0; c < kRhs; ++c) Y[i * kRhs + c] -= a2[c];
+      }
+      __syncthreads();
+      continue;
+    }
+    if (tid < nrow * P) {
+      const int i = lo + tid % nrow, part = tid / nrow;
+      S a2[kRhs] = {};
+#pragma unroll 4
+      for (int t = part; t < bs; t += P) {
+        const S a = M[i + (int64_t)(j0 + t) * r];
+#pragma unroll
+        for (int c = 0; c < kRhs; ++c) a2[c] += a * Y[(j0 + t) * kRhs + c];
+      }
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) red[(part * nrow + (i - lo)) * kRhs + c] = a2[c];
+    }
+    __syncthreads();
+    for (int o = tid; o < nrow * kRhs; o += kSolveThreads) {
+      S sum = 0;
+      for (int part = 0; part < P; ++part) sum += red[part * nrow * kRhs + o];
+      Y[lo * kRhs + o] -= sum;
+    }
+    __syncthreads();
+  }
+}
+
+// rows i of Y (r x kRhs, row-major) permuted: gather Y(i) = Y(p(i)) or
+// scatter Y(p(i)) = Y(i), through tmp
+template <typename S>
+__device__ void solve_perm(S* Y, S* tmp, int r, const int* __restrict__ p, bool gather) {
+  for (int o = threadIdx.x; o < r * kRhs; o += kSolveThreads) {
+    const int i = o / kRhs, c = o - i * kRhs, pi = p[i];
+    if (gather) tmp[o] = Y[pi * kRhs + c];
+    else tmp[pi * kRhs + c] = Y[o];
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < r * kRhs; o += kSolveThreads) Y[o] = tmp[o];
+  __syncthreads();
+}
+
+// PROG: 0 sv 'n'/'n', 1 sv 'n'/'c', 2 sv 'p'.  Shared memory: xs (kmax x 8),
+// xr (rmax x 8), red (8 x 256), and for PROG < 2 a permutation buffer
+// (rmax x 8).  X has nrhs columns, a multiple of 8.
+template <typename S, int PROG, bool UP>
+__global__ void __launch_bounds__(kSolveThreads) k_solve(const FacRec* __restrict__ recs,
+                                                         const S* __restrict__ fval,
+                                                         const int* __restrict__ fidx, S* X,
+                                                         int64_t ldx, int nrhs, int kmax,
+                                                         int rmax) {
+  extern __shared__ char smem_raw[];
+  S* xs = reinterpret_cast<S*>(smem_raw);
+  S* xr = xs + (size_t)kmax * kRhs;
+  S* red = xr + (size_t)rmax * kRhs;
+  S* tmp = red + (size_t)kRhs * kSolveThreads;
+  const FacRec f = recs[blockIdx.x];
+  const int k = f.k, r = f.r;
+  const int* sk = fidx + f.sk;
+  const int* rd = fidx + f.rd;
+  const int* piv = fidx + f.piv;
+  for (int c0 = 0; c0 < nrhs; c0 += kRhs) {
+    for (int a = threadIdx.x; a < k + r; a += kSolveThreads) {
+      const bool isk = a < k;
+      const int64_t row = isk ? sk[a] : rd[a - k];
+      const S* src = X + row + c0 * ldx;
+      S v[kRhs];
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) v[c] = src[c * ldx];
+      S* dst = isk ? xs + a * kRhs : xr + (a - k) * kRhs;
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) dst[c] = v[c];
+    }
+    __syncthreads();
+    if (PROG == 0) {            // sv_nn.m
+      if (UP) {
+        solve_prod(xr, r, xs, k, fval + f.Tt, r, red);
+        solve_perm(xr, tmp, r, piv, true);
+        solve_tri(xr, r, fval + f.LU, fval + f.Li, true, red);
+        solve_prod(xs, k, xr, r, fval + f.E, k, red);
+      } else {
+        solve_prod(xr, r, xs, k, fval + f.G, r, red);
+        solve_tri(xr, r, fval + f.LU, fval + f.Ui, false, red);
+        solve_prod(xs, k, xr, r, fval + f.T, k, red);
+      }
+    } else if (PROG == 1) {     // sv_nc.m
+      if (UP) {
+        solve_prod(xr, r, xs, k, fval + f.Tt, r, red);
+        solve_tri(xr, r, fval + f.LUt, fval + f.Uti, true, red);
+        solve_prod(xs, k, xr, r, fval + f.Gt, k, red);
+      } else {
+        solve_prod(xr, r, xs, k, fval + f.Et, r, red);
+        solve_tri(xr, r, fval + f.LUt, fval + f.Lti, false, red);
+        solve_perm(xr, tmp, r, piv, false);
+        solve_prod(xs, k, xr, r, fval + f.T, k, red);
+      }
+    } else {                    // sv_p.m
+      if (UP) {
+        solve_prod(xr, r, xs, k, fval + f.Tt, r, red);
+        solve_tri(xr, r, fval + f.LU, fval + f.Li, true, red);
+        solve_prod(xs, k, xr, r, fval + f.E, k, red);
+      } else {
+        solve_prod(xr, r, xs, k, fval + f.Et, r, red);
+        solve_tri(xr, r, fval + f.LUt, fval + f.Lti, false, red);
+        solve_prod(xs, k, xr, r, fval + f.T, k, red);
+      }
+    }
+    for (int a = threadIdx.x; a < k + r; a += kSolveThreads) {
+      const bool isk = a < k;
+      const int64_t row = isk ? sk[a] : rd[a - k];
+      S* dstg = X + row + c0 * ldx;
+      const S* srcl = isk ? xs + a * kRhs : xr + (a - k) * kRhs;
+#pragma unroll
+      for (int c = 0; c < kRhs; ++c) dstg[c * ldx] = srcl[c];
+    }
+    __syncthreads();
+  }
+}
+
+size_t solve_shmem(int prog, int kmax, int rmax, size_t es) {
+  return ((size_t)(kmax + rmax) * kRhs + (size_t)kRhs * kSolveThreads +
+          (prog < 2 ? (size_t)rmax * kRhs : 0)) * es;
+}
+
+template <typename S>
+void launch_solve(int prog, bool up, const FacRec* recs, int nrec, const S* fval, const int* fidx,
+                  S* X, int64_t ldx, int nrhs, int kmax, int rmax, size_t shmem, hipStream_t st) {
+#define RSKX_SOLVE(P)                                                                           \
+  {                                                                                             \
+    auto kern = up ? k_solve<S, P, true> : k_solve<S, P, false>;                                \
+    if (shmem > 65536)                                                                          \
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)shmem);                                                    \
+    hipLaunchKernelGGL(kern, dim3(nrec), dim3(kSolveThreads), shmem, st, recs, fval, fidx, X,    \
+                       ldx, nrhs, kmax, rmax);                                                  \
+  }
+  if (prog == 0) RSKX_SOLVE(0) else if (prog == 1) RSKX_SOLVE(1) else RSKX_SOLVE(2)
+#undef RSKX_SOLVE
+}
+
 // ------------------------------------------------ X to / from tree order --
 // Y has ncol >= nrhs columns; the extra ones are zero
 template <typename U, typename S>
@@ -469,7 +746,10 @@ void launch_gather_points(const double* x, const double* nu, const double* w, co
   template void launch_perm<double, S>(bool, void*, int64_t, S*, int64_t, int, int,         \
                                        const int*, hipStream_t);                              \
   template void launch_perm<float, S>(bool, void*, int64_t, S*, int64_t, int, int,          \
-                                      const int*, hipStream_t);
+                                      const int*, hipStream_t);                               \
+  template void launch_pack<S>(bool, const FacRec*, int, S*, hipStream_t);                    \
+  template void launch_solve<S>(int, bool, const FacRec*, int, const S*, const int*, S*,      \
+                                int64_t, int, int, int, size_t, hipStream_t);
 RSKX_SWEEP_INST(double)
 RSKX_SWEEP_INST(float)
 
