@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -81,34 +82,90 @@ static int fail(int code, const std::string& msg) { set_error(msg); return code;
 static hipStream_t g_stream = nullptr;
 
 // ------------------------------------------------------- device buffers --
+// Device blocks released by factors and workspaces are kept for reuse
+// (hipMalloc of large blocks and hipFree, which synchronizes the device,
+// would otherwise sit inside every factorization).  A request takes the
+// smallest cached block of at least its size and at most 4/3 of it; the
+// cache is trimmed, largest blocks first, above kCacheBytes.
+struct BlockCache {
+  std::multimap<size_t, void*> free;
+  size_t bytes = 0;
+  std::mutex mu;
+};
+static BlockCache g_blocks;
+constexpr size_t kCacheBytes = size_t(48) << 30;
+
+static void* block_get(size_t want, size_t* got) {
+  {
+    std::lock_guard<std::mutex> lk(g_blocks.mu);
+    auto it = g_blocks.free.lower_bound(want);
+    if (it != g_blocks.free.end() && it->first <= want + want / 3) {
+      void* p = it->second;
+      *got = it->first;
+      g_blocks.bytes -= it->first;
+      g_blocks.free.erase(it);
+      return p;
+    }
+  }
+  void* q = nullptr;
+  if (hipMalloc(&q, want) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  *got = want;
+  return q;
+}
+
+static void block_put(void* p, size_t cap) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_blocks.mu);
+  g_blocks.free.emplace(cap, p);
+  g_blocks.bytes += cap;
+  while (g_blocks.bytes > kCacheBytes && !g_blocks.free.empty()) {
+    auto it = std::prev(g_blocks.free.end());
+    (void)hipFree(it->second);
+    g_blocks.bytes -= it->first;
+    g_blocks.free.erase(it);
+  }
+}
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
   int ensure(size_t bytes, bool keep = false) {
     if (bytes <= cap) return RSKX_OK;
-    size_t nc = std::max(bytes, cap + cap / 2);
-    void* q = nullptr;
-    if (hipMalloc(&q, nc) != hipSuccess) {
-      (void)hipGetLastError();
+    size_t nc = std::max(bytes, cap + cap / 2), got = 0;
+    void* q = block_get(nc, &got);
+    if (!q)
       return fail(RSKX_ERR_MEM, "FLAM:rskx:oom: device allocation of " + std::to_string(nc) + " bytes failed");
-    }
     if (keep && p && cap) {
       if (hipMemcpyAsync(q, p, cap, hipMemcpyDeviceToDevice, g_stream) != hipSuccess)
         return fail(RSKX_ERR_HIP, "FLAM:rskx:hip: arena copy failed");
       hipStreamSynchronize(g_stream);
     }
-    if (p) hipFree(p);
+    if (p) {
+      hipStreamSynchronize(g_stream);    // no kernel may still use the old block
+      block_put(p, cap);
+    }
     p = q;
-    cap = nc;
+    cap = got;
     return RSKX_OK;
   }
-  void release() { if (p) hipFree(p); p = nullptr; cap = 0; }
+  void release() {
+    if (p) {
+      hipStreamSynchronize(g_stream);
+      block_put(p, cap);
+    }
+    p = nullptr;
+    cap = 0;
+  }
   template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
 // scratch reused across factorizations of this process
 struct Workspace {
   DevBuf boxes, nsegs, csegs, outs, ids, pool, nbr, mnbr, mrows, kout, iter, K, perm, T, W, M[2], flag;
+  DevBuf lrecs;                 // factor records of the level being packed
   DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs;
   int* hflag = nullptr;
 };
@@ -180,12 +237,23 @@ struct rskx_factor {
   std::vector<int64_t> lvp;               // cumulative factor counts per processed level
   std::vector<int> lev_kmax, lev_rmax;
   std::vector<FacRec> recs;               // host copy
-  DevBuf d_recs, fval, fidx, order;       // device data
+  DevBuf d_recs, order;                   // device data
+  // factor values and indices, one block per processed level (FacRec
+  // offsets are relative to their level's blocks)
+  std::vector<DevBuf> lval, lidx;
   mutable std::vector<int64_t> h_order;   // tree order -> original index (filled on export)
   std::vector<std::array<int64_t, 4>> table;
   Prof prof;
-  int64_t fval_used = 0, fidx_used = 0;
-  ~rskx_factor() { d_recs.release(); fval.release(); fidx.release(); order.release(); }
+  int64_t fval_used = 0, fidx_used = 0;   // totals over the levels
+  int level_of(int64_t f) const {         // processed level holding factor f
+    return (int)(std::upper_bound(lvp.begin(), lvp.end(), f) - lvp.begin()) - 1;
+  }
+  ~rskx_factor() {
+    d_recs.release();
+    order.release();
+    for (auto& b : lval) b.release();
+    for (auto& b : lidx) b.release();
+  }
 };
 
 namespace {
@@ -396,6 +464,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   int64_t nfac = 0;
   int mpar = 0;                 // M arena of the level below: ws.M[mpar]
   std::vector<int64_t> pblk(t.nlvl + 1, 0);
+  std::vector<FacRec> lrec;
   for (int lv = 0; lv < t.nlvl; ++lv) {
     int64_t c = 0;
     for (int64_t b = t.lvp[lv]; b < t.lvp[lv + 1]; ++b) c += t.cnt[b] > 0;
@@ -621,22 +690,22 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
         id_flops += 4.0 * m * n * k - 2.0 * k * k * (m + n) + 4.0 / 3.0 * k * k * k + (double)k * k * r;
       if (r > 0) {
         O.fac = (int)(nfac++);
-        O.T = F->fval_used + need_v; need_v += k * r;
-        O.LU = F->fval_used + need_v; need_v += r * r;
-        O.E = F->fval_used + need_v; need_v += k * r;
-        O.G = F->fval_used + need_v; if (lu) need_v += r * k;
+        O.T = need_v; need_v += k * r;
+        O.LU = need_v; need_v += r * r;
+        O.E = need_v; need_v += k * r;
+        O.G = need_v; if (lu) need_v += r * k;
         const int64_t nbk = (r + kDiagBlock - 1) / kDiagBlock;
-        O.Li = F->fval_used + need_v; need_v += nbk * kDiagBlock * kDiagBlock;
-        O.Ui = F->fval_used + need_v; if (lu) need_v += nbk * kDiagBlock * kDiagBlock;
-        O.Tt = F->fval_used + need_v; need_v += k * r;
-        O.Et = F->fval_used + need_v; need_v += k * r;
-        O.Gt = F->fval_used + need_v; if (lu) need_v += r * k;
-        O.LUt = F->fval_used + need_v; need_v += r * r;
-        O.Lti = F->fval_used + need_v; need_v += nbk * kDiagBlock * kDiagBlock;
-        O.Uti = F->fval_used + need_v; if (lu) need_v += nbk * kDiagBlock * kDiagBlock;
-        O.sk = F->fidx_used + need_i; need_i += k;
-        O.rd = F->fidx_used + need_i; need_i += r;
-        O.piv = F->fidx_used + need_i; if (lu) need_i += r;
+        O.Li = need_v; need_v += nbk * kDiagBlock * kDiagBlock;
+        O.Ui = need_v; if (lu) need_v += nbk * kDiagBlock * kDiagBlock;
+        O.Tt = need_v; need_v += k * r;
+        O.Et = need_v; need_v += k * r;
+        O.Gt = need_v; if (lu) need_v += r * k;
+        O.LUt = need_v; need_v += r * r;
+        O.Lti = need_v; need_v += nbk * kDiagBlock * kDiagBlock;
+        O.Uti = need_v; if (lu) need_v += nbk * kDiagBlock * kDiagBlock;
+        O.sk = need_i; need_i += k;
+        O.rd = need_i; need_i += r;
+        O.piv = need_i; if (lu) need_i += r;
         O.M_off = offM; offM += k * k;
         O.nl_off = pool_top + off_slf + off_nl; off_nl += k;
         kmx = std::max<int>(kmx, (int)k);
@@ -650,8 +719,12 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
       }
     }
     prof.st["id"].flops += id_flops;
-    RSKX_TRY(F->fval.ensure((F->fval_used + need_v + 1) * sizeof(S), true));
-    RSKX_TRY(F->fidx.ensure((F->fidx_used + need_i + 1) * sizeof(int), true));
+    F->lval.emplace_back();
+    F->lidx.emplace_back();
+    RSKX_TRY(F->lval.back().ensure((need_v + 1) * sizeof(S)));
+    RSKX_TRY(F->lidx.back().ensure((need_i + 1) * sizeof(int)));
+    S* fval = F->lval.back().as<S>();
+    int* fidx = F->lidx.back().as<int>();
     const int mcur = 1 - mpar;
     RSKX_TRY(ws.M[mcur].ensure(std::max<int64_t>(offM, 1) * sizeof(S)));
     RSKX_TRY(ws.M[mpar].ensure(sizeof(S)));
@@ -685,7 +758,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
         launch_elim_reg<S>(lu, shm_reg, ws.ids.as<int>(), (int)creg.size(), ws.boxes.as<LvlBox>(),
                            ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
                            ws.perm.as<int>(), ws.T.as<double>(), ws.M[mpar].as<S>(),
-                           ws.M[mcur].as<S>(), F->fval.as<S>(), F->fidx.as<int>(), P,
+                           ws.M[mcur].as<S>(), fval, fidx, P,
                            ws.flag.as<int>(), g_stream);
         off += (int)creg.size();
       }
@@ -694,8 +767,8 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
         launch_elim<S>(lu, c < 2 ? 1 : 0, shm_cls[c], ws.ids.as<int>() + off,
                        (int)cls[c].size(), ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(),
                        ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(), ws.T.as<double>(),
-                       ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), ws.W.as<S>(), F->fval.as<S>(),
-                       F->fidx.as<int>(), P, ws.flag.as<int>(), g_stream);
+                       ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), ws.W.as<S>(), fval,
+                       fidx, P, ws.flag.as<int>(), g_stream);
         off += (int)cls[c].size();
       }
     }
@@ -731,6 +804,24 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
       }
       bs[b].moff = O.M_off;
     }
+    // transposed copies for the solve sweeps; runs while the host lays
+    // out the next level
+    {
+      const int64_t f0 = F->lvp.back(), nl = (int64_t)F->recs.size() - f0;
+      if (nl > 0) {
+        lrec.assign(F->recs.begin() + f0, F->recs.end());   // kept until the next sync
+        RSKX_TRY(upload(ws.lrecs, lrec));
+        int64_t bytes = 0;
+        for (const FacRec& fr : lrec) {
+          const int64_t kr = (int64_t)fr.k * fr.r, nbk = (fr.r + kDiagBlock - 1) / kDiagBlock;
+          bytes += 2 * (int64_t)sizeof(S) * ((lu ? 3 : 2) * kr + (int64_t)fr.r * fr.r +
+                                             (lu ? 2 : 1) * nbk * kDiagBlock * kDiagBlock);
+        }
+        prof.begin("pack", ev);
+        launch_pack<S>(lu, ws.lrecs.as<FacRec>(), (int)nl, fval, g_stream);
+        prof.end("pack", ev, bytes);
+      }
+    }
     F->fval_used += need_v;
     F->fidx_used += need_i;
     pool_top += off_slf + off_nl;
@@ -749,18 +840,6 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   }
   if (o.verb) printf("%s\n", std::string(69, '-').c_str());
   RSKX_TRY(upload(F->d_recs, F->recs));
-  {
-    hipEvent_t ev;
-    int64_t bytes = 0;
-    for (const FacRec& fr : F->recs) {
-      const int64_t kr = (int64_t)fr.k * fr.r, nbk = (fr.r + kDiagBlock - 1) / kDiagBlock;
-      bytes += 2 * (int64_t)sizeof(S) * ((lu ? 3 : 2) * kr + (int64_t)fr.r * fr.r +
-                                         (lu ? 2 : 1) * nbk * kDiagBlock * kDiagBlock);
-    }
-    prof.begin("pack", ev);
-    launch_pack<S>(lu, F->d_recs.as<FacRec>(), (int)F->recs.size(), F->fval.as<S>(), g_stream);
-    prof.end("pack", ev, bytes);
-  }
   RSKX_HIP(hipGetLastError());
   RSKX_HIP(hipStreamSynchronize(g_stream));
   prof.flush();
@@ -809,8 +888,8 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
                    4 * (fr.k + fr.r);
         }
         prof.begin("sweep", ev);
-        launch_solve<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->fval.as<S>(),
-                        F->fidx.as<int>(), Y, N, ncol, kmx, rmx, shs, g_stream);
+        launch_solve<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->lval[lv].as<S>(),
+                        F->lidx[lv].as<int>(), Y, N, ncol, kmx, rmx, shs, g_stream);
         prof.end("sweep", ev, bytes);
         continue;
       }
@@ -828,8 +907,8 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
                  4 * (fr.k + fr.r);
       }
       prof.begin("sweep", ev);
-      launch_sweep<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->fval.as<S>(),
-                      F->fidx.as<int>(), Y, N, ncol, kmx, rmx, bcap, shm, g_stream);
+      launch_sweep<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->lval[lv].as<S>(),
+                      F->lidx[lv].as<int>(), Y, N, ncol, kmx, rmx, bcap, shm, g_stream);
       prof.end("sweep", ev, bytes);
     }
   }
@@ -970,6 +1049,7 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
   v->k = f.k;
   v->r = f.r;
   if (!v->sk) return RSKX_OK;
+  const int flv = F->level_of(i);
   const int64_t k = f.k, r = f.r;
   const bool lu = F->symm == 'n';
   const int es = F->prec == RSKX_PREC_FP64 ? 8 : 4;
@@ -977,14 +1057,14 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
     dst.resize(cnt);
     if (!cnt) return RSKX_OK;
     std::vector<char> raw(cnt * es);
-    RSKX_HIP(hipMemcpy(raw.data(), F->fval.as<char>() + off * es, cnt * es, hipMemcpyDeviceToHost));
+    RSKX_HIP(hipMemcpy(raw.data(), F->lval[flv].as<char>() + off * es, cnt * es, hipMemcpyDeviceToHost));
     for (int64_t q = 0; q < cnt; ++q)
       dst[q] = es == 8 ? reinterpret_cast<double*>(raw.data())[q] : reinterpret_cast<float*>(raw.data())[q];
     return RSKX_OK;
   };
   auto geti = [&](int64_t off, int64_t cnt, std::vector<int>& dst) -> int {
     dst.resize(cnt);
-    if (cnt) RSKX_HIP(hipMemcpy(dst.data(), F->fidx.as<int>() + off, cnt * 4, hipMemcpyDeviceToHost));
+    if (cnt) RSKX_HIP(hipMemcpy(dst.data(), F->lidx[flv].as<int>() + off, cnt * 4, hipMemcpyDeviceToHost));
     return RSKX_OK;
   };
   std::vector<double> T, LU, E, G;

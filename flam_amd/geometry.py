@@ -41,11 +41,14 @@ class Curve:
 
     def circulant_matvec(self, X):
         """Exact FP64 A X for rotation-invariant curves (the circle): A is
-        circulant, so A X = ifft(fft(A(:,1)) .* fft(X)) (ie_circle.m:47-48)."""
-        G = torch.fft.fft(self.column0().to(torch.complex128))
+        circulant, so A X = ifft(fft(A(:,1)) .* fft(X)) (ie_circle.m:47-48),
+        here with real transforms along each column and the symbol cached."""
+        if getattr(self, "_symbol", None) is None:
+            self._symbol = torch.fft.rfft(self.column0())
         vec = X.dim() == 1
-        X2 = X.reshape(self.N, -1)
-        Y = torch.fft.ifft(G[:, None] * torch.fft.fft(X2.to(torch.complex128), dim=0), dim=0).real
+        X2 = X.reshape(self.N, -1).to(torch.float64)
+        Xt = X2.t()                        # one transform per column, contiguous rows
+        Y = torch.fft.irfft(self._symbol * torch.fft.rfft(Xt, dim=1), n=self.N, dim=1).t()
         return Y.reshape(-1) if vec else Y
 
 

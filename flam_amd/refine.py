@@ -8,8 +8,26 @@ preconditioned directions are kept, so a preconditioner that is linear only
 up to FP32 rounding does not limit the attainable FP64 residual), run on
 all right-hand sides at once and stopped on the true relative residual
 ||b - A x|| / ||b|| of every column.
+
+Vectors are stored one right-hand side per row, (nrhs, N), which is the
+column-major N x nrhs layout the sweeps take; the Krylov basis of every
+column is one (nrhs, restart + 1, N) array, so each orthogonalization pass
+is one batched product.  Classical Gram-Schmidt is applied twice per step.
 """
 import torch
+
+
+def _rowsum(P):
+    """Sum over the last dimension in two stages (1024 partial sums per
+    row first), so a few very long rows still spread over the whole GPU."""
+    n = P.shape[-1]
+    if n % 1024 == 0 and n >= 1 << 16:
+        return P.reshape(*P.shape[:-1], 1024, n // 1024).sum(-1).sum(-1)
+    return P.sum(-1)
+
+
+def _norms(W):
+    return torch.sqrt(_rowsum(W * W))
 
 
 def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8):
@@ -18,45 +36,55 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8):
     vec = B.dim() == 1
     B2 = B.reshape(B.shape[0], -1).to(torch.float64)
     N, nrhs = B2.shape
-    bnorm = torch.linalg.vector_norm(B2, dim=0)
+    Bt = B2.t().contiguous()                       # (nrhs, N)
+    bnorm = _norms(Bt)
     bnorm = torch.where(bnorm > 0, bnorm, torch.ones_like(bnorm))
-    X = torch.zeros_like(B2)
+    bnorm_h = bnorm.cpu()
+    Xt = torch.zeros_like(Bt)
+    V = torch.empty(nrhs, restart + 1, N, dtype=torch.float64, device=B2.device)
+    Z = torch.empty(nrhs, restart, N, dtype=torch.float64, device=B2.device)
+
+    def op(f, Yt):                                 # (nrhs, N) -> (nrhs, N)
+        return f(Yt.t()).t().contiguous()
+
     iters = 0
     relres = None
     outer = 0
     for outer in range(maxit + 1):
-        R = B2 - matvec(X) if outer else B2.clone()
-        rn = torch.linalg.vector_norm(R, dim=0)
+        R = Bt - op(matvec, Xt) if outer else Bt.clone()
+        rn = _norms(R)
         relres = (rn / bnorm).cpu()
         if bool((relres <= tol).all()) or outer == maxit:
             break
-        V = [R / rn]
-        Z = []
+        V[:, 0, :] = R / rn[:, None]
         H = torch.zeros(nrhs, restart + 1, restart, dtype=torch.float64)
         g = torch.zeros(nrhs, restart + 1, dtype=torch.float64)
         g[:, 0] = rn.cpu()
         y = None
+        j = 0
         for j in range(restart):
-            z = precond(V[j])
-            Z.append(z)
-            w = matvec(z)
-            hcol = torch.zeros(nrhs, j + 2, dtype=torch.float64, device=B2.device)
-            for i in range(j + 1):          # modified Gram-Schmidt, per column
-                h = (V[i] * w).sum(0)
-                hcol[:, i] = h
-                w = w - V[i] * h
-            hn = torch.linalg.vector_norm(w, dim=0)
-            hcol[:, j + 1] = hn
-            H[:, : j + 2, j] = hcol.cpu()
-            V.append(w / torch.where(hn > 0, hn, torch.ones_like(hn)))
+            Z[:, j, :] = op(precond, V[:, j, :])
+            w = op(matvec, Z[:, j, :])
+            Vj = V[:, : j + 1, :]
+            # GEMMs with a reduction length of N and one output column map
+            # badly onto BLAS kernels; elementwise products and row sums
+            # stream each vector once per pass
+            h = _rowsum(Vj * w[:, None, :])                    # (nrhs, j+1)
+            w = w - (Vj * h[:, :, None]).sum(1)
+            h2 = _rowsum(Vj * w[:, None, :])
+            w = w - (Vj * h2[:, :, None]).sum(1)
+            hn = _norms(w)
+            V[:, j + 1, :] = w / torch.where(hn > 0, hn, torch.ones_like(hn))[:, None]
+            hcol = torch.cat([h + h2, hn[:, None]], dim=1).cpu()
+            H[:, : j + 2, j] = hcol
             iters += 1
             Hj = H[:, : j + 2, : j + 1]
             y = torch.linalg.lstsq(Hj, g[:, : j + 2, None]).solution[..., 0]
             est = torch.linalg.vector_norm(g[:, : j + 2] - (Hj @ y[..., None])[..., 0], dim=1)
-            if bool((est / bnorm.cpu() <= 0.5 * tol).all()) or bool((hn == 0).all()):
+            if bool((est / bnorm_h <= 0.5 * tol).all()) or bool((hcol[:, -1] == 0).all()):
                 break
-        Zm = torch.stack(Z)                 # (j+1, N, nrhs)
-        yd = y.to(B2.device).t()            # (j+1, nrhs)
-        X = X + (Zm * yd[:, None, :]).sum(0)
+        yd = y.to(B2.device)[:, :, None]                       # (nrhs, j+1, 1)
+        Xt = Xt + (Z[:, : j + 1, :] * yd).sum(1)
     info = dict(iters=iters, outer=outer, relres=relres.tolist())
+    X = Xt.t()
     return (X.reshape(-1) if vec else X), info
