@@ -378,6 +378,9 @@ int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const
     RSKX_TRY(upload(ws.tchunks, chunks));
     launch_tree_split(true, ws.tchunks.as<TreeChunk>(), nch, d, divmask, N, xs, ord,
                       ws.txs.as<double>(), ws.tord.as<int>(), ws.tcounts.as<int>(), g_stream);
+    // the parent level's split status is now final: its neighbour lists
+    // are built while the device scatters this level's points
+    append_level_neighbours(t, chld, nlvl - 1);
     ++nlvl;
     t.lvp.push_back(t.nbox());
     t.lsize.insert(t.lsize.end(), l.begin(), l.end());

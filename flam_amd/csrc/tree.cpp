@@ -106,44 +106,58 @@ HostTree build_tree(const double* x, int d, int64_t N, int occ, int lvlmax, cons
   return t;
 }
 
+void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& chld, int lv) {
+  // neighbours of the boxes of level lv (hypoct.m:141-170): siblings,
+  // adjacent non-split boxes among the parent's neighbours, and adjacent
+  // children of the parent's neighbours.  Boxes are numbered level by
+  // level and a box's list depends only on its parent's, so lists are
+  // appended to the CSR arrays in box order as levels are completed.
+  const int d = t.d;
+  if (t.nbor_off.empty()) t.nbor_off.push_back(0);
+  const int64_t b0 = t.lvp[lv], b1 = t.lvp[lv + 1];
+  if ((int64_t)t.nbor_off.size() != b0 + 1) return;      // out of order: nothing to do
+  if (lv == 0) {
+    for (int64_t i = b0; i < b1; ++i) t.nbor_off.push_back((int)t.nbor.size());
+    return;
+  }
+  const double* lc = &t.lsize[(size_t)d * lv];
+  for (int64_t i = b0; i < b1; ++i) {
+    const double* c = &t.ctr[(size_t)d * i];
+    const int p = t.prnt[i];
+    for (int j : chld[p])
+      if (j != i) t.nbor.push_back(j);
+    const int pn0 = t.nbor_off[p], pn1 = t.nbor_off[p + 1];
+    for (int e = pn0; e < pn1; ++e) {
+      const int j = t.nbor[e];
+      if (t.cnt[j] == 0) continue;            // only non-empty (leaf) boxes
+      const double* jl = &t.lsize[(size_t)d * t.level_of(j)];
+      const double* jc = &t.ctr[(size_t)d * j];
+      double dmax = -INFINITY;
+      for (int q = 0; q < d; ++q)
+        dmax = std::max(dmax, matlab_round((std::fabs(c[q] - jc[q]) - 0.5 * (lc[q] + jl[q])) / lc[q]));
+      if (dmax <= 0) t.nbor.push_back(j);
+    }
+    for (int e = pn0; e < pn1; ++e) {
+      const int j = t.nbor[e];
+      for (int q2 : chld[j]) {
+        const double* qc = &t.ctr[(size_t)d * q2];
+        double dmax = -INFINITY;
+        for (int u = 0; u < d; ++u) dmax = std::max(dmax, matlab_round(std::fabs(c[u] - qc[u]) / lc[u]));
+        if (dmax <= 1) t.nbor.push_back(q2);
+      }
+    }
+    t.nbor_off.push_back((int)t.nbor.size());
+  }
+}
+
 void finish_tree(HostTree& t, const std::vector<std::vector<int>>& chld) {
-  const int d = t.d, nlvl = t.nlvl;
-  int nbox = t.nbox();
+  const int nbox = t.nbox();
   t.chld_off.assign(nbox + 1, 0);
   for (int b = 0; b < nbox; ++b) t.chld_off[b + 1] = t.chld_off[b] + (int)chld[b].size();
+  t.chld.clear();
+  t.chld.reserve(t.chld_off[nbox]);
   for (int b = 0; b < nbox; ++b) t.chld.insert(t.chld.end(), chld[b].begin(), chld[b].end());
-
-  // neighbours, level by level (hypoct.m:141-170)
-  std::vector<std::vector<int>> nb(nbox);
-  for (int lv = 1; lv < nlvl; ++lv) {
-    const double* lc = &t.lsize[(size_t)d * lv];
-    for (int64_t i = t.lvp[lv]; i < t.lvp[lv + 1]; ++i) {
-      const double* c = &t.ctr[(size_t)d * i];
-      int p = t.prnt[i];
-      for (int j : chld[p])
-        if (j != i) nb[i].push_back(j);
-      for (int j : nb[p]) {
-        if (t.cnt[j] == 0) continue;          // only non-empty (leaf) boxes
-        const double* jl = &t.lsize[(size_t)d * t.level_of(j)];
-        const double* jc = &t.ctr[(size_t)d * j];
-        double dmax = -INFINITY;
-        for (int q = 0; q < d; ++q)
-          dmax = std::max(dmax, matlab_round((std::fabs(c[q] - jc[q]) - 0.5 * (lc[q] + jl[q])) / lc[q]));
-        if (dmax <= 0) nb[i].push_back(j);
-      }
-      for (int j : nb[p])
-        for (int q2 : chld[j]) {
-          const double* qc = &t.ctr[(size_t)d * q2];
-          double dmax = -INFINITY;
-          for (int q = 0; q < d; ++q)
-            dmax = std::max(dmax, matlab_round(std::fabs(c[q] - qc[q]) / lc[q]));
-          if (dmax <= 1) nb[i].push_back(q2);
-        }
-    }
-  }
-  t.nbor_off.assign(nbox + 1, 0);
-  for (int b = 0; b < nbox; ++b) t.nbor_off[b + 1] = t.nbor_off[b] + (int)nb[b].size();
-  for (int b = 0; b < nbox; ++b) t.nbor.insert(t.nbor.end(), nb[b].begin(), nb[b].end());
+  for (int lv = 0; lv < t.nlvl; ++lv) append_level_neighbours(t, chld, lv);
 }
 
 }  // namespace rskx

@@ -37,8 +37,14 @@ struct HostTree {
 // column-major (lower bounds then upper bounds) or nullptr.
 HostTree build_tree(const double* x, int d, int64_t N, int occ, int lvlmax, const double* ext);
 
-// Children in CSR form and the neighbour lists (hypoct.m:141-170), given
-// the boxes (ctr, prnt, beg/cnt, lvp, lsize, nlvl) and each box's children.
+// Neighbour lists of the boxes of level lv (hypoct.m:141-170), appended to
+// t.nbor / t.nbor_off; needs the lists of every earlier level and the final
+// split status (cnt) of the levels above lv.
+void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& chld, int lv);
+
+// Children in CSR form and the neighbour lists of the levels not yet
+// appended, given the boxes (ctr, prnt, beg/cnt, lvp, lsize, nlvl) and each
+// box's children.
 void finish_tree(HostTree& t, const std::vector<std::vector<int>>& chld);
 
 }  // namespace rskx
