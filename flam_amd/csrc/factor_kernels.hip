@@ -1327,6 +1327,12 @@ void launch_id(int mode, const int* ids, const LvlBox* boxes, const int* mrows, 
                      kout, iterout);
 }
 
+// value of lane l (uniform l) broadcast to the wave
+__device__ inline float lane_bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double lane_bcast(double v, int l) { return read_lane(v, l); }
+
 // Elimination with the augmented matrix held in registers.
 //
 // In the order [rd; sk] the box's sparsified matrix with the skeleton
@@ -1359,6 +1365,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   const BoxOut O = outs[bidx];
   const int n = B.n, k = O.k, r = O.r;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // columns go to waves in panels of NB: slot tb of wave w holds column
+  // ((tb / NB) * 16 + w) * NB + tb % NB (NB = 1: column w + 16 tb)
+  constexpr int NB = LU ? 1 : 4;
+  static_assert(TB % NB == 0, "TB must be a multiple of the panel width");
+  auto colj = [&](int tb) { return ((tb / NB) * 16 + wid) * NB + tb % NB; };
   __shared__ ChildMap cm;
   __shared__ int s_piv[2];
   __shared__ S s_pv[2];
@@ -1430,7 +1441,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb) {
-      const int i = lane + 64 * ta, j = wid + 16 * tb;
+      const int i = lane + 64 * ta, j = colj(tb);
       S z = 0;
       if (i < n && j < n) {
         if (i < r && j < r) z = kv(i, j);                    // Krr
@@ -1450,7 +1461,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
     }
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb) {
-      const int j = wid + 16 * tb;
+      const int j = colj(tb);
       const S kx = j < r ? KX[l + j * k] : (S)0;
 #pragma unroll
       for (int ta = 0; ta < TA; ++ta) Z[ta][tb] -= tv[ta] * kx;
@@ -1468,7 +1479,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
     }
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb) {
-      const int j = wid + 16 * tb;
+      const int j = colj(tb);
       const S tt = j < r ? Ts[l + j * k] : (S)0;
 #pragma unroll
       for (int ta = 0; ta < TA; ++ta) Z[ta][tb] -= kr[ta] * tt;
@@ -1486,16 +1497,107 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb) {
-      const int i = lane + 64 * ta, j = wid + 16 * tb;
+      const int i = lane + 64 * ta, j = colj(tb);
       if (i < r && j >= r && j < n) Z[ta][tb] = KX[i + (j - r) * r];
     }
 
-  // ---- r elimination steps, one barrier each; column s is held by wave
-  // s mod 16, slot s / 16
+  // ---- Cholesky: r / NB panels, one barrier each.  The wave owning panel
+  // p factors its NB columns in registers (pivots broadcast by readlane),
+  // leaves L(:, panel) in LDS, and every wave then applies the rank-NB
+  // update to its own trailing columns.
+  if constexpr (!LU) {
+    for (int p = 0; p * NB < r; ++p) {
+      const int s0 = p * NB, s1 = s0 + NB < r ? s0 + NB : r;
+      const int wp = p % 16, gp = p / 16;
+      S* Lp = colb + (p & 1) * NB * n;           // L(i, s0 + c) at Lp[c * n + i]
+      if (wid == wp) {
+        S pc[NB][TA];
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+#pragma unroll
+          for (int ta = 0; ta < TA; ++ta) {
+            S v = 0;
+#pragma unroll
+            for (int g = 0; g < TB / NB; ++g) if (g == gp) v = Z[ta][g * NB + c];
+            pc[c][ta] = v;
+          }
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          const int sc = s0 + c;
+          if (sc >= s1) break;
+          S dv = 0;
+#pragma unroll
+          for (int ta = 0; ta < TA; ++ta) if (ta == (sc >> 6)) dv = pc[c][ta];
+          dv = lane_bcast(dv, sc & 63);
+          if (!(dv > (S)0)) { if (lane == 0) atomicOr(fail, 1); dv = (S)1; }
+          const S pv = sqrt(dv), inv = (S)1 / pv;
+#pragma unroll
+          for (int ta = 0; ta < TA; ++ta) {
+            const int i = lane + 64 * ta;
+            if (i > sc && i < n) pc[c][ta] *= inv;
+            else if (i == sc) pc[c][ta] = pv;
+          }
+#pragma unroll
+          for (int c2 = c + 1; c2 < NB; ++c2) {   // the panel's later columns
+            const int j = s0 + c2;
+            if (j >= s1) break;
+            S u = 0;
+#pragma unroll
+            for (int ta = 0; ta < TA; ++ta) if (ta == (j >> 6)) u = pc[c][ta];
+            u = lane_bcast(u, j & 63);
+#pragma unroll
+            for (int ta = 0; ta < TA; ++ta) {
+              const int i = lane + 64 * ta;
+              if (i >= j && i < n) pc[c2][ta] -= pc[c][ta] * u;
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+#pragma unroll
+          for (int ta = 0; ta < TA; ++ta) {
+            const int i = lane + 64 * ta;
+#pragma unroll
+            for (int g = 0; g < TB / NB; ++g) if (g == gp) Z[ta][g * NB + c] = pc[c][ta];
+            if (i < n) Lp[c * n + i] = (s0 + c < s1 && i > s0 + c) ? pc[c][ta] : (S)0;
+          }
+      }
+      __syncthreads();
+      S li[NB][TA];
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) {
+          const int i = lane + 64 * ta;
+          li[c][ta] = i < n ? Lp[c * n + i] : (S)0;
+        }
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb) {
+        const int j = colj(tb);
+        if (j < s1 || j >= n) continue;
+        S u[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) u[c] = Lp[c * n + j];
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) {
+          const int i = lane + 64 * ta;
+          if (j <= i || (i >= r && j >= r)) {
+            S z = Z[ta][tb];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) z -= li[c][ta] * u[c];
+            Z[ta][tb] = z;
+          }
+        }
+      }
+    }
+  }
+
+  // ---- LU: r elimination steps, one barrier each; column s is held by
+  // wave s mod 16, slot s / 16
   bool done[TA];                                 // LU: row already used as a pivot
 #pragma unroll
   for (int ta = 0; ta < TA; ++ta) done[ta] = false;
-  for (int s = 0; s < r; ++s) {
+  for (int s = 0; LU && s < r; ++s) {
     S* cb = colb + (s & 1) * n;
     const int ws = s & 15, bs = s >> 4;
     if (wid == ws) {
@@ -1582,7 +1684,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
     const int pa = pr >> 6, pl = pr & 63;
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb) {
-      const int j = wid + 16 * tb;
+      const int j = colj(tb);
       if (j <= s || j >= n) continue;
       S u;
       if (LU) {
@@ -1611,7 +1713,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb) {
-      const int i = lane + 64 * ta, j = wid + 16 * tb;
+      const int i = lane + 64 * ta, j = colj(tb);
       if (i >= n || j >= n) continue;
       const S z = Z[ta][tb];
       if (i < r) {
