@@ -452,6 +452,15 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   }
   int64_t pool_top = N;
   if (!ws.hflag) RSKX_HIP(hipHostMalloc((void**)&ws.hflag, 64 * sizeof(int), hipHostMallocDefault));
+  ws.hflag[0] = 0;
+  // elimination failures flagged on the device (read after a synchronization)
+  auto elim_status = [&]() -> int {
+    if (ws.hflag[0] & 1)
+      return fail(RSKX_ERR_NUM, "FLAM:rskx:chol: matrix is not positive definite (symm 'p')");
+    if (ws.hflag[0] & 2)
+      return fail(RSKX_ERR_NUM, "FLAM:rskx:lu: exactly singular redundant block");
+    return RSKX_OK;
+  };
   RSKX_TRY(ws.flag.ensure(64));
   RSKX_HIP(hipMemsetAsync(ws.flag.p, 0, 64, g_stream));
 
@@ -564,11 +573,25 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     prof.end("lists", ev);
     // ID: boxes whose K fits a register tile assemble and factor it in
     // registers (k_id_reg); the rest assemble K in global memory first
-    std::vector<int> hmn(nb);
-    RSKX_HIP(hipMemcpyAsync(hmn.data(), ws.mnbr.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipStreamSynchronize(g_stream));
-    // FP32 factorizations at tol >= 1e-7 take the Gram route (k_gram_id)
+    // FP32 factorizations at tol >= 1e-7 take the Gram route (k_gram_id),
+    // whose shared memory is sized by the unfiltered neighbour count, so
+    // the host need not wait for k_lists; the QR routes read back the
+    // filtered counts to choose register classes
     const bool gram = sizeof(S) == 4 && prm.tol >= 1e-7;
+    std::vector<int> hmn(nb);
+    bool exact = !gram;
+    for (int64_t ib = 0; ib < nb && !exact; ++ib) {
+      const int n = hb[ib].n, mc = hb[ib].mc;
+      if (n > 0 && n <= gram_id_nmax(1) &&
+          gram_id_shmem(n <= gram_id_nmax(0) ? 0 : 1, n, mc, mc * (two ? 2 : 1) + L.p, L.p) > 160 * 1024)
+        exact = true;                        // the bound is too loose: use the filtered counts
+    }
+    if (!exact) {
+      for (int64_t ib = 0; ib < nb; ++ib) hmn[ib] = hb[ib].mc;
+    } else {
+      RSKX_HIP(hipMemcpyAsync(hmn.data(), ws.mnbr.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
+      RSKX_HIP(hipStreamSynchronize(g_stream));
+    }
     std::vector<int> idg, idr[2], idq[2], idall;
     size_t shm_r[2] = {0, 0}, shm_q[2] = {0, 0};
     int nmax_g = 0, mcmax_g = 0, mcapmax_g = 0;
@@ -670,6 +693,8 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     RSKX_HIP(hipMemcpyAsync(hit.data(), ws.iter.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
     RSKX_HIP(hipMemcpyAsync(hm.data(), ws.mrows.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
     RSKX_HIP(hipStreamSynchronize(g_stream));
+    RSKX_TRY(elim_status());                  // the previous level's elimination
+    prof.flush();
     for (int64_t ib = 0; ib < nb; ++ib)
       if (hit[ib] < 0) {
         fprintf(stderr, "Warning: FLAM:id:maxIterCount: Maximum number of RRQR iterations reached.\n");
@@ -777,13 +802,9 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     }
     prof.end("factor", ev, 0, el_flops);
     RSKX_HIP(hipGetLastError());
+    // the failure flag comes back with the next synchronization; the host
+    // meanwhile lays out the next level
     RSKX_HIP(hipMemcpyAsync(ws.hflag, ws.flag.p, sizeof(int), hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipStreamSynchronize(g_stream));
-    if (ws.hflag[0] & 1)
-      return fail(RSKX_ERR_NUM, "FLAM:rskx:chol: matrix is not positive definite (symm 'p')");
-    if (ws.hflag[0] & 2)
-      return fail(RSKX_ERR_NUM, "FLAM:rskx:lu: exactly singular redundant block");
-    prof.flush();
 
     // ---- commit the level
     int64_t elim = 0;
@@ -845,6 +866,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   RSKX_TRY(upload(F->d_recs, F->recs));
   RSKX_HIP(hipGetLastError());
   RSKX_HIP(hipStreamSynchronize(g_stream));
+  RSKX_TRY(elim_status());
   prof.flush();
   return RSKX_OK;
 }
