@@ -761,12 +761,14 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     // per CU, or neither (global scratch; also the no-compression copies)
     const size_t kHalf = 78 * 1024, kFull = 158 * 1024;   // 1024-thread workgroups
     std::vector<int> cls[3], creg;
+    std::vector<char> regbox(nb, 0);           // k_elim_reg also writes the transposed copies
     size_t shm_cls[3] = {0, 0, 0}, shm_reg = 0;
     const int nreg = elim_reg_nmax(sizeof(S));
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
       if (O.r > 0 && hb[ib].n <= nreg) {     // augmented matrix in registers
         creg.push_back((int)ib);
+        regbox[ib] = 1;
         shm_reg = std::max(shm_reg, elim_reg_shmem(hb[ib].n, O.k, O.r, sizeof(S)));
         continue;
       }
@@ -820,6 +822,8 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
         fr.T = O.T; fr.LU = O.LU; fr.E = O.E; fr.G = O.G; fr.Li = O.Li; fr.Ui = O.Ui;
         fr.sk = O.sk; fr.rd = O.rd; fr.piv = O.piv;
         fr.Tt = O.Tt; fr.Et = O.Et; fr.Gt = O.Gt; fr.LUt = O.LUt; fr.Lti = O.Lti; fr.Uti = O.Uti;
+        fr.packed = regbox[ib];
+        fr.pad_ = 0;
         F->recs.push_back(fr);
         elim += O.r;
       } else {
@@ -832,11 +836,14 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     // out the next level
     {
       const int64_t f0 = F->lvp.back(), nl = (int64_t)F->recs.size() - f0;
-      if (nl > 0) {
+      bool any = false;
+      for (int64_t f = f0; f < f0 + nl; ++f) any = any || !F->recs[f].packed;
+      if (nl > 0 && any) {
         lrec.assign(F->recs.begin() + f0, F->recs.end());   // kept until the next sync
         RSKX_TRY(upload(ws.lrecs, lrec));
         int64_t bytes = 0;
         for (const FacRec& fr : lrec) {
+          if (fr.packed) continue;
           const int64_t kr = (int64_t)fr.k * fr.r, nbk = (fr.r + kDiagBlock - 1) / kDiagBlock;
           bytes += 2 * (int64_t)sizeof(S) * ((lu ? 3 : 2) * kr + (int64_t)fr.r * fr.r +
                                              (lu ? 2 : 1) * nbk * kDiagBlock * kDiagBlock);

@@ -84,6 +84,7 @@ struct BoxOut {
 // per-factor record used by the sweeps (values in the factor arena)
 struct FacRec {
   int k, r;
+  int packed, pad_;       // transposed copies already written by the elimination
   int64_t T, LU, E, G;    // T k x r, LU r x r packed, E k x r, G r x k
   int64_t Li, Ui;         // ceil(r/32) inverted diagonal blocks of L (and U), 32 x 32 each
   int64_t sk, rd, piv;    // index arena offsets (tree order point indices)

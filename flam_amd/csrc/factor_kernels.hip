@@ -1725,6 +1725,48 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
         else Mo[(i - r) + (j - r) * k] = z;
       }
     }
+  // transposed copies for the solve sweeps (FacRec::LUt, Et, Gt, Tt), a
+  // few register slots (16 columns each) at a time through LDS so that the
+  // global writes run along rows
+  {
+    S* fLUt = fval + O.LUt;
+    S* fEt = fval + O.Et;
+    S* fGt = fval + O.Gt;
+    constexpr int SPR = TB >= 16 ? 2 : 1;        // slots per round: 16 SPR columns
+    static_assert(TB % SPR == 0, "slot rounds");
+    S* sb = colb;                                // 16 SPR x n staging (colb holds 2 TB x n)
+#pragma unroll
+    for (int rd = 0; rd < TB / SPR; ++rd) {
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < SPR; ++u)
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) {
+          const int i = lane + 64 * ta;
+          if (i < n) sb[(SPR * wid + u) * n + i] = Z[ta][SPR * rd + u];
+        }
+      __syncthreads();
+      for (int e = tid; e < 16 * SPR * n; e += 1024) {
+        const int cs = e % (16 * SPR), i = e / (16 * SPR);
+        const int w = cs / SPR, tb = SPR * rd + cs % SPR;
+        const int j = ((tb / NB) * 16 + w) * NB + tb % NB;
+        if (j >= n) continue;
+        const S v = sb[cs * n + i];
+        if (i < r) {
+          const int li = LU ? lpos[i] : i;
+          if (j < r) fLUt[j + li * r] = v;
+          else if (LU) fGt[(j - r) + li * k] = v;
+        } else if (j < r) {
+          fEt[j + (i - r) * r] = v;
+        }
+      }
+    }
+    S* fTt = fval + O.Tt;
+    for (int e = tid; e < k * r; e += 1024) {
+      const int j = e % r, a = e / r;
+      fTt[e] = Ts[a + j * k];
+    }
+  }
   int* piv = fidx + O.piv;
   if (LU)
     for (int p = tid; p < r; p += 1024) piv[p] = prow[p];
@@ -1791,8 +1833,13 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
       }
       if (c < kDiagBlock) {
         S* dst = fval + (upper ? O.Ui : O.Li) + (int64_t)b * kDiagBlock * kDiagBlock;
+        S* dstt = fval + (upper ? O.Uti : O.Lti) + (int64_t)b * kDiagBlock * kDiagBlock;
 #pragma unroll
-        for (int t = 0; t < kDiagBlock; ++t) dst[t + c * kDiagBlock] = (t < bs && c < bs) ? x[t] : (S)0;
+        for (int t = 0; t < kDiagBlock; ++t) {
+          const S xv = (t < bs && c < bs) ? x[t] : (S)0;
+          dst[t + c * kDiagBlock] = xv;
+          dstt[c + t * kDiagBlock] = xv;
+        }
       }
     }
   }

@@ -375,6 +375,7 @@ template <typename S, bool LU>
 __global__ void __launch_bounds__(256) k_pack(const FacRec* __restrict__ recs, S* fval) {
   __shared__ S tile[32][33];
   const FacRec f = recs[blockIdx.x];
+  if (f.packed) return;
   const int k = f.k, r = f.r, nbk = (r + kDiagBlock - 1) / kDiagBlock;
   transpose_into(fval + f.Tt, fval + f.T, k, r, tile);
   transpose_into(fval + f.Et, fval + f.E, k, r, tile);
