@@ -38,7 +38,9 @@ void launch_gram_id(int, const int*, const LvlBox*, const int*, const int*, cons
                     double*, int*, const PointsDev&, const LevelParams&, const IdParams&, int*, int,
                     size_t, hipStream_t);
 int gram_id_nmax(int cls);
+int gram_id_class(int n);
 size_t gram_id_shmem(int cls, int n, int mn, int m, int p);
+constexpr int kGramClasses = 3;
 template <typename S>
 void launch_id_reg(int, const int*, const LvlBox*, const int*, const int*, const int*, int*, S*,
                    int*, const PointsDev&, const LevelParams&, const IdParams&, int*, int, size_t,
@@ -582,8 +584,8 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     bool exact = !gram;
     for (int64_t ib = 0; ib < nb && !exact; ++ib) {
       const int n = hb[ib].n, mc = hb[ib].mc;
-      if (n > 0 && n <= gram_id_nmax(1) &&
-          gram_id_shmem(n <= gram_id_nmax(0) ? 0 : 1, n, mc, mc * (two ? 2 : 1) + L.p, L.p) > 160 * 1024)
+      if (n > 0 && gram_id_class(n) >= 0 &&
+          gram_id_shmem(gram_id_class(n), n, mc, mc * (two ? 2 : 1) + L.p, L.p) > 160 * 1024)
         exact = true;                        // the bound is too loose: use the filtered counts
     }
     if (!exact) {
@@ -592,13 +594,13 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
       RSKX_HIP(hipMemcpyAsync(hmn.data(), ws.mnbr.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
       RSKX_HIP(hipStreamSynchronize(g_stream));
     }
-    std::vector<int> idg, idr[2], idq[2], idall;
-    size_t shm_r[2] = {0, 0}, shm_q[2] = {0, 0};
+    std::vector<int> idg, idr[2], idq[kGramClasses], idall;
+    size_t shm_r[2] = {0, 0}, shm_q[kGramClasses] = {0, 0, 0};
     int nmax_g = 0, mcmax_g = 0, mcapmax_g = 0;
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
-      if (gram && n > 0 && mb > 0 && n <= gram_id_nmax(1)) {
-        const int c = n <= gram_id_nmax(0) ? 0 : 1;
+      if (gram && n > 0 && mb > 0 && gram_id_class(n) >= 0) {
+        const int c = gram_id_class(n);
         idq[c].push_back((int)ib);
         shm_q[c] = std::max(shm_q[c], gram_id_shmem(c, n, hmn[ib], mb, L.p));
         continue;
@@ -626,14 +628,14 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     idall = idg;
     idall.insert(idall.end(), idr[0].begin(), idr[0].end());
     idall.insert(idall.end(), idr[1].begin(), idr[1].end());
-    idall.insert(idall.end(), idq[0].begin(), idq[0].end());
-    idall.insert(idall.end(), idq[1].begin(), idq[1].end());
+    for (int c = kGramClasses - 1; c >= 0; --c) idall.insert(idall.end(), idq[c].begin(), idq[c].end());
     RSKX_TRY(upload(ws.ids, idall));
     const int* dids = ws.ids.as<int>();
     const int ng = (int)idg.size(), nr0 = (int)idr[0].size(), nr1 = (int)idr[1].size();
-    const int nq0 = (int)idq[0].size(), nq1 = (int)idq[1].size();
-    if (shm_r[0] > 160 * 1024 || shm_r[1] > 160 * 1024 || shm_q[0] > 160 * 1024 ||
-        shm_q[1] > 160 * 1024)
+    int nq = 0;
+    for (int c = 0; c < kGramClasses; ++c) nq += (int)idq[c].size();
+    if (shm_r[0] > 160 * 1024 || shm_r[1] > 160 * 1024 ||
+        *std::max_element(shm_q, shm_q + kGramClasses) > 160 * 1024)
       return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box neighbourhood too large for the ID kernels");
     if ((size_t)(nmax_g + mcmax_g) * 5 * sizeof(double) > 160 * 1024)
       return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box neighbourhood too large for the assembly kernel");
@@ -670,19 +672,23 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
                    nr0 + nr1, (size_t)(3 * nmx + 32) * sizeof(double), g_stream);
     }
     const int* qids = dids + ng + nr0 + nr1;
-    for (int c = 0; c < 2; ++c) {
-      const int cnt = c == 0 ? nq0 : nq1;
-      if (!cnt) continue;
-      launch_gram_id(c, qids + (c == 0 ? 0 : nq0), ws.boxes.as<LvlBox>(), ws.pool.as<int>(),
-                     ws.nbr.as<int>(), ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.T.as<double>(),
-                     ws.perm.as<int>(), P, L, prm, ws.kout.as<int>(), cnt, shm_q[c], g_stream);
+    {
+      int off = 0;
+      for (int c = kGramClasses - 1; c >= 0; --c) {     // largest boxes first
+        const int cnt = (int)idq[c].size();
+        if (!cnt) continue;
+        launch_gram_id(c, qids + off, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
+                       ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.T.as<double>(), ws.perm.as<int>(),
+                       P, L, prm, ws.kout.as<int>(), cnt, shm_q[c], g_stream);
+        off += cnt;
+      }
     }
-    if (nq0 + nq1) {
+    if (nq) {
       int nmx = 0;
-      for (int c = 0; c < 2; ++c) for (int ib : idq[c]) nmx = std::max(nmx, hb[ib].n);
+      for (int c = 0; c < kGramClasses; ++c) for (int ib : idq[c]) nmx = std::max(nmx, hb[ib].n);
       launch_id<S>(2, qids, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
                    ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
-                   nq0 + nq1, (size_t)(3 * nmx + 32) * sizeof(double), g_stream);
+                   nq, (size_t)(3 * nmx + 32) * sizeof(double), g_stream);
     }
     prof.end("id", ev);
     RSKX_HIP(hipGetLastError());
@@ -712,7 +718,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
       O.k = hk[ib];
       O.r = B.n - O.k;
       const int64_t k = O.k, r = O.r, m = hm[ib], n = B.n;
-      if (gram && n > 0 && m > 0 && n <= gram_id_nmax(1))   // G = K'K, pivoted Cholesky, T
+      if (gram && n > 0 && m > 0 && gram_id_class((int)n) >= 0)   // G = K'K, pivoted Cholesky, T
         id_flops += (double)m * n * (n + 1) + 2.0 * (double)n * n * k + (double)k * k * r;
       else                                                  // truncated QRCP, T
         id_flops += 4.0 * m * n * k - 2.0 * k * k * (m + n) + 4.0 / 3.0 * k * k * k + (double)k * k * r;

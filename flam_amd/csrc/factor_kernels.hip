@@ -1256,13 +1256,18 @@ void launch_gram_id(int cls, const int* ids, const LvlBox* boxes, const int* poo
     hipLaunchKernelGGL(kern, dim3(nb), dim3(NTV), shmem, st, ids, boxes, pool, nbrar, mnbr,      \
                        mrows, Tar, permar, P, L, prm, kout);                                     \
   }
-  if (cls == 0) RSKX_GID(256, 2) else RSKX_GID(512, 5)
+  if (cls == 0) RSKX_GID(256, 2) else if (cls == 1) RSKX_GID(512, 2) else RSKX_GID(512, 5)
 #undef RSKX_GID
 }
 
 // k_gram_id classes: {threads, 4 x 4 tiles per thread}; the largest n each
 // holds (n4 (n4 + 1) / 2 <= threads x tiles, n4 = ceil(n / 4))
-int gram_id_nmax(int cls) { return cls == 0 ? 124 : 284; }
+int gram_id_nmax(int cls) { return cls == 0 ? 124 : cls == 1 ? 176 : 284; }
+int gram_id_class(int n) {
+  for (int c = 0; c < 3; ++c)
+    if (n <= gram_id_nmax(c)) return c;
+  return -1;
+}
 size_t gram_id_shmem(int cls, int n, int mn, int m, int p) {
   const int np = (n + 3) / 4 * 4, nw = (cls == 0 ? 256 : 512) / 64;
   return (size_t)(16 * np + 2 * p + 5 * n + 5 * mn + 2 * np + 2 * nw) * sizeof(double) +
