@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--target", type=float, default=1e-10)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-N", type=int, default=1 << 17)
+    ap.add_argument("--cpu-N", type=int, default=1 << 20)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a known-size "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE counters)")
@@ -74,6 +74,20 @@ def run_path(flam_amd, g, B, args, precision, steps, warmup, log):
     return times, infos, F
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel class from the calibrated rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary kept under profiles/ (None if absent)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md"))):
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_json:"):
+                    rec = json.loads(line.split(":", 1)[1])
+                    if rec.get("kernel") == kernel:
+                        return rec["bytes_per_launch"], os.path.basename(path)
+    return None, None
+
+
 def roofline_of(F, precision):
     """Dominant kernel class by time, with algorithmic flops / bytes."""
     kt = F.kernel_times()
@@ -82,13 +96,15 @@ def roofline_of(F, precision):
     if st["flops"] > 0:
         ach = st["flops"] / st["launches"] / (avg_ms * 1e-3)
         peak = PEAK["fp64"] if name == "id" or precision == "fp64" else PEAK["fp32"]
+        traffic, src = measured_traffic(name) if precision == "fp32" else (None, None)
         return dict(kernel=name, bound="mfma", achieved=ach / 1e12, peak=peak / 1e12,
-                    unit="TFLOP/s", frac=ach / peak, traffic=None,
+                    unit="TFLOP/s", frac=ach / peak, traffic=traffic, traffic_source=src,
                     avg_launch_ms=avg_ms, launches=st["launches"]), kt
     ach = st["bytes"] / st["launches"] / (avg_ms * 1e-3)
+    traffic, src = measured_traffic(name) if precision == "fp32" else (None, None)
     return dict(kernel=name, bound="hbm", achieved=ach / 1e9, peak=PEAK["hbm"] / 1e9,
-                unit="GB/s", frac=ach / PEAK["hbm"], traffic=None, avg_launch_ms=avg_ms,
-                launches=st["launches"]), kt
+                unit="GB/s", frac=ach / PEAK["hbm"], traffic=traffic, traffic_source=src,
+                avg_launch_ms=avg_ms, launches=st["launches"]), kt
 
 
 def cpu_baseline(N, occ, tol, nrhs, target, log):
@@ -141,13 +157,17 @@ def main():
         print(msg, file=sys.stderr, flush=True)
 
     if args.calibrate_copy:
+        # a device-to-device copy of known size; 2 GiB written in between
+        # evicts the source from the last-level caches first
         src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
+        junk = torch.empty(1 << 29, dtype=torch.float32, device="cuda")
+        junk.fill_(1.0)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
         log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
-        del src, dst
+        del src, dst, junk
     N = args.N
     log(f"workload: circle N={N} occ={args.occ} tol={args.tol:g} nrhs={args.nrhs} "
         f"target {args.target:g}")
