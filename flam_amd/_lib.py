@@ -38,7 +38,7 @@ class Proxy(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("lvlmax", c_int), ("ext", POINTER(c_double)), ("Tmax", c_double),
                 ("rrqr_iter", c_double), ("symm", c_char), ("verb", c_int),
-                ("precision", c_int), ("profile", c_int)]
+                ("precision", c_int), ("profile", c_int), ("stop", c_int)]
 
 
 class Info(ctypes.Structure):
@@ -61,6 +61,10 @@ SIGNATURES = {
                             POINTER(Proxy), POINTER(Opts), POINTER(c_void_p)]),
     "rskx_rskelf_sv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
     "rskx_rskelf_mv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
+    "rskx_sweep_sv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int, c_int]),
+    "rskx_sweep_mv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int, c_int]),
+    "rskx_partial_info": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64),
+                                  POINTER(c_int64), POINTER(c_int64), POINTER(c_double)]),
     "rskx_free": (None, [c_void_p]),
     "rskx_last_error": (c_char_p, []),
     "rskx_set_stream": (c_int, [c_void_p]),

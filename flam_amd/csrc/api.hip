@@ -247,6 +247,10 @@ struct rskx_factor {
   std::vector<std::array<int64_t, 4>> table;
   Prof prof;
   int64_t fval_used = 0, fidx_used = 0;   // totals over the levels
+  // partial factorization (rskelf.m:279-308): remaining points (ascending
+  // original indices) and the sparse modification of their system
+  std::vector<int64_t> Si, SI, SJ;
+  std::vector<double> SV;
   int level_of(int64_t f) const {         // processed level holding factor f
     return (int)(std::upper_bound(lvp.begin(), lvp.end(), f) - lvp.begin()) - 1;
   }
@@ -402,6 +406,59 @@ struct BoxState {
 
 constexpr int kBlock = 256;
 
+// Remaining points and skeleton system of a partial factorization
+// (rskelf.m:279-308): the points still listed by the boxes of the last
+// factored level and by the leaves above it, and the M blocks of the last
+// level's boxes placed at those points' positions.  Host side, once.
+template <typename S>
+int partial_assembly(const HostTree& t, const std::vector<BoxState>& bs, int lv_last,
+                     const DevBuf& Marena, rskx_factor* F) {
+  Workspace& ws = g_ws;
+  const int64_t N = F->N;
+  std::vector<int> ord(N), pool_h;
+  RSKX_HIP(hipMemcpy(ord.data(), F->order.p, N * sizeof(int), hipMemcpyDeviceToHost));
+  int64_t pool_n = 0;
+  std::vector<int> boxes;                  // boxes holding remaining points
+  for (int b = 0; b < t.nbox(); ++b) {
+    const int lvb = t.level_of(b);
+    const bool frontier = lvb == lv_last || (lvb < lv_last && t.leaf(b));
+    if (frontier && bs[b].cur_len > 0) {
+      boxes.push_back(b);
+      pool_n = std::max<int64_t>(pool_n, bs[b].cur_off + bs[b].cur_len);
+    }
+  }
+  pool_h.resize(pool_n);
+  if (pool_n)
+    RSKX_HIP(hipMemcpy(pool_h.data(), ws.pool.p, pool_n * sizeof(int), hipMemcpyDeviceToHost));
+  F->Si.clear();
+  for (int b : boxes)
+    for (int q = 0; q < bs[b].cur_len; ++q) F->Si.push_back(ord[pool_h[bs[b].cur_off + q]]);
+  std::sort(F->Si.begin(), F->Si.end());
+  std::vector<int64_t> pos(N, -1);
+  for (size_t q = 0; q < F->Si.size(); ++q) pos[F->Si[q]] = (int64_t)q;
+  // M blocks of the last factored level
+  int64_t mmax = 0;
+  for (int b : boxes)
+    if (t.level_of(b) == lv_last && bs[b].moff >= 0)
+      mmax = std::max<int64_t>(mmax, bs[b].moff + (int64_t)bs[b].cur_len * bs[b].cur_len);
+  std::vector<S> Mh(mmax);
+  if (mmax) RSKX_HIP(hipMemcpy(Mh.data(), Marena.p, mmax * sizeof(S), hipMemcpyDeviceToHost));
+  F->SI.clear(); F->SJ.clear(); F->SV.clear();
+  for (int b : boxes) {
+    if (t.level_of(b) != lv_last || bs[b].moff < 0) continue;
+    const int nb = bs[b].cur_len;
+    for (int j = 0; j < nb; ++j)
+      for (int i = 0; i < nb; ++i) {
+        const double v = (double)Mh[bs[b].moff + i + (int64_t)j * nb];
+        if (v == 0) continue;                // sparse() keeps no zeros
+        F->SI.push_back(pos[ord[pool_h[bs[b].cur_off + i]]]);
+        F->SJ.push_back(pos[ord[pool_h[bs[b].cur_off + j]]]);
+        F->SV.push_back(v);
+      }
+  }
+  return RSKX_OK;
+}
+
 template <typename S>
 int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
                 const rskx_kernel& A, const rskx_proxy& PX, const rskx_opts& o, rskx_factor* F) {
@@ -490,7 +547,11 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   std::vector<Seg> hn, hc;
   std::vector<BoxOut> ho;
   std::vector<int> hk, hit;
+  int lv_last = t.nlvl;           // last level factored
   for (int lv = t.nlvl - 1; lv >= 0; --lv) {
+    // rskelf.m:172: opts.stop(t.nlvl - lvl, l), numeric stop = levels done
+    if (o.stop >= 0 && F->nlvl_done >= o.stop) break;
+    lv_last = lv;
     auto tl = clk::now();
     const int lvl_ref = lv + 1;   // the reference's level number
     const int64_t b0 = t.lvp[lv], nb = t.lvp[lv + 1] - b0;
@@ -876,6 +937,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
              std::chrono::duration<double>(clk::now() - tl).count());
   }
   if (o.verb) printf("%s\n", std::string(69, '-').c_str());
+  if (nrem > 0) RSKX_TRY(partial_assembly<S>(t, bs, lv_last, ws.M[mpar], F));
   RSKX_TRY(upload(F->d_recs, F->recs));
   RSKX_HIP(hipGetLastError());
   RSKX_HIP(hipStreamSynchronize(g_stream));
@@ -885,7 +947,8 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
 }
 
 template <typename S, typename U>
-int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans, bool solve) {
+int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans, bool solve,
+               int mode) {
   rskx_factor* F = const_cast<rskx_factor*>(Fc);
   const int64_t N = F->N;
   if (nrhs <= 0 || N == 0) return RSKX_OK;
@@ -906,6 +969,7 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
   // rskelf_sv.m:25-35 / sv/*.m: mode 3 = upward then downward sweep
   for (int pass = 0; pass < 2; ++pass) {
     const bool up = pass == 0;
+    if (!(mode & (up ? 1 : 2))) continue;     // mode 1: upward only, 2: downward only
     for (int q = 0; q < nl; ++q) {
       const int lv = up ? q : nl - 1 - q;
       const int64_t f0 = F->lvp[lv], f1 = F->lvp[lv + 1];
@@ -974,6 +1038,7 @@ int rskx_opts_default(rskx_opts* o) {
   o->verb = 0;
   o->precision = RSKX_PREC_FP64;
   o->profile = 0;
+  o->stop = -1;
   return RSKX_OK;
 }
 
@@ -1032,8 +1097,9 @@ int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
 }
 
 static int sweep_entry(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
-                       bool solve) {
+                       bool solve, int mode = 3) {
   if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
+  if (mode < 1 || mode > 3) return fail(RSKX_ERR_ARG, "FLAM:rskx:mode: sweep mode must be 1, 2 or 3");
   char t = (char)tolower(trans);
   if (t != 'n' && t != 't' && t != 'c')
     return fail(RSKX_ERR_ARG, "FLAM:chktrans:invalidTrans: Transpose parameter must be one of 'N', 'T', or 'C'.");
@@ -1042,10 +1108,32 @@ static int sweep_entry(const rskx_factor* F, void* X, int64_t ldx, int nrhs, cha
   if (nrhs > 0 && !X) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: X is NULL");
   if (dtype != RSKX_F64 && dtype != RSKX_F32) return fail(RSKX_ERR_ARG, "FLAM:rskx:dtype: unknown element type");
   if (F->prec == RSKX_PREC_FP64)
-    return dtype == RSKX_F64 ? sweep_impl<double, double>(F, X, ldx, nrhs, t, solve)
-                             : sweep_impl<double, float>(F, X, ldx, nrhs, t, solve);
-  return dtype == RSKX_F64 ? sweep_impl<float, double>(F, X, ldx, nrhs, t, solve)
-                           : sweep_impl<float, float>(F, X, ldx, nrhs, t, solve);
+    return dtype == RSKX_F64 ? sweep_impl<double, double>(F, X, ldx, nrhs, t, solve, mode)
+                             : sweep_impl<double, float>(F, X, ldx, nrhs, t, solve, mode);
+  return dtype == RSKX_F64 ? sweep_impl<float, double>(F, X, ldx, nrhs, t, solve, mode)
+                           : sweep_impl<float, float>(F, X, ldx, nrhs, t, solve, mode);
+}
+
+int rskx_sweep_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
+                  int mode) {
+  return sweep_entry(F, X, ldx, nrhs, trans, dtype, true, mode);
+}
+
+int rskx_sweep_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
+                  int mode) {
+  return sweep_entry(F, X, ldx, nrhs, trans, dtype, false, mode);
+}
+
+int rskx_partial_info(const rskx_factor* F, int64_t* nsk, int64_t* nnz, int64_t* sk, int64_t* I,
+                      int64_t* J, double* V) {
+  if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
+  if (nsk) *nsk = (int64_t)F->Si.size();
+  if (nnz) *nnz = (int64_t)F->SV.size();
+  if (sk) std::copy(F->Si.begin(), F->Si.end(), sk);
+  if (I) std::copy(F->SI.begin(), F->SI.end(), I);
+  if (J) std::copy(F->SJ.begin(), F->SJ.end(), J);
+  if (V) std::copy(F->SV.begin(), F->SV.end(), V);
+  return RSKX_OK;
 }
 
 int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype) {

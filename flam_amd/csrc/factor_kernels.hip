@@ -1583,15 +1583,15 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
         S u[NB];
 #pragma unroll
         for (int c = 0; c < NB; ++c) u[c] = Lp[c * n + j];
+        // entries above the diagonal of Krr, the Krs' block and padding
+        // rows are never read again: update without masks
 #pragma unroll
         for (int ta = 0; ta < TA; ++ta) {
-          const int i = lane + 64 * ta;
-          if (j <= i || (i >= r && j >= r)) {
-            S z = Z[ta][tb];
+          if (64 * ta >= n) continue;
+          S z = Z[ta][tb];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) z -= li[c][ta] * u[c];
-            Z[ta][tb] = z;
-          }
+          for (int c = 0; c < NB; ++c) z -= li[c][ta] * u[c];
+          Z[ta][tb] = z;
         }
       }
     }

@@ -158,7 +158,8 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     """F = rskelf(A, x, occ, rank_or_tol, pxyfun, opts) on the GPU.
 
     opts keys (rskelf.m:112-120): lvlmax (inf), ext (None), Tmax (2),
-    rrqr_iter (inf), symm ('n'), verb (0); plus precision ('fp64' or
+    rrqr_iter (inf), symm ('n'), verb (0), stop (inf: a numeric stop
+    factors that many levels, rskelf.m:128,172); plus precision ('fp64' or
     'fp32') and profile (0)."""
     opts = dict(opts or {})
     x = _dev(x)
@@ -186,8 +187,10 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
         raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:precision: precision must be 'fp64' or 'fp32'")
     o.precision = _PREC[prec]
     o.profile = int(opts.get("profile", 0))
-    if "stop" in opts and opts["stop"] not in (None, math.inf):
-        raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:stop: partial factorization is not built on the device")
+    stop = opts.get("stop", math.inf)
+    if callable(stop):
+        raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:stop: only a numeric opts.stop (levels) is built")
+    o.stop = -1 if stop is None or stop == math.inf else int(stop)
     k = A._struct()
     p = pxyfun._struct() if pxyfun is not None else L.Proxy(L.RSKX_PROXY_NONE, 0, 1.5, 0.0)
     h = ctypes.c_void_p()
@@ -216,6 +219,72 @@ def _sweep(fn, F, X, trans):
     _check(fn(F.handle, Xc.data_ptr(), F.N, Xc.shape[0], t, dt))
     Y = Xc.t()
     return Y.reshape(-1) if vec else Y
+
+
+def _sweep_mode(fn, F, X, trans, mode):
+    """One sweep (mode 1 upward, 2 downward) on a fresh column-major copy."""
+    X2 = X if X.dim() == 2 else X.reshape(-1, 1)
+    Xc = X2.t().clone(memory_format=torch.contiguous_format)
+    dt = L.RSKX_F64 if Xc.dtype == torch.float64 else L.RSKX_F32
+    torch.cuda.synchronize()
+    _check(fn(F.handle, Xc.data_ptr(), F.N, Xc.shape[0], str(trans).encode()[:1], dt, mode))
+    return Xc.t()
+
+
+def rskelf_partial_info(F):
+    """[sk, S] = rskelf_partial_info(F) (rskelf/rskelf_partial_info.m):
+    the points left by a partial factorization (ascending, 0-based, a CPU
+    int64 tensor) and the sparse modification S (CPU, FP64, COO) of their
+    system, which is A(sk, sk) + S."""
+    nsk, nnz = ctypes.c_int64(), ctypes.c_int64()
+    _check(L.lib().rskx_partial_info(F.handle, ctypes.byref(nsk), ctypes.byref(nnz),
+                                     None, None, None, None))
+    sk = torch.empty(nsk.value, dtype=torch.int64)
+    I = torch.empty(nnz.value, dtype=torch.int64)
+    J = torch.empty(nnz.value, dtype=torch.int64)
+    V = torch.empty(nnz.value, dtype=torch.float64)
+    p64 = lambda t: ctypes.cast(t.data_ptr(), ctypes.POINTER(ctypes.c_int64))
+    _check(L.lib().rskx_partial_info(F.handle, None, None, p64(sk), p64(I), p64(J),
+                                     ctypes.cast(V.data_ptr(), ctypes.POINTER(ctypes.c_double))))
+    S = torch.sparse_coo_tensor(torch.stack([I, J]), V, (nsk.value, nsk.value))
+    return sk, S
+
+
+def _partial(fn, F, X, fun, trans):
+    X = X if isinstance(X, torch.Tensor) else torch.as_tensor(X)
+    if not X.is_cuda:
+        X = X.to("cuda")
+    if X.dtype not in (torch.float64, torch.float32):
+        X = X.to(torch.float64)
+    vec = X.dim() == 1
+    X2 = X.reshape(-1, 1) if vec else X
+    if X2.shape[0] != F.N:
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: X must have N rows")
+    t = str(trans).lower()
+    if t not in ("n", "t", "c"):
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:chktrans:invalidTrans: Transpose parameter must be one of 'N', 'T', or 'C'.")
+    if t == "t":                        # real data: transpose = conjugate transpose
+        t = "c"
+    sk, _ = rskelf_partial_info(F)
+    Y = _sweep_mode(fn, F, X2, t, 1)
+    if sk.numel():
+        skd = sk.to(Y.device)
+        Y = Y.clone()
+        Y[skd] = fun(Y[skd], t).to(Y.dtype)
+    Y = _sweep_mode(fn, F, Y, t, 2)
+    return Y.reshape(-1) if vec else Y
+
+
+def rskelf_partial_sv(F, X, svfun, trans="n"):
+    """rskelf/rskelf_partial_sv.m:35-61: upward sweep, svfun on the remaining
+    skeletons (svfun(Y, trans) = (A(sk,sk) + S)^{-1} Y, or its transpose),
+    downward sweep."""
+    return _partial(L.lib().rskx_sweep_sv, F, X, svfun, trans)
+
+
+def rskelf_partial_mv(F, X, mvfun, trans="n"):
+    """rskelf/rskelf_partial_mv.m:35-61, with mvfun applying A(sk,sk) + S."""
+    return _partial(L.lib().rskx_sweep_mv, F, X, mvfun, trans)
 
 
 def rskelf_sv(F, X, trans="n"):

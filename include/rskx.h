@@ -80,6 +80,8 @@ typedef struct {
   int verb;             /* 1: print the per-level table to stdout           */
   int precision;        /* RSKX_PREC_*                                      */
   int profile;          /* 1: time each kernel class with HIP events        */
+  int stop;             /* factor at most this many levels, leaves first
+                           (rskelf.m:128,172: numeric opts.stop); < 0: all   */
 } rskx_opts;
 
 typedef struct rskx_factor rskx_factor;
@@ -124,6 +126,25 @@ int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char tr
 /* Y = rskelf_mv(F,X,trans)                      -- rskelf/rskelf_mv.m:13 */
 int rskx_rskelf_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
                    int dtype);
+
+/* The upward (mode 1) or downward (mode 2) sweep alone, or both (mode 3),
+ * of rskelf_sv / rskelf_mv: the sweeps' mode argument
+ * (rskelf/sv/rskelf_sv_nn.m:3, rskelf/mv/rskelf_mv_nn.m:3), as
+ * rskelf_partial_sv.m:35-61 and rskelf_partial_mv.m:35-61 call them around
+ * the remaining skeleton system of a partial factorization. */
+int rskx_sweep_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
+                  int mode);
+int rskx_sweep_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
+                  int mode);
+
+/* [sk, S] = rskelf_partial_info(F)              -- rskelf/rskelf_partial_info.m:14
+ * The points left by a partial factorization (opts.stop) and the sparse
+ * modification S of their system (rskelf.m:279-308): the remaining matrix
+ * is A(sk, sk) + S.  Call with NULL arrays to get nsk and nnz, then with
+ * host arrays of nsk (sk, ascending original indices, 0-based) and nnz
+ * (I, J: 0-based positions in sk; V) elements. */
+int rskx_partial_info(const rskx_factor* F, int64_t* nsk, int64_t* nnz, int64_t* sk, int64_t* I,
+                      int64_t* J, double* V);
 
 void rskx_free(rskx_factor* F);
 const char* rskx_last_error(void);

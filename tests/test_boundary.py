@@ -47,6 +47,7 @@ def test_opts_defaults_match_reference():
     # rskelf.m:112-120
     assert o.lvlmax == 0 and not o.ext and o.Tmax == 2.0 and math.isinf(o.rrqr_iter)
     assert o.symm == b"n" and o.verb == 0 and o.precision == L.RSKX_PREC_FP64
+    assert o.stop < 0                     # opts.stop = Inf
 
 
 def _call(L, *, symm=b"n", occ=64, tol=1e-6, Tmax=2.0, rrqr=math.inf, lvlmax=0, d=2,

@@ -224,3 +224,59 @@ def test_sweeps_leave_input_untouched():
         assert torch.equal(b, keep)
         assert x.shape == b.shape and y.shape == b.shape
         assert not torch.equal(x, b)
+
+
+@pytest.mark.parametrize("shape,symm", [("circle", "p"), ("ellipse", "n")])
+def test_partial_factorization(shape, symm):
+    """opts.stop (rskelf.m:128,172,279-308) and rskelf_partial_info/_sv/_mv:
+    as many points remain as in the oracle (within the rank tolerance of
+    the tables above); when the remaining sets coincide the skeleton system
+    S agrees to the accuracy the interpolation matrices allow (the skeleton
+    blocks R11 have condition up to ~1/tol, so T, and through it S, is
+    reproduced only to ~1e-16/tol); and the partial sweeps around a dense
+    solve (product) of A(sk,sk) + S reproduce A^{-1} b (A b) to ~tol."""
+    flam_amd, geometry = _dev()
+    N, occ, tol, stop = 2048, 32, 1e-10, 3
+    print(f"partial {shape} {symm} N={N} stop={stop}", flush=True)
+    if shape == "circle":
+        g, pr = geometry.circle(N), opr.CurveDL(N, "circle")
+    else:
+        g, pr = geometry.ellipse(N, a=1.0, b=0.6), opr.CurveDL(N, "ellipse", a=1.0, b=0.6)
+    F = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, dict(symm=symm, stop=stop))
+    Fo = ofl.rskelf(pr.afun, pr.x, occ, tol, pr.pxyfun, dict(symm=symm, stop=stop, order="level"))
+    assert F.info()["nlvl_done"] == stop
+    sk, S = flam_amd.rskelf_partial_info(F)
+    Sd = S.to_dense().numpy()
+    assert abs(sk.numel() - len(Fo["Si"])) <= max(2, 0.02 * len(Fo["Si"]))
+    assert np.all(np.diff(sk.numpy()) > 0)
+    if np.array_equal(sk.numpy(), np.asarray(Fo["Si"])):
+        So = Fo["S"].toarray()
+        assert np.abs(Sd - So).max() <= 1e-2 * np.abs(So).max()
+    skn = sk.numpy()
+    Ask = pr.afun(skn, skn) + Sd
+    b = np.random.default_rng(5).standard_normal((N, 2))
+    bt = torch.from_numpy(b).cuda()
+    Ad = pr.dense()
+    for trans in ("n", "c"):
+        Aop = Ad if trans == "n" else Ad.T
+        Askop = Ask if trans == "n" else Ask.T
+        x = flam_amd.rskelf_partial_sv(
+            F, bt, lambda Y, t: torch.from_numpy(np.linalg.solve(Askop, Y.cpu().numpy())).cuda(),
+            trans).cpu().numpy()
+        xe = np.linalg.solve(Aop, b)
+        assert np.linalg.norm(x - xe) <= 1e-7 * np.linalg.norm(xe), trans
+        y = flam_amd.rskelf_partial_mv(
+            F, bt, lambda Y, t: torch.from_numpy(Askop @ Y.cpu().numpy()).cuda(), trans).cpu().numpy()
+        ye = Aop @ b
+        assert np.linalg.norm(y - ye) <= 1e-7 * np.linalg.norm(ye), trans
+
+
+def test_partial_factorization_stop_zero():
+    """stop = 0: nothing is factored; every point remains and S is empty."""
+    flam_amd, geometry = _dev()
+    N = 600
+    g = geometry.ie_circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, 32, 1e-10, g.proxy, dict(symm="n", stop=0))
+    sk, S = flam_amd.rskelf_partial_info(F)
+    assert F.info()["nlvl_done"] == 0 and F.info()["nfactor"] == 0
+    assert np.array_equal(sk.numpy(), np.arange(N)) and S._nnz() == 0
