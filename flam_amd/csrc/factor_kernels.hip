@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) k_lists(const LvlBox* __restrict__ boxes,
                                                PointsDev P, LevelParams L) {
   const LvlBox B = boxes[blockIdx.x];
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & (kWave - 1), wid = tid / kWave, nw = nt / kWave;
+  const int lane = tid & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(tid / kWave), nw = nt / kWave;
   int* slf = pool + B.slf_off;
   int pos = 0;
   for (int s = 0; s < B.ncs; ++s) {
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
   const LvlBox B = boxes[b];
   const int n = B.n, m = mrows[b], ld = B.mcap;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & (kWave - 1), wid = tid / kWave, nw = nt / kWave;
+  const int lane = tid & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(tid / kWave), nw = nt / kWave;
   S* K = Kar + B.K_off;
   int* perm = permar + B.perm_off;
   double* vn1 = sh;
@@ -572,7 +572,7 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
   S* Krs = Ksr + k * r;              // r x k  (becomes G; LU only)
   S* Kss = Krs + (LU ? r * k : 0);   // k x k
   S* Ts = Kss + k * k;               // k x r
-  const int lane = tid & (kWave - 1), wid = tid / kWave, nw = nt / kWave;
+  const int lane = tid & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(tid / kWave), nw = nt / kWave;
   // rows over lanes, columns over waves: no integer division in the loops
 #define FOR_COLS_ROWS(i, i0, i1, j, j0, j1) \
   for (int j = (j0) + wid; j < (j1); j += nw) for (int i = (i0) + lane; i < (i1); i += kWave)
@@ -812,7 +812,7 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   const int mt = L.two_sided ? 2 * mn : mn;
   const int m = mt + L.p;
   const int ld = B.mcap;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   S* K = Kar + B.K_off;
   int* perm = permar + B.perm_off;
   const int* slf = pool + B.slf_off;
@@ -1369,7 +1369,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   const LvlBox B = boxes[bidx];
   const BoxOut O = outs[bidx];
   const int n = B.n, k = O.k, r = O.r;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   // columns go to waves in panels of NB: slot tb of wave w holds column
   // ((tb / NB) * 16 + w) * NB + tb % NB (NB = 1: column w + 16 tb)
   constexpr int NB = LU ? 1 : 4;
