@@ -3,11 +3,13 @@
 Public surface (mirrors the reference's rskelf/rskelf_sv/rskelf_mv):
 
     from flam_amd import rskelf, rskelf_sv, rskelf_mv, LaplaceDL2D, ProxyRing
-"""
-from .rskelf import (FlamError, Factor, LaplaceDL2D, ProxyRing, kernel_matvec, rskelf,
-                     rskelf_mv, rskelf_partial_info, rskelf_partial_mv, rskelf_partial_sv,
-                     rskelf_sv)
 
-__all__ = ["FlamError", "Factor", "LaplaceDL2D", "ProxyRing", "kernel_matvec", "rskelf",
-           "rskelf_mv", "rskelf_partial_info", "rskelf_partial_mv", "rskelf_partial_sv",
-           "rskelf_sv"]
+plus rskelf_batch (many independent problems in one factorization).
+"""
+from .rskelf import (Batch, FlamError, Factor, LaplaceDL2D, ProxyRing, kernel_matvec,
+                     kernel_matvec_batch, rskelf, rskelf_batch, rskelf_mv, rskelf_partial_info,
+                     rskelf_partial_mv, rskelf_partial_sv, rskelf_sv, stack_problems)
+
+__all__ = ["Batch", "FlamError", "Factor", "LaplaceDL2D", "ProxyRing", "kernel_matvec",
+           "kernel_matvec_batch", "rskelf", "rskelf_batch", "rskelf_mv", "rskelf_partial_info",
+           "rskelf_partial_mv", "rskelf_partial_sv", "rskelf_sv", "stack_problems"]

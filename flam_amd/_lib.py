@@ -59,6 +59,9 @@ SIGNATURES = {
     "rskx_opts_default": (c_int, [POINTER(Opts)]),
     "rskx_rskelf": (c_int, [c_void_p, c_int, c_int64, c_int, c_double, POINTER(Kernel),
                             POINTER(Proxy), POINTER(Opts), POINTER(c_void_p)]),
+    "rskx_rskelf_batch": (c_int, [c_int, POINTER(c_int64), c_void_p, c_int, c_int, c_double,
+                                  POINTER(Kernel), POINTER(Proxy), POINTER(Opts),
+                                  POINTER(c_void_p)]),
     "rskx_rskelf_sv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
     "rskx_rskelf_mv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
     "rskx_sweep_sv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int, c_int]),
@@ -75,6 +78,9 @@ SIGNATURES = {
                                   POINTER(c_int64), POINTER(c_int64), POINTER(c_double), c_int]),
     "rskx_kernel_matvec": (c_int, [c_void_p, c_int, c_int64, POINTER(Kernel), c_void_p, c_int64,
                                    c_void_p, c_int64, c_int]),
+    "rskx_kernel_matvec_batch": (c_int, [c_int, POINTER(c_int64), c_void_p, c_int,
+                                         POINTER(Kernel), c_void_p, c_int64, c_void_p, c_int64,
+                                         c_int]),
 }
 
 _lib = None

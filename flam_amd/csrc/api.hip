@@ -69,11 +69,12 @@ void launch_solve(int, bool, const FacRec*, int, const S*, const int*, S*, int64
                   size_t, hipStream_t);
 size_t solve_shmem(int prog, int kmax, int rmax, size_t es);
 void launch_matvec(const double*, int64_t, const double*, const double*, const double*, double,
-                   const double*, int64_t, double*, int64_t, int, hipStream_t);
+                   const double*, int64_t, double*, int64_t, int, const int64_t*, int, hipStream_t);
 void launch_gather_points(const double*, const double*, const double*, const double*, const int*,
                           int64_t, int, double*, hipStream_t);
-void launch_tree_init(const double*, int, int64_t, double*, int*, double*, int, hipStream_t);
-void launch_tree_split(bool, const TreeChunk*, int, int, int, int64_t, double*, int*, double*, int*,
+void launch_tree_init(const double*, int, int64_t, const int64_t*, int, double*, int*, double*, int,
+                      hipStream_t);
+void launch_tree_split(bool, const TreeChunk*, int, int, int64_t, double*, int*, double*, int*,
                        int*, hipStream_t);
 
 // --------------------------------------------------------------- errors --
@@ -168,7 +169,7 @@ struct DevBuf {
 struct Workspace {
   DevBuf boxes, nsegs, csegs, outs, ids, pool, nbr, mnbr, mrows, kout, iter, K, perm, T, W, M[2], flag;
   DevBuf lrecs;                 // factor records of the level being packed
-  DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs;
+  DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs, poff;
   int* hflag = nullptr;
 };
 static Workspace g_ws;
@@ -233,6 +234,7 @@ using namespace rskx;
 struct rskx_factor {
   int64_t N = 0;
   int d = 0;
+  int nprob = 1;                          // independent problems (rskx_rskelf_batch)
   char symm = 'n';
   int prec = RSKX_PREC_FP64;
   int nlvl = 0, nlvl_done = 0;
@@ -264,12 +266,14 @@ struct rskx_factor {
 
 namespace {
 
-// Tree on the device: the same boxes, centres and point order as
-// build_tree (core/hypoct.m:51-171), with the point partition done by
-// k_tree_split.  The tree order (tree position -> original index) is left
-// in `order` on the device.
-int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const double* ext,
-                   HostTree& t, DevBuf& order) {
+// Forest on the device: one tree per problem (problem j owns the points
+// [poff[j], poff[j+1]) of x), each with the boxes, centres and point order
+// that core/hypoct.m:51-171 builds for that problem alone; the point
+// partition of every splitting box of a level, whatever its tree, is one
+// k_tree_split pass.  The tree order (tree position -> original index) is
+// left in `order` on the device.  A single problem is nprob = 1.
+int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* poff, int occ,
+                   int lvlmax, const double* ext, HostTree& t, DevBuf& order) {
   Workspace& ws = g_ws;
   constexpr int kChunk = 8192;
   constexpr int kMaxLevels = 64;
@@ -277,49 +281,61 @@ int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const
   RSKX_TRY(ws.txs.ensure((size_t)d * N * sizeof(double)));
   RSKX_TRY(order.ensure((size_t)N * sizeof(int)));
   RSKX_TRY(ws.tord.ensure((size_t)N * sizeof(int)));
-  const int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (N + 255) / 256));
-  RSKX_TRY(ws.tpart.ensure((size_t)nblk * 6 * sizeof(double)));
+  int64_t nmax = 0;
+  for (int j = 0; j < nprob; ++j) nmax = std::max<int64_t>(nmax, poff[j + 1] - poff[j]);
+  int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (nmax + 255) / 256));
+  nblk = std::max(1, std::min(nblk, 65536 / nprob));
+  RSKX_TRY(ws.tpart.ensure((size_t)nprob * nblk * 6 * sizeof(double)));
+  const std::vector<int64_t> hoff(poff, poff + nprob + 1);   // alive until the sync below
+  RSKX_TRY(upload(ws.poff, hoff));
   double* xs = ws.xs.as<double>();
   int* ord = order.as<int>();
-  launch_tree_init(x, d, N, xs, ord, ws.tpart.as<double>(), nblk, g_stream);
-  std::vector<double> part((size_t)nblk * 6);
+  launch_tree_init(x, d, N, ws.poff.as<int64_t>(), nprob, xs, ord, ws.tpart.as<double>(), nblk,
+                   g_stream);
+  std::vector<double> part((size_t)nprob * nblk * 6);
   RSKX_HIP(hipMemcpyAsync(part.data(), ws.tpart.p, part.size() * 8, hipMemcpyDeviceToHost, g_stream));
   RSKX_HIP(hipStreamSynchronize(g_stream));
-  std::vector<double> lo(d), hi(d), l(d);
-  if (ext) {
-    for (int i = 0; i < d; ++i) { lo[i] = ext[i]; hi[i] = ext[d + i]; }
-  } else {
-    for (int i = 0; i < d; ++i) {
-      lo[i] = INFINITY; hi[i] = -INFINITY;
-      for (int b = 0; b < nblk; ++b) {
-        lo[i] = std::min(lo[i], part[b * 6 + i]);
-        hi[i] = std::max(hi[i], part[b * 6 + 3 + i]);
-      }
-    }
-  }
   t = HostTree();
   t.d = d;
-  for (int i = 0; i < d; ++i) {
-    l[i] = hi[i] - lo[i];
-    t.ctr.push_back(0.5 * (lo[i] + hi[i]));
+  t.nprob = nprob;
+  std::vector<double> l((size_t)nprob * d);     // each tree's current box size
+  std::vector<std::vector<int>> chld(nprob);
+  for (int j = 0; j < nprob; ++j) {
+    double lo[3], hi[3];
+    for (int i = 0; i < d; ++i) {
+      if (ext) { lo[i] = ext[i]; hi[i] = ext[d + i]; continue; }
+      lo[i] = INFINITY; hi[i] = -INFINITY;
+      for (int b = 0; b < nblk; ++b) {
+        const double* q = &part[((size_t)j * nblk + b) * 6];
+        lo[i] = std::min(lo[i], q[i]);
+        hi[i] = std::max(hi[i], q[3 + i]);
+      }
+    }
+    for (int i = 0; i < d; ++i) {
+      l[(size_t)j * d + i] = hi[i] - lo[i];
+      t.ctr.push_back(0.5 * (lo[i] + hi[i]));
+      t.bsize.push_back(hi[i] - lo[i]);
+    }
+    t.beg.push_back(poff[j]);
+    t.cnt.push_back(poff[j + 1] - poff[j]);
+    t.prnt.push_back(-1);
+    t.prob.push_back(j);
   }
-  t.beg.push_back(0);
-  t.cnt.push_back(N);
-  t.prnt.push_back(-1);
-  std::vector<std::vector<int>> chld(1);
-  t.lvp = {0, 1};
-  t.lsize = l;
+  t.lvp = {0, nprob};
   int nlvl = 1;
   const int ncode = 1 << d;
   std::vector<TreeChunk> chunks;
-  std::vector<int> counts;
+  std::vector<int> counts, divmask(nprob);
   while (lvlmax <= 0 || nlvl < lvlmax) {
     if (nlvl >= kMaxLevels)
       return fail(RSKX_ERR_UNSUP, "FLAM:rskx:tree: more than 64 levels (occ too small for the data)");
-    double lmax = *std::max_element(l.begin(), l.end());
-    int divmask = 0;
-    for (int i = 0; i < d; ++i)
-      if (l[i] >= lmax / std::sqrt(2.0)) { divmask |= 1 << i; l[i] *= 0.5; }
+    for (int j = 0; j < nprob; ++j) {       // hypoct.m:81-82, per tree
+      double* lj = &l[(size_t)j * d];
+      const double lmax = *std::max_element(lj, lj + d);
+      divmask[j] = 0;
+      for (int i = 0; i < d; ++i)
+        if (lj[i] >= lmax / std::sqrt(2.0)) { divmask[j] |= 1 << i; lj[i] *= 0.5; }
+    }
     chunks.clear();
     std::vector<int> split;   // boxes of this level being split
     for (int64_t p = t.lvp[nlvl - 1]; p < t.lvp[nlvl]; ++p) {
@@ -331,6 +347,7 @@ int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const
         c.start = t.beg[p] + s0;
         c.len = (int)std::min<int64_t>(kChunk, t.cnt[p] - s0);
         c.box = sb;
+        c.divmask = divmask[t.prob[p]];
         for (int i = 0; i < d; ++i) c.ctr[i] = t.ctr[(size_t)d * p + i];
         chunks.push_back(c);
       }
@@ -339,7 +356,7 @@ int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const
     const int nch = (int)chunks.size();
     RSKX_TRY(upload(ws.tchunks, chunks));
     RSKX_TRY(ws.tcounts.ensure((size_t)nch * 8 * sizeof(int)));
-    launch_tree_split(false, ws.tchunks.as<TreeChunk>(), nch, d, divmask, N, xs, ord,
+    launch_tree_split(false, ws.tchunks.as<TreeChunk>(), nch, d, N, xs, ord,
                       ws.txs.as<double>(), ws.tord.as<int>(), ws.tcounts.as<int>(), g_stream);
     counts.resize((size_t)nch * 8);
     RSKX_HIP(hipMemcpyAsync(counts.data(), ws.tcounts.p, counts.size() * sizeof(int),
@@ -349,6 +366,9 @@ int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const
     int c0 = 0;
     for (int sb = 0; sb < (int)split.size(); ++sb) {
       const int p = split[sb];
+      const int j = t.prob[p];
+      const int dm = divmask[j];
+      const double* lj = &l[(size_t)j * d];
       int c1 = c0;
       while (c1 < nch && chunks[c1].box == sb) ++c1;
       int64_t tot[8] = {0};
@@ -369,12 +389,14 @@ int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const
         if (tot[u] == 0) continue;
         for (int i = 0; i < d; ++i) {
           double bit = (double)((u >> i) & 1);
-          t.ctr.push_back(pc[i] + (((divmask >> i) & 1) ? l[i] : 0.0) * (bit - 0.5));
+          t.ctr.push_back(pc[i] + (((dm >> i) & 1) ? lj[i] : 0.0) * (bit - 0.5));
+          t.bsize.push_back(lj[i]);
         }
         t.beg.push_back(cb);
         t.cnt.push_back(tot[u]);
         cb += tot[u];
         t.prnt.push_back(p);
+        t.prob.push_back(j);
         chld.emplace_back();
         chld[p].push_back(t.nbox() - 1);
       }
@@ -382,14 +404,13 @@ int build_tree_dev(const double* x, int d, int64_t N, int occ, int lvlmax, const
       c0 = c1;
     }
     RSKX_TRY(upload(ws.tchunks, chunks));
-    launch_tree_split(true, ws.tchunks.as<TreeChunk>(), nch, d, divmask, N, xs, ord,
+    launch_tree_split(true, ws.tchunks.as<TreeChunk>(), nch, d, N, xs, ord,
                       ws.txs.as<double>(), ws.tord.as<int>(), ws.tcounts.as<int>(), g_stream);
     // the parent level's split status is now final: its neighbour lists
     // are built while the device scatters this level's points
     append_level_neighbours(t, chld, nlvl - 1);
     ++nlvl;
     t.lvp.push_back(t.nbox());
-    t.lsize.insert(t.lsize.end(), l.begin(), l.end());
   }
   RSKX_HIP(hipGetLastError());
   t.nlvl = nlvl;
@@ -460,8 +481,9 @@ int partial_assembly(const HostTree& t, const std::vector<BoxState>& bs, int lv_
 }
 
 template <typename S>
-int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
-                const rskx_kernel& A, const rskx_proxy& PX, const rskx_opts& o, rskx_factor* F) {
+int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* poff, int occ,
+                double rank_or_tol, const rskx_kernel& A, const rskx_proxy& PX,
+                const double* pweight, const rskx_opts& o, rskx_factor* F) {
   using clk = std::chrono::steady_clock;
   Workspace& ws = g_ws;
   Prof& prof = F->prof;
@@ -470,7 +492,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
 
   // ---- tree (device partition, host box bookkeeping)
   HostTree t;
-  RSKX_TRY(build_tree_dev(x, d, N, occ, o.lvlmax, o.ext, t, F->order));
+  RSKX_TRY(build_tree_dev(x, d, N, nprob, poff, occ, o.lvlmax, o.ext, t, F->order));
   F->nlvl = t.nlvl;
   prof.host("tree", std::chrono::duration<double, std::milli>(clk::now() - t0).count());
   if (o.verb) {
@@ -557,9 +579,7 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
     const int64_t b0 = t.lvp[lv], nb = t.lvp[lv + 1] - b0;
     const int64_t nrem1 = nrem;
     LevelParams L{};
-    for (int i = 0; i < d; ++i) L.l[i] = t.lsize[(size_t)d * lv + i];
     L.radius = PX.radius;
-    L.pweight = PX.weight;
     L.pkind = PX.kind;
     const bool use_proxy = lvl_ref > 2 && PX.kind != RSKX_PROXY_NONE;
     L.p = use_proxy ? PX.p : 0;
@@ -610,7 +630,11 @@ int factor_impl(const double* x, int d, int64_t N, int occ, double rank_or_tol,
       B.perm_off = off_perm;           off_perm += n;
       B.T_off = off_T;                 off_T += 3 * (int64_t)n * n;
       B.W_off = off_W;                 off_W += 2 * (int64_t)n * n;
-      for (int i = 0; i < d; ++i) B.ctr[i] = t.ctr[(size_t)d * b + i];
+      for (int i = 0; i < d; ++i) {
+        B.ctr[i] = t.ctr[(size_t)d * b + i];
+        B.l[i] = t.bsize[(size_t)d * b + i];
+      }
+      B.pw = pweight[t.prob[b]];
       nmax = std::max(nmax, n);
       mcapmax = std::max(mcapmax, B.mcap);
       mcmax = std::max(mcmax, mc);
@@ -1049,12 +1073,10 @@ int rskx_set_stream(void* s) {
   return RSKX_OK;
 }
 
-int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
-                const rskx_kernel* A, const rskx_proxy* pxy, const rskx_opts* opts,
-                rskx_factor** out) {
-  if (!out) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: output handle is NULL");
-  *out = nullptr;
-  rskx_opts o;
+// argument checks shared by rskx_rskelf and rskx_rskelf_batch
+// (rskelf.m:112-128, core/chksymm.m, core/id.m:52-57, core/hypoct.m:58-61)
+static int check_factor_args(const double* x, int d, int64_t N, int occ, double rank_or_tol,
+                             const rskx_kernel* A, const rskx_opts* opts, rskx_opts& o) {
   rskx_opts_default(&o);
   if (opts) o = *opts;
   char s = (char)tolower(o.symm);
@@ -1075,25 +1097,69 @@ int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   if (!x || !A->nu || !A->w || !A->diag) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: point or kernel array is NULL");
   if (N < 1) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: N must be positive");
   if (N > (int64_t)1 << 30) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: N above 2^30");
-  rskx_proxy px{RSKX_PROXY_NONE, 0, 1.5, 0.0};
-  if (pxy) px = *pxy;
-  if (px.kind != RSKX_PROXY_RING_SL && px.kind != RSKX_PROXY_RING_DL)
-    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: naive global compression (no pxyfun) is not built; pass a proxy ring");
-  if (px.p < 1 || !(px.radius > 0)) return fail(RSKX_ERR_ARG, "FLAM:rskx:proxy: invalid proxy ring");
   if (o.precision != RSKX_PREC_FP64 && o.precision != RSKX_PREC_FP32)
     return fail(RSKX_ERR_ARG, "FLAM:rskx:precision: unknown precision");
+  return RSKX_OK;
+}
 
+static int check_proxy(const rskx_proxy* pxy) {
+  if (!pxy || (pxy->kind != RSKX_PROXY_RING_SL && pxy->kind != RSKX_PROXY_RING_DL))
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: naive global compression (no pxyfun) is not built; pass a proxy ring");
+  if (pxy->p < 1 || !(pxy->radius > 0)) return fail(RSKX_ERR_ARG, "FLAM:rskx:proxy: invalid proxy ring");
+  return RSKX_OK;
+}
+
+static int run_factor(const double* x, int d, int64_t N, int nprob, const int64_t* poff, int occ,
+                      double rank_or_tol, const rskx_kernel* A, const rskx_proxy& px,
+                      const double* pw, const rskx_opts& o, rskx_factor** out) {
   rskx_factor* F = new rskx_factor();
   F->N = N;
   F->d = d;
+  F->nprob = nprob;
   F->symm = o.symm;
   F->prec = o.precision;
   int rc = o.precision == RSKX_PREC_FP64
-               ? factor_impl<double>(x, d, N, occ, rank_or_tol, *A, px, o, F)
-               : factor_impl<float>(x, d, N, occ, rank_or_tol, *A, px, o, F);
+               ? factor_impl<double>(x, d, N, nprob, poff, occ, rank_or_tol, *A, px, pw, o, F)
+               : factor_impl<float>(x, d, N, nprob, poff, occ, rank_or_tol, *A, px, pw, o, F);
   if (rc != RSKX_OK) { delete F; return rc; }
   *out = F;
   return RSKX_OK;
+}
+
+int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
+                const rskx_kernel* A, const rskx_proxy* pxy, const rskx_opts* opts,
+                rskx_factor** out) {
+  if (!out) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: output handle is NULL");
+  *out = nullptr;
+  rskx_opts o;
+  RSKX_TRY(check_factor_args(x, d, N, occ, rank_or_tol, A, opts, o));
+  RSKX_TRY(check_proxy(pxy));
+  const int64_t poff[2] = {0, N};
+  return run_factor(x, d, N, 1, poff, occ, rank_or_tol, A, *pxy, &pxy->weight, o, out);
+}
+
+int rskx_rskelf_batch(int nprob, const int64_t* offsets, const double* x, int d, int occ,
+                      double rank_or_tol, const rskx_kernel* A, const rskx_proxy* pxy,
+                      const rskx_opts* opts, rskx_factor** out) {
+  if (!out) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: output handle is NULL");
+  *out = nullptr;
+  if (nprob < 1 || !offsets || !pxy) return fail(RSKX_ERR_ARG, "FLAM:rskx:batch: need nprob >= 1, offsets and proxies");
+  if (nprob > 65536) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:batch: more than 65536 problems");
+  if (offsets[0] != 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:batch: offsets[0] must be 0");
+  for (int j = 0; j < nprob; ++j)
+    if (offsets[j + 1] <= offsets[j]) return fail(RSKX_ERR_ARG, "FLAM:rskx:batch: every problem needs at least one point");
+  const int64_t N = offsets[nprob];
+  rskx_opts o;
+  RSKX_TRY(check_factor_args(x, d, N, occ, rank_or_tol, A, opts, o));
+  if (o.ext) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:batch: opts.ext (one root extent) is not taken by a batch");
+  std::vector<double> pw(nprob);
+  for (int j = 0; j < nprob; ++j) {
+    RSKX_TRY(check_proxy(&pxy[j]));
+    if (pxy[j].kind != pxy[0].kind || pxy[j].p != pxy[0].p || pxy[j].radius != pxy[0].radius)
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:batch: the problems' proxy rings must share kind, p and radius");
+    pw[j] = pxy[j].weight;
+  }
+  return run_factor(x, d, N, nprob, offsets, occ, rank_or_tol, A, pxy[0], pw.data(), o, out);
 }
 
 static int sweep_entry(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
@@ -1248,7 +1314,29 @@ int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, 
     return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: only the 2D Laplace double-layer kernel is built");
   if (!x || !X || !Y) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL array");
   if (ldx < N || ldy < N || nrhs < 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad leading dimension");
-  launch_matvec(x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, g_stream);
+  launch_matvec(x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, nullptr, 1, g_stream);
+  RSKX_HIP(hipGetLastError());
+  RSKX_HIP(hipStreamSynchronize(g_stream));
+  return RSKX_OK;
+}
+
+int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x, int d,
+                             const rskx_kernel* A, const double* X, int64_t ldx, double* Y,
+                             int64_t ldy, int nrhs) {
+  if (!A || A->kind != RSKX_KERNEL_LAPLACE2D_DL || d != 2)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: only the 2D Laplace double-layer kernel is built");
+  if (nprob < 1 || !offsets || offsets[0] != 0)
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:batch: need nprob >= 1 and offsets starting at 0");
+  for (int j = 0; j < nprob; ++j)
+    if (offsets[j + 1] <= offsets[j]) return fail(RSKX_ERR_ARG, "FLAM:rskx:batch: every problem needs at least one point");
+  const int64_t N = offsets[nprob];
+  if (!x || !X || !Y) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL array");
+  if (ldx < N || ldy < N || nrhs < 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad leading dimension");
+  static DevBuf doff;                      // offsets on the device (kept for later calls)
+  const std::vector<int64_t> hoff(offsets, offsets + nprob + 1);
+  RSKX_TRY(upload(doff, hoff));
+  launch_matvec(x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs,
+                doff.as<int64_t>(), nprob, g_stream);
   RSKX_HIP(hipGetLastError());
   RSKX_HIP(hipStreamSynchronize(g_stream));
   return RSKX_OK;

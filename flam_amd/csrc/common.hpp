@@ -31,11 +31,10 @@ struct PointsDev {
   double scale;           // off-diagonal multiplier
 };
 
-// proxy / filter parameters of one level
+// proxy / filter parameters of one level (box sizes and proxy weights are
+// per box: the boxes of one level may belong to different trees)
 struct LevelParams {
-  double l[3];            // box size of the level
   double radius;          // proxy ring radius = neighbour filter radius (box units)
-  double pweight;         // single-layer proxy multiplier
   int p;                  // proxy points (0 at levels <= 2)
   int pkind;              // RSKX_PROXY_*
   int filter;             // apply the neighbour filter (levels > 2)
@@ -57,6 +56,8 @@ struct LvlBox {
   int64_t T_off;          // offset in the double T/R scratch (2 n^2)
   int64_t W_off;          // offset in the factor scratch (n^2)
   double ctr[3];
+  double l[3];            // box size (its tree's size at this level)
+  double pw;              // single-layer proxy multiplier of its problem
 };
 
 // list segment: `len` point indices at pool[off]; for child segments moff
@@ -110,6 +111,7 @@ struct TreeChunk {
   int64_t start;          // first point of the chunk (current order)
   int len;                // points in the chunk
   int box;                // splitting box index within this level's list
+  int divmask;            // dimensions its tree halves at this level
   double ctr[3];          // that box's centre
   int64_t off[8];         // scatter: first destination of each child code
 };

@@ -28,23 +28,10 @@ __device__ inline double entry(const PointsDev& P, int i, int j) {
   return i == j ? P.diag[j] : entry_offdiag(P, i, j);
 }
 
-// proxy row q against column point j (ie_circle.m:106-111, ie_ellipse.m:126-128)
-__device__ inline double proxy_entry(const PointsDev& P, const LevelParams& L, const double* ctr,
-                                     int q, int j) {
-  double th = (double)(q + 1) * kTwoPi / (double)L.p;
-  double px = L.radius * cos(th) * L.l[0] + ctr[0];
-  double py = L.radius * sin(th) * L.l[1] + ctr[1];
-  double dx = px - P.x[0][j];
-  double dy = py - P.x[1][j];
-  double r2 = dx * dx + dy * dy;
-  if (L.pkind == RSKX_PROXY_RING_SL) return -0.5 * log(r2) / kTwoPi * L.pweight;
-  return P.w[j] * (dx * P.nu[0][j] + dy * P.nu[1][j]) / (kTwoPi * r2);
-}
-
-__device__ inline bool in_ring(const PointsDev& P, const LevelParams& L, const double* ctr, int pt) {
+__device__ inline bool in_ring(const PointsDev& P, const LevelParams& L, const LvlBox& B, int pt) {
   double s = 0;
   for (int q = 0; q < P.d; ++q) {
-    double t = (P.x[q][pt] - ctr[q]) / L.l[q];
+    double t = (P.x[q][pt] - B.ctr[q]) / B.l[q];
     s += t * t;
   }
   return s < L.radius * L.radius;
@@ -77,7 +64,7 @@ __global__ void __launch_bounds__(256) k_lists(const LvlBox* __restrict__ boxes,
       int pt = 0;
       if (i < g.len) {
         pt = pool[g.off + i];
-        keep = !L.filter || in_ring(P, L, B.ctr, pt);
+        keep = !L.filter || in_ring(P, L, B, pt);
       }
       uint64_t mask = __ballot(keep);
       int pre = __popcll(mask & ((1ull << lane) - 1ull));
@@ -153,11 +140,11 @@ __global__ void __launch_bounds__(256) k_assemble(const int* __restrict__ ids,
       }
     for (int i = threadIdx.x; i < L.p; i += blockDim.x) {   // proxy rows
       double th = (double)(i + 1) * kTwoPi / (double)L.p;
-      double px = L.radius * cos(th) * L.l[0] + B.ctr[0];
-      double py = L.radius * sin(th) * L.l[1] + B.ctr[1];
+      double px = L.radius * cos(th) * B.l[0] + B.ctr[0];
+      double py = L.radius * sin(th) * B.l[1] + B.ctr[1];
       double dx = px - xj, dy = py - yj;
       double r2 = dx * dx + dy * dy;
-      double v = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * L.pweight
+      double v = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
                                                : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
       Kj[mt + i] = (S)v;
     }
@@ -850,8 +837,8 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   }
   for (int q = tid; q < L.p; q += 1024) {    // as proxy_entry
     const double th = (double)(q + 1) * kTwoPi / (double)L.p;
-    ppx[q] = L.radius * cos(th) * L.l[0] + B.ctr[0];
-    ppy[q] = L.radius * sin(th) * L.l[1] + B.ctr[1];
+    ppx[q] = L.radius * cos(th) * B.l[0] + B.ctr[0];
+    ppy[q] = L.radius * sin(th) * B.l[1] + B.ctr[1];
   }
   if (tid == 0) mrows[b] = m;
   __syncthreads();
@@ -888,7 +875,7 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
           const int q = i - mt;
           const double dx = ppx[q] - xj, dy = ppy[q] - yj;
           const double r2 = dx * dx + dy * dy;
-          val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * L.pweight
+          val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
                                               : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
         }
       }
@@ -1093,8 +1080,8 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   }
   for (int q = tid; q < L.p; q += NT) {    // as proxy_entry
     const double th = (double)(q + 1) * kTwoPi / (double)L.p;
-    ppx[q] = L.radius * cos(th) * L.l[0] + B.ctr[0];
-    ppy[q] = L.radius * sin(th) * L.l[1] + B.ctr[1];
+    ppx[q] = L.radius * cos(th) * B.l[0] + B.ctr[0];
+    ppy[q] = L.radius * sin(th) * B.l[1] + B.ctr[1];
   }
   for (int j = tid; j < np; j += NT) prow[j] = 0;
   if (tid == 0) mrows[b] = m;
@@ -1146,7 +1133,7 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
           const int q = i - mt;
           const double dx = ppx[q] - xj, dy = ppy[q] - yj;
           const double r2 = dx * dx + dy * dy;
-          val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * L.pweight
+          val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
                                               : sw[j] * (dx * snx[j] + dy * sny[j]) / (kTwoPi * r2);
         }
       }

@@ -664,24 +664,51 @@ void launch_perm(bool to, void* X, int64_t ldx, S* Y, int64_t N, int nrhs, int n
 // ------------------------------------------- FP64 on-the-fly A X product --
 constexpr double kTwoPiM = 6.283185307179586476925286766559;
 
+// problem of point i: the last j with poff[j] <= i (poff ascending, nprob + 1 entries)
+__device__ inline int problem_of(const int64_t* __restrict__ poff, int nprob, int64_t i) {
+  int lo = 0, hi = nprob;                     // poff[lo] <= i < poff[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (poff[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Y = A X for the block-diagonal matrix of nprob independent problems
+// (problem j: points [poff[j], poff[j+1])); poff == nullptr is one problem
 __global__ void __launch_bounds__(256) k_matvec(const double* __restrict__ x, int64_t N,
                                                 const double* __restrict__ nu,
                                                 const double* __restrict__ w,
                                                 const double* __restrict__ diag, double scale,
                                                 const double* __restrict__ X, int64_t ldx,
-                                                double* __restrict__ Y, int64_t ldy, int nrhs) {
+                                                double* __restrict__ Y, int64_t ldy, int nrhs,
+                                                const int64_t* __restrict__ poff, int nprob) {
   __shared__ double sx[256], sy[256], snx[256], sny[256], sw[256];
   __shared__ double sX[8][256];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const bool act = i < N;
   const double xi = act ? x[2 * i] : 0, yi = act ? x[2 * i + 1] : 0;
+  // sources of this thread's problem, and of the block's first and last targets
+  int64_t my0 = 0, my1 = N, blk0 = 0, blk1 = N;
+  if (poff) {
+    const int64_t i0 = blockIdx.x * (int64_t)blockDim.x;
+    const int64_t i1 = i0 + blockDim.x - 1 < N - 1 ? i0 + blockDim.x - 1 : N - 1;
+    const int pa = problem_of(poff, nprob, i0), pb = problem_of(poff, nprob, i1);
+    blk0 = poff[pa];
+    blk1 = poff[pb + 1];
+    if (act) {
+      const int pi = problem_of(poff, nprob, i);
+      my0 = poff[pi];
+      my1 = poff[pi + 1];
+    }
+  }
   for (int c0 = 0; c0 < nrhs; c0 += 8) {
     const int cc = nrhs - c0 < 8 ? nrhs - c0 : 8;
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t j0 = 0; j0 < N; j0 += 256) {
+    for (int64_t j0 = blk0; j0 < blk1; j0 += 256) {
       __syncthreads();
       int64_t j = j0 + threadIdx.x;
-      if (j < N) {
+      if (j < blk1) {
         sx[threadIdx.x] = x[2 * j];
         sy[threadIdx.x] = x[2 * j + 1];
         snx[threadIdx.x] = nu[2 * j];
@@ -690,9 +717,10 @@ __global__ void __launch_bounds__(256) k_matvec(const double* __restrict__ x, in
         for (int c = 0; c < cc; ++c) sX[c][threadIdx.x] = X[j + (c0 + c) * ldx];
       }
       __syncthreads();
-      const int jn = N - j0 < 256 ? (int)(N - j0) : 256;
+      const int jn = blk1 - j0 < 256 ? (int)(blk1 - j0) : 256;
       for (int t = 0; t < jn; ++t) {
         int64_t jj = j0 + t;
+        if (jj < my0 || jj >= my1) continue;   // another problem's source
         double a;
         if (jj == i) a = diag[jj];
         else {
@@ -711,10 +739,10 @@ __global__ void __launch_bounds__(256) k_matvec(const double* __restrict__ x, in
 
 void launch_matvec(const double* x, int64_t N, const double* nu, const double* w,
                    const double* diag, double scale, const double* X, int64_t ldx, double* Y,
-                   int64_t ldy, int nrhs, hipStream_t st) {
+                   int64_t ldy, int nrhs, const int64_t* poff, int nprob, hipStream_t st) {
   int grid = (int)((N + 255) / 256);
   hipLaunchKernelGGL(k_matvec, dim3(grid), dim3(256), 0, st, x, N, nu, w, diag, scale, X, ldx, Y,
-                     ldy, nrhs);
+                     ldy, nrhs, poff, nprob);
 }
 
 // points, normals, weights and diagonal into tree-order structure of arrays

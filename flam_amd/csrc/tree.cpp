@@ -21,95 +21,6 @@ static double matlab_round(double v) {
   return v < 0 ? -std::floor(-v + 0.5) : std::floor(v + 0.5);
 }
 
-HostTree build_tree(const double* x, int d, int64_t N, int occ, int lvlmax, const double* ext) {
-  HostTree t;
-  t.d = d;
-  std::vector<double> lo(d), hi(d), l(d);
-  if (ext) {
-    for (int i = 0; i < d; ++i) { lo[i] = ext[i]; hi[i] = ext[d + i]; }
-  } else {
-    for (int i = 0; i < d; ++i) { lo[i] = INFINITY; hi[i] = -INFINITY; }
-    for (int64_t p = 0; p < N; ++p)
-      for (int i = 0; i < d; ++i) {
-        double v = x[i + (int64_t)d * p];
-        lo[i] = std::min(lo[i], v);
-        hi[i] = std::max(hi[i], v);
-      }
-    if (N == 0)
-      for (int i = 0; i < d; ++i) { lo[i] = 0; hi[i] = 0; }
-  }
-  for (int i = 0; i < d; ++i) {
-    l[i] = hi[i] - lo[i];
-    t.ctr.push_back(0.5 * (lo[i] + hi[i]));
-  }
-  t.order.resize(N);
-  for (int64_t p = 0; p < N; ++p) t.order[p] = p;
-  t.beg.push_back(0);
-  t.cnt.push_back(N);
-  t.prnt.push_back(-1);
-  std::vector<std::vector<int>> chld(1);
-  t.lvp = {0, 1};
-  t.lsize = l;
-  int nlvl = 1;
-
-  const int nchild = 1 << d;
-  std::vector<int64_t> tmp;
-  std::vector<int> code;
-  std::vector<int64_t> bucket(nchild + 1);
-  while (lvlmax <= 0 || nlvl < lvlmax) {
-    int nbox_before = t.nbox();
-    double lmax = *std::max_element(l.begin(), l.end());
-    std::vector<char> ldiv(d);
-    for (int i = 0; i < d; ++i) {
-      ldiv[i] = l[i] >= lmax / std::sqrt(2.0);
-      if (ldiv[i]) l[i] *= 0.5;
-    }
-    for (int64_t p = t.lvp[nlvl - 1]; p < t.lvp[nlvl]; ++p) {
-      int64_t b0 = t.beg[p], n = t.cnt[p];
-      if (n <= occ) continue;
-      const double* c = &t.ctr[(size_t)d * p];
-      code.resize(n);
-      std::fill(bucket.begin(), bucket.end(), 0);
-      for (int64_t q = 0; q < n; ++q) {
-        int64_t pt = t.order[b0 + q];
-        int cd = 0;
-        for (int i = 0; i < d; ++i)
-          if (ldiv[i] && x[i + (int64_t)d * pt] > c[i]) cd |= 1 << i;
-        code[q] = cd;
-        bucket[cd + 1]++;
-      }
-      for (int u = 0; u < nchild; ++u) bucket[u + 1] += bucket[u];
-      // stable counting sort of the box's range by child code
-      tmp.resize(n);
-      std::vector<int64_t> pos(bucket.begin(), bucket.end() - 1);
-      for (int64_t q = 0; q < n; ++q) tmp[pos[code[q]]++] = t.order[b0 + q];
-      std::copy(tmp.begin(), tmp.end(), t.order.begin() + b0);
-      std::vector<double> pc(c, c + d);   // parent centre (t.ctr may reallocate)
-      for (int u = 0; u < nchild; ++u) {
-        int64_t cn = bucket[u + 1] - bucket[u];
-        if (cn == 0) continue;
-        for (int i = 0; i < d; ++i) {
-          double bit = (double)((u >> i) & 1);
-          t.ctr.push_back(pc[i] + (ldiv[i] ? l[i] : 0.0) * (bit - 0.5));
-        }
-        t.beg.push_back(b0 + bucket[u]);
-        t.cnt.push_back(cn);
-        t.prnt.push_back((int)p);
-        chld.emplace_back();
-        chld[p].push_back(t.nbox() - 1);
-      }
-      t.cnt[p] = 0;   // points moved to the children
-    }
-    if (t.nbox() <= nbox_before) break;
-    ++nlvl;
-    t.lvp.push_back(t.nbox());
-    t.lsize.insert(t.lsize.end(), l.begin(), l.end());
-  }
-  t.nlvl = nlvl;
-  finish_tree(t, chld);
-  return t;
-}
-
 // A small persistent pool of host threads for the per-level neighbour
 // search: parallel_for(n, parts, f) runs f(part, lo, hi) on `parts`
 // contiguous ranges covering [0, n) and returns when all have finished.
@@ -196,9 +107,9 @@ void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& c
     for (int64_t i = b0; i < b1; ++i) t.nbor_off.push_back((int)t.nbor.size());
     return;
   }
-  const double* lc = &t.lsize[(size_t)d * lv];
   auto box_list = [&](int64_t i, std::vector<int>& out) {
     const double* c = &t.ctr[(size_t)d * i];
+    const double* lc = &t.bsize[(size_t)d * i];
     const int p = t.prnt[i];
     for (int j : chld[p])
       if (j != i) out.push_back(j);
@@ -206,7 +117,7 @@ void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& c
     for (int e = pn0; e < pn1; ++e) {
       const int j = t.nbor[e];
       if (t.cnt[j] == 0) continue;            // only non-empty (leaf) boxes
-      const double* jl = &t.lsize[(size_t)d * t.level_of(j)];
+      const double* jl = &t.bsize[(size_t)d * j];
       const double* jc = &t.ctr[(size_t)d * j];
       double dmax = -INFINITY;
       for (int q = 0; q < d; ++q)

@@ -16,11 +16,17 @@
 
 namespace rskx {
 
+// A forest of such trees, one per independent problem, is stored the same
+// way: level lv holds the level-lv boxes of every tree (trees aligned at
+// their roots), and each box carries its own size, since every tree halves
+// the sides of its own root box.
 struct HostTree {
   int d = 0;
   int nlvl = 0;
+  int nprob = 1;                  // trees (independent problems)
   std::vector<int64_t> lvp;       // box offsets per level, nlvl+1 entries
-  std::vector<double> lsize;      // d x nlvl box sizes
+  std::vector<double> bsize;      // d x nbox box sizes
+  std::vector<int> prob;          // tree (problem) of each box
   std::vector<double> ctr;        // d x nbox centres
   std::vector<int> prnt;          // parent box, -1 for the root
   std::vector<int> chld_off, chld;  // CSR children
@@ -33,17 +39,13 @@ struct HostTree {
   bool leaf(int b) const { return chld_off[b + 1] == chld_off[b]; }
 };
 
-// x: host d x N column-major.  lvlmax <= 0 means no limit.  ext: d x 2
-// column-major (lower bounds then upper bounds) or nullptr.
-HostTree build_tree(const double* x, int d, int64_t N, int occ, int lvlmax, const double* ext);
-
 // Neighbour lists of the boxes of level lv (hypoct.m:141-170), appended to
 // t.nbor / t.nbor_off; needs the lists of every earlier level and the final
 // split status (cnt) of the levels above lv.
 void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& chld, int lv);
 
 // Children in CSR form and the neighbour lists of the levels not yet
-// appended, given the boxes (ctr, prnt, beg/cnt, lvp, lsize, nlvl) and each
+// appended, given the boxes (ctr, prnt, beg/cnt, lvp, bsize, nlvl) and each
 // box's children.
 void finish_tree(HostTree& t, const std::vector<std::vector<int>>& chld);
 

@@ -21,13 +21,14 @@ constexpr int kMaxCodes = 8;
 
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_tree_split(const TreeChunk* __restrict__ chunks, int d,
-                                                    int divmask, int64_t N, double* xs,
+                                                    int64_t N, double* xs,
                                                     int* ord, double* txs, int* tord,
                                                     int* __restrict__ counts) {
   const TreeChunk C = chunks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   const int nw = blockDim.x / kWave;
   const int ncode = 1 << d;
+  const int divmask = C.divmask;
   __shared__ int wcnt[kMaxCodes][4];
   __shared__ int64_t run[kMaxCodes];
   if (tid < ncode) run[tid] = WRITE ? C.off[tid] : 0;
@@ -79,11 +80,14 @@ __global__ void k_tree_copyback(const TreeChunk* __restrict__ chunks, int d, int
 }
 
 // column-major d x N points into per-dimension arrays; identity order;
-// per-block min/max for the root extent (hypoct.m:65)
-__global__ void k_tree_init(const double* __restrict__ x, int d, int64_t N, double* xs, int* ord,
+// per-block min/max of each problem's points (blockIdx.y: problem, whose
+// points are [poff[j], poff[j+1])) for the root extents (hypoct.m:65)
+__global__ void k_tree_init(const double* __restrict__ x, int d, int64_t N,
+                            const int64_t* __restrict__ poff, double* xs, int* ord,
                             double* __restrict__ part) {
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N;
+  const int64_t p0 = poff[blockIdx.y], p1 = poff[blockIdx.y + 1];
+  for (int64_t p = p0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < p1;
        p += (int64_t)gridDim.x * blockDim.x) {
     for (int q = 0; q < d; ++q) {
       double v = x[d * p + q];
@@ -104,28 +108,31 @@ __global__ void k_tree_init(const double* __restrict__ x, int d, int64_t N, doub
       }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
+    double* o = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6;
     for (int q = 0; q < d; ++q) {
-      part[blockIdx.x * 6 + q] = slo[q][0];
-      part[blockIdx.x * 6 + 3 + q] = shi[q][0];
+      o[q] = slo[q][0];
+      o[3 + q] = shi[q][0];
     }
+  }
 }
 
-void launch_tree_init(const double* x, int d, int64_t N, double* xs, int* ord, double* part,
-                      int nblk, hipStream_t st) {
-  hipLaunchKernelGGL(k_tree_init, dim3(nblk), dim3(256), 0, st, x, d, N, xs, ord, part);
+void launch_tree_init(const double* x, int d, int64_t N, const int64_t* poff, int nprob,
+                      double* xs, int* ord, double* part, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_tree_init, dim3(nblk, nprob), dim3(256), 0, st, x, d, N, poff, xs, ord,
+                     part);
 }
 
-void launch_tree_split(bool write, const TreeChunk* chunks, int nchunk, int d, int divmask,
-                       int64_t N, double* xs, int* ord, double* txs, int* tord, int* counts,
+void launch_tree_split(bool write, const TreeChunk* chunks, int nchunk, int d, int64_t N,
+                       double* xs, int* ord, double* txs, int* tord, int* counts,
                        hipStream_t st) {
   if (write) {
-    hipLaunchKernelGGL(k_tree_split<true>, dim3(nchunk), dim3(256), 0, st, chunks, d, divmask, N,
+    hipLaunchKernelGGL(k_tree_split<true>, dim3(nchunk), dim3(256), 0, st, chunks, d, N,
                        xs, ord, txs, tord, counts);
     hipLaunchKernelGGL(k_tree_copyback, dim3(nchunk), dim3(256), 0, st, chunks, d, N, xs, ord,
                        txs, tord);
   } else {
-    hipLaunchKernelGGL(k_tree_split<false>, dim3(nchunk), dim3(256), 0, st, chunks, d, divmask, N,
+    hipLaunchKernelGGL(k_tree_split<false>, dim3(nchunk), dim3(256), 0, st, chunks, d, N,
                        xs, ord, txs, tord, counts);
   }
 }

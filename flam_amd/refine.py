@@ -30,9 +30,17 @@ def _norms(W):
     return torch.sqrt(_rowsum(W * W))
 
 
-def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8):
+def _seg_norms(W, segments):
+    """(nrhs, len(segments)) norms of the row blocks W[:, a:b]."""
+    return torch.stack([_norms(W[:, a:b]) for a, b in segments], dim=1)
+
+
+def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None):
     """Solve A X = B.  matvec, precond: callables on (N, nrhs) FP64 CUDA
-    tensors.  Returns (X, info) with info = dict(iters, outer, relres)."""
+    tensors.  segments: optional row ranges [(a, b), ...] of independent
+    problems stacked in one system (a block-diagonal A); then every
+    problem's own relative residual must reach tol.  Returns (X, info) with
+    info = dict(iters, outer, relres)."""
     vec = B.dim() == 1
     B2 = B.reshape(B.shape[0], -1).to(torch.float64)
     N, nrhs = B2.shape
@@ -40,6 +48,14 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8):
     bnorm = _norms(Bt)
     bnorm = torch.where(bnorm > 0, bnorm, torch.ones_like(bnorm))
     bnorm_h = bnorm.cpu()
+    if segments is not None:
+        bseg = _seg_norms(Bt, segments)
+        bseg = torch.where(bseg > 0, bseg, torch.ones_like(bseg))
+        # the Arnoldi estimate bounds the stacked residual, which bounds
+        # every block's: stop it at the smallest block's share
+        inner_ref = bseg.min(dim=1).values.cpu()
+    else:
+        inner_ref = bnorm_h
     Xt = torch.zeros_like(Bt)
     V = torch.empty(nrhs, restart + 1, N, dtype=torch.float64, device=B2.device)
     Z = torch.empty(nrhs, restart, N, dtype=torch.float64, device=B2.device)
@@ -53,7 +69,10 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8):
     for outer in range(maxit + 1):
         R = Bt - op(matvec, Xt) if outer else Bt.clone()
         rn = _norms(R)
-        relres = (rn / bnorm).cpu()
+        if segments is not None:
+            relres = (_seg_norms(R, segments) / bseg).max(dim=1).values.cpu()
+        else:
+            relres = (rn / bnorm).cpu()
         if bool((relres <= tol).all()) or outer == maxit:
             break
         V[:, 0, :] = R / rn[:, None]
@@ -81,7 +100,7 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8):
             Hj = H[:, : j + 2, : j + 1]
             y = torch.linalg.lstsq(Hj, g[:, : j + 2, None]).solution[..., 0]
             est = torch.linalg.vector_norm(g[:, : j + 2] - (Hj @ y[..., None])[..., 0], dim=1)
-            if bool((est / bnorm_h <= 0.5 * tol).all()) or bool((hcol[:, -1] == 0).all()):
+            if bool((est / inner_ref <= 0.5 * tol).all()) or bool((hcol[:, -1] == 0).all()):
                 break
         yd = y.to(B2.device)[:, :, None]                       # (nrhs, j+1, 1)
         Xt = Xt + (Z[:, : j + 1, :] * yd).sum(1)

@@ -154,19 +154,10 @@ class Factor:
                 for i in range(n)}
 
 
-def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
-    """F = rskelf(A, x, occ, rank_or_tol, pxyfun, opts) on the GPU.
-
-    opts keys (rskelf.m:112-120): lvlmax (inf), ext (None), Tmax (2),
-    rrqr_iter (inf), symm ('n'), verb (0), stop (inf: a numeric stop
-    factors that many levels, rskelf.m:128,172); plus precision ('fp64' or
-    'fp32') and profile (0)."""
+def _opts(opts, d):
+    """rskx_opts from the options dict (rskelf.m:112-120 names and defaults);
+    returns the struct and the host buffer ext points into."""
     opts = dict(opts or {})
-    x = _dev(x)
-    if x.dim() != 2:
-        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: x must be d x N")
-    d, N = x.shape
-    xc = x.t().contiguous()                # column-major d x N
     o = L.Opts()
     L.lib().rskx_opts_default(ctypes.byref(o))
     lvlmax = opts.get("lvlmax", math.inf)
@@ -191,6 +182,22 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     if callable(stop):
         raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:stop: only a numeric opts.stop (levels) is built")
     o.stop = -1 if stop is None or stop == math.inf else int(stop)
+    return o, ext_buf
+
+
+def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
+    """F = rskelf(A, x, occ, rank_or_tol, pxyfun, opts) on the GPU.
+
+    opts keys (rskelf.m:112-120): lvlmax (inf), ext (None), Tmax (2),
+    rrqr_iter (inf), symm ('n'), verb (0), stop (inf: a numeric stop
+    factors that many levels, rskelf.m:128,172); plus precision ('fp64' or
+    'fp32') and profile (0)."""
+    x = _dev(x)
+    if x.dim() != 2:
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: x must be d x N")
+    d, N = x.shape
+    xc = x.t().contiguous()                # column-major d x N
+    o, ext_buf = _opts(opts, d)
     k = A._struct()
     p = pxyfun._struct() if pxyfun is not None else L.Proxy(L.RSKX_PROXY_NONE, 0, 1.5, 0.0)
     h = ctypes.c_void_p()
@@ -198,6 +205,83 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     _check(L.lib().rskx_rskelf(xc.data_ptr(), d, N, int(occ), float(rank_or_tol),
                                ctypes.byref(k), ctypes.byref(p), ctypes.byref(o), ctypes.byref(h)))
     return Factor(h.value, N, keep=(A, xc))
+
+
+@dataclass
+class Batch:
+    """Independent problems stacked for rskelf_batch: the points, normals,
+    weights and diagonals of every problem concatenated (problem j owns
+    columns offsets[j]:offsets[j+1]), and one proxy ring per problem."""
+    x: torch.Tensor
+    kernel: LaplaceDL2D
+    offsets: list
+    proxies: list
+
+    @property
+    def N(self):
+        return self.offsets[-1]
+
+    def split(self, X):
+        """The rows of each problem of a stacked (N, ...) array."""
+        return [X[a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
+
+
+def stack_problems(problems):
+    """Batch of problems given as objects with x (d x N_j), kernel
+    (LaplaceDL2D, one common scale) and proxy (ProxyRing)."""
+    scales = {float(p.kernel.scale) for p in problems}
+    if len(scales) != 1:
+        raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:batch: the problems' kernels must share one scale")
+    offs = [0]
+    for p in problems:
+        offs.append(offs[-1] + int(p.x.shape[1]))
+    x = torch.cat([_dev(p.x) for p in problems], dim=1)
+    nu = torch.cat([_dev(p.kernel.nu).reshape(2, -1) for p in problems], dim=1)
+    w = torch.cat([_dev(p.kernel.w).reshape(-1) for p in problems])
+    diag = torch.cat([_dev(p.kernel.diag).reshape(-1) for p in problems])
+    return Batch(x, LaplaceDL2D(nu, w, diag, scales.pop()), offs, [p.proxy for p in problems])
+
+
+def rskelf_batch(batch, occ, rank_or_tol, opts=None):
+    """One factorization of many independent problems (rskx_rskelf_batch):
+    the factor of the block-diagonal matrix diag(A_1, ..., A_P), each
+    problem's tree factored level by level together with the others'.
+    rskelf_sv / rskelf_mv apply it to the stacked right-hand sides."""
+    x = _dev(batch.x)
+    d, N = x.shape
+    if N != batch.N:
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:batch: offsets do not match the points")
+    xc = x.t().contiguous()
+    o, ext_buf = _opts(opts, d)
+    k = batch.kernel._struct()
+    nprob = len(batch.offsets) - 1
+    offs = (ctypes.c_int64 * (nprob + 1))(*batch.offsets)
+    pxs = (L.Proxy * nprob)(*[p._struct() for p in batch.proxies])
+    h = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    _check(L.lib().rskx_rskelf_batch(nprob, offs, xc.data_ptr(), d, int(occ), float(rank_or_tol),
+                                     ctypes.byref(k), pxs, ctypes.byref(o), ctypes.byref(h)))
+    return Factor(h.value, N, keep=(batch.kernel, xc))
+
+
+def kernel_matvec_batch(batch, X):
+    """FP64 Y = diag(A_1, ..., A_P) X, each kernel evaluated on the fly."""
+    x = _dev(batch.x)
+    d, N = x.shape
+    xc = x.t().contiguous()
+    X = _dev(X)
+    vec = X.dim() == 1
+    X2 = X.reshape(-1, 1) if vec else X
+    Xc = X2.t().contiguous()
+    Yc = torch.empty_like(Xc)
+    k = batch.kernel._struct()
+    nprob = len(batch.offsets) - 1
+    offs = (ctypes.c_int64 * (nprob + 1))(*batch.offsets)
+    torch.cuda.synchronize()
+    _check(L.lib().rskx_kernel_matvec_batch(nprob, offs, xc.data_ptr(), d, ctypes.byref(k),
+                                            Xc.data_ptr(), N, Yc.data_ptr(), N, Xc.shape[0]))
+    Y = Yc.t()
+    return Y.reshape(-1) if vec else Y
 
 
 def _sweep(fn, F, X, trans):

@@ -117,6 +117,23 @@ int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
                 const rskx_kernel* A, const rskx_proxy* pxy, const rskx_opts* opts,
                 rskx_factor** out);
 
+/* Many independent problems factored together.  No single reference entry
+ * corresponds: it is what calling rskelf (rskelf/rskelf.m:109) once per
+ * problem computes, for the block-diagonal matrix diag(A_1, ..., A_P).
+ * Problem j owns the points [offsets[j], offsets[j+1]) of x (device,
+ * d x N with N = offsets[nprob]) and of the kernel's nu / w / diag arrays;
+ * offsets is a HOST array of nprob + 1 ascending entries, offsets[0] = 0,
+ * every problem non-empty.  Each problem keeps its own tree (its own root
+ * box, core/hypoct.m:65); level l of every tree is factored by the same
+ * kernel launches.  pxy is a HOST array of nprob proxy rings that share
+ * kind, p and radius (the weights may differ); opts.ext must be NULL.
+ * The handle is one factor of order N: rskx_rskelf_sv / _mv apply it to
+ * the stacked right-hand sides of all problems, and each problem's block
+ * equals the factor rskx_rskelf computes for that problem alone. */
+int rskx_rskelf_batch(int nprob, const int64_t* offsets, const double* x, int d, int occ,
+                      double rank_or_tol, const rskx_kernel* A, const rskx_proxy* pxy,
+                      const rskx_opts* opts, rskx_factor** out);
+
 /* Y = rskelf_sv(F,X,trans)                      -- rskelf/rskelf_sv.m:13
  * In place on device X (N x nrhs, leading dimension ldx, type dtype).
  * trans 'n', 't' or 'c' (core/chktrans.m:8). */
@@ -171,6 +188,13 @@ int rskx_kernel_times(const rskx_factor* F, const char** names, double* ms, int6
  * e.g. rskelf/test/ie_ellipse.m:56).  All pointers device. */
 int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A,
                        const double* X, int64_t ldx, double* Y, int64_t ldy, int nrhs);
+
+/* The same product for the block-diagonal matrix of rskx_rskelf_batch's
+ * problems (offsets: HOST, nprob + 1 entries as there): each problem's
+ * rows see only its own points. */
+int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x, int d,
+                             const rskx_kernel* A, const double* X, int64_t ldx, double* Y,
+                             int64_t ldy, int nrhs);
 
 #ifdef __cplusplus
 }

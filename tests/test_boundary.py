@@ -101,3 +101,32 @@ def test_python_mirror_rejects_without_gpu():
     from flam_amd.rskelf import FlamError
     assert issubclass(FlamError, ValueError)
     assert hasattr(flam_amd, "rskelf") and hasattr(flam_amd, "rskelf_sv")
+
+
+def _batch_call(L, offsets, proxies, ext=False):
+    o = L.Opts()
+    L.lib().rskx_opts_default(ctypes.byref(o))
+    buf = (ctypes.c_double * 4)(0, 0, 1, 1)
+    if ext:
+        o.ext = buf
+    dummy = 16  # never dereferenced: validation fails first
+    k = L.Kernel(L.RSKX_KERNEL_LAPLACE2D_DL, dummy, dummy, dummy, 1.0)
+    offs = (ctypes.c_int64 * len(offsets))(*offsets)
+    pxs = (L.Proxy * len(proxies))(*proxies)
+    h = ctypes.c_void_p()
+    rc = L.lib().rskx_rskelf_batch(len(offsets) - 1, offs, dummy, 2, 64, 1e-6, ctypes.byref(k),
+                                   pxs, ctypes.byref(o), ctypes.byref(h))
+    return rc, L.last_error()
+
+
+def test_batch_arguments_checked_before_device_use():
+    L = _lib()
+    ring = L.Proxy(L.RSKX_PROXY_RING_DL, 64, 1.5, 0.0)
+    rc, msg = _batch_call(L, [1, 10, 20], [ring, ring])
+    assert rc == L.RSKX_ERR_ARG and msg.startswith("FLAM:rskx:batch")
+    rc, msg = _batch_call(L, [0, 10, 10], [ring, ring])           # empty problem
+    assert rc == L.RSKX_ERR_ARG and msg.startswith("FLAM:rskx:batch")
+    rc, msg = _batch_call(L, [0, 10, 20], [ring, L.Proxy(L.RSKX_PROXY_RING_SL, 64, 1.5, 1.0)])
+    assert rc == L.RSKX_ERR_UNSUP and msg.startswith("FLAM:rskx:batch")
+    rc, msg = _batch_call(L, [0, 10, 20], [ring, ring], ext=True)
+    assert rc == L.RSKX_ERR_UNSUP and msg.startswith("FLAM:rskx:batch")
