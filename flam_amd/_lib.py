@@ -21,9 +21,11 @@ RSKX_PREC_FP32 = 1
 RSKX_F64 = 0
 RSKX_F32 = 1
 RSKX_KERNEL_LAPLACE2D_DL = 1
+RSKX_KERNEL_LAPLACE3D_SL = 2
 RSKX_PROXY_NONE = 0
 RSKX_PROXY_RING_SL = 1
 RSKX_PROXY_RING_DL = 2
+RSKX_PROXY_POINTS_SL = 3
 
 
 class Kernel(ctypes.Structure):
@@ -32,7 +34,8 @@ class Kernel(ctypes.Structure):
 
 
 class Proxy(ctypes.Structure):
-    _fields_ = [("kind", c_int), ("p", c_int), ("radius", c_double), ("weight", c_double)]
+    _fields_ = [("kind", c_int), ("p", c_int), ("radius", c_double), ("weight", c_double),
+                ("pts", c_void_p)]
 
 
 class Opts(ctypes.Structure):

@@ -30,10 +30,10 @@ void launch_lists(const LvlBox*, const Seg*, const Seg*, int*, int*, int*, const
                   const LevelParams&, int, hipStream_t);
 template <typename S>
 void launch_assemble(const int*, const LvlBox*, const int*, const int*, const int*, int*, S*,
-                     const PointsDev&, const LevelParams&, int, size_t, hipStream_t);
+                     const PointsDev&, const LevelParams&, int, hipStream_t);
 template <typename S>
-void launch_id(int, const int*, const LvlBox*, const int*, S*, int*, double*, const IdParams&,
-               int*, int*, int, size_t, hipStream_t);
+void launch_id(int, const int*, const LvlBox*, const int*, S*, int*, double*, const IdParams&, int*,
+               int*, int, size_t, double*, int64_t, hipStream_t);
 void launch_gram_id(int, const int*, const LvlBox*, const int*, const int*, const int*, int*,
                     double*, int*, const PointsDev&, const LevelParams&, const IdParams&, int*, int,
                     size_t, hipStream_t);
@@ -59,16 +59,16 @@ void launch_elim(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut
                  int*, hipStream_t);
 template <typename S>
 void launch_sweep(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int, int,
-                  int, size_t, hipStream_t);
+                  int, size_t, S*, int64_t, hipStream_t);
 template <typename U, typename S>
 void launch_perm(bool, void*, int64_t, S*, int64_t, int, int, const int*, hipStream_t);
 template <typename S>
 void launch_pack(bool, const FacRec*, int, S*, hipStream_t);
 template <typename S>
 void launch_solve(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int, int,
-                  size_t, hipStream_t);
+                  size_t, S*, int64_t, hipStream_t);
 size_t solve_shmem(int prog, int kmax, int rmax, size_t es);
-void launch_matvec(const double*, int64_t, const double*, const double*, const double*, double,
+void launch_matvec(int, const double*, int64_t, const double*, const double*, const double*, double,
                    const double*, int64_t, double*, int64_t, int, const int64_t*, int, hipStream_t);
 void launch_gather_points(const double*, const double*, const double*, const double*, const int*,
                           int64_t, int, double*, hipStream_t);
@@ -168,7 +168,9 @@ struct DevBuf {
 // scratch reused across factorizations of this process
 struct Workspace {
   DevBuf boxes, nsegs, csegs, outs, ids, pool, nbr, mnbr, mrows, kout, iter, K, perm, T, W, M[2], flag;
+  DevBuf V;                     // Householder vectors of the global QR when rows exceed shared memory
   DevBuf lrecs;                 // factor records of the level being packed
+  DevBuf sweep;                 // sweep work rows of factors too large for shared memory
   DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs, poff;
   int* hflag = nullptr;
 };
@@ -516,6 +518,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   P.w = pts + (size_t)2 * d * N;
   P.diag = pts + (size_t)(2 * d + 1) * N;
   P.d = d;
+  P.kind = A.kind;
   P.scale = A.scale;
 
   // ---- pool of point lists: leaves own their tree-order ranges
@@ -581,8 +584,11 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     LevelParams L{};
     L.radius = PX.radius;
     L.pkind = PX.kind;
+    L.ppts = PX.pts;
     const bool use_proxy = lvl_ref > 2 && PX.kind != RSKX_PROXY_NONE;
-    L.p = use_proxy ? PX.p : 0;
+    // point proxies of an unsymmetric matrix add the incoming rows
+    L.np = use_proxy ? PX.p : 0;
+    L.p = L.np * (PX.kind == RSKX_PROXY_POINTS_SL && two ? 2 : 1);
     L.filter = use_proxy ? 1 : 0;
     L.two_sided = two ? 1 : 0;
 
@@ -629,7 +635,9 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       B.K_off = off_K;                 off_K += (int64_t)B.mcap * n;
       B.perm_off = off_perm;           off_perm += n;
       B.T_off = off_T;                 off_T += 3 * (int64_t)n * n;
-      B.W_off = off_W;                 off_W += 2 * (int64_t)n * n;
+      // elimination scratch of the global class: 2 n^2 elements, then a
+      // 6 n double point cache (12 n elements keep the next box 8-byte aligned)
+      B.W_off = off_W;                 off_W += 2 * (int64_t)n * n + 12 * (int64_t)n;
       for (int i = 0; i < d; ++i) {
         B.ctr[i] = t.ctr[(size_t)d * b + i];
         B.l[i] = t.bsize[(size_t)d * b + i];
@@ -681,14 +689,19 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     }
     std::vector<int> idg, idr[2], idq[kGramClasses], idall;
     size_t shm_r[2] = {0, 0}, shm_q[kGramClasses] = {0, 0, 0};
-    int nmax_g = 0, mcmax_g = 0, mcapmax_g = 0;
+    int nmax_g = 0, mcapmax_g = 0;
+    // a box whose point data would not fit a class's shared memory goes to
+    // the next route: Gram -> register QRCP -> QRCP on K in global memory
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
       if (gram && n > 0 && mb > 0 && gram_id_class(n) >= 0) {
         const int c = gram_id_class(n);
-        idq[c].push_back((int)ib);
-        shm_q[c] = std::max(shm_q[c], gram_id_shmem(c, n, hmn[ib], mb, L.p));
-        continue;
+        const size_t by = gram_id_shmem(c, n, hmn[ib], mb, L.p);
+        if (by <= 160 * 1024) {
+          idq[c].push_back((int)ib);
+          shm_q[c] = std::max(shm_q[c], by);
+          continue;
+        }
       }
       int cls = -1;
       for (int c = 1; c >= 0 && cls < 0; --c) {
@@ -696,17 +709,16 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         id_reg_limits(sizeof(S), c, &ml, &nl);
         if (mb <= ml && n <= nl) cls = c;
       }
-      if (cls >= 0 && n > 0 && mb > 0) {
+      // staging (8 x 1024 S), proxy points, point data, norms, v, red, maps
+      const size_t by = (size_t)8 * 1024 * sizeof(S) +
+                        (size_t)(3 * L.p + 7 * n + 5 * hmn[ib] + mb + 32) * sizeof(double) +
+                        (size_t)2 * n * sizeof(int) + 64;
+      if (cls >= 0 && n > 0 && mb > 0 && by <= 160 * 1024) {
         idr[cls].push_back((int)ib);
-        // staging (8 x 1024 S), proxy points, point data, norms, v, red, maps
-        const size_t by = (size_t)8 * 1024 * sizeof(S) +
-                          (size_t)(2 * L.p + 7 * n + 5 * hmn[ib] + mb + 32) * sizeof(double) +
-                          (size_t)2 * n * sizeof(int) + 64;
         shm_r[cls] = std::max(shm_r[cls], by);
       } else {
         idg.push_back((int)ib);
         nmax_g = std::max(nmax_g, n);
-        mcmax_g = std::max(mcmax_g, hb[ib].mc);
         mcapmax_g = std::max(mcapmax_g, hb[ib].mcap);
       }
     }
@@ -719,28 +731,29 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     const int ng = (int)idg.size(), nr0 = (int)idr[0].size(), nr1 = (int)idr[1].size();
     int nq = 0;
     for (int c = 0; c < kGramClasses; ++c) nq += (int)idq[c].size();
-    if (shm_r[0] > 160 * 1024 || shm_r[1] > 160 * 1024 ||
-        *std::max_element(shm_q, shm_q + kGramClasses) > 160 * 1024)
-      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box neighbourhood too large for the ID kernels");
-    if ((size_t)(nmax_g + mcmax_g) * 5 * sizeof(double) > 160 * 1024)
-      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box neighbourhood too large for the assembly kernel");
     int64_t Kbytes = 0;
     for (int ib : idg) Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
     prof.begin("assemble", ev);
     if (ng)
       launch_assemble<S>(dids, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
-                         ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng,
-                         (size_t)(nmax_g + mcmax_g) * 5 * sizeof(double), g_stream);
+                         ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng, g_stream);
     prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
     prof.begin("id", ev);
     if (ng) {
       size_t shg = (size_t)(2 * nmax_g + std::max(nmax_g, mcapmax_g) + 32) * sizeof(double);
+      double* vglob = nullptr;             // Householder vectors too long for shared memory
+      const int64_t vstride = std::max(nmax_g, mcapmax_g);
+      if (shg > 160 * 1024) {
+        shg = (size_t)(3 * nmax_g + 32) * sizeof(double);
+        RSKX_TRY(ws.V.ensure((size_t)ng * vstride * sizeof(double)));
+        vglob = ws.V.as<double>();
+      }
       if (shg > 160 * 1024)
         return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box too large for the ID kernel (n=" +
-                                         std::to_string(nmax_g) + ", m=" + std::to_string(mcapmax_g) + ")");
+                                         std::to_string(nmax_g) + ")");
       launch_id<S>(0, dids, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
                    ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
-                   ng, shg, g_stream);
+                   ng, shg, vglob, vstride, g_stream);
     }
     for (int c = 0; c < 2; ++c) {
       const int cnt = c == 0 ? nr0 : nr1;
@@ -754,7 +767,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       for (int c = 0; c < 2; ++c) for (int ib : idr[c]) nmx = std::max(nmx, hb[ib].n);
       launch_id<S>(1, dids + ng, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
                    ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
-                   nr0 + nr1, (size_t)(3 * nmx + 32) * sizeof(double), g_stream);
+                   nr0 + nr1, (size_t)(3 * nmx + 32) * sizeof(double), nullptr, 0, g_stream);
     }
     const int* qids = dids + ng + nr0 + nr1;
     {
@@ -773,7 +786,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       for (int c = 0; c < kGramClasses; ++c) for (int ib : idq[c]) nmx = std::max(nmx, hb[ib].n);
       launch_id<S>(2, qids, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
                    ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
-                   nq, (size_t)(3 * nmx + 32) * sizeof(double), g_stream);
+                   nq, (size_t)(3 * nmx + 32) * sizeof(double), nullptr, 0, g_stream);
     }
     prof.end("id", ev);
     RSKX_HIP(hipGetLastError());
@@ -857,7 +870,8 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     const int nreg = elim_reg_nmax(sizeof(S));
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
-      if (O.r > 0 && hb[ib].n <= nreg) {     // augmented matrix in registers
+      if (O.r > 0 && hb[ib].n <= nreg &&     // augmented matrix in registers
+          elim_reg_shmem(hb[ib].n, O.k, O.r, sizeof(S)) <= 160 * 1024) {
         creg.push_back((int)ib);
         regbox[ib] = 1;
         shm_reg = std::max(shm_reg, elim_reg_shmem(hb[ib].n, O.k, O.r, sizeof(S)));
@@ -867,7 +881,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       size_t by = O.r > 0 ? elim_elems(lu, O.k, O.r) * sizeof(S) + cache : 0;
       int c = (O.r > 0 && by <= kHalf) ? 0 : (O.r > 0 && by <= kFull) ? 1 : 2;
       cls[c].push_back((int)ib);
-      shm_cls[c] = std::max(shm_cls[c], c < 2 ? by : cache);
+      if (c < 2) shm_cls[c] = std::max(shm_cls[c], by);   // class 2: all in global scratch
     }
     std::vector<int> allids(creg);
     for (int c = 2; c >= 0; --c) allids.insert(allids.end(), cls[c].begin(), cls[c].end());
@@ -1004,8 +1018,17 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
       // workgroups per CU), else what is left; larger blocks are read from
       // global memory directly
       if (solve) {
-        const size_t shs = solve_shmem(prog, kmx, rmx, sizeof(S));
-        if (shs > 160 * 1024) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: factor too large for the solve kernel");
+        size_t shs = solve_shmem(prog, kmx, rmx, sizeof(S));
+        // rows too many for shared memory: each workgroup works in its own
+        // slice of a global workspace instead
+        S* gws = nullptr;
+        int64_t gstride = 0;
+        if (shs > 160 * 1024) {
+          gstride = (int64_t)((shs / sizeof(S) + 63) / 64 * 64);
+          RSKX_TRY(g_ws.sweep.ensure((size_t)gstride * (f1 - f0) * sizeof(S)));
+          gws = g_ws.sweep.as<S>();
+          shs = 0;
+        }
         int64_t bytes = 0;
         for (int64_t f = f0; f < f1; ++f) {
           const FacRec& fr = F->recs[f];
@@ -1015,16 +1038,24 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
         }
         prof.begin("sweep", ev);
         launch_solve<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->lval[lv].as<S>(),
-                        F->lidx[lv].as<int>(), Y, N, ncol, kmx, rmx, shs, g_stream);
+                        F->lidx[lv].as<int>(), Y, N, ncol, kmx, rmx, shs, gws, gstride, g_stream);
         prof.end("sweep", ev, bytes);
         continue;
       }
       const size_t xb = ((size_t)(kmx + 2 * rmx) * kRhs + (size_t)kRhs * 256) * sizeof(S);
       const size_t want = (size_t)std::max(rmx * rmx, kmx * rmx) * sizeof(S);
       const size_t room = xb < 76 * 1024 ? 76 * 1024 - xb : 0;
-      const int bcap = (int)(std::min(want, room) / sizeof(S));
+      int bcap = (int)(std::min(want, room) / sizeof(S));
       size_t shm = xb + (size_t)bcap * sizeof(S);
-      if (shm > 160 * 1024) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: factor too large for the sweep kernel");
+      S* gws = nullptr;
+      int64_t gstride = 0;
+      if (shm > 160 * 1024) {                 // as the solve: a global workspace slice
+        gstride = (int64_t)((xb / sizeof(S) + 63) / 64 * 64);
+        RSKX_TRY(g_ws.sweep.ensure((size_t)gstride * (f1 - f0) * sizeof(S)));
+        gws = g_ws.sweep.as<S>();
+        bcap = 0;
+        shm = 0;
+      }
       int64_t bytes = 0;
       for (int64_t f = f0; f < f1; ++f) {
         const FacRec& fr = F->recs[f];
@@ -1034,7 +1065,7 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
       }
       prof.begin("sweep", ev);
       launch_sweep<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->lval[lv].as<S>(),
-                      F->lidx[lv].as<int>(), Y, N, ncol, kmx, rmx, bcap, shm, g_stream);
+                      F->lidx[lv].as<int>(), Y, N, ncol, kmx, rmx, bcap, shm, gws, gstride, g_stream);
       prof.end("sweep", ev, bytes);
     }
   }
@@ -1073,6 +1104,17 @@ int rskx_set_stream(void* s) {
   return RSKX_OK;
 }
 
+// the built-in matrices and their dimensions
+static int check_kernel(const rskx_kernel* A, int d) {
+  if (A && A->kind == RSKX_KERNEL_LAPLACE2D_DL && d == 2) {
+    if (!A->nu) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: the double layer needs normals");
+    return RSKX_OK;
+  }
+  if (A && A->kind == RSKX_KERNEL_LAPLACE3D_SL && d == 3) return RSKX_OK;
+  return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: built kernels are the 2D Laplace double layer (d = 2) "
+                              "and the 3D Laplace single layer (d = 3)");
+}
+
 // argument checks shared by rskx_rskelf and rskx_rskelf_batch
 // (rskelf.m:112-128, core/chksymm.m, core/id.m:52-57, core/hypoct.m:58-61)
 static int check_factor_args(const double* x, int d, int64_t N, int occ, double rank_or_tol,
@@ -1092,9 +1134,8 @@ static int check_factor_args(const double* x, int d, int64_t N, int occ, double 
   if (!(o.rrqr_iter >= 0)) return fail(RSKX_ERR_ARG, "FLAM:id:invalidRRQRIter: Maximum number of RRQR iterations must be nonnegative.");
   if (rank_or_tol >= 1 && std::isfinite(o.Tmax) && o.rrqr_iter > 0)
     return fail(RSKX_ERR_UNSUP, "FLAM:rskx:rank: fixed-rank ID with RRQR refinement is not built (pass a tolerance < 1, or Tmax = Inf)");
-  if (!A || A->kind != RSKX_KERNEL_LAPLACE2D_DL || d != 2)
-    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: only the 2D Laplace double-layer kernel is built");
-  if (!x || !A->nu || !A->w || !A->diag) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: point or kernel array is NULL");
+  RSKX_TRY(check_kernel(A, d));
+  if (!x || !A->w || !A->diag) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: point or kernel array is NULL");
   if (N < 1) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: N must be positive");
   if (N > (int64_t)1 << 30) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: N above 2^30");
   if (o.precision != RSKX_PREC_FP64 && o.precision != RSKX_PREC_FP32)
@@ -1102,10 +1143,17 @@ static int check_factor_args(const double* x, int d, int64_t N, int occ, double 
   return RSKX_OK;
 }
 
-static int check_proxy(const rskx_proxy* pxy) {
-  if (!pxy || (pxy->kind != RSKX_PROXY_RING_SL && pxy->kind != RSKX_PROXY_RING_DL))
-    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: naive global compression (no pxyfun) is not built; pass a proxy ring");
-  if (pxy->p < 1 || !(pxy->radius > 0)) return fail(RSKX_ERR_ARG, "FLAM:rskx:proxy: invalid proxy ring");
+static int check_proxy(const rskx_proxy* pxy, int d) {
+  if (!pxy || (pxy->kind != RSKX_PROXY_RING_SL && pxy->kind != RSKX_PROXY_RING_DL &&
+               pxy->kind != RSKX_PROXY_POINTS_SL))
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: naive global compression (no pxyfun) is not built; pass a proxy surface");
+  if (pxy->p < 1 || !(pxy->radius > 0)) return fail(RSKX_ERR_ARG, "FLAM:rskx:proxy: invalid proxy surface");
+  if (pxy->kind == RSKX_PROXY_POINTS_SL) {
+    if (d != 3) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: point proxies are built for d = 3");
+    if (!pxy->pts) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: proxy points are NULL");
+  } else if (d != 2) {
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: proxy rings are built for d = 2");
+  }
   return RSKX_OK;
 }
 
@@ -1133,7 +1181,7 @@ int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
   *out = nullptr;
   rskx_opts o;
   RSKX_TRY(check_factor_args(x, d, N, occ, rank_or_tol, A, opts, o));
-  RSKX_TRY(check_proxy(pxy));
+  RSKX_TRY(check_proxy(pxy, d));
   const int64_t poff[2] = {0, N};
   return run_factor(x, d, N, 1, poff, occ, rank_or_tol, A, *pxy, &pxy->weight, o, out);
 }
@@ -1154,9 +1202,10 @@ int rskx_rskelf_batch(int nprob, const int64_t* offsets, const double* x, int d,
   if (o.ext) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:batch: opts.ext (one root extent) is not taken by a batch");
   std::vector<double> pw(nprob);
   for (int j = 0; j < nprob; ++j) {
-    RSKX_TRY(check_proxy(&pxy[j]));
-    if (pxy[j].kind != pxy[0].kind || pxy[j].p != pxy[0].p || pxy[j].radius != pxy[0].radius)
-      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:batch: the problems' proxy rings must share kind, p and radius");
+    RSKX_TRY(check_proxy(&pxy[j], d));
+    if (pxy[j].kind != pxy[0].kind || pxy[j].p != pxy[0].p || pxy[j].radius != pxy[0].radius ||
+        pxy[j].pts != pxy[0].pts)
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:batch: the problems' proxy surfaces must share kind, p, radius and points");
     pw[j] = pxy[j].weight;
   }
   return run_factor(x, d, N, nprob, offsets, occ, rank_or_tol, A, pxy[0], pw.data(), o, out);
@@ -1310,11 +1359,11 @@ int rskx_kernel_times(const rskx_factor* F, const char** names, double* ms, int6
 
 int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, const double* X,
                        int64_t ldx, double* Y, int64_t ldy, int nrhs) {
-  if (!A || A->kind != RSKX_KERNEL_LAPLACE2D_DL || d != 2)
-    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: only the 2D Laplace double-layer kernel is built");
+  RSKX_TRY(check_kernel(A, d));
   if (!x || !X || !Y) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL array");
   if (ldx < N || ldy < N || nrhs < 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad leading dimension");
-  launch_matvec(x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, nullptr, 1, g_stream);
+  launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, nullptr, 1,
+                g_stream);
   RSKX_HIP(hipGetLastError());
   RSKX_HIP(hipStreamSynchronize(g_stream));
   return RSKX_OK;
@@ -1323,8 +1372,7 @@ int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, 
 int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x, int d,
                              const rskx_kernel* A, const double* X, int64_t ldx, double* Y,
                              int64_t ldy, int nrhs) {
-  if (!A || A->kind != RSKX_KERNEL_LAPLACE2D_DL || d != 2)
-    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: only the 2D Laplace double-layer kernel is built");
+  RSKX_TRY(check_kernel(A, d));
   if (nprob < 1 || !offsets || offsets[0] != 0)
     return fail(RSKX_ERR_ARG, "FLAM:rskx:batch: need nprob >= 1 and offsets starting at 0");
   for (int j = 0; j < nprob; ++j)
@@ -1335,7 +1383,7 @@ int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x,
   static DevBuf doff;                      // offsets on the device (kept for later calls)
   const std::vector<int64_t> hoff(offsets, offsets + nprob + 1);
   RSKX_TRY(upload(doff, hoff));
-  launch_matvec(x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs,
+  launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs,
                 doff.as<int64_t>(), nprob, g_stream);
   RSKX_HIP(hipGetLastError());
   RSKX_HIP(hipStreamSynchronize(g_stream));

@@ -28,6 +28,7 @@ struct PointsDev {
   const double* w;        // quadrature weights
   const double* diag;     // diagonal entries
   int d;
+  int kind;               // RSKX_KERNEL_*
   double scale;           // off-diagonal multiplier
 };
 
@@ -35,7 +36,9 @@ struct PointsDev {
 // per box: the boxes of one level may belong to different trees)
 struct LevelParams {
   double radius;          // proxy ring radius = neighbour filter radius (box units)
-  int p;                  // proxy points (0 at levels <= 2)
+  const double* ppts;     // RSKX_PROXY_POINTS_SL: 3 x np points in box units (device)
+  int p;                  // proxy rows of K (0 at levels <= 2)
+  int np;                 // distinct proxy points (p, or p / 2 with incoming rows)
   int pkind;              // RSKX_PROXY_*
   int filter;             // apply the neighbour filter (levels > 2)
   int two_sided;          // symm 'n': stack A(slf,nbr)' under A(nbr,slf)

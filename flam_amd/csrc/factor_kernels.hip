@@ -14,6 +14,61 @@
 namespace rskx {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kFourPi = 12.566370614359172953850573533118;
+
+// ------------------------------------------------- point fields, kernels --
+// Every kernel stages five fields per point: x, y, f2, f3, w, where
+// (f2, f3) = (nu_x, nu_y) for the 2D double layer and (z, 0) for the 3D
+// single layer, which needs no normals.
+__device__ inline void point_fields(const PointsDev& P, int pt, double& f0, double& f1, double& f2,
+                                    double& f3, double& fw) {
+  f0 = P.x[0][pt];
+  f1 = P.x[1][pt];
+  if (P.kind == RSKX_KERNEL_LAPLACE3D_SL) {
+    f2 = P.x[2][pt];
+    f3 = 0;
+  } else {
+    f2 = P.nu[0][pt];
+    f3 = P.nu[1][pt];
+  }
+  fw = P.w[pt];
+}
+
+// A_ij = scale w_j / (4 pi |x_i - x_j|) (RSKX_KERNEL_LAPLACE3D_SL, i != j)
+// from target (t0, t1, t2) and source (s0, s1, s2) coordinates
+__device__ inline double sl3_entry(const PointsDev& P, double t0, double t1, double t2, double s0,
+                                   double s1, double s2, double sw) {
+  const double dx = t0 - s0, dy = t1 - s1, dz = t2 - s2;
+  return P.scale * sw / (kFourPi * sqrt(dx * dx + dy * dy + dz * dz));
+}
+
+// proxy point q of box B: the ring of ie_circle.m:35,107 / ie_ellipse.m:127,
+// or the caller's points scaled by the box size (ie_sphere.m:175)
+__device__ inline void proxy_point(const LevelParams& L, const LvlBox& B, int q, double& px,
+                                   double& py, double& pz) {
+  if (L.pkind == RSKX_PROXY_POINTS_SL) {
+    const int u = q % L.np;
+    px = L.ppts[3 * u] * B.l[0] + B.ctr[0];
+    py = L.ppts[3 * u + 1] * B.l[1] + B.ctr[1];
+    pz = L.ppts[3 * u + 2] * B.l[2] + B.ctr[2];
+  } else {
+    const double th = (double)(q + 1) * kTwoPi / (double)L.p;
+    px = L.radius * cos(th) * B.l[0] + B.ctr[0];
+    py = L.radius * sin(th) * B.l[1] + B.ctr[1];
+    pz = 0;
+  }
+}
+
+// RSKX_PROXY_POINTS_SL row q against source point (s0, s1, s2) of weight sw:
+// rows q < np are the outgoing interactions w_j G(pxy, x_j), rows q >= np
+// (symm 'n') the incoming ones, weighted by the problem's proxy weight
+// (ie_sphere.m:178-183)
+__device__ inline double proxy_sl3(const LevelParams& L, const LvlBox& B, int q, double px,
+                                   double py, double pz, double s0, double s1, double s2,
+                                   double sw) {
+  const double dx = px - s0, dy = py - s1, dz = pz - s2;
+  return (q < L.np ? sw : B.pw) / (kFourPi * sqrt(dx * dx + dy * dy + dz * dz));
+}
 
 // ------------------------------------------------------- matrix entries --
 // A_ij of RSKX_KERNEL_LAPLACE2D_DL, i != j (points in tree order)
@@ -84,6 +139,12 @@ __global__ void __launch_bounds__(256) k_lists(const LvlBox* __restrict__ boxes,
 }
 
 // ----------------------------------------------------------- k_assemble --
+// K of one box in global memory, for the boxes no register ID class takes.
+// The neighbour (row) fields are staged in LDS a chunk of kAsmRows at a
+// time; a column's own fields are the same for every thread and are read
+// directly.  No limit on the box or neighbourhood size.
+constexpr int kAsmRows = 1024;
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_assemble(const int* __restrict__ ids,
                                                   const LvlBox* __restrict__ boxes,
@@ -92,7 +153,7 @@ __global__ void __launch_bounds__(256) k_assemble(const int* __restrict__ ids,
                                                   const int* __restrict__ mnbr,
                                                   int* __restrict__ mrows, S* __restrict__ Kar,
                                                   PointsDev P, LevelParams L) {
-  extern __shared__ double sp[];
+  __shared__ double rf[5][kAsmRows];
   const int bidx = ids[blockIdx.x];
   const LvlBox B = boxes[bidx];
   const int n = B.n, mn = mnbr[bidx];
@@ -101,51 +162,59 @@ __global__ void __launch_bounds__(256) k_assemble(const int* __restrict__ ids,
   if (threadIdx.x == 0) mrows[bidx] = m;
   const int* slf = pool + B.slf_off;
   const int* nbr = nbrar + B.nbr_off;
-  // stage column (slf) and row (nbr) point data: x, y, nu_x, nu_y, w
-  double* sx = sp;
-  double* sy = sx + n;
-  double* snx = sy + n;
-  double* sny = snx + n;
-  double* sw = sny + n;
-  double* rx = sw + n;
-  double* ry = rx + mn;
-  double* rnx = ry + mn;
-  double* rny = rnx + mn;
-  double* rw = rny + mn;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    int pt = slf[j];
-    sx[j] = P.x[0][pt]; sy[j] = P.x[1][pt]; snx[j] = P.nu[0][pt]; sny[j] = P.nu[1][pt];
-    sw[j] = P.w[pt];
-  }
-  for (int i = threadIdx.x; i < mn; i += blockDim.x) {
-    int pt = nbr[i];
-    rx[i] = P.x[0][pt]; ry[i] = P.x[1][pt]; rnx[i] = P.nu[0][pt]; rny[i] = P.nu[1][pt];
-    rw[i] = P.w[pt];
-  }
-  __syncthreads();
   S* K = Kar + B.K_off;
   const int ld = B.mcap;
+  const bool sl3 = P.kind == RSKX_KERNEL_LAPLACE3D_SL;
   const double c0 = P.scale / kTwoPi;
-  for (int j = 0; j < n; ++j) {
-    const double xj = sx[j], yj = sy[j], nxj = snx[j], nyj = sny[j], wj = sw[j];
-    S* Kj = K + (int64_t)j * ld;
-    for (int i = threadIdx.x; i < mn; i += blockDim.x) {    // A(nbr, slf)
-      double dx = rx[i] - xj, dy = ry[i] - yj;
-      Kj[i] = (S)(c0 * wj * (dx * nxj + dy * nyj) / (dx * dx + dy * dy));
-    }
-    if (L.two_sided)
-      for (int i = threadIdx.x; i < mn; i += blockDim.x) {  // A(slf, nbr)'
-        double dx = xj - rx[i], dy = yj - ry[i];
-        Kj[mn + i] = (S)(c0 * rw[i] * (dx * rnx[i] + dy * rny[i]) / (dx * dx + dy * dy));
+  for (int r0 = 0; r0 < mn; r0 += kAsmRows) {
+    const int rc = mn - r0 < kAsmRows ? mn - r0 : kAsmRows;
+    __syncthreads();
+    for (int i = threadIdx.x; i < rc; i += blockDim.x)
+      point_fields(P, nbr[r0 + i], rf[0][i], rf[1][i], rf[2][i], rf[3][i], rf[4][i]);
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+      double xj, yj, nxj, nyj, wj;
+      point_fields(P, slf[j], xj, yj, nxj, nyj, wj);
+      S* Kj = K + (int64_t)j * ld + r0;
+      for (int i = threadIdx.x; i < rc; i += blockDim.x) {    // A(nbr, slf)
+        double v;
+        if (sl3) {
+          v = sl3_entry(P, rf[0][i], rf[1][i], rf[2][i], xj, yj, nxj, wj);
+        } else {
+          const double dx = rf[0][i] - xj, dy = rf[1][i] - yj;
+          v = c0 * wj * (dx * nxj + dy * nyj) / (dx * dx + dy * dy);
+        }
+        Kj[i] = (S)v;
       }
-    for (int i = threadIdx.x; i < L.p; i += blockDim.x) {   // proxy rows
-      double th = (double)(i + 1) * kTwoPi / (double)L.p;
-      double px = L.radius * cos(th) * B.l[0] + B.ctr[0];
-      double py = L.radius * sin(th) * B.l[1] + B.ctr[1];
-      double dx = px - xj, dy = py - yj;
-      double r2 = dx * dx + dy * dy;
-      double v = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
-                                               : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
+      if (L.two_sided)
+        for (int i = threadIdx.x; i < rc; i += blockDim.x) {  // A(slf, nbr)'
+          double v;
+          if (sl3) {
+            v = sl3_entry(P, xj, yj, nxj, rf[0][i], rf[1][i], rf[2][i], rf[4][i]);
+          } else {
+            const double dx = xj - rf[0][i], dy = yj - rf[1][i];
+            v = c0 * rf[4][i] * (dx * rf[2][i] + dy * rf[3][i]) / (dx * dx + dy * dy);
+          }
+          Kj[mn + i] = (S)v;
+        }
+    }
+  }
+  for (int j = 0; j < n; ++j) {                              // proxy rows
+    double xj, yj, nxj, nyj, wj;
+    point_fields(P, slf[j], xj, yj, nxj, nyj, wj);
+    S* Kj = K + (int64_t)j * ld;
+    for (int i = threadIdx.x; i < L.p; i += blockDim.x) {
+      double px, py, pz;
+      proxy_point(L, B, i, px, py, pz);
+      double v;
+      if (L.pkind == RSKX_PROXY_POINTS_SL) {
+        v = proxy_sl3(L, B, i, px, py, pz, xj, yj, nxj, wj);
+      } else {
+        const double dx = px - xj, dy = py - yj;
+        const double r2 = dx * dx + dy * dy;
+        v = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
+                                          : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
+      }
       Kj[mt + i] = (S)v;
     }
   }
@@ -226,7 +295,8 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
                                             const int* __restrict__ mrows, S* __restrict__ Kar,
                                             int* __restrict__ permar, double* __restrict__ Tar,
                                             IdParams prm, int* __restrict__ kout,
-                                            int* __restrict__ iterout) {
+                                            int* __restrict__ iterout, double* __restrict__ vglob,
+                                            int64_t vstride) {
   extern __shared__ double sh[];
   const int b = ids[blockIdx.x];
   const LvlBox B = boxes[b];
@@ -235,13 +305,16 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
   const int lane = tid & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(tid / kWave), nw = nt / kWave;
   S* K = Kar + B.K_off;
   int* perm = permar + B.perm_off;
+  constexpr bool QR = MODE == 0;
+  // shared: norms (2n), then the QR's Householder vector (m, or in this
+  // workgroup's slice of vglob when m rows do not fit) or the tails' k-vector
+  // (at most n), then the reduction scratch
   double* vn1 = sh;
   double* vn2 = sh + n;
-  double* v = sh + 2 * n;
-  double* red = v + (m > n ? m : n);
+  double* v = QR && vglob ? vglob + blockIdx.x * vstride : sh + 2 * n;
+  double* red = QR && !vglob ? sh + 2 * n + (m > n ? m : n) : sh + 3 * n;
   __shared__ int s_piv;
 
-  constexpr bool QR = MODE == 0;
   if (QR)
     for (int j = tid; j < n; j += nt) perm[j] = j;
   if (m == 0 || n == 0) {           // id.m:64-68: nothing to compress against
@@ -539,8 +612,9 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
   }
   const int* q = permar + B.perm_off;
   const double* Tg = Tar + B.T_off;  // k x r, leading dimension n
-  // point data of slf in skeleton-first order, cached in shared memory
-  double* cx = smem_d;
+  // point data of slf in skeleton-first order, cached in shared memory, or
+  // for the global-scratch class after the box's 2 n^2 matrix elements
+  double* cx = SH ? smem_d : reinterpret_cast<double*>(War + B.W_off + 2 * (int64_t)n * n);
   double* cy = cx + n;
   double* cnx = cy + n;
   double* cny = cnx + n;
@@ -548,9 +622,8 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
   double* cdg = cw + n;
   for (int a = tid; a < n; a += nt) {
     int pt = slf[q[a]];
-    cx[a] = P.x[0][pt]; cy[a] = P.x[1][pt];
-    cnx[a] = P.nu[0][pt]; cny[a] = P.nu[1][pt];
-    cw[a] = P.w[pt]; cdg[a] = P.diag[pt];
+    point_fields(P, pt, cx[a], cy[a], cnx[a], cny[a], cw[a]);
+    cdg[a] = P.diag[pt];
   }
   __syncthreads();
   S* base = SH ? reinterpret_cast<S*>(smem_d + 6 * n) : War + B.W_off;
@@ -566,6 +639,7 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
   auto kval = [&](int a, int c) -> S {
     double v;
     if (a == c) v = cdg[c];
+    else if (P.kind == RSKX_KERNEL_LAPLACE3D_SL) v = sl3_entry(P, cx[a], cy[a], cnx[a], cx[c], cy[c], cnx[c], cw[c]);
     else {
       double dx = cx[a] - cx[c], dy = cy[a] - cy[c];
       v = P.scale * cw[c] * (dx * cnx[c] + dy * cny[c]) / (kTwoPi * (dx * dx + dy * dy));
@@ -807,7 +881,8 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   S* stg = reinterpret_cast<S*>(sh);   // per-thread assembly staging, RA x 1024
   double* ppx = sh + (RA * 1024 * sizeof(S) + 7) / 8;   // proxy points
   double* ppy = ppx + L.p;
-  double* sx = ppy + L.p;          // column points
+  double* ppz = ppy + L.p;
+  double* sx = ppz + L.p;          // column points (fields as point_fields)
   double* sy = sx + n;
   double* snx = sy + n;
   double* sny = snx + n;
@@ -825,21 +900,12 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   __shared__ double s_tau, s_tol;
   __shared__ int s_stop;
   for (int j = tid; j < n; j += 1024) {
-    const int pt = slf[j];
-    sx[j] = P.x[0][pt]; sy[j] = P.x[1][pt]; snx[j] = P.nu[0][pt]; sny[j] = P.nu[1][pt];
-    sw[j] = P.w[pt];
+    point_fields(P, slf[j], sx[j], sy[j], snx[j], sny[j], sw[j]);
     p2c[j] = j;
   }
-  for (int i = tid; i < mn; i += 1024) {
-    const int pt = nbr[i];
-    rx[i] = P.x[0][pt]; ry[i] = P.x[1][pt]; rnx[i] = P.nu[0][pt]; rny[i] = P.nu[1][pt];
-    rw[i] = P.w[pt];
-  }
-  for (int q = tid; q < L.p; q += 1024) {    // as proxy_entry
-    const double th = (double)(q + 1) * kTwoPi / (double)L.p;
-    ppx[q] = L.radius * cos(th) * B.l[0] + B.ctr[0];
-    ppy[q] = L.radius * sin(th) * B.l[1] + B.ctr[1];
-  }
+  for (int i = tid; i < mn; i += 1024)
+    point_fields(P, nbr[i], rx[i], ry[i], rnx[i], rny[i], rw[i]);
+  for (int q = tid; q < L.p; q += 1024) proxy_point(L, B, q, ppx[q], ppy[q], ppz[q]);
   if (tid == 0) mrows[b] = m;
   __syncthreads();
   if (m == 0 || n == 0) {
@@ -852,6 +918,7 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   // LDS slots and then moved to registers, which keeps the entry
   // evaluation from inflating register pressure.
   const double c0 = P.scale / kTwoPi;
+  const bool sl3 = P.kind == RSKX_KERNEL_LAPLACE3D_SL;
   S Kr[RA][CB];
 #pragma unroll
   for (int cb = 0; cb < CB; ++cb) {
@@ -865,18 +932,30 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
       double val = 0;
       if (cj && i < m) {
         if (i < mn) {
-          const double dx = rx[i] - xj, dy = ry[i] - yj;
-          val = c0 * wj * (dx * nxj + dy * nyj) / (dx * dx + dy * dy);
+          if (sl3) {
+            val = sl3_entry(P, rx[i], ry[i], rnx[i], xj, yj, nxj, wj);
+          } else {
+            const double dx = rx[i] - xj, dy = ry[i] - yj;
+            val = c0 * wj * (dx * nxj + dy * nyj) / (dx * dx + dy * dy);
+          }
         } else if (i < mt) {
           const int q = i - mn;
-          const double dx = xj - rx[q], dy = yj - ry[q];
-          val = c0 * rw[q] * (dx * rnx[q] + dy * rny[q]) / (dx * dx + dy * dy);
+          if (sl3) {
+            val = sl3_entry(P, xj, yj, nxj, rx[q], ry[q], rnx[q], rw[q]);
+          } else {
+            const double dx = xj - rx[q], dy = yj - ry[q];
+            val = c0 * rw[q] * (dx * rnx[q] + dy * rny[q]) / (dx * dx + dy * dy);
+          }
         } else {
           const int q = i - mt;
-          const double dx = ppx[q] - xj, dy = ppy[q] - yj;
-          const double r2 = dx * dx + dy * dy;
-          val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
-                                              : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
+          if (L.pkind == RSKX_PROXY_POINTS_SL) {
+            val = proxy_sl3(L, B, q, ppx[q], ppy[q], ppz[q], xj, yj, nxj, wj);
+          } else {
+            const double dx = ppx[q] - xj, dy = ppy[q] - yj;
+            const double r2 = dx * dx + dy * dy;
+            val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
+                                                : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
+          }
         }
       }
       stg[ra * 1024 + tid] = (S)val;
@@ -1054,7 +1133,8 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   double* red = prow + np;           // 2 NW
   double* ppx = red + 2 * NW;        // proxy points
   double* ppy = ppx + L.p;
-  double* sx = ppy + L.p;            // column points
+  double* ppz = ppy + L.p;
+  double* sx = ppz + L.p;            // column points (fields as point_fields)
   double* sy = sx + n;
   double* snx = sy + n;
   double* sny = snx + n;
@@ -1068,21 +1148,12 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   double* Rc = Tar + B.T_off + (int64_t)n * n;       // R rows by column
   double* Rp = Rc + (int64_t)n * n;                  // R rows by position
   for (int j = tid; j < n; j += NT) {
-    const int pt = slf[j];
-    sx[j] = P.x[0][pt]; sy[j] = P.x[1][pt]; snx[j] = P.nu[0][pt]; sny[j] = P.nu[1][pt];
-    sw[j] = P.w[pt];
+    point_fields(P, slf[j], sx[j], sy[j], snx[j], sny[j], sw[j]);
     p2c[j] = j;
   }
-  for (int i = tid; i < mn; i += NT) {
-    const int pt = nbr[i];
-    rx[i] = P.x[0][pt]; ry[i] = P.x[1][pt]; rnx[i] = P.nu[0][pt]; rny[i] = P.nu[1][pt];
-    rw[i] = P.w[pt];
-  }
-  for (int q = tid; q < L.p; q += NT) {    // as proxy_entry
-    const double th = (double)(q + 1) * kTwoPi / (double)L.p;
-    ppx[q] = L.radius * cos(th) * B.l[0] + B.ctr[0];
-    ppy[q] = L.radius * sin(th) * B.l[1] + B.ctr[1];
-  }
+  for (int i = tid; i < mn; i += NT)
+    point_fields(P, nbr[i], rx[i], ry[i], rnx[i], rny[i], rw[i]);
+  for (int q = tid; q < L.p; q += NT) proxy_point(L, B, q, ppx[q], ppy[q], ppz[q]);
   for (int j = tid; j < np; j += NT) prow[j] = 0;
   if (tid == 0) mrows[b] = m;
   __syncthreads();
@@ -1115,6 +1186,7 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   // G = K'K over chunks of RC rows of K = [A(nbr,slf); A(slf,nbr)'; Kpxy]
   // (rskelf.m:201-212), entries as in k_assemble
   const double c0 = P.scale / kTwoPi;
+  const bool sl3 = P.kind == RSKX_KERNEL_LAPLACE3D_SL;
   for (int r0 = 0; r0 < m; r0 += RC) {
     __syncthreads();
     for (int e = tid; e < RC * np; e += NT) {
@@ -1123,18 +1195,30 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
       if (i < m && j < n) {
         const double xj = sx[j], yj = sy[j];
         if (i < mn) {
-          const double dx = rx[i] - xj, dy = ry[i] - yj;
-          val = c0 * sw[j] * (dx * snx[j] + dy * sny[j]) / (dx * dx + dy * dy);
+          if (sl3) {
+            val = sl3_entry(P, rx[i], ry[i], rnx[i], xj, yj, snx[j], sw[j]);
+          } else {
+            const double dx = rx[i] - xj, dy = ry[i] - yj;
+            val = c0 * sw[j] * (dx * snx[j] + dy * sny[j]) / (dx * dx + dy * dy);
+          }
         } else if (i < mt) {
           const int q = i - mn;
-          const double dx = xj - rx[q], dy = yj - ry[q];
-          val = c0 * rw[q] * (dx * rnx[q] + dy * rny[q]) / (dx * dx + dy * dy);
+          if (sl3) {
+            val = sl3_entry(P, xj, yj, snx[j], rx[q], ry[q], rnx[q], rw[q]);
+          } else {
+            const double dx = xj - rx[q], dy = yj - ry[q];
+            val = c0 * rw[q] * (dx * rnx[q] + dy * rny[q]) / (dx * dx + dy * dy);
+          }
         } else {
           const int q = i - mt;
-          const double dx = ppx[q] - xj, dy = ppy[q] - yj;
-          const double r2 = dx * dx + dy * dy;
-          val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
-                                              : sw[j] * (dx * snx[j] + dy * sny[j]) / (kTwoPi * r2);
+          if (L.pkind == RSKX_PROXY_POINTS_SL) {
+            val = proxy_sl3(L, B, q, ppx[q], ppy[q], ppz[q], xj, yj, snx[j], sw[j]);
+          } else {
+            const double dx = ppx[q] - xj, dy = ppy[q] - yj;
+            const double r2 = dx * dx + dy * dy;
+            val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
+                                                : sw[j] * (dx * snx[j] + dy * sny[j]) / (kTwoPi * r2);
+          }
         }
       }
       chunk[rr * np + j] = val;
@@ -1257,7 +1341,7 @@ int gram_id_class(int n) {
 }
 size_t gram_id_shmem(int cls, int n, int mn, int m, int p) {
   const int np = (n + 3) / 4 * 4, nw = (cls == 0 ? 256 : 512) / 64;
-  return (size_t)(16 * np + 2 * p + 5 * n + 5 * mn + 2 * np + 2 * nw) * sizeof(double) +
+  return (size_t)(16 * np + 3 * p + 5 * n + 5 * mn + 2 * np + 2 * nw) * sizeof(double) +
          (size_t)n * sizeof(int) + 64;
 }
 
@@ -1299,24 +1383,21 @@ void launch_lists(const LvlBox* boxes, const Seg* nsegs, const Seg* csegs, int* 
 template <typename S>
 void launch_assemble(const int* ids, const LvlBox* boxes, const int* pool, const int* nbrar,
                      const int* mnbr, int* mrows, S* Kar, const PointsDev& P, const LevelParams& L,
-                     int nb, size_t shmem, hipStream_t st) {
-  if (shmem > 65536)
-    (void)hipFuncSetAttribute((const void*)k_assemble<S>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  hipLaunchKernelGGL(k_assemble<S>, dim3(nb), dim3(256), shmem, st, ids, boxes, pool, nbrar,
-                     mnbr, mrows, Kar, P, L);
+                     int nb, hipStream_t st) {
+  hipLaunchKernelGGL(k_assemble<S>, dim3(nb), dim3(256), 0, st, ids, boxes, pool, nbrar, mnbr,
+                     mrows, Kar, P, L);
 }
 
 template <typename S>
 void launch_id(int mode, const int* ids, const LvlBox* boxes, const int* mrows, S* Kar,
                int* permar, double* Tar, const IdParams& prm, int* kout, int* iterout, int nb,
-               size_t shmem, hipStream_t st) {
+               size_t shmem, double* vglob, int64_t vstride, hipStream_t st) {
   auto kern = mode == 0 ? k_id<S, 0> : mode == 1 ? k_id<S, 1> : k_id<S, 2>;
   if (shmem > 65536)
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)shmem);
   hipLaunchKernelGGL(kern, dim3(nb), dim3(256), shmem, st, ids, boxes, mrows, Kar, permar, Tar, prm,
-                     kout, iterout);
+                     kout, iterout, vglob, vstride);
 }
 
 // value of lane l (uniform l) broadcast to the wave
@@ -1399,15 +1480,16 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
     const int sp = p < r ? q[k + p] : q[p - r];
     qp[p] = sp;
     const int pt = slf[sp];
-    cx[p] = P.x[0][pt]; cy[p] = P.x[1][pt];
-    cnx[p] = P.nu[0][pt]; cny[p] = P.nu[1][pt];
-    cw[p] = P.w[pt]; cdg[p] = P.diag[pt];
+    point_fields(P, pt, cx[p], cy[p], cnx[p], cny[p], cw[p]);
+    cdg[p] = P.diag[pt];
     if (LU && p < r) { lpos[p] = p; prow[p] = p; }
   }
   __syncthreads();
   const bool leafbox = cm.nc == 1 && cm.moff[0] < 0;
   auto aval = [&](int p1, int p2) -> double {    // A entry between Z positions
     if (p1 == p2) return cdg[p2];
+    if (P.kind == RSKX_KERNEL_LAPLACE3D_SL)
+      return sl3_entry(P, cx[p1], cy[p1], cnx[p1], cx[p2], cy[p2], cnx[p2], cw[p2]);
     double dx = cx[p1] - cx[p2], dy = cy[p1] - cy[p2];
     return P.scale * cw[p2] * (dx * cnx[p2] + dy * cny[p2]) / (kTwoPi * (dx * dx + dy * dy));
   };
@@ -1895,9 +1977,10 @@ void launch_elim(bool lu, int mode, size_t shmem, const int* ids, int nb, const 
 #define RSKX_INST(S)                                                                          \
   template void launch_assemble<S>(const int*, const LvlBox*, const int*, const int*,          \
                                    const int*, int*, S*, const PointsDev&, const LevelParams&,  \
-                                   int, size_t, hipStream_t);                                   \
+                                   int, hipStream_t);                                           \
   template void launch_id<S>(int, const int*, const LvlBox*, const int*, S*, int*, double*,    \
-                             const IdParams&, int*, int*, int, size_t, hipStream_t);           \
+                             const IdParams&, int*, int*, int, size_t, double*, int64_t,       \
+                             hipStream_t);                                                     \
   template void launch_id_reg<S>(int, const int*, const LvlBox*, const int*, const int*,        \
                                  const int*, int*, S*, int*, const PointsDev&, const LevelParams&, \
                                  const IdParams&, int*, int, size_t, hipStream_t);             \

@@ -192,9 +192,12 @@ template <typename S, int PROG, bool UP>
 __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
                                                const S* __restrict__ fval,
                                                const int* __restrict__ fidx, S* X, int64_t ldx,
-                                               int nrhs, int kmax, int rmax, int bcap) {
+                                               int nrhs, int kmax, int rmax, int bcap, S* gws,
+                                               int64_t gstride) {
   extern __shared__ char smem_raw[];
-  S* sm = reinterpret_cast<S*>(smem_raw);
+  // rows of X and the work vectors live in shared memory, or for factors
+  // too large for it in this workgroup's slice of a global workspace
+  S* sm = gws ? gws + blockIdx.x * gstride : reinterpret_cast<S*>(smem_raw);
   const FacRec f = recs[blockIdx.x];
   const int k = f.k, r = f.r;
   const int* sk = fidx + f.sk;
@@ -317,8 +320,8 @@ __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
 
 template <typename S, int PROG>
 static void launch_prog(bool up, const FacRec* recs, int nrec, const S* fval, const int* fidx, S* X,
-                        int64_t ldx, int nrhs, int kmax, int rmax, int bcap, size_t shmem,
-                        hipStream_t st) {
+                        int64_t ldx, int nrhs, int kmax, int rmax, int bcap, size_t shmem, S* gws,
+                        int64_t gstride, hipStream_t st) {
   if (shmem > 65536) {
     (void)hipFuncSetAttribute((const void*)k_sweep<S, PROG, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
@@ -327,23 +330,23 @@ static void launch_prog(bool up, const FacRec* recs, int nrec, const S* fval, co
   }
   if (up)
     hipLaunchKernelGGL((k_sweep<S, PROG, true>), dim3(nrec), dim3(256), shmem, st, recs, fval, fidx,
-                       X, ldx, nrhs, kmax, rmax, bcap);
+                       X, ldx, nrhs, kmax, rmax, bcap, gws, gstride);
   else
     hipLaunchKernelGGL((k_sweep<S, PROG, false>), dim3(nrec), dim3(256), shmem, st, recs, fval,
-                       fidx, X, ldx, nrhs, kmax, rmax, bcap);
+                       fidx, X, ldx, nrhs, kmax, rmax, bcap, gws, gstride);
 }
 
 template <typename S>
 void launch_sweep(int prog, bool up, const FacRec* recs, int nrec, const S* fval, const int* fidx,
-                  S* X, int64_t ldx, int nrhs, int kmax, int rmax, int bcap, size_t shmem,
-                  hipStream_t st) {
+                  S* X, int64_t ldx, int nrhs, int kmax, int rmax, int bcap, size_t shmem, S* gws,
+                  int64_t gstride, hipStream_t st) {
   switch (prog) {
-    case 0: launch_prog<S, 0>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
-    case 1: launch_prog<S, 1>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
-    case 2: launch_prog<S, 2>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
-    case 3: launch_prog<S, 3>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
-    case 4: launch_prog<S, 4>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
-    default: launch_prog<S, 5>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, st); break;
+    case 0: launch_prog<S, 0>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, gws, gstride, st); break;
+    case 1: launch_prog<S, 1>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, gws, gstride, st); break;
+    case 2: launch_prog<S, 2>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, gws, gstride, st); break;
+    case 3: launch_prog<S, 3>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, gws, gstride, st); break;
+    case 4: launch_prog<S, 4>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, gws, gstride, st); break;
+    default: launch_prog<S, 5>(up, recs, nrec, fval, fidx, X, ldx, nrhs, kmax, rmax, bcap, shmem, gws, gstride, st); break;
   }
 }
 
@@ -535,9 +538,11 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(const FacRec* __restric
                                                          const S* __restrict__ fval,
                                                          const int* __restrict__ fidx, S* X,
                                                          int64_t ldx, int nrhs, int kmax,
-                                                         int rmax) {
+                                                         int rmax, S* gws, int64_t gstride) {
   extern __shared__ char smem_raw[];
-  S* xs = reinterpret_cast<S*>(smem_raw);
+  // shared memory, or this workgroup's slice of a global workspace for
+  // factors whose rows do not fit
+  S* xs = gws ? gws + blockIdx.x * gstride : reinterpret_cast<S*>(smem_raw);
   S* xr = xs + (size_t)kmax * kRhs;
   S* red = xr + (size_t)rmax * kRhs;
   S* tmp = red + (size_t)kRhs * kSolveThreads;
@@ -611,7 +616,8 @@ size_t solve_shmem(int prog, int kmax, int rmax, size_t es) {
 
 template <typename S>
 void launch_solve(int prog, bool up, const FacRec* recs, int nrec, const S* fval, const int* fidx,
-                  S* X, int64_t ldx, int nrhs, int kmax, int rmax, size_t shmem, hipStream_t st) {
+                  S* X, int64_t ldx, int nrhs, int kmax, int rmax, size_t shmem, S* gws,
+                  int64_t gstride, hipStream_t st) {
 #define RSKX_SOLVE(P)                                                                           \
   {                                                                                             \
     auto kern = up ? k_solve<S, P, true> : k_solve<S, P, false>;                                \
@@ -619,7 +625,7 @@ void launch_solve(int prog, bool up, const FacRec* recs, int nrec, const S* fval
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                 (int)shmem);                                                    \
     hipLaunchKernelGGL(kern, dim3(nrec), dim3(kSolveThreads), shmem, st, recs, fval, fidx, X,    \
-                       ldx, nrhs, kmax, rmax);                                                  \
+                       ldx, nrhs, kmax, rmax, gws, gstride);                                    \
   }
   if (prog == 0) RSKX_SOLVE(0) else if (prog == 1) RSKX_SOLVE(1) else RSKX_SOLVE(2)
 #undef RSKX_SOLVE
@@ -737,12 +743,79 @@ __global__ void __launch_bounds__(256) k_matvec(const double* __restrict__ x, in
   }
 }
 
-void launch_matvec(const double* x, int64_t N, const double* nu, const double* w,
+// Y = A X for RSKX_KERNEL_LAPLACE3D_SL: A_ij = scale w_j / (4 pi |x_i - x_j|),
+// A_ii = diag_i; x column-major 3 x N; blocks of problems as k_matvec
+constexpr double kFourPiM = 12.566370614359172953850573533118;
+
+__global__ void __launch_bounds__(256) k_matvec_sl3(const double* __restrict__ x, int64_t N,
+                                                    const double* __restrict__ w,
+                                                    const double* __restrict__ diag, double scale,
+                                                    const double* __restrict__ X, int64_t ldx,
+                                                    double* __restrict__ Y, int64_t ldy, int nrhs,
+                                                    const int64_t* __restrict__ poff, int nprob) {
+  __shared__ double sx[256], sy[256], sz[256], sw[256];
+  __shared__ double sX[8][256];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool act = i < N;
+  const double xi = act ? x[3 * i] : 0, yi = act ? x[3 * i + 1] : 0, zi = act ? x[3 * i + 2] : 0;
+  int64_t my0 = 0, my1 = N, blk0 = 0, blk1 = N;
+  if (poff) {
+    const int64_t i0 = blockIdx.x * (int64_t)blockDim.x;
+    const int64_t i1 = i0 + blockDim.x - 1 < N - 1 ? i0 + blockDim.x - 1 : N - 1;
+    const int pa = problem_of(poff, nprob, i0), pb = problem_of(poff, nprob, i1);
+    blk0 = poff[pa];
+    blk1 = poff[pb + 1];
+    if (act) {
+      const int pi = problem_of(poff, nprob, i);
+      my0 = poff[pi];
+      my1 = poff[pi + 1];
+    }
+  }
+  const double c0 = scale / kFourPiM;
+  for (int c0r = 0; c0r < nrhs; c0r += 8) {
+    const int cc = nrhs - c0r < 8 ? nrhs - c0r : 8;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t j0 = blk0; j0 < blk1; j0 += 256) {
+      __syncthreads();
+      int64_t j = j0 + threadIdx.x;
+      if (j < blk1) {
+        sx[threadIdx.x] = x[3 * j];
+        sy[threadIdx.x] = x[3 * j + 1];
+        sz[threadIdx.x] = x[3 * j + 2];
+        sw[threadIdx.x] = w[j];
+        for (int c = 0; c < cc; ++c) sX[c][threadIdx.x] = X[j + (c0r + c) * ldx];
+      }
+      __syncthreads();
+      const int jn = blk1 - j0 < 256 ? (int)(blk1 - j0) : 256;
+      for (int t = 0; t < jn; ++t) {
+        int64_t jj = j0 + t;
+        if (jj < my0 || jj >= my1) continue;   // another problem's source
+        double a;
+        if (jj == i) a = diag[jj];
+        else {
+          const double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
+          a = c0 * sw[t] / sqrt(dx * dx + dy * dy + dz * dz);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) if (c < cc) acc[c] += a * sX[c][t];
+      }
+    }
+    if (act)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) if (c < cc) Y[i + (c0r + c) * ldy] = acc[c];
+  }
+}
+
+void launch_matvec(int kind, const double* x, int64_t N, const double* nu, const double* w,
                    const double* diag, double scale, const double* X, int64_t ldx, double* Y,
                    int64_t ldy, int nrhs, const int64_t* poff, int nprob, hipStream_t st) {
   int grid = (int)((N + 255) / 256);
-  hipLaunchKernelGGL(k_matvec, dim3(grid), dim3(256), 0, st, x, N, nu, w, diag, scale, X, ldx, Y,
-                     ldy, nrhs, poff, nprob);
+  if (kind == RSKX_KERNEL_LAPLACE3D_SL)
+    hipLaunchKernelGGL(k_matvec_sl3, dim3(grid), dim3(256), 0, st, x, N, w, diag, scale, X, ldx,
+                       Y, ldy, nrhs, poff, nprob);
+  else
+    hipLaunchKernelGGL(k_matvec, dim3(grid), dim3(256), 0, st, x, N, nu, w, diag, scale, X, ldx,
+                       Y, ldy, nrhs, poff, nprob);
 }
 
 // points, normals, weights and diagonal into tree-order structure of arrays
@@ -755,7 +828,7 @@ __global__ void k_gather_points(const double* __restrict__ x, const double* __re
     int64_t q = order[p];
     for (int i = 0; i < d; ++i) {
       pts[i * N + p] = x[d * q + i];
-      pts[(d + i) * N + p] = nu[d * q + i];
+      pts[(d + i) * N + p] = nu ? nu[d * q + i] : 0.0;   // kernels without normals
     }
     pts[2 * d * N + p] = w[q];
     pts[(2 * d + 1) * N + p] = diag[q];
@@ -771,14 +844,14 @@ void launch_gather_points(const double* x, const double* nu, const double* w, co
 
 #define RSKX_SWEEP_INST(S)                                                                    \
   template void launch_sweep<S>(int, bool, const FacRec*, int, const S*, const int*, S*,      \
-                                int64_t, int, int, int, int, size_t, hipStream_t);            \
+                                int64_t, int, int, int, int, size_t, S*, int64_t, hipStream_t); \
   template void launch_perm<double, S>(bool, void*, int64_t, S*, int64_t, int, int,         \
                                        const int*, hipStream_t);                              \
   template void launch_perm<float, S>(bool, void*, int64_t, S*, int64_t, int, int,          \
                                       const int*, hipStream_t);                               \
   template void launch_pack<S>(bool, const FacRec*, int, S*, hipStream_t);                    \
   template void launch_solve<S>(int, bool, const FacRec*, int, const S*, const int*, S*,      \
-                                int64_t, int, int, int, size_t, hipStream_t);
+                                int64_t, int, int, int, size_t, S*, int64_t, hipStream_t);
 RSKX_SWEEP_INST(double)
 RSKX_SWEEP_INST(float)
 

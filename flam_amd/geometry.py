@@ -8,12 +8,14 @@ diagonal) on the GPU, following the problem set-up of the reference demos:
                 diagonal limit w (-1/4pi), w = 2pi/N
   ie_circle(N)  rskelf/test/ie_circle.m:34,85-103 (A = D w, diag -1/2(1+1/N))
   ellipse(N,a,b) rskelf/test/ie_ellipse.m:28-37,103-123 with semi-axes (a, b)
+  sphere(nface) 3D Laplace single layer on a triangulated unit sphere
+                (points and proxies after rskelf/test/ie_sphere.m:36-40)
 """
 import math
 
 import torch
 
-from .rskelf import LaplaceDL2D, ProxyRing
+from .rskelf import LaplaceDL2D, LaplaceSL3D, ProxyPoints, ProxyRing
 
 
 class Curve:
@@ -85,3 +87,89 @@ def ellipse(N, a=1.0, b=1.0, nproxy=64, device="cuda"):
     diag = -0.5 * (1 + h * kappa / (2 * math.pi))
     proxy = ProxyRing(p=nproxy, radius=1.5, kind="dl", weight=0.0)
     return Curve(x, nu, h, diag, 1.0, proxy)
+
+
+# ---------------------------------------------------------------- sphere --
+_ICO_FACES = [[1, 12, 6], [1, 6, 2], [1, 2, 8], [1, 8, 11], [1, 11, 12], [2, 6, 10], [6, 12, 5],
+              [12, 11, 3], [11, 8, 7], [8, 2, 9], [4, 10, 5], [4, 5, 3], [4, 3, 7], [4, 7, 9],
+              [4, 9, 10], [5, 10, 6], [3, 5, 12], [7, 3, 11], [9, 7, 8], [10, 9, 2]]
+
+
+def icosphere(n, by="faces"):
+    """Unit-sphere triangulation from a subdivided icosahedron with at least
+    n faces (or vertices), in the vertex and face order of
+    geom/trisphere_subdiv.m:12-86: new vertices are the projected edge
+    midpoints in ascending lexicographic order, appended after the old
+    ones; each face is replaced by its four children, listed child-major
+    (all first children, then all second, ...).  FP64 tensors on the CPU:
+    vertices (nv, 3), faces (nf, 3) 0-based."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = torch.tensor([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                      [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]],
+                     dtype=torch.float64)
+    V = V / torch.linalg.vector_norm(V[0])
+    F = torch.tensor(_ICO_FACES, dtype=torch.int64) - 1
+    size = (lambda: V.shape[0]) if by == "vertices" else (lambda: F.shape[0])
+    while size() < n:
+        a, b, c = F[:, 0], F[:, 1], F[:, 2]
+        mids = 0.5 * torch.cat([V[a] + V[b], V[a] + V[c], V[b] + V[c]])
+        mids = mids / torch.linalg.vector_norm(mids, dim=1, keepdim=True)
+        uniq, inv = torch.unique(mids, dim=0, return_inverse=True)
+        nv, nf = V.shape[0], F.shape[0]
+        ab, ac, bc = (nv + inv[:nf], nv + inv[nf:2 * nf], nv + inv[2 * nf:])
+        V = torch.cat([V, uniq])
+        F = torch.cat([torch.stack([a, ab, ac], 1), torch.stack([ab, b, bc], 1),
+                       torch.stack([ac, bc, c], 1), torch.stack([ab, bc, ac], 1)])
+    return V, F
+
+
+def _self_panel(V, F, C):
+    """Integral of 1/(4 pi r) over each flat triangle from its centroid:
+    sum over the edges of h (asinh(s2/h) - asinh(s1/h)) / (4 pi), h the
+    centroid's distance to the edge's line, s1 < s2 the signed positions
+    of the edge's ends along it from the foot of the perpendicular."""
+    tot = torch.zeros(F.shape[0], dtype=torch.float64)
+    for e in range(3):
+        a, b = V[F[:, e]], V[F[:, (e + 1) % 3]]
+        edge = b - a
+        length = torch.linalg.vector_norm(edge, dim=1)
+        u = edge / length[:, None]
+        s1 = ((a - C) * u).sum(1)
+        h = torch.linalg.vector_norm(a - s1[:, None] * u - C, dim=1)
+        tot += h * (torch.asinh((s1 + length) / h) - torch.asinh(s1 / h))
+    return tot / (4 * math.pi)
+
+
+class Surface:
+    """Collocation points of a triangulated surface with the 3D Laplace
+    single-layer kernel (LaplaceSL3D) and its point proxies."""
+
+    def __init__(self, x, w, diag, scale, proxy):
+        self.x, self.w, self.diag, self.scale, self.proxy = x, w, diag, scale, proxy
+        self.N = x.shape[1]
+
+    @property
+    def kernel(self):
+        return LaplaceSL3D(self.w, self.diag, self.scale)
+
+
+def sphere(nface, nproxy=512, seed=0, device="cuda"):
+    """First-kind Laplace single layer on the unit sphere (the C4 workload
+    at nface = 327680): triangle centroids of icosphere(nface), A_ij =
+    area_j / (4 pi |x_i - x_j|), A_ii the self-panel integral; proxies are
+    nproxy vertices of icosphere(nproxy, 'vertices') chosen by a seeded
+    permutation (rskelf/test/ie_sphere.m:39-40 draws an unseeded one), at
+    one box size from the centre, with incoming rows weighted by the mean
+    area 4 pi / N (ie_sphere.m:175-183) and neighbours kept within 1.5 box
+    sizes (ie_sphere.m:186)."""
+    import numpy as np
+    V, F = icosphere(nface)
+    P0, P1, P2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    C = (P0 + P1 + P2) / 3
+    area = 0.5 * torch.linalg.vector_norm(torch.linalg.cross(P1 - P0, P2 - P1), dim=1)
+    diag = _self_panel(V, F, C)
+    N = C.shape[0]
+    Pv, _ = icosphere(nproxy, by="vertices")
+    sel = torch.from_numpy(np.random.default_rng(seed).permutation(Pv.shape[0])[:nproxy])
+    proxy = ProxyPoints(Pv[sel].t().contiguous().to(device), radius=1.5, weight=4 * math.pi / N)
+    return Surface(C.t().contiguous().to(device), area.to(device), diag.to(device), 1.0, proxy)

@@ -65,6 +65,47 @@ class LaplaceDL2D:
 
 
 @dataclass
+class LaplaceSL3D:
+    """A_ij = scale w_j / (4 pi |x_i - x_j|), A_ii = diag_i, points in 3D.
+
+    w: (N,) quadrature weights (triangle areas), diag: (N,) self-panel
+    integrals.  The single-layer kernel of rskelf/test/ie_sphere.m:149-160
+    with one-point quadrature off the diagonal."""
+    w: torch.Tensor
+    diag: torch.Tensor
+    scale: float = 1.0
+    nu = None
+
+    def _struct(self):
+        self.w = _dev(self.w).reshape(-1)
+        self.diag = _dev(self.diag).reshape(-1)
+        return L.Kernel(L.RSKX_KERNEL_LAPLACE3D_SL, None, self.w.data_ptr(),
+                        self.diag.data_ptr(), float(self.scale))
+
+
+@dataclass
+class ProxyPoints:
+    """pxyfun of rskelf/test/ie_sphere.m:174-187: proxy points pts (3, p) in
+    box-size units about the box centre (pxy = pts l + ctr), single-layer
+    rows weighted by the column's w (outgoing) and, for symm 'n', by
+    `weight` (incoming, 4 pi / N there); neighbours kept inside `radius`
+    box sizes (1.5)."""
+    pts: torch.Tensor
+    radius: float = 1.5
+    weight: float = 0.0
+
+    @property
+    def p(self):
+        return int(self.pts.shape[1])
+
+    def _struct(self):
+        self.pts = _dev(self.pts).reshape(3, -1)
+        self._pts_cm = self.pts.t().contiguous()      # column-major 3 x p
+        return L.Proxy(L.RSKX_PROXY_POINTS_SL, self.p, float(self.radius), float(self.weight),
+                       self._pts_cm.data_ptr())
+
+
+@dataclass
 class ProxyRing:
     """pxyfun of the demos: p points on a ring of radius `radius` box sizes
     around the box centre; neighbours inside the ring are kept explicitly.
@@ -77,7 +118,7 @@ class ProxyRing:
 
     def _struct(self):
         kind = {"sl": L.RSKX_PROXY_RING_SL, "dl": L.RSKX_PROXY_RING_DL}[self.kind]
-        return L.Proxy(kind, int(self.p), float(self.radius), float(self.weight))
+        return L.Proxy(kind, int(self.p), float(self.radius), float(self.weight), None)
 
 
 _PREC = {"fp64": L.RSKX_PREC_FP64, "fp32": L.RSKX_PREC_FP32}
@@ -227,19 +268,26 @@ class Batch:
 
 
 def stack_problems(problems):
-    """Batch of problems given as objects with x (d x N_j), kernel
-    (LaplaceDL2D, one common scale) and proxy (ProxyRing)."""
+    """Batch of problems given as objects with x (d x N_j), kernel (all
+    LaplaceDL2D or all LaplaceSL3D, one common scale) and proxy."""
     scales = {float(p.kernel.scale) for p in problems}
     if len(scales) != 1:
         raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:batch: the problems' kernels must share one scale")
     offs = [0]
     for p in problems:
         offs.append(offs[-1] + int(p.x.shape[1]))
+    kinds = {type(p.kernel) for p in problems}
+    if len(kinds) != 1:
+        raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:batch: the problems' kernels must be of one kind")
     x = torch.cat([_dev(p.x) for p in problems], dim=1)
-    nu = torch.cat([_dev(p.kernel.nu).reshape(2, -1) for p in problems], dim=1)
     w = torch.cat([_dev(p.kernel.w).reshape(-1) for p in problems])
     diag = torch.cat([_dev(p.kernel.diag).reshape(-1) for p in problems])
-    return Batch(x, LaplaceDL2D(nu, w, diag, scales.pop()), offs, [p.proxy for p in problems])
+    if kinds.pop() is LaplaceSL3D:
+        kern = LaplaceSL3D(w, diag, scales.pop())
+    else:
+        nu = torch.cat([_dev(p.kernel.nu).reshape(2, -1) for p in problems], dim=1)
+        kern = LaplaceDL2D(nu, w, diag, scales.pop())
+    return Batch(x, kern, offs, [p.proxy for p in problems])
 
 
 def rskelf_batch(batch, occ, rank_or_tol, opts=None):
@@ -256,7 +304,16 @@ def rskelf_batch(batch, occ, rank_or_tol, opts=None):
     k = batch.kernel._struct()
     nprob = len(batch.offsets) - 1
     offs = (ctypes.c_int64 * (nprob + 1))(*batch.offsets)
-    pxs = (L.Proxy * nprob)(*[p._struct() for p in batch.proxies])
+    structs = [p._struct() for p in batch.proxies]
+    if structs[0].kind == L.RSKX_PROXY_POINTS_SL:
+        # the library takes one proxy point set for the batch (weights may differ)
+        p0 = batch.proxies[0]
+        for p, st in zip(batch.proxies[1:], structs[1:]):
+            if p.pts.shape != p0.pts.shape or not torch.equal(p.pts, p0.pts):
+                raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:batch: the problems' proxy "
+                                                  "surfaces must share their points")
+            st.pts = structs[0].pts
+    pxs = (L.Proxy * nprob)(*structs)
     h = ctypes.c_void_p()
     torch.cuda.synchronize()
     _check(L.lib().rskx_rskelf_batch(nprob, offs, xc.data_ptr(), d, int(occ), float(rank_or_tol),

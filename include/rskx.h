@@ -49,7 +49,12 @@ extern "C" {
  *   A_ii = diag_i
  * rskelf/test/ie_circle.m:85-103 (scale 1, diag -1/2(1+1/N)),
  * rskelf/test/ie_ellipse.m:103-123 (scale 1, diag -1/2(1 + h kappa/2pi)),
- * and the benchmark circle (scale -1, diag 1/2 - w/4pi). */
+ * and the benchmark circle (scale -1, diag 1/2 - w/4pi).  d = 2. */
+#define RSKX_KERNEL_LAPLACE3D_SL 2
+/*   A_ij = scale * w_j / (4 pi |x_i - x_j|), i != j;  A_ii = diag_i
+ * the single-layer kernel of rskelf/test/ie_sphere.m:149-160 with
+ * one-point quadrature (w = triangle areas) and the self-panel integral on
+ * the diagonal.  d = 3; nu is not read (may be NULL). */
 typedef struct {
   int kind;             /* RSKX_KERNEL_*                                    */
   const double* nu;     /* d x N unit normals (device)                      */
@@ -62,12 +67,23 @@ typedef struct {
 #define RSKX_PROXY_NONE 0     /* naive global compression (rskelf.m:204)     */
 #define RSKX_PROXY_RING_SL 1  /* ring, single layer -w/2pi log r (ie_circle.m:106-115) */
 #define RSKX_PROXY_RING_DL 2  /* ring, double layer of slf (ie_ellipse.m:126-132) */
+#define RSKX_PROXY_POINTS_SL 3 /* given points, single layer (ie_sphere.m:174-187):
+                                 pxy = pts * l + ctr; rows w_j G(pxy, x_j) and,
+                                 for symm 'n', weight * G(x_j, pxy) (incoming);
+                                 d = 3 */
 typedef struct {
   int kind;             /* RSKX_PROXY_*                                     */
   int p;                /* number of proxy points (64 in the demos)         */
   double radius;        /* ring radius and neighbour filter radius, in units
-                           of the box size (1.5 in ie_circle.m:35,114)      */
-  double weight;        /* RING_SL multiplier (2 pi / N in ie_circle.m:111) */
+                           of the box size (1.5 in ie_circle.m:35,114);
+                           POINTS_SL: the filter radius only (1.5 in
+                           ie_sphere.m:186)                                  */
+  double weight;        /* RING_SL multiplier (2 pi / N in ie_circle.m:111);
+                           POINTS_SL: incoming-row multiplier (4 pi / N,
+                           the mean triangle area, ie_sphere.m:179)          */
+  const double* pts;    /* POINTS_SL: 3 x p points (device, column-major) in
+                           box-size units about the box centre (the unit
+                           sphere of ie_sphere.m:39-40 and :175); else NULL  */
 } rskx_proxy;
 
 /* options: rskelf.m:112-128 */
@@ -112,7 +128,8 @@ typedef struct {
 int rskx_opts_default(rskx_opts* opts);
 
 /* F = rskelf(A,x,occ,rank_or_tol,pxyfun,opts)   -- rskelf/rskelf.m:109
- * x: device d x N coordinates.  d must be 2 for RSKX_KERNEL_LAPLACE2D_DL. */
+ * x: device d x N coordinates.  d must be 2 for RSKX_KERNEL_LAPLACE2D_DL
+ * with a RING proxy, 3 for RSKX_KERNEL_LAPLACE3D_SL with POINTS_SL. */
 int rskx_rskelf(const double* x, int d, int64_t N, int occ, double rank_or_tol,
                 const rskx_kernel* A, const rskx_proxy* pxy, const rskx_opts* opts,
                 rskx_factor** out);

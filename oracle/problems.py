@@ -120,3 +120,156 @@ class CurveDL:
         B = np.asarray(B, dtype=float)
         s = B.sum(axis=0)
         return (B - beta / (alpha + N * beta) * s) / alpha
+
+
+# --------------------------------------------------------------------------
+# 3D: triangulated unit sphere, Laplace single layer
+# --------------------------------------------------------------------------
+def trisphere_subdiv(n, vfmin="f"):
+    """Vertices (3 x nv) and faces (3 x nf, 0-based) of the unit sphere from
+    a recursively subdivided icosahedron, subdividing until the vertex
+    ('v') or face ('f') count is at least n; geom/trisphere_subdiv.m:12-86
+    (base icosahedron, edge midpoints projected to the sphere, duplicate
+    midpoints merged by a sorted unique over rows, each face split in 4)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=float)
+    V = V / np.linalg.norm(V[0])
+    F = np.array([[1, 12, 6], [1, 6, 2], [1, 2, 8], [1, 8, 11], [1, 11, 12],
+                  [2, 6, 10], [6, 12, 5], [12, 11, 3], [11, 8, 7], [8, 2, 9],
+                  [4, 10, 5], [4, 5, 3], [4, 3, 7], [4, 7, 9], [4, 9, 10],
+                  [5, 10, 6], [3, 5, 12], [7, 3, 11], [9, 7, 8], [10, 9, 2]]) - 1
+    count = (lambda: V.shape[0]) if vfmin == "v" else (lambda: F.shape[0])
+    while count() < n:
+        M = 0.5 * np.concatenate([V[F[:, 0]] + V[F[:, 1]],
+                                  V[F[:, 0]] + V[F[:, 2]],
+                                  V[F[:, 1]] + V[F[:, 2]]])
+        M = M / np.sqrt((M ** 2).sum(axis=1))[:, None]
+        M, I = np.unique(M, axis=0, return_inverse=True)
+        I = I.reshape(-1)
+        nv, nf = V.shape[0], F.shape[0]
+        F12, F13, F23 = nv + I[:nf], nv + I[nf:2 * nf], nv + I[2 * nf:]
+        V = np.concatenate([V, M])
+        F = np.concatenate([np.stack([F[:, 0], F12, F13], 1), np.stack([F12, F[:, 1], F23], 1),
+                            np.stack([F13, F23, F[:, 2]], 1), np.stack([F12, F23, F13], 1)])
+    return V.T.copy(), F.T.copy()
+
+
+def tri3geom(V, F):
+    """Centroid, unit normal and area of each triangle (geom/tri3geom.m:9-22)."""
+    P0, P1, P2 = V[:, F[0]], V[:, F[1]], V[:, F[2]]
+    C = (P0 + P1 + P2) / 3.0
+    Nv = np.cross((P1 - P0).T, (P2 - P1).T).T
+    A = np.sqrt((Nv ** 2).sum(axis=0))
+    return C, Nv / A, 0.5 * A
+
+
+def self_panel_sl(V, F, C):
+    """Integral of 1/(4 pi |c - y|) over each flat triangle for a point c in
+    its plane and inside it (the centroid): split the triangle at c into
+    three triangles with apex c; over one of them, with its base edge at
+    distance h from c and the base's ends at signed offsets s1 < s2 from
+    the foot of the perpendicular, the integral of 1/r in polar
+    coordinates about c is h (asinh(s2/h) - asinh(s1/h))."""
+    out = np.zeros(F.shape[1])
+    for e in range(3):
+        a = V[:, F[e]]
+        b = V[:, F[(e + 1) % 3]]
+        u = b - a
+        L = np.sqrt((u ** 2).sum(axis=0))
+        u = u / L
+        s1 = ((a - C) * u).sum(axis=0)            # offsets of the ends along the edge
+        s2 = s1 + L
+        foot = a - s1 * u
+        h = np.sqrt(((foot - C) ** 2).sum(axis=0))
+        out += h * (np.arcsinh(s2 / h) - np.arcsinh(s1 / h))
+    return out / (4.0 * math.pi)
+
+
+def self_panel_sl_quadrature(V, F, C, order=24):
+    """The same integral by brute force: Duffy-mapped Gauss-Legendre
+    quadrature on the three sub-triangles with apex at c (the map's
+    Jacobian cancels the 1/r singularity), the construction of
+    quad/quad_sqtri3.m.  A check of self_panel_sl, not used otherwise."""
+    g, wg = np.polynomial.legendre.leggauss(order)
+    g = 0.5 * (g + 1.0)
+    wg = 0.5 * wg
+    out = np.zeros(F.shape[1])
+    for e in range(3):
+        a = V[:, F[e]]
+        b = V[:, F[(e + 1) % 3]]
+        for s, ws in zip(g, wg):
+            for t, wt in zip(g, wg):
+                # y = c + s ((a - c) + t (b - a)), dA = s |(a-c) x (b-a)| ds dt
+                y = C + s * ((a - C) + t * (b - a))
+                jac = s * np.linalg.norm(np.cross((a - C).T, (b - a).T), axis=1)
+                r = np.sqrt(((y - C) ** 2).sum(axis=0))
+                out += ws * wt * jac / r
+    return out / (4.0 * math.pi)
+
+
+class SphereSL:
+    """First-kind Laplace single-layer matrix on the triangulated unit
+    sphere: collocation at triangle centroids with one-point quadrature,
+    A_ij = area_j / (4 pi |x_i - x_j|) for i != j, and A_ii the exact
+    integral of the kernel over triangle i at its centroid.  Points and
+    areas from trisphere_subdiv / tri3geom (rskelf/test/ie_sphere.m:36-37).
+    Proxies: nproxy vertices of trisphere_subdiv(nproxy, 'v'), picked by a
+    seeded permutation (ie_sphere.m:39-40 uses an unseeded randperm), at
+    pxy = proxy l + ctr (ie_sphere.m:175); rows for the outgoing (area-
+    weighted) and, for symm 'n', the incoming (4 pi / N weighted)
+    interactions (ie_sphere.m:178-183); neighbour filter radius 1.5
+    (ie_sphere.m:186)."""
+
+    def __init__(self, nface, nproxy=512, seed=0, symm="n"):
+        self.V, self.F = trisphere_subdiv(nface)
+        self.x, self.nu, self.area = tri3geom(self.V, self.F)
+        self.N = self.x.shape[1]
+        self.diag = self_panel_sl(self.V, self.F, self.x)
+        Pv, _ = trisphere_subdiv(nproxy, "v")
+        sel = np.random.default_rng(seed).permutation(Pv.shape[1])[:nproxy]
+        self.proxy = Pv[:, sel]
+        self.two_sided = symm == "n"
+
+    @classmethod
+    def from_arrays(cls, x, area, diag, proxy, symm="n"):
+        """The same problem on given points, weights, diagonal and proxies
+        (to run the oracle on exactly the arrays another build produced)."""
+        self = cls.__new__(cls)
+        self.x, self.area, self.diag, self.proxy = (np.asarray(a, dtype=float)
+                                                    for a in (x, area, diag, proxy))
+        self.N = self.x.shape[1]
+        self.two_sided = symm == "n"
+        return self
+
+    @staticmethod
+    def sl(xt, ys):
+        d = xt[:, :, None] - ys[:, None, :]
+        with np.errstate(divide="ignore"):
+            return 1.0 / (4.0 * math.pi * np.sqrt((d ** 2).sum(axis=0)))
+
+    def afun(self, I, J):
+        I = np.asarray(I, dtype=np.int64)
+        J = np.asarray(J, dtype=np.int64)
+        A = self.sl(self.x[:, I], self.x[:, J]) * self.area[J][None, :]
+        same = I[:, None] == J[None, :]
+        return np.where(same, self.diag[J][None, :], A)
+
+    def pxyfun(self, x, slf, nbr, l, ctr):
+        pxy = self.proxy * l[:, None] + ctr[:, None]
+        Kout = self.sl(pxy, x[:, slf]) * self.area[slf][None, :]
+        if self.two_sided:
+            Kin = self.sl(x[:, slf], pxy).T * (4.0 * math.pi / self.N)
+            Kpxy = np.concatenate([Kout, Kin])
+        else:
+            Kpxy = Kout
+        keep = np.sum(((x[:, nbr] - ctr[:, None]) / l[:, None]) ** 2, axis=0) < 1.5 ** 2
+        return Kpxy, nbr[keep]
+
+    def dense(self):
+        idx = np.arange(self.N)
+        return self.afun(idx, idx)
+
+    def matvec(self, X):
+        return self.dense() @ np.asarray(X)

@@ -320,3 +320,64 @@ def test_snorm_matches_svd():
     A = rng.standard_normal((60, 60))
     s = flam.snorm(60, lambda v: A @ v, lambda v: A.T @ v, tol=1e-10, niter_max=2000)
     assert abs(s - np.linalg.norm(A, 2)) < 1e-3 * s
+
+
+# ---------------------------------------------------------------- sphere --
+def test_trisphere_counts_and_geometry():
+    """geom/trisphere_subdiv.m: 20 * 4^k faces, 10 * 4^k + 2 vertices; the
+    triangles tile an inscribed polyhedron (total area below 4 pi and
+    growing with k, above 0.99 * 4 pi at k = 4), centroids inside the unit
+    ball, normals outward."""
+    prev = 0.0
+    for k in range(5):
+        V, F = problems.trisphere_subdiv(20 * 4 ** k)
+        assert F.shape[1] == 20 * 4 ** k and V.shape[1] == 10 * 4 ** k + 2
+        assert np.allclose(np.linalg.norm(V, axis=0), 1.0)
+        C, Nrm, A = problems.tri3geom(V, F)
+        assert prev < A.sum() < 4 * np.pi       # refining the inscribed polyhedron
+        prev = A.sum()
+        assert np.all(np.linalg.norm(C, axis=0) < 1.0)
+        assert np.allclose(np.sum(Nrm * C, axis=0) / np.linalg.norm(C, axis=0), 1.0, atol=0.05)
+    assert prev > 0.99 * 4 * np.pi
+    V, _ = problems.trisphere_subdiv(512, "v")
+    assert V.shape[1] == 642
+
+
+def test_self_panel_closed_form_vs_duffy_quadrature():
+    """The centroid self-panel integral of 1/(4 pi r) in closed form against
+    brute-force Duffy quadrature (the construction of quad/quad_sqtri3.m),
+    and against the equilateral triangle's textbook value
+    sqrt(3) a ln(2 + sqrt(3)) / (4 pi) ... for side a, centroid apex."""
+    V, F = problems.trisphere_subdiv(320)
+    C, _, _ = problems.tri3geom(V, F)
+    d1 = problems.self_panel_sl(V, F, C)
+    d2 = problems.self_panel_sl_quadrature(V, F, C)
+    assert np.max(np.abs(d1 - d2) / d1) < 1e-9
+    # equilateral triangle of side a: each edge at distance h = a / (2 sqrt 3)
+    # from the centroid, ends at -a/2, a/2: 3 h 2 asinh(sqrt 3) / (4 pi)
+    a = 0.7
+    Ve = np.array([[0, 0, 0], [a, 0, 0], [a / 2, a * np.sqrt(3) / 2, 0]]).T
+    Fe = np.array([[0], [1], [2]])
+    Ce = Ve.mean(axis=1, keepdims=True)
+    h = a / (2 * np.sqrt(3))
+    assert np.isclose(problems.self_panel_sl(Ve, Fe, Ce)[0], 6 * h * np.arcsinh(np.sqrt(3)) / (4 * np.pi),
+                      rtol=1e-14)
+
+
+def test_sphere_rskelf_vs_dense_solve():
+    """3D single layer on the 1280-triangle sphere: the oracle's factor
+    solves to the dense LU's answer within the ID tolerance, and a
+    compressing configuration (tol 1e-4) stays within its bound."""
+    pr = problems.SphereSL(1280)
+    A = pr.dense()
+    b = np.random.default_rng(0).standard_normal(pr.N)
+    xe = np.linalg.solve(A, b)
+    for occ, tol, bound in ((64, 1e-10, 1e-9), (32, 1e-4, 1e-2)):
+        F = flam.rskelf(pr.afun, pr.x, occ, tol, pr.pxyfun, dict(symm="n", order="level"))
+        x = flam.rskelf_sv(F, b)
+        assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < bound
+        y = flam.rskelf_mv(F, xe)
+        assert np.linalg.norm(y - b) / np.linalg.norm(b) < bound
+    tab = flam.rank_table(F)
+    assert tab[-1][3] == 0 and tab[0][2] == pr.N
+    assert sum(r[2] - r[3] for r in tab) == pr.N
