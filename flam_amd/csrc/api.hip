@@ -74,6 +74,7 @@ void launch_gather_points(const double*, const double*, const double*, const dou
                           int64_t, int, double*, hipStream_t);
 void launch_tree_init(const double*, int, int64_t, const int64_t*, int, double*, int*, double*, int,
                       hipStream_t);
+void launch_iota(int*, int64_t, hipStream_t);
 void launch_tree_split(bool, const TreeChunk*, int, int, int64_t, double*, int*, double*, int*,
                        int*, hipStream_t);
 
@@ -178,10 +179,43 @@ static Workspace g_ws;
 
 #define RSKX_TRY(expr) do { int rc_ = (expr); if (rc_ != RSKX_OK) return rc_; } while (0)
 
+// Host-to-device uploads go through a pinned staging ring: a copy from
+// pageable memory returns only once the stream has reached it (it waits for
+// every kernel queued before it), while a copy from pinned memory is queued
+// and returns at once, so the host lays out the next level while the device
+// still works.  A staged region is reused only after a stream
+// synchronization (sync_stream), which every queued copy precedes.
+struct PinnedRing {
+  char* p = nullptr;
+  size_t cap = 0, top = 0;
+};
+static PinnedRing g_pin;
+
+static int sync_stream() {
+  RSKX_HIP(hipStreamSynchronize(g_stream));
+  g_pin.top = 0;
+  return RSKX_OK;
+}
+
 template <typename T>
 static int upload(DevBuf& b, const std::vector<T>& v) {
-  RSKX_TRY(b.ensure(std::max<size_t>(v.size() * sizeof(T), 16)));
-  if (!v.empty()) RSKX_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, g_stream));
+  const size_t bytes = v.size() * sizeof(T);
+  RSKX_TRY(b.ensure(std::max<size_t>(bytes, 16)));
+  if (!bytes) return RSKX_OK;
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (g_pin.top + need > g_pin.cap) {
+    RSKX_TRY(sync_stream());                 // every staged copy has run
+    if (need > g_pin.cap) {
+      if (g_pin.p) RSKX_HIP(hipHostFree(g_pin.p));
+      g_pin.p = nullptr;
+      g_pin.cap = std::max(need, std::max(g_pin.cap * 2, (size_t)16 << 20));
+      RSKX_HIP(hipHostMalloc((void**)&g_pin.p, g_pin.cap, hipHostMallocDefault));
+    }
+  }
+  char* src = g_pin.p + g_pin.top;
+  std::memcpy(src, v.data(), bytes);
+  g_pin.top += need;
+  RSKX_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, g_stream));
   return RSKX_OK;
 }
 
@@ -296,7 +330,7 @@ int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* 
                    g_stream);
   std::vector<double> part((size_t)nprob * nblk * 6);
   RSKX_HIP(hipMemcpyAsync(part.data(), ws.tpart.p, part.size() * 8, hipMemcpyDeviceToHost, g_stream));
-  RSKX_HIP(hipStreamSynchronize(g_stream));
+  RSKX_TRY(sync_stream());
   t = HostTree();
   t.d = d;
   t.nprob = nprob;
@@ -363,7 +397,7 @@ int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* 
     counts.resize((size_t)nch * 8);
     RSKX_HIP(hipMemcpyAsync(counts.data(), ws.tcounts.p, counts.size() * sizeof(int),
                             hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipStreamSynchronize(g_stream));
+    RSKX_TRY(sync_stream());
     // child boxes and each chunk's destinations (stable within a child)
     int c0 = 0;
     for (int sb = 0; sb < (int)split.size(); ++sb) {
@@ -528,12 +562,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     if (t.cnt[b] > 0) { bs[b].cur_off = t.beg[b]; bs[b].cur_len = (int)t.cnt[b]; }
   int64_t pool_cap = N * (int64_t)(2 * t.nlvl + 2) + 64;
   RSKX_TRY(ws.pool.ensure(pool_cap * sizeof(int)));
-  {
-    std::vector<int> iota(N);
-    for (int64_t p = 0; p < N; ++p) iota[p] = (int)p;
-    RSKX_HIP(hipMemcpyAsync(ws.pool.p, iota.data(), N * sizeof(int), hipMemcpyHostToDevice, g_stream));
-    RSKX_HIP(hipStreamSynchronize(g_stream));
-  }
+  launch_iota(ws.pool.as<int>(), N, g_stream);
   int64_t pool_top = N;
   if (!ws.hflag) RSKX_HIP(hipHostMalloc((void**)&ws.hflag, 64 * sizeof(int), hipHostMallocDefault));
   ws.hflag[0] = 0;
@@ -685,7 +714,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       for (int64_t ib = 0; ib < nb; ++ib) hmn[ib] = hb[ib].mc;
     } else {
       RSKX_HIP(hipMemcpyAsync(hmn.data(), ws.mnbr.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
-      RSKX_HIP(hipStreamSynchronize(g_stream));
+      RSKX_TRY(sync_stream());
     }
     std::vector<int> idg, idr[2], idq[kGramClasses], idall;
     size_t shm_r[2] = {0, 0}, shm_q[kGramClasses] = {0, 0, 0};
@@ -796,7 +825,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     RSKX_HIP(hipMemcpyAsync(hk.data(), ws.kout.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
     RSKX_HIP(hipMemcpyAsync(hit.data(), ws.iter.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
     RSKX_HIP(hipMemcpyAsync(hm.data(), ws.mrows.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipStreamSynchronize(g_stream));
+    RSKX_TRY(sync_stream());
     RSKX_TRY(elim_status());                  // the previous level's elimination
     prof.flush();
     for (int64_t ib = 0; ib < nb; ++ib)
@@ -978,7 +1007,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   if (nrem > 0) RSKX_TRY(partial_assembly<S>(t, bs, lv_last, ws.M[mpar], F));
   RSKX_TRY(upload(F->d_recs, F->recs));
   RSKX_HIP(hipGetLastError());
-  RSKX_HIP(hipStreamSynchronize(g_stream));
+  RSKX_TRY(sync_stream());
   RSKX_TRY(elim_status());
   prof.flush();
   return RSKX_OK;
@@ -1073,7 +1102,7 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
   launch_perm<U, S>(false, X, ldx, Y, N, nrhs, ncol, F->order.as<int>(), g_stream);
   prof.end("permute", ev, (int64_t)N * nrhs * (sizeof(S) + sizeof(U)) + N * 4);
   RSKX_HIP(hipGetLastError());
-  RSKX_HIP(hipStreamSynchronize(g_stream));
+  RSKX_TRY(sync_stream());
   prof.flush();
   return RSKX_OK;
 }
@@ -1365,7 +1394,7 @@ int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, 
   launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, nullptr, 1,
                 g_stream);
   RSKX_HIP(hipGetLastError());
-  RSKX_HIP(hipStreamSynchronize(g_stream));
+  RSKX_TRY(sync_stream());
   return RSKX_OK;
 }
 
@@ -1386,7 +1415,7 @@ int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x,
   launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs,
                 doff.as<int64_t>(), nprob, g_stream);
   RSKX_HIP(hipGetLastError());
-  RSKX_HIP(hipStreamSynchronize(g_stream));
+  RSKX_TRY(sync_stream());
   return RSKX_OK;
 }
 

@@ -117,6 +117,18 @@ __global__ void k_tree_init(const double* __restrict__ x, int d, int64_t N,
   }
 }
 
+// a[i] = i
+__global__ void k_iota(int* __restrict__ a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = (int)i;
+}
+
+void launch_iota(int* a, int64_t n, hipStream_t st) {
+  const int grid = (int)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096);
+  hipLaunchKernelGGL(k_iota, dim3(grid), dim3(256), 0, st, a, n);
+}
+
 void launch_tree_init(const double* x, int d, int64_t N, const int64_t* poff, int nprob,
                       double* xs, int* ord, double* part, int nblk, hipStream_t st) {
   hipLaunchKernelGGL(k_tree_init, dim3(nblk, nprob), dim3(256), 0, st, x, d, N, poff, xs, ord,
