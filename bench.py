@@ -41,6 +41,11 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-N", type=int, default=1 << 20)
+    ap.add_argument("--workload", choices=["circle", "ellipses", "sphere"], default="circle",
+                    help="circle: the metric's configuration (default); ellipses: 96 uneven "
+                         "ellipses in one batched factorization; sphere: 3D single layer")
+    ap.add_argument("--nprob", type=int, default=96, help="ellipses: number of problems")
+    ap.add_argument("--nface", type=int, default=20480, help="sphere: triangles")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a known-size "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE counters)")
@@ -141,6 +146,73 @@ def cpu_baseline(N, occ, tol, nrhs, target, log):
                        f"{dt:.1f} s")
 
 
+def other_workload(args, log):
+    """Extra measurements (not the metric's configuration): the batched
+    ellipses and the 3D sphere, mixed FP32 factor + FP64 refinement beside
+    the plain FP64 factor, with FGMRES on an exact on-the-fly FP64 matvec."""
+    import numpy as np
+    import torch
+
+    import flam_amd
+    from flam_amd import geometry
+    from flam_amd.refine import fgmres
+    if args.workload == "ellipses":
+        # 96 ellipses, a = 1, b ~ U[0.3, 1], N log-uniform in [2048, 131072], seed 0
+        rng = np.random.default_rng(0)
+        bs = rng.uniform(0.3, 1.0, args.nprob)
+        Ns = np.round(np.exp(rng.uniform(np.log(2048), np.log(131072), args.nprob))).astype(int)
+        batch = flam_amd.stack_problems([geometry.ellipse(int(n), 1.0, float(b))
+                                         for n, b in zip(Ns, bs)])
+        N, occ, tol, target, symm = batch.N, 256, 1e-6, 1e-10, "n"
+        segs = list(zip(batch.offsets[:-1], batch.offsets[1:]))
+        factor = lambda o: flam_amd.rskelf_batch(batch, occ, tol, o)
+        mv = lambda V: flam_amd.kernel_matvec_batch(batch, V)
+        desc = (f"{args.nprob} ellipses (a=1, b~U[0.3,1], N log-uniform [2048,131072], seed 0), "
+                f"N total {N}, one batched factorization, occ={occ}, tol={tol:g}, symm n, "
+                f"{args.nrhs} RHS, FP32 factor + FP64 FGMRES to {target:g} per problem")
+    else:
+        g = geometry.sphere(args.nface)
+        N, occ, tol, target, symm = g.N, 256, 1e-5, 1e-9, "n"
+        segs = None
+        factor = lambda o: flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, o)
+        mv = lambda V: flam_amd.kernel_matvec(g.kernel, g.x, V)
+        desc = (f"3D Laplace single layer on the unit sphere, {args.nface} triangles, occ={occ}, "
+                f"tol={tol:g}, symm n, {args.nrhs} RHS, FP32 factor + FP64 FGMRES to {target:g}")
+    log(f"workload: {desc}")
+    B = torch.from_numpy(np.random.default_rng(0).standard_normal((N, args.nrhs))).cuda()
+    res = {}
+    for prec in ("fp32", "fp64"):
+        times = []
+        for it in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            F = factor(dict(symm=symm, precision=prec, profile=1))
+            t1 = time.perf_counter()
+            X, info = fgmres(mv, B, lambda V: flam_amd.rskelf_sv(F, V), tol=target, segments=segs)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            log(f"{prec} step {it}: factor {1e3*(t1-t0):.1f} ms, refine {1e3*(t2-t1):.1f} ms, "
+                f"gmres iters {info['iters']}, relres {max(info['relres']):.2e}")
+            if it >= args.warmup:
+                times.append((t2 - t0, 1e3 * (t1 - t0), 1e3 * (t2 - t1), info["iters"]))
+        roof, kt = roofline_of(F, prec)
+        res[prec] = dict(value=N / float(np.mean([t[0] for t in times])),
+                         ms_per_step=1e3 * float(np.mean([t[0] for t in times])),
+                         factor_ms=float(np.mean([t[1] for t in times])),
+                         refine_ms=float(np.mean([t[2] for t in times])),
+                         iters=times[-1][3], levels=F.level_table(), roofline=roof,
+                         kernels={k: dict(ms=v["ms"], launches=v["launches"]) for k, v in kt.items()})
+        del F
+    out = {"metric": f"{args.workload} factor+solve points/sec at the target residual",
+           "value": res["fp32"]["value"], "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic", "config": {"workload": desc, "N": N},
+           "mixed": res["fp32"], "plain_fp64": res["fp64"],
+           "mixed_over_plain": res["fp32"]["value"] / res["fp64"]["value"]}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -155,6 +227,9 @@ def main():
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
+
+    if args.workload != "circle":
+        return other_workload(args, log)
 
     if args.calibrate_copy:
         # a device-to-device copy of known size; 2 GiB written in between
