@@ -47,10 +47,10 @@ void launch_id_reg(int, const int*, const LvlBox*, const int*, const int*, const
                    hipStream_t);
 void id_reg_limits(size_t es, int cls, int* mlim, int* nlim);
 size_t elim_elems(bool lu, int k, int r);
-size_t elim_reg_shmem(int n, int k, int r, size_t es);
-int elim_reg_nmax(size_t es);
+size_t elim_reg_shmem(int n, int k, int r, size_t es, int nw, bool lu);
+int elim_reg_nmax(size_t es, int nw);
 template <typename S>
-void launch_elim_reg(bool, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
+void launch_elim_reg(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
                      const int*, const double*, const S*, S*, S*, int*, const PointsDev&, int*,
                      hipStream_t);
 template <typename S>
@@ -893,18 +893,33 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // elimination: boxes whose blocks fit two per CU in shared memory, one
     // per CU, or neither (global scratch; also the no-compression copies)
     const size_t kHalf = 78 * 1024, kFull = 158 * 1024;   // 1024-thread workgroups
-    std::vector<int> cls[3], creg;
+    // Boxes whose augmented matrix fits registers go to k_elim_reg: 8-wave
+    // workgroups (two per CU) when the box and its shared memory allow, else
+    // 16-wave workgroups
+    std::vector<int> cls[3], creg[2];
     std::vector<char> regbox(nb, 0);           // k_elim_reg also writes the transposed copies
-    size_t shm_cls[3] = {0, 0, 0}, shm_reg = 0;
-    const int nreg = elim_reg_nmax(sizeof(S));
+    size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0};
+    const int nreg8 = elim_reg_nmax(sizeof(S), 8), nreg16 = elim_reg_nmax(sizeof(S), 16);
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
-      if (O.r > 0 && hb[ib].n <= nreg &&     // augmented matrix in registers
-          elim_reg_shmem(hb[ib].n, O.k, O.r, sizeof(S)) <= 160 * 1024) {
-        creg.push_back((int)ib);
-        regbox[ib] = 1;
-        shm_reg = std::max(shm_reg, elim_reg_shmem(hb[ib].n, O.k, O.r, sizeof(S)));
-        continue;
+      const int n = hb[ib].n;
+      if (O.r > 0 && n <= nreg8) {
+        const size_t by = elim_reg_shmem(n, O.k, O.r, sizeof(S), 8, lu);
+        if (by <= 80 * 1024) {
+          creg[0].push_back((int)ib);
+          regbox[ib] = 1;
+          shm_reg[0] = std::max(shm_reg[0], by);
+          continue;
+        }
+      }
+      if (O.r > 0 && n <= nreg16) {
+        const size_t by = elim_reg_shmem(n, O.k, O.r, sizeof(S), 16, lu);
+        if (by <= 160 * 1024) {
+          creg[1].push_back((int)ib);
+          regbox[ib] = 1;
+          shm_reg[1] = std::max(shm_reg[1], by);
+          continue;
+        }
       }
       const size_t cache = 6 * sizeof(double) * (size_t)hb[ib].n;
       size_t by = O.r > 0 ? elim_elems(lu, O.k, O.r) * sizeof(S) + cache : 0;
@@ -912,19 +927,21 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       cls[c].push_back((int)ib);
       if (c < 2) shm_cls[c] = std::max(shm_cls[c], by);   // class 2: all in global scratch
     }
-    std::vector<int> allids(creg);
+    std::vector<int> allids(creg[1]);
+    allids.insert(allids.end(), creg[0].begin(), creg[0].end());
     for (int c = 2; c >= 0; --c) allids.insert(allids.end(), cls[c].begin(), cls[c].end());
     RSKX_TRY(upload(ws.ids, allids));
     prof.begin("factor", ev);
     {
       int off = 0;
-      if (!creg.empty()) {
-        launch_elim_reg<S>(lu, shm_reg, ws.ids.as<int>(), (int)creg.size(), ws.boxes.as<LvlBox>(),
-                           ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
-                           ws.perm.as<int>(), ws.T.as<double>(), ws.M[mpar].as<S>(),
-                           ws.M[mcur].as<S>(), fval, fidx, P,
-                           ws.flag.as<int>(), g_stream);
-        off += (int)creg.size();
+      for (int c = 1; c >= 0; --c) {           // 16-wave boxes first, then 8-wave
+        if (creg[c].empty()) continue;
+        launch_elim_reg<S>(lu, c == 0 ? 8 : 16, shm_reg[c], ws.ids.as<int>() + off,
+                           (int)creg[c].size(), ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(),
+                           ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(),
+                           ws.T.as<double>(), ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), fval, fidx,
+                           P, ws.flag.as<int>(), g_stream);
+        off += (int)creg[c].size();
       }
       for (int c = 2; c >= 0; --c) {
         if (cls[c].empty()) continue;

@@ -1400,6 +1400,13 @@ void launch_id(int mode, const int* ids, const LvlBox* boxes, const int* mrows, 
                      kout, iterout, vglob, vstride);
 }
 
+// elements of the register elimination's column region: two panel buffers
+// of nb columns, or the staging of nw * spr columns, whichever is larger
+__host__ __device__ inline int elim_reg_colb(int n, int nb, int nw, int spr) {
+  const int a = 2 * nb * n, b = nw * spr * n;
+  return a > b ? a : b;
+}
+
 // value of lane l (uniform l) broadcast to the wave
 __device__ inline float lane_bcast(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -1418,12 +1425,15 @@ __device__ inline double lane_bcast(double v, int l) { return read_lane(v, l); }
 // E = Ksr' / U (or Ksr' / L') beside and below it (rskelf.m:226-227,236),
 // and Mss - E G (or Mss - E E') in the trailing block (rskelf.m:240-246).
 //
-// Thread (wave w, lane l) holds Z(l + 64 a, w + 16 b), a < TA, b < TB, so
-// n <= min(64 TA, 16 TB).  Pivoting is lazy: rows are never moved; each
-// step's pivot row is found among the rows not yet used, ties going to the
-// row LAPACK's swapped order would reach first.
-template <typename S, bool LU, int TA, int TB>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) k_elim_reg(const int* __restrict__ ids,
+// Thread (wave w, lane l) of NW waves holds Z(l + 64 a, w + NW b), a < TA,
+// b < TB, so n <= min(64 TA, NW TB).  Two shapes: 16 waves for boxes up to
+// the register limit, and 8 waves for smaller boxes, two workgroups per CU
+// so that one box's serial steps (a panel's factorization, a pivot search)
+// overlap the other's updates.  Pivoting is lazy: rows are never moved;
+// each step's pivot row is found among the rows not yet used, ties going to
+// the row LAPACK's swapped order would reach first.
+template <typename S, bool LU, int TA, int TB, int NW>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 1, 4))) k_elim_reg(const int* __restrict__ ids,
                                                    const LvlBox* __restrict__ boxes,
                                                    const BoxOut* __restrict__ outs,
                                                    const Seg* __restrict__ csegs, int* pool,
@@ -1438,11 +1448,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   const BoxOut O = outs[bidx];
   const int n = B.n, k = O.k, r = O.r;
   const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  constexpr int NT = 64 * NW;
   // columns go to waves in panels of NB: slot tb of wave w holds column
-  // ((tb / NB) * 16 + w) * NB + tb % NB (NB = 1: column w + 16 tb)
+  // ((tb / NB) * NW + w) * NB + tb % NB (NB = 1: column w + NW tb)
   constexpr int NB = LU ? 1 : 4;
   static_assert(TB % NB == 0, "TB must be a multiple of the panel width");
-  auto colj = [&](int tb) { return ((tb / NB) * 16 + wid) * NB + tb % NB; };
+  auto colj = [&](int tb) { return ((tb / NB) * NW + wid) * NB + tb % NB; };
   __shared__ ChildMap cm;
   __shared__ int s_piv[2];
   __shared__ S s_pv[2];
@@ -1472,11 +1483,15 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   int* lpos = qp + n;                            // LU: logical position of physical row
   int* prow = lpos + n;                          // LU: physical row at logical position,
                                                  // then 2 x TB panel pivot rows
-  S* colb = reinterpret_cast<S*>(prow + n + 2 * TB + ((n + 2 * TB) & 1));   // 2 x n column buffers
-  S* Ts = colb + 2 * TB * n;                     // k x r
+  // two panel buffers (NB columns each) and the staging of the transposed
+  // copies (NW SPR columns) share one region
+  constexpr int SPR = TB >= 16 ? 2 : 1;          // slots per staging round
+  const int ncolb = elim_reg_colb(n, LU ? 1 : 4, NW, SPR);
+  S* colb = reinterpret_cast<S*>(prow + n + 2 * TB + ((n + 2 * TB) & 1));
+  S* Ts = colb + ncolb;                          // k x r
   S* Kss = Ts + k * r;                           // k x k
   S* KX = Kss + k * k;                           // k x r, then r x k
-  for (int p = tid; p < n; p += 1024) {
+  for (int p = tid; p < n; p += NT) {
     const int sp = p < r ? q[k + p] : q[p - r];
     qp[p] = sp;
     const int pt = slf[sp];
@@ -1496,14 +1511,14 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   auto mv = [&](int p1, int p2) -> S { return leafbox ? (S)0 : mval(cm, Mprev, qp[p1], qp[p2]); };
   auto kv = [&](int p1, int p2) -> S { return (S)aval(p1, p2) + mv(p1, p2); };
   // T, Kss, Ksr (positions: rd j -> j, sk a -> r + a)
-  for (int e = tid; e < k * r; e += 1024) {
+  for (int e = tid; e < k * r; e += NT) {
     const int a = e % k, j = e / k;
     Ts[e] = (S)Tg[a + (int64_t)j * n];
     KX[e] = kv(r + a, j);
   }
-  for (int e = tid; e < k * k; e += 1024) Kss[e] = kv(r + e % k, r + e / k);
+  for (int e = tid; e < k * k; e += NT) Kss[e] = kv(r + e % k, r + e / k);
   __syncthreads();
-  for (int e = tid; e < k * r; e += 1024) {      // Ksr' = Ksr - Kss T
+  for (int e = tid; e < k * r; e += NT) {      // Ksr' = Ksr - Kss T
     const int a = e % k, j = e / k;
     S acc = 0;
     for (int l = 0; l < k; ++l) acc += Kss[a + l * k] * Ts[l + j * k];
@@ -1542,7 +1557,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
     }
   }
   __syncthreads();
-  for (int e = tid; e < r * k; e += 1024) KX[e] = kv(e % r, r + e / r);   // Krs, r x k
+  for (int e = tid; e < r * k; e += NT) KX[e] = kv(e % r, r + e / r);   // Krs, r x k
   __syncthreads();
   for (int l = 0; l < k; ++l) {                  // Krr -= Krs T
     S kr[TA];
@@ -1560,7 +1575,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
     }
   }
   __syncthreads();
-  for (int e = tid; e < r * k; e += 1024) {      // Krs' = Krs - T' Kss
+  for (int e = tid; e < r * k; e += NT) {      // Krs' = Krs - T' Kss
     const int i = e % r, c = e / r;
     S acc = 0;
     for (int l = 0; l < k; ++l) acc += Ts[l + i * k] * Kss[l + c * k];
@@ -1582,7 +1597,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   if constexpr (!LU) {
     for (int p = 0; p * NB < r; ++p) {
       const int s0 = p * NB, s1 = s0 + NB < r ? s0 + NB : r;
-      const int wp = p % 16, gp = p / 16;
+      const int wp = p % NW, gp = p / NW;
       S* Lp = colb + (p & 1) * NB * n;           // L(i, s0 + c) at Lp[c * n + i]
       if (wid == wp) {
         S pc[NB][TA];
@@ -1667,13 +1682,13 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   }
 
   // ---- LU: r elimination steps, one barrier each; column s is held by
-  // wave s mod 16, slot s / 16
+  // wave s mod NW, slot s / NW
   bool done[TA];                                 // LU: row already used as a pivot
 #pragma unroll
   for (int ta = 0; ta < TA; ++ta) done[ta] = false;
   for (int s = 0; LU && s < r; ++s) {
     S* cb = colb + (s & 1) * n;
-    const int ws = s & 15, bs = s >> 4;
+    const int ws = s % NW, bs = s / NW;
     if (wid == ws) {
       S colv[TA];
 #pragma unroll
@@ -1800,15 +1815,14 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
       }
     }
   // transposed copies for the solve sweeps (FacRec::LUt, Et, Gt, Tt), a
-  // few register slots (16 columns each) at a time through LDS so that the
+  // few register slots (NW columns each) at a time through LDS so that the
   // global writes run along rows
   {
     S* fLUt = fval + O.LUt;
     S* fEt = fval + O.Et;
     S* fGt = fval + O.Gt;
-    constexpr int SPR = TB >= 16 ? 2 : 1;        // slots per round: 16 SPR columns
     static_assert(TB % SPR == 0, "slot rounds");
-    S* sb = colb;                                // 16 SPR x n staging (colb holds 2 TB x n)
+    S* sb = colb;                                // NW SPR x n staging
 #pragma unroll
     for (int rd = 0; rd < TB / SPR; ++rd) {
       __syncthreads();
@@ -1820,10 +1834,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
           if (i < n) sb[(SPR * wid + u) * n + i] = Z[ta][SPR * rd + u];
         }
       __syncthreads();
-      for (int e = tid; e < 16 * SPR * n; e += 1024) {
-        const int cs = e % (16 * SPR), i = e / (16 * SPR);
+      for (int e = tid; e < NW * SPR * n; e += NT) {
+        const int cs = e % (NW * SPR), i = e / (NW * SPR);
         const int w = cs / SPR, tb = SPR * rd + cs % SPR;
-        const int j = ((tb / NB) * 16 + w) * NB + tb % NB;
+        const int j = ((tb / NB) * NW + w) * NB + tb % NB;
         if (j >= n) continue;
         const S v = sb[cs * n + i];
         if (i < r) {
@@ -1836,24 +1850,24 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
       }
     }
     S* fTt = fval + O.Tt;
-    for (int e = tid; e < k * r; e += 1024) {
+    for (int e = tid; e < k * r; e += NT) {
       const int j = e % r, a = e / r;
       fTt[e] = Ts[a + j * k];
     }
   }
   int* piv = fidx + O.piv;
   if (LU)
-    for (int p = tid; p < r; p += 1024) piv[p] = prow[p];
+    for (int p = tid; p < r; p += NT) piv[p] = prow[p];
   S* fT = fval + O.T;
-  for (int e = tid; e < k * r; e += 1024) fT[e] = Ts[e];
+  for (int e = tid; e < k * r; e += NT) fT[e] = Ts[e];
   int* fsk = fidx + O.sk;
   int* frd = fidx + O.rd;
-  for (int a = tid; a < k; a += 1024) {
+  for (int a = tid; a < k; a += NT) {
     const int pt = slf[q[a]];
     fsk[a] = pt;
     pool[O.nl_off + a] = pt;
   }
-  for (int i = tid; i < r; i += 1024) frd[i] = slf[q[k + i]];
+  for (int i = tid; i < r; i += NT) frd[i] = slf[q[k + i]];
   __syncthreads();
   // inverted 32 x 32 diagonal blocks of L and U, from copies of the
   // diagonal blocks staged in the (now unused) panel/T/K scratch when they
@@ -1861,17 +1875,17 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   {
     const int nbk = (r + kDiagBlock - 1) / kDiagBlock;
     constexpr int kBB = kDiagBlock * kDiagBlock;
-    const bool staged = (size_t)nbk * kBB <= (size_t)2 * TB * n + (size_t)2 * k * r + (size_t)k * k;
+    const bool staged = (size_t)nbk * kBB <= (size_t)ncolb + (size_t)2 * k * r + (size_t)k * k;
     S* stg = colb;
     if (staged) {
-      for (int e2 = tid; e2 < nbk * kBB; e2 += 1024) {
+      for (int e2 = tid; e2 < nbk * kBB; e2 += NT) {
         const int b = e2 / kBB, t = (e2 % kBB) % kDiagBlock, u = (e2 % kBB) / kDiagBlock;
         const int gi = b * kDiagBlock + t, gj = b * kDiagBlock + u;
         stg[e2] = (gi < r && gj < r) ? fLU[gi + (int64_t)gj * r] : (S)0;
       }
     }
     __syncthreads();
-    for (int bb = wid; bb < 2 * nbk; bb += 16) {
+    for (int bb = wid; bb < 2 * nbk; bb += NW) {
       const bool upper = bb >= nbk;
       if (upper && !LU) continue;
       const int b = upper ? bb - nbk : bb;
@@ -1919,35 +1933,52 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
   }
 }
 
+// register elimination shapes: TA row slots, TB column slots, NW waves
+constexpr int kElimTA = 4, kElimTB32 = 16, kElimTB64 = 12;          // 16 waves
+constexpr int kElimTA8_32 = 3, kElimTB8_32 = 20;                     // 8 waves, FP32
+constexpr int kElimTA8_64 = 2, kElimTB8_64 = 16;                     // 8 waves, FP64
+
+// largest box of each register elimination shape (nw: 8 or 16 waves)
+int elim_reg_nmax(size_t es, int nw) {
+  if (nw == 8) return es == 4 ? std::min(64 * kElimTA8_32, 8 * kElimTB8_32)
+                              : std::min(64 * kElimTA8_64, 8 * kElimTB8_64);
+  return es == 4 ? 256 : 192;
+}
+
 // shared bytes of k_elim_reg for a box
-size_t elim_reg_shmem(int n, int k, int r, size_t es) {
-  const int TB = es == 4 ? 16 : 12;
+size_t elim_reg_shmem(int n, int k, int r, size_t es, int nw, bool lu) {
+  const int TB = nw == 8 ? (es == 4 ? kElimTB8_32 : kElimTB8_64) : (es == 4 ? kElimTB32 : kElimTB64);
+  const int spr = TB >= 16 ? 2 : 1;
   size_t b = (size_t)6 * n * sizeof(double) + (size_t)(3 * n + 2 * TB + 2) * sizeof(int);
-  b += ((size_t)2 * TB * n + (size_t)k * r + (size_t)k * k + (size_t)k * r) * es + 16;
+  b += ((size_t)elim_reg_colb(n, lu ? 1 : 4, nw, spr) + (size_t)k * r + (size_t)k * k +
+        (size_t)k * r) * es + 16;
   return b;
 }
 
 template <typename S>
-void launch_elim_reg(bool lu, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
+void launch_elim_reg(bool lu, int nw, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
                      const BoxOut* outs, const Seg* csegs, int* pool, const int* permar,
                      const double* Tar, const S* Mprev, S* Mcur, S* fval, int* fidx,
                      const PointsDev& P, int* fail, hipStream_t st) {
-  constexpr int TA = 4, TB = sizeof(S) == 4 ? 16 : 12;
-#define RSKX_ELIMR(LUV)                                                                         \
+  constexpr bool F32 = sizeof(S) == 4;
+  constexpr int TA16 = kElimTA, TB16 = F32 ? kElimTB32 : kElimTB64;
+  constexpr int TA8 = F32 ? kElimTA8_32 : kElimTA8_64, TB8 = F32 ? kElimTB8_32 : kElimTB8_64;
+#define RSKX_ELIMR(LUV, TAV, TBV, NWV)                                                          \
   {                                                                                             \
-    auto kern = k_elim_reg<S, LUV, TA, TB>;                                                     \
+    auto kern = k_elim_reg<S, LUV, TAV, TBV, NWV>;                                              \
     if (shmem > 65536)                                                                          \
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                 (int)shmem);                                                    \
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(1024), shmem, st, ids, boxes, outs, csegs, pool,    \
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(64 * NWV), shmem, st, ids, boxes, outs, csegs, pool, \
                        permar, Tar, Mprev, Mcur, fval, fidx, P, fail);                          \
   }
-  if (lu) RSKX_ELIMR(true) else RSKX_ELIMR(false)
+  if (nw == 8) {
+    if (lu) RSKX_ELIMR(true, TA8, TB8, 8) else RSKX_ELIMR(false, TA8, TB8, 8)
+  } else {
+    if (lu) RSKX_ELIMR(true, TA16, TB16, 16) else RSKX_ELIMR(false, TA16, TB16, 16)
+  }
 #undef RSKX_ELIMR
 }
-
-// largest box the register elimination takes
-int elim_reg_nmax(size_t es) { return es == 4 ? 256 : 192; }
 
 // shared-memory elements of one box's elimination blocks
 size_t elim_elems(bool lu, int k, int r) {
@@ -1984,7 +2015,7 @@ void launch_elim(bool lu, int mode, size_t shmem, const int* ids, int nb, const 
   template void launch_id_reg<S>(int, const int*, const LvlBox*, const int*, const int*,        \
                                  const int*, int*, S*, int*, const PointsDev&, const LevelParams&, \
                                  const IdParams&, int*, int, size_t, hipStream_t);             \
-  template void launch_elim_reg<S>(bool, size_t, const int*, int, const LvlBox*, const BoxOut*, \
+  template void launch_elim_reg<S>(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, \
                                    const Seg*, int*, const int*, const double*, const S*, S*, S*, \
                                    int*, const PointsDev&, int*, hipStream_t);                   \
   template void launch_elim<S>(bool, int, size_t, const int*, int, const LvlBox*,             \

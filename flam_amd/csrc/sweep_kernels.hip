@@ -188,7 +188,7 @@ __device__ void bulk_copy(S* __restrict__ dst, const S* __restrict__ src, int n)
 // PROG: 0 sv 'n'/'n', 1 sv 'n'/'c', 2 sv 'p', 3 mv 'n'/'n', 4 mv 'n'/'c', 5 mv 'p'
 // Shared memory: xs (k x 8), xr (r x 8), tmp (r x 8), red (8 x 256), then
 // the stage buffer (bcap elements).  X has nrhs columns, a multiple of 8.
-template <typename S, int PROG, bool UP>
+template <typename S, int PROG, bool UP, bool GWS>
 __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
                                                const S* __restrict__ fval,
                                                const int* __restrict__ fidx, S* X, int64_t ldx,
@@ -196,8 +196,9 @@ __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
                                                int64_t gstride) {
   extern __shared__ char smem_raw[];
   // rows of X and the work vectors live in shared memory, or for factors
-  // too large for it in this workgroup's slice of a global workspace
-  S* sm = gws ? gws + blockIdx.x * gstride : reinterpret_cast<S*>(smem_raw);
+  // too large for it (GWS) in this workgroup's slice of a global workspace;
+  // a compile-time choice, so the shared-memory variant keeps LDS accesses
+  S* sm = GWS ? gws + blockIdx.x * gstride : reinterpret_cast<S*>(smem_raw);
   const FacRec f = recs[blockIdx.x];
   const int k = f.k, r = f.r;
   const int* sk = fidx + f.sk;
@@ -322,18 +323,13 @@ template <typename S, int PROG>
 static void launch_prog(bool up, const FacRec* recs, int nrec, const S* fval, const int* fidx, S* X,
                         int64_t ldx, int nrhs, int kmax, int rmax, int bcap, size_t shmem, S* gws,
                         int64_t gstride, hipStream_t st) {
-  if (shmem > 65536) {
-    (void)hipFuncSetAttribute((const void*)k_sweep<S, PROG, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    (void)hipFuncSetAttribute((const void*)k_sweep<S, PROG, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  }
-  if (up)
-    hipLaunchKernelGGL((k_sweep<S, PROG, true>), dim3(nrec), dim3(256), shmem, st, recs, fval, fidx,
-                       X, ldx, nrhs, kmax, rmax, bcap, gws, gstride);
-  else
-    hipLaunchKernelGGL((k_sweep<S, PROG, false>), dim3(nrec), dim3(256), shmem, st, recs, fval,
-                       fidx, X, ldx, nrhs, kmax, rmax, bcap, gws, gstride);
+  auto kern = gws ? (up ? k_sweep<S, PROG, true, true> : k_sweep<S, PROG, false, true>)
+                  : (up ? k_sweep<S, PROG, true, false> : k_sweep<S, PROG, false, false>);
+  if (shmem > 65536)
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)shmem);
+  hipLaunchKernelGGL(kern, dim3(nrec), dim3(256), shmem, st, recs, fval, fidx, X, ldx, nrhs, kmax,
+                     rmax, bcap, gws, gstride);
 }
 
 template <typename S>
@@ -533,16 +529,17 @@ __device__ void solve_perm(S* Y, S* tmp, int r, const int* __restrict__ p, bool 
 // PROG: 0 sv 'n'/'n', 1 sv 'n'/'c', 2 sv 'p'.  Shared memory: xs (kmax x 8),
 // xr (rmax x 8), red (8 x 256), and for PROG < 2 a permutation buffer
 // (rmax x 8).  X has nrhs columns, a multiple of 8.
-template <typename S, int PROG, bool UP>
+template <typename S, int PROG, bool UP, bool GWS>
 __global__ void __launch_bounds__(kSolveThreads) k_solve(const FacRec* __restrict__ recs,
                                                          const S* __restrict__ fval,
                                                          const int* __restrict__ fidx, S* X,
                                                          int64_t ldx, int nrhs, int kmax,
                                                          int rmax, S* gws, int64_t gstride) {
   extern __shared__ char smem_raw[];
-  // shared memory, or this workgroup's slice of a global workspace for
-  // factors whose rows do not fit
-  S* xs = gws ? gws + blockIdx.x * gstride : reinterpret_cast<S*>(smem_raw);
+  // shared memory, or (GWS, a compile-time choice that leaves the shared
+  // variant its LDS accesses) this workgroup's slice of a global workspace
+  // for factors whose rows do not fit
+  S* xs = GWS ? gws + blockIdx.x * gstride : reinterpret_cast<S*>(smem_raw);
   S* xr = xs + (size_t)kmax * kRhs;
   S* red = xr + (size_t)rmax * kRhs;
   S* tmp = red + (size_t)kRhs * kSolveThreads;
@@ -620,7 +617,8 @@ void launch_solve(int prog, bool up, const FacRec* recs, int nrec, const S* fval
                   int64_t gstride, hipStream_t st) {
 #define RSKX_SOLVE(P)                                                                           \
   {                                                                                             \
-    auto kern = up ? k_solve<S, P, true> : k_solve<S, P, false>;                                \
+    auto kern = gws ? (up ? k_solve<S, P, true, true> : k_solve<S, P, false, true>)            \
+                    : (up ? k_solve<S, P, true, false> : k_solve<S, P, false, false>);          \
     if (shmem > 65536)                                                                          \
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                 (int)shmem);                                                    \
