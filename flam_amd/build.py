@@ -23,6 +23,13 @@ FLAGS = [
 ]
 
 
+# per-source flags: the register-resident elimination unrolls nested loops
+# over up to 3 x 24 register slots; above LLVM's default size limit for
+# "#pragma unroll" a partly unrolled loop would index the slot array
+# dynamically and move it to scratch memory
+EXTRA = {"factor_kernels.hip": ["-mllvm", "-pragma-unroll-threshold=100000"]}
+
+
 def _run(cmd):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
@@ -39,7 +46,8 @@ def build(verbose=False, src_dir=None, out=None):
         jobs = []
         for s in SOURCES:
             obj = os.path.join(tmpd, os.path.splitext(s)[0] + ".o")
-            jobs.append(([HIPCC, *FLAGS, *inc, "-c", os.path.join(src_dir, s), "-o", obj], obj))
+            jobs.append(([HIPCC, *FLAGS, *EXTRA.get(s, []), *inc, "-c", os.path.join(src_dir, s),
+                          "-o", obj], obj))
         if verbose:
             for cmd, _ in jobs:
                 print(" ".join(cmd))

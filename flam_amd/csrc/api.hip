@@ -894,8 +894,8 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // per CU, or neither (global scratch; also the no-compression copies)
     const size_t kHalf = 78 * 1024, kFull = 158 * 1024;   // 1024-thread workgroups
     // Boxes whose augmented matrix fits registers go to k_elim_reg: 8-wave
-    // workgroups (two per CU) when the box and its shared memory allow, else
-    // 16-wave workgroups
+    // workgroups (two per CU, n <= 192 in FP32, 128 in FP64) when the box
+    // and its shared memory allow, else 16-wave workgroups
     std::vector<int> cls[3], creg[2];
     std::vector<char> regbox(nb, 0);           // k_elim_reg also writes the transposed copies
     size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0};

@@ -1935,7 +1935,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
 
 // register elimination shapes: TA row slots, TB column slots, NW waves
 constexpr int kElimTA = 4, kElimTB32 = 16, kElimTB64 = 12;          // 16 waves
-constexpr int kElimTA8_32 = 3, kElimTB8_32 = 20;                     // 8 waves, FP32
+constexpr int kElimTA8_32 = 3, kElimTB8_32 = 24;                     // 8 waves, FP32
 constexpr int kElimTA8_64 = 2, kElimTB8_64 = 16;                     // 8 waves, FP64
 
 // largest box of each register elimination shape (nw: 8 or 16 waves)
