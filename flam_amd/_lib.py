@@ -81,6 +81,8 @@ SIGNATURES = {
                                   POINTER(c_int64), POINTER(c_int64), POINTER(c_double), c_int]),
     "rskx_kernel_matvec": (c_int, [c_void_p, c_int, c_int64, POINTER(Kernel), c_void_p, c_int64,
                                    c_void_p, c_int64, c_int]),
+    "rskx_cgs2": (c_int, [c_void_p, c_int64, c_int64, c_int, c_void_p, c_int64, c_int64, c_int,
+                          c_void_p]),
     "rskx_kernel_matvec_batch": (c_int, [c_int, POINTER(c_int64), c_void_p, c_int,
                                          POINTER(Kernel), c_void_p, c_int64, c_void_p, c_int64,
                                          c_int]),

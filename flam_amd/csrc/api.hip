@@ -75,6 +75,10 @@ void launch_gather_points(const double*, const double*, const double*, const dou
 void launch_tree_init(const double*, int, int64_t, const int64_t*, int, double*, int*, double*, int,
                       hipStream_t);
 void launch_iota(int*, int64_t, hipStream_t);
+int cgs_max_vectors();
+int cgs_blocks(int64_t N);
+void launch_cgs(const double*, int64_t, int64_t, int, double*, int64_t, int64_t, int, double*,
+                double*, double*, hipStream_t);
 void launch_tree_split(bool, const TreeChunk*, int, int, int64_t, double*, int*, double*, int*,
                        int*, hipStream_t);
 
@@ -172,6 +176,7 @@ struct Workspace {
   DevBuf V;                     // Householder vectors of the global QR when rows exceed shared memory
   DevBuf lrecs;                 // factor records of the level being packed
   DevBuf sweep;                 // sweep work rows of factors too large for shared memory
+  DevBuf cgs;                   // Gram-Schmidt partial sums
   DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs, poff;
   int* hflag = nullptr;
 };
@@ -1412,6 +1417,23 @@ int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, 
                 g_stream);
   RSKX_HIP(hipGetLastError());
   RSKX_TRY(sync_stream());
+  return RSKX_OK;
+}
+
+int rskx_cgs2(const double* V, int64_t ldv, int64_t ldr, int nv, double* w, int64_t ldw,
+              int64_t N, int nrhs, double* hout) {
+  if (!V || !w || !hout) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL array");
+  if (nv < 1 || nv > cgs_max_vectors())
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:size: between 1 and " + std::to_string(cgs_max_vectors()) +
+                                  " basis vectors per call");
+  if (N < 1 || nrhs < 1 || ldv < N || ldw < N || ldr < (int64_t)nv * ldv)
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad sizes or leading dimensions");
+  const int nblk = cgs_blocks(N);
+  RSKX_TRY(g_ws.cgs.ensure((size_t)nrhs * nblk * (nv + 1) * sizeof(double)));
+  double* partial = g_ws.cgs.as<double>();
+  launch_cgs(V, ldv, ldr, nv, w, ldw, N, nrhs, partial, partial + (size_t)nrhs * nblk * nv, hout,
+             g_stream);
+  RSKX_HIP(hipGetLastError());
   return RSKX_OK;
 }
 

@@ -12,9 +12,26 @@ all right-hand sides at once and stopped on the true relative residual
 Vectors are stored one right-hand side per row, (nrhs, N), which is the
 column-major N x nrhs layout the sweeps take; the Krylov basis of every
 column is one (nrhs, restart + 1, N) array, so each orthogonalization pass
-is one batched product.  Classical Gram-Schmidt is applied twice per step.
+is one batched product.  Classical Gram-Schmidt is applied twice per step,
+by the library's fused kernels (rskx_cgs2): each pass reads the basis and
+the new vector once, with no temporaries.
 """
 import torch
+
+from . import _lib as L
+
+
+def _cgs2(V, nv, w):
+    """Orthogonalize w (nrhs, N) against V[:, :nv, :] twice, in place; returns
+    a (nrhs, nv + 1) CPU tensor of the projections and the new norm."""
+    nrhs, nb, N = V.shape
+    hout = torch.empty(nrhs, nv + 1, dtype=torch.float64, device=V.device)
+    lib = L.lib()
+    lib.rskx_set_stream(torch.cuda.current_stream().cuda_stream)
+    rc = lib.rskx_cgs2(V.data_ptr(), N, nb * N, nv, w.data_ptr(), N, N, nrhs, hout.data_ptr())
+    if rc != L.RSKX_OK:
+        raise RuntimeError(L.last_error())
+    return hout
 
 
 def _rowsum(P):
@@ -84,17 +101,10 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None):
         for j in range(restart):
             Z[:, j, :] = op(precond, V[:, j, :])
             w = op(matvec, Z[:, j, :])
-            Vj = V[:, : j + 1, :]
-            # GEMMs with a reduction length of N and one output column map
-            # badly onto BLAS kernels; elementwise products and row sums
-            # stream each vector once per pass
-            h = _rowsum(Vj * w[:, None, :])                    # (nrhs, j+1)
-            w = w - (Vj * h[:, :, None]).sum(1)
-            h2 = _rowsum(Vj * w[:, None, :])
-            w = w - (Vj * h2[:, :, None]).sum(1)
-            hn = _norms(w)
+            hdev = _cgs2(V, j + 1, w)                          # (nrhs, j + 2): h, ||w||
+            hn = hdev[:, -1]
             V[:, j + 1, :] = w / torch.where(hn > 0, hn, torch.ones_like(hn))[:, None]
-            hcol = torch.cat([h + h2, hn[:, None]], dim=1).cpu()
+            hcol = hdev.cpu()
             H[:, : j + 2, j] = hcol
             iters += 1
             Hj = H[:, : j + 2, : j + 1]

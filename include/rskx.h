@@ -213,6 +213,19 @@ int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x,
                              const rskx_kernel* A, const double* X, int64_t ldx, double* Y,
                              int64_t ldy, int nrhs);
 
+/* Orthogonalization step of the flexible GMRES refinement that uses a
+ * factor as preconditioner (the reference preconditions MATLAB's gmres with
+ * rskelf_sv, rskelf/test/ie_square3.m:97): classical Gram-Schmidt applied
+ * twice to w against nv basis vectors, for nrhs right-hand sides at once.
+ * Right-hand side r of the basis starts at V + r * ldr, its vector i at
+ * + i * ldv; w's at w + r * ldw; all vectors hold N doubles (device).  On
+ * return w is orthogonalized in place and hout (device, nrhs x (nv + 1),
+ * row-major) holds each right-hand side's projections (both passes
+ * summed) followed by the norm of the new w.  1 <= nv <= 32.  Asynchronous
+ * on the library stream. */
+int rskx_cgs2(const double* V, int64_t ldv, int64_t ldr, int nv, double* w, int64_t ldw,
+              int64_t N, int nrhs, double* hout);
+
 #ifdef __cplusplus
 }
 #endif

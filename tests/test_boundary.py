@@ -22,7 +22,7 @@ def _lib():
 def _header_functions():
     src = open(os.path.join(ROOT, "include", "rskx.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(rskx_[a-z_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(rskx_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_header_declares_expected_surface():
