@@ -50,11 +50,11 @@ size_t elim_elems(bool lu, int k, int r);
 size_t elim_reg_shmem(int n, int k, int r, size_t es, int nw, bool lu);
 int elim_reg_nmax(size_t es, int nw);
 template <typename S>
-void launch_elim_reg(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
+void launch_elim_reg(bool, bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
                      const int*, const double*, const S*, S*, S*, int*, const PointsDev&, int*,
                      hipStream_t);
 template <typename S>
-void launch_elim(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*,
+void launch_elim(bool, bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*,
                  int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
                  int*, hipStream_t);
 template <typename S>
@@ -575,6 +575,8 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   auto elim_status = [&]() -> int {
     if (ws.hflag[0] & 1)
       return fail(RSKX_ERR_NUM, "FLAM:rskx:chol: matrix is not positive definite (symm 'p')");
+    if (ws.hflag[0] & 4)
+      return fail(RSKX_ERR_NUM, "FLAM:rskx:ldl: zero pivot in L D L' (symm 'h' factors without pivoting)");
     if (ws.hflag[0] & 2)
       return fail(RSKX_ERR_NUM, "FLAM:rskx:lu: exactly singular redundant block");
     return RSKX_OK;
@@ -584,6 +586,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
 
   const bool lu = (o.symm == 'n');
   const bool two = (o.symm == 'n');
+  const bool ldl = (o.symm == 'h');
   IdParams prm;
   prm.tol = std::fmod(rank_or_tol, 1.0);
   prm.kmax = (int)std::floor(rank_or_tol);
@@ -872,6 +875,8 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         O.sk = need_i; need_i += k;
         O.rd = need_i; need_i += r;
         O.piv = need_i; if (lu) need_i += r;
+        O.D = -1;
+        if (ldl) { O.D = need_v; need_v += r; }
         O.M_off = offM; offM += k * k;
         O.nl_off = pool_top + off_slf + off_nl; off_nl += k;
         kmx = std::max<int>(kmx, (int)k);
@@ -880,6 +885,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
                     (lu ? 2.0 : 1.0) * k * r * r + 2.0 * k * k * r;
       } else {
         O.fac = -1;
+        O.D = -1;
         O.M_off = offM; offM += n * n;
         O.nl_off = -1;
       }
@@ -941,7 +947,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       int off = 0;
       for (int c = 1; c >= 0; --c) {           // 16-wave boxes first, then 8-wave
         if (creg[c].empty()) continue;
-        launch_elim_reg<S>(lu, c == 0 ? 8 : 16, shm_reg[c], ws.ids.as<int>() + off,
+        launch_elim_reg<S>(lu, ldl, c == 0 ? 8 : 16, shm_reg[c], ws.ids.as<int>() + off,
                            (int)creg[c].size(), ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(),
                            ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(),
                            ws.T.as<double>(), ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), fval, fidx,
@@ -950,7 +956,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       }
       for (int c = 2; c >= 0; --c) {
         if (cls[c].empty()) continue;
-        launch_elim<S>(lu, c < 2 ? 1 : 0, shm_cls[c], ws.ids.as<int>() + off,
+        launch_elim<S>(lu, ldl, c < 2 ? 1 : 0, shm_cls[c], ws.ids.as<int>() + off,
                        (int)cls[c].size(), ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(),
                        ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(), ws.T.as<double>(),
                        ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), ws.W.as<S>(), fval,
@@ -979,6 +985,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         fr.sk = O.sk; fr.rd = O.rd; fr.piv = O.piv;
         fr.Tt = O.Tt; fr.Et = O.Et; fr.Gt = O.Gt; fr.LUt = O.LUt; fr.Lti = O.Lti; fr.Uti = O.Uti;
         fr.packed = regbox[ib];
+        fr.D = O.D;
         fr.pad_ = 0;
         F->recs.push_back(fr);
         elim += O.r;
@@ -1052,7 +1059,8 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
   launch_perm<U, S>(true, X, ldx, Y, N, nrhs, ncol, F->order.as<int>(), g_stream);
   prof.end("permute", ev, (int64_t)N * nrhs * sizeof(U) + N * ncol * sizeof(S) + N * 4);
   int prog;
-  if (F->symm == 'p') prog = solve ? 2 : 5;
+  // 'h' runs the 'p' programs with a unit L; the kernels apply D (FacRec::D)
+  if (F->symm == 'p' || F->symm == 'h') prog = solve ? 2 : 5;
   else prog = solve ? (trans == 'n' ? 0 : 1) : (trans == 'n' ? 3 : 4);
   const int nl = (int)F->lvp.size() - 1;
   // rskelf_sv.m:25-35 / sv/*.m: mode 3 = upward then downward sweep
@@ -1175,8 +1183,8 @@ static int check_factor_args(const double* x, int d, int64_t N, int occ, double 
   char s = (char)tolower(o.symm);
   if (s != 'n' && s != 's' && s != 'h' && s != 'p')
     return fail(RSKX_ERR_ARG, "FLAM:chksymm:invalidSymm: Symmetry parameter must be one of 'N', 'S', 'H', or 'P'.");
-  if (s == 's' || s == 'h')
-    return fail(RSKX_ERR_UNSUP, std::string("FLAM:rskx:symm: symm '") + s + "' is not built on the device (use 'n' or 'p')");
+  if (s == 's')
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:symm: symm 's' (complex symmetric) is not built on the device (use 'n', 'h' or 'p')");
   o.symm = s;
   if (occ < 0) return fail(RSKX_ERR_ARG, "FLAM:hypoct:invalidOcc: Leaf occupancy must be nonnegative.");
   if (o.lvlmax < 0) return fail(RSKX_ERR_ARG, "FLAM:hypoct:invalidLvlmax: Maximum tree depth must be at least 1.");
@@ -1359,11 +1367,13 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
     if (cnt) RSKX_HIP(hipMemcpy(dst.data(), F->lidx[flv].as<int>() + off, cnt * 4, hipMemcpyDeviceToHost));
     return RSKX_OK;
   };
-  std::vector<double> T, LU, E, G;
+  std::vector<double> T, LU, E, G, D;
   std::vector<int> sk, rd, pv;
   RSKX_TRY(getv(f.T, k * r, T));
   RSKX_TRY(getv(f.LU, r * r, LU));
   RSKX_TRY(getv(f.E, k * r, E));
+  const bool ldl = F->symm == 'h';
+  if (ldl) RSKX_TRY(getv(f.D, r, D));
   if (lu) RSKX_TRY(getv(f.G, r * k, G));
   RSKX_TRY(geti(f.sk, k, sk));
   RSKX_TRY(geti(f.rd, r, rd));
@@ -1380,7 +1390,10 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
   if (v->L)
     for (int64_t c = 0; c < r; ++c)
       for (int64_t a = 0; a < r; ++a)
-        v->L[a + c * r] = a > c ? LU[a + c * r] : (a == c ? (lu ? 1.0 : LU[a + c * r]) : 0.0);
+        v->L[a + c * r] = a > c ? LU[a + c * r] : (a == c ? ((lu || ldl) ? 1.0 : LU[a + c * r]) : 0.0);
+  if (ldl && v->U)                            // U = D (rskelf.m:231: U = sparse(D))
+    for (int64_t c = 0; c < r; ++c)
+      for (int64_t a = 0; a < r; ++a) v->U[a + c * r] = a == c ? D[a] : 0.0;
   if (lu) {
     if (v->U)
       for (int64_t c = 0; c < r; ++c)

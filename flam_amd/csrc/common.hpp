@@ -83,6 +83,7 @@ struct BoxOut {
   int64_t Li, Ui;         // inverted 32 x 32 diagonal blocks of L and U
   int64_t sk, rd, piv;    // offsets into the factor index arena
   int64_t Tt, Et, Gt, LUt, Lti, Uti;   // transposed copies (FacRec)
+  int64_t D;              // symm 'h': the r pivots of L D L' (-1 otherwise)
 };
 
 // per-factor record used by the sweeps (values in the factor arena)
@@ -97,6 +98,7 @@ struct FacRec {
   // lanes: T' (r x k), E' (r x k), G' (k x r, 'n'), LU' (r x r), and the
   // diagonal-block inverses transposed block by block
   int64_t Tt, Et, Gt, LUt, Lti, Uti;
+  int64_t D;              // symm 'h': D of K(rd,rd) = L D L' (rskelf.m:229-233), -1 otherwise
 };
 
 // diagonal block size of the blocked triangular solves in the sweeps

@@ -570,7 +570,7 @@ __device__ inline S mval(const ChildMap& cm, const S* Mprev, int a, int b) {
 // G = L \ Krs'(p,:), and M(sk,sk) - E G (or - E E') is the new skeleton
 // block.  The blocks live in shared memory when SH (one or two boxes per
 // CU), else in the box's global scratch.
-template <typename S, bool LU, bool SH>
+template <typename S, bool LU, bool SH, bool LDL>
 __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
                                               const LvlBox* __restrict__ boxes,
                                               const BoxOut* __restrict__ outs,
@@ -744,16 +744,27 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
       __syncthreads();
     }
   } else {
-    for (int s = 0; s < r; ++s) {    // right-looking Cholesky, lower
+    // right-looking Cholesky ('p'), or L D L' without pivoting ('h': unit L,
+    // the pivots to D; rskelf.m:229 takes Bunch-Kaufman pivots, which on the
+    // definite matrices this path is for are the diagonal in order)
+    S* fD = LDL ? fval + O.D : nullptr;
+    for (int s = 0; s < r; ++s) {
       if (tid == 0) {
         S d = Krr[s + s * r];
-        if (!(d > (S)0)) { atomicOr(fail, 1); d = (S)1; }
-        d = sqrt(d);
-        Krr[s + s * r] = d;
-        s_d = (S)1 / d;
+        if (LDL) {
+          if (!(d != (S)0)) { atomicOr(fail, 4); d = (S)1; }
+          fD[s] = d;
+          Krr[s + s * r] = (S)1;
+          s_d = (S)1 / d;
+        } else {
+          if (!(d > (S)0)) { atomicOr(fail, 1); d = (S)1; }
+          d = sqrt(d);
+          Krr[s + s * r] = d;
+          s_d = (S)1 / d;
+        }
       }
       __syncthreads();
-      const S d2 = s_d * s_d;
+      const S d2 = LDL ? s_d : s_d * s_d;
       for (int j = s + 1 + wid; j < r; j += nw) {
         const S ljs = Krr[j + s * r] * d2;
         for (int i = j + lane; i < r; i += kWave) Krr[i + j * r] -= Krr[i + s * r] * ljs;
@@ -785,7 +796,7 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
             S v = (t == c) ? (S)1 : (S)0;
 #pragma unroll
             for (int u = 0; u < t; ++u) v -= Krr[(j0 + t) + (j0 + u) * r] * x[u];
-            x[t] = LU ? v : v / Krr[(j0 + t) + (j0 + t) * r];
+            x[t] = (LU || LDL) ? v : v / Krr[(j0 + t) + (j0 + t) * r];
           }
         }
       } else {
@@ -807,7 +818,8 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
       }
     }
   }
-  // E = Ksr'/U ('n') or Ksr'/L' ('p'); G = L \ Krs'(p,:) ('n'); column sweeps
+  // E = Ksr'/U ('n'), Ksr'/L' ('p') or (Ksr'/L')/D ('h'); G = L \ Krs'(p,:)
+  // ('n'); column sweeps
   for (int j = 0; j < r; ++j) {
     const S inv = (S)1 / Krr[j + j * r];
     for (int a = tid; a < k; a += nt) Ksr[a + j * k] *= inv;
@@ -820,12 +832,20 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
       FOR_COLS_ROWS(i, j + 1, r, c, 0, k) Krs[i + c * r] -= Krr[i + j * r] * Krs[j + c * r];
     __syncthreads();
   }
-  // M = M(sk,sk) - E G  or  - E E' (rskelf.m:240-246)
+  if (LDL) {
+    const S* fD = fval + O.D;
+    FOR_COLS_ROWS(a, 0, k, c, 0, r) Ksr[a + c * k] /= fD[c];
+    __syncthreads();
+  }
+  // M = M(sk,sk) - E G, - E E' or - E D E' (rskelf.m:240-246)
   S* Mo = Mcur + O.M_off;
   FOR_COLS_ROWS(a, 0, k, c, 0, k) {
     S acc = 0;
     if (LU) for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * Krs[l + c * r];
-    else    for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * Ksr[c + l * k];
+    else if (LDL) {
+      const S* fD = fval + O.D;
+      for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * fD[l] * Ksr[c + l * k];
+    } else for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * Ksr[c + l * k];
     Mo[a + c * k] = mval(cm, Mprev, q[a], q[c]) - acc;
   }
 #undef FOR_COLS_ROWS
@@ -1432,7 +1452,7 @@ __device__ inline double lane_bcast(double v, int l) { return read_lane(v, l); }
 // overlap the other's updates.  Pivoting is lazy: rows are never moved;
 // each step's pivot row is found among the rows not yet used, ties going to
 // the row LAPACK's swapped order would reach first.
-template <typename S, bool LU, int TA, int TB, int NW>
+template <typename S, bool LU, int TA, int TB, int NW, bool LDL>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 1, 4))) k_elim_reg(const int* __restrict__ ids,
                                                    const LvlBox* __restrict__ boxes,
                                                    const BoxOut* __restrict__ outs,
@@ -1457,6 +1477,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
   __shared__ ChildMap cm;
   __shared__ int s_piv[2];
   __shared__ S s_pv[2];
+  __shared__ S s_dpan[2][4];                      // 'h': the pivots of the last two panels
+  if (tid < 8) s_dpan[tid >> 2][tid & 3] = (S)1;  // a short last panel leaves entries unset
   if (tid == 0) {
     int nc = B.ncs > 8 ? 8 : B.ncs;
     cm.nc = nc;
@@ -1618,8 +1640,14 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
 #pragma unroll
           for (int ta = 0; ta < TA; ++ta) if (ta == (sc >> 6)) dv = pc[c][ta];
           dv = lane_bcast(dv, sc & 63);
-          if (!(dv > (S)0)) { if (lane == 0) atomicOr(fail, 1); dv = (S)1; }
-          const S pv = sqrt(dv), inv = (S)1 / pv;
+          // 'p': Cholesky pivot sqrt(d); 'h': unit L, d to D (no pivoting;
+          // rskelf.m:229's Bunch-Kaufman pivots are the diagonal in order on
+          // the definite matrices this path is for)
+          if (LDL) {
+            if (!(dv != (S)0)) { if (lane == 0) atomicOr(fail, 4); dv = (S)1; }
+            if (lane == 0) { fval[O.D + sc] = dv; s_dpan[p & 1][c] = dv; }
+          } else if (!(dv > (S)0)) { if (lane == 0) atomicOr(fail, 1); dv = (S)1; }
+          const S pv = LDL ? (S)1 : sqrt(dv), inv = (S)1 / (LDL ? dv : pv);
 #pragma unroll
           for (int ta = 0; ta < TA; ++ta) {
             const int i = lane + 64 * ta;
@@ -1634,6 +1662,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
 #pragma unroll
             for (int ta = 0; ta < TA; ++ta) if (ta == (j >> 6)) u = pc[c][ta];
             u = lane_bcast(u, j & 63);
+            if (LDL) u *= dv;                          // l_ic d_c l_jc
 #pragma unroll
             for (int ta = 0; ta < TA; ++ta) {
               const int i = lane + 64 * ta;
@@ -1666,7 +1695,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
         if (j < s1 || j >= n) continue;
         S u[NB];
 #pragma unroll
-        for (int c = 0; c < NB; ++c) u[c] = Lp[c * n + j];
+        for (int c = 0; c < NB; ++c) u[c] = LDL ? Lp[c * n + j] * s_dpan[p & 1][c] : Lp[c * n + j];
         // entries above the diagonal of Krr, the Krs' block and padding
         // rows are never read again: update without masks
 #pragma unroll
@@ -1904,7 +1933,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
             S v = (t == c) ? (S)1 : (S)0;
 #pragma unroll
             for (int u = 0; u < t; ++u) v -= Lb[t + u * ldb] * x[u];
-            x[t] = LU ? v : v / Lb[t + t * ldb];
+            x[t] = (LU || LDL) ? v : v / Lb[t + t * ldb];
           }
         }
       } else {
@@ -1956,7 +1985,7 @@ size_t elim_reg_shmem(int n, int k, int r, size_t es, int nw, bool lu) {
 }
 
 template <typename S>
-void launch_elim_reg(bool lu, int nw, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
+void launch_elim_reg(bool lu, bool ldl, int nw, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
                      const BoxOut* outs, const Seg* csegs, int* pool, const int* permar,
                      const double* Tar, const S* Mprev, S* Mcur, S* fval, int* fidx,
                      const PointsDev& P, int* fail, hipStream_t st) {
@@ -1965,7 +1994,8 @@ void launch_elim_reg(bool lu, int nw, size_t shmem, const int* ids, int nb, cons
   constexpr int TA8 = F32 ? kElimTA8_32 : kElimTA8_64, TB8 = F32 ? kElimTB8_32 : kElimTB8_64;
 #define RSKX_ELIMR(LUV, TAV, TBV, NWV)                                                          \
   {                                                                                             \
-    auto kern = k_elim_reg<S, LUV, TAV, TBV, NWV>;                                              \
+    auto kern = ldl ? k_elim_reg<S, false, TAV, TBV, NWV, true>                                  \
+                    : k_elim_reg<S, LUV, TAV, TBV, NWV, false>;                                  \
     if (shmem > 65536)                                                                          \
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                 (int)shmem);                                                    \
@@ -1986,14 +2016,14 @@ size_t elim_elems(bool lu, int k, int r) {
 }
 
 template <typename S>
-void launch_elim(bool lu, int mode, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
+void launch_elim(bool lu, bool ldl, int mode, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
                  const BoxOut* outs, const Seg* csegs, int* pool, const int* permar,
                  const double* Tar, const S* Mprev, S* Mcur, S* War, S* fval, int* fidx,
                  const PointsDev& P, int* fail, hipStream_t st) {
   // mode 1: blocks in shared memory, mode 0: blocks in global scratch
 #define RSKX_ELIM(LUV, SHV)                                                                   \
   {                                                                                           \
-    auto kern = k_elim<S, LUV, SHV>;                                                          \
+    auto kern = ldl ? k_elim<S, false, SHV, true> : k_elim<S, LUV, SHV, false>;               \
     if (shmem > 65536)                                                                        \
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                 (int)shmem);                                                  \
@@ -2015,10 +2045,10 @@ void launch_elim(bool lu, int mode, size_t shmem, const int* ids, int nb, const 
   template void launch_id_reg<S>(int, const int*, const LvlBox*, const int*, const int*,        \
                                  const int*, int*, S*, int*, const PointsDev&, const LevelParams&, \
                                  const IdParams&, int*, int, size_t, hipStream_t);             \
-  template void launch_elim_reg<S>(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, \
+  template void launch_elim_reg<S>(bool, bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, \
                                    const Seg*, int*, const int*, const double*, const S*, S*, S*, \
                                    int*, const PointsDev&, int*, hipStream_t);                   \
-  template void launch_elim<S>(bool, int, size_t, const int*, int, const LvlBox*,             \
+  template void launch_elim<S>(bool, bool, int, size_t, const int*, int, const LvlBox*,       \
                                const BoxOut*, const Seg*, int*, const int*, const double*,     \
                                const S*, S*, S*, S*, int*, const PointsDev&, int*, hipStream_t);
 RSKX_INST(double)

@@ -295,14 +295,21 @@ __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
         trmul(xr, tmp, r, stage(LUg, rr), false, false, true, red);
         upd(xr, r, xs, k, stage(Tg, kr), k, true, one, red);
       }
-    } else {                    // mv_p.m
+    } else {                    // mv_p.m; mv_h.m (unit L, then X(rd) = D X(rd))
+      const bool unit = f.D >= 0;
       if (UP) {
         upd(xs, k, xr, r, stage(Tg, kr), k, false, one, red);
-        trmul(xr, tmp, r, stage(LUg, rr), true, false, true, red);
+        trmul(xr, tmp, r, stage(LUg, rr), true, unit, true, red);
         upd(xr, r, xs, k, stage(Eg, kr), k, true, one, red);
+        if (unit) {
+          __syncthreads();
+          const S* D = fval + f.D;
+          for (int o = threadIdx.x; o < r * kRhs; o += blockDim.x) xr[o] *= D[o % r];
+          __syncthreads();
+        }
       } else {
         upd(xs, k, xr, r, stage(Eg, kr), k, false, one, red);
-        trmul(xr, tmp, r, stage(LUg, rr), true, false, false, red);
+        trmul(xr, tmp, r, stage(LUg, rr), true, unit, false, red);
         upd(xr, r, xs, k, stage(Tg, kr), k, true, one, red);
       }
     }
@@ -583,11 +590,17 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(const FacRec* __restric
         solve_perm(xr, tmp, r, piv, false);
         solve_prod(xs, k, xr, r, fval + f.T, k, red);
       }
-    } else {                    // sv_p.m
+    } else {                    // sv_p.m; sv_h.m (unit L, then X(rd) = D \ X(rd))
       if (UP) {
         solve_prod(xr, r, xs, k, fval + f.Tt, r, red);
         solve_tri(xr, r, fval + f.LU, fval + f.Li, true, red);
         solve_prod(xs, k, xr, r, fval + f.E, k, red);
+        if (f.D >= 0) {
+          __syncthreads();
+          const S* D = fval + f.D;
+          for (int o = threadIdx.x; o < r * kRhs; o += kSolveThreads) xr[o] /= D[o / kRhs];
+          __syncthreads();
+        }
       } else {
         solve_prod(xr, r, xs, k, fval + f.Et, r, red);
         solve_tri(xr, r, fval + f.LUt, fval + f.Lti, false, red);

@@ -178,8 +178,9 @@ class Factor:
         for key in ("T", "L", "U", "E", "F"):
             setattr(v, key, ptr(arr[key], ctypes.c_double))
         _check(L.lib().rskx_export_factor(self._h, i, ctypes.byref(v)))
+        ldl = self.info()["symm"] == "h"            # U = D for 'h' (rskelf.m:231)
         out = dict(sk=arr["sk"], rd=arr["rd"], T=arr["T"].T.copy(), L=arr["L"].T.copy(),
-                   E=arr["E"].T.copy(), U=arr["U"].T.copy() if lu else None,
+                   E=arr["E"].T.copy(), U=arr["U"].T.copy() if (lu or ldl) else None,
                    p=arr["p"] if lu else None, F=arr["F"].T.copy() if lu else None)
         return out
 

@@ -280,3 +280,61 @@ def test_partial_factorization_stop_zero():
     sk, S = flam_amd.rskelf_partial_info(F)
     assert F.info()["nlvl_done"] == 0 and F.info()["nfactor"] == 0
     assert np.array_equal(sk.numpy(), np.arange(N)) and S._nnz() == 0
+
+
+# ------------------------------------------------------------ symm 'h' --
+@pytest.mark.parametrize("kind", ["circle", "ie_circle"])
+def test_symm_h_ldl(kind):
+    """symm 'h' (rskelf.m:228-233, sv/rskelf_sv_h.m, mv/rskelf_mv_h.m): the
+    same ID as 'p' (identical rank table to the oracle's 'h'), L D L'
+    without pivoting on the SPD benchmark circle and on the demo's negative
+    definite ie_circle matrix (where Cholesky fails); solve against the
+    closed form, apply against the exact circulant product, exported L unit
+    lower and U = D diagonal, and 't' / 'c' equal to 'n'."""
+    flam_amd, geometry = _dev()
+    N, occ, tol = 4096, 64, 1e-12
+    print(f"symm h: {kind} N={N} occ={occ} tol={tol}", flush=True)
+    g = geometry.circle(N) if kind == "circle" else geometry.ie_circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, dict(symm="h"))
+    pr = opr.CurveDL(N, kind)
+    Fo = ofl.rskelf(pr.afun, pr.x, occ, tol, pr.pxyfun, dict(symm="h", order="level"))
+    _cmp_tables(F.level_table(), ofl.rank_table(Fo))
+    b = np.random.default_rng(5).standard_normal(N)
+    bt = torch.from_numpy(b).cuda()
+    x = flam_amd.rskelf_sv(F, bt).cpu().numpy()
+    xe = pr.closed_form_inverse_apply(b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-10
+    y = flam_amd.rskelf_mv(F, torch.from_numpy(xe).cuda()).cpu().numpy()
+    assert np.linalg.norm(y - b) / np.linalg.norm(b) < 1e-10
+    for t in ("t", "c"):
+        assert torch.allclose(flam_amd.rskelf_sv(F, bt, t), flam_amd.rskelf_sv(F, bt), rtol=0,
+                              atol=1e-14 * np.abs(x).max())
+    f = F.export(0)
+    assert np.allclose(np.diag(f["L"]), 1.0) and np.allclose(np.triu(f["L"], 1), 0.0)
+    assert np.allclose(f["U"], np.diag(np.diag(f["U"])))
+    d = np.diag(f["U"])
+    assert (d > 0).all() if kind == "circle" else (d < 0).all()
+    if kind == "ie_circle":
+        with pytest.raises(flam_amd.FlamError) as e:
+            flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, dict(symm="p"))
+        assert e.value.ident == "FLAM:rskx:chol"
+    # boxes above the register elimination's size (the LDS / global kernel)
+    F2 = flam_amd.rskelf(g.kernel, g.x, 300, tol, g.proxy, dict(symm="h"))
+    x2 = flam_amd.rskelf_sv(F2, bt).cpu().numpy()
+    assert np.linalg.norm(x2 - xe) / np.linalg.norm(xe) < 1e-10
+    y2 = flam_amd.rskelf_mv(F2, torch.from_numpy(xe).cuda()).cpu().numpy()
+    assert np.linalg.norm(y2 - b) / np.linalg.norm(b) < 1e-10
+
+
+def test_symm_h_fp32_refined():
+    flam_amd, geometry = _dev()
+    from flam_amd.refine import fgmres
+    N = 1 << 16
+    print(f"symm h fp32: circle N={N} occ=256 tol=1e-6 + fgmres", flush=True)
+    g = geometry.circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, 256, 1e-6, g.proxy, dict(symm="h", precision="fp32"))
+    B = torch.from_numpy(np.random.default_rng(6).standard_normal((N, 3))).cuda()
+    X, info = fgmres(g.circulant_matvec, B, lambda V: flam_amd.rskelf_sv(F, V), tol=1e-10)
+    R = B - g.circulant_matvec(X)
+    rel = (torch.linalg.vector_norm(R, dim=0) / torch.linalg.vector_norm(B, dim=0)).max().item()
+    assert rel <= 1e-10 and info["iters"] <= 8
