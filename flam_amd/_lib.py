@@ -67,6 +67,9 @@ SIGNATURES = {
                                   POINTER(c_void_p)]),
     "rskx_rskelf_sv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
     "rskx_rskelf_mv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
+    "rskx_rskelf_logdet": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    "rskx_rskelf_cholmv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
+    "rskx_rskelf_cholsv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
     "rskx_sweep_sv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int, c_int]),
     "rskx_sweep_mv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int, c_int]),
     "rskx_partial_info": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64),

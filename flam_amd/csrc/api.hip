@@ -68,6 +68,8 @@ template <typename S>
 void launch_solve(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int, int,
                   size_t, S*, int64_t, hipStream_t);
 size_t solve_shmem(int prog, int kmax, int rmax, size_t es);
+template <typename S>
+void launch_logdet(const FacRec*, int, const S*, const int*, int, double*, hipStream_t);
 void launch_matvec(int, const double*, int64_t, const double*, const double*, const double*, double,
                    const double*, int64_t, double*, int64_t, int, const int64_t*, int, hipStream_t);
 void launch_gather_points(const double*, const double*, const double*, const double*, const int*,
@@ -1139,6 +1141,39 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
 
 }  // namespace
 
+template <typename S>
+static int logdet_impl(const rskx_factor* F, double* re, double* im) {
+  const int mode = F->symm == 'p' ? 0 : F->symm == 'h' ? 1 : 2;
+  const int64_t nf = (int64_t)F->recs.size();
+  int rmax = 0;
+  for (const FacRec& f : F->recs) rmax = std::max(rmax, f.r);
+  if (mode == 2 && rmax > 65536)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: a factor beyond 65536 redundant points for the permutation sign");
+  DevBuf out;
+  RSKX_TRY(out.ensure(std::max<int64_t>(nf, 1) * 3 * sizeof(double)));
+  for (size_t lv = 0; lv + 1 < F->lvp.size(); ++lv) {
+    const int64_t f0 = F->lvp[lv], f1 = F->lvp[lv + 1];
+    if (f1 > f0)
+      launch_logdet<S>(F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->lval[lv].as<S>(),
+                       F->lidx[lv].as<int>(), mode, out.as<double>() + 3 * f0, g_stream);
+  }
+  RSKX_HIP(hipGetLastError());
+  std::vector<double> h((size_t)nf * 3);
+  if (nf) RSKX_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  RSKX_TRY(sync_stream());
+  out.release();
+  double lr = 0;
+  int64_t nneg = 0;
+  for (int64_t f = 0; f < nf; ++f) {
+    lr += h[3 * f];
+    nneg += (int64_t)h[3 * f + 1] + (int64_t)h[3 * f + 2];
+  }
+  // 'p': 2 sum log diag(L); each negative pivot or odd permutation adds i pi
+  *re = mode == 0 ? 2 * lr : lr;
+  *im = (nneg & 1) ? 3.141592653589793238462643383279502884 : 0.0;
+  return RSKX_OK;
+}
+
 // ================================================================ C ABI ==
 extern "C" {
 
@@ -1308,6 +1343,35 @@ int rskx_partial_info(const rskx_factor* F, int64_t* nsk, int64_t* nnz, int64_t*
   if (J) std::copy(F->SJ.begin(), F->SJ.end(), J);
   if (V) std::copy(F->SV.begin(), F->SV.end(), V);
   return RSKX_OK;
+}
+
+int rskx_rskelf_logdet(const rskx_factor* F, double* re, double* im) {
+  if (!F || !re || !im) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  return F->prec == RSKX_PREC_FP64 ? logdet_impl<double>(F, re, im) : logdet_impl<float>(F, re, im);
+}
+
+// the generalized Cholesky factor C of F = C C' is half a sweep
+// (rskelf_cholmv.m, rskelf_cholsv.m): C X = downward apply, C' X = upward
+// apply, C \ X = upward solve, C' \ X = downward solve
+static int chol_entry(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
+                      bool solve) {
+  if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
+  if (F->symm != 'p')
+    return fail(RSKX_ERR_ARG, std::string("FLAM:rskelf_chol") + (solve ? "sv" : "mv") +
+                                  ":invalidSymm: Symmetry parameter must be 'P'.");
+  const char t = (char)tolower(trans);
+  if (t != 'n' && t != 't' && t != 'c')
+    return fail(RSKX_ERR_ARG, "FLAM:chktrans:invalidTrans: Transpose parameter must be one of 'N', 'T', or 'C'.");
+  const bool fwd = t == 'n';
+  return sweep_entry(F, X, ldx, nrhs, 'n', dtype, solve, solve == fwd ? 1 : 2);
+}
+
+int rskx_rskelf_cholmv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype) {
+  return chol_entry(F, X, ldx, nrhs, trans, dtype, false);
+}
+
+int rskx_rskelf_cholsv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype) {
+  return chol_entry(F, X, ldx, nrhs, trans, dtype, true);
 }
 
 int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype) {

@@ -678,6 +678,66 @@ void launch_perm(bool to, void* X, int64_t ldx, S* Y, int64_t N, int nrhs, int n
                        N, nrhs, order);
 }
 
+// ------------------------------------------------------------ log det --
+// One workgroup per factor: the sum of log |d| over its r pivots (diag(L)
+// for 'p', D for 'h', diag(U) for 'n'), how many are negative, and for 'n'
+// the parity of the row permutation, counted by walking its cycles with
+// a visited bitmap in LDS (rskelf/rskelf_logdet.m:17-27, core/logdet_ldl.m,
+// compat/detperm.m).  out[3 f + {0, 1, 2}].
+constexpr int kLogdetMaxR = 65536;
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_logdet(const FacRec* __restrict__ recs,
+                                                const S* __restrict__ fval,
+                                                const int* __restrict__ fidx, int mode,
+                                                double* __restrict__ out) {
+  __shared__ double rs[4];
+  __shared__ int rn[4];
+  __shared__ unsigned seen[kLogdetMaxR / 32];
+  const FacRec f = recs[blockIdx.x];
+  const int r = f.r;
+  double s = 0;
+  int neg = 0;
+  for (int i = threadIdx.x; i < r; i += blockDim.x) {
+    const double d = mode == 1 ? (double)fval[f.D + i] : (double)fval[f.LU + i + (int64_t)i * r];
+    s += log(fabs(d));
+    neg += d < 0;
+  }
+  s = wave_sum(s);
+  neg = (int)wave_sum((double)neg);
+  if ((threadIdx.x & 63) == 0) { rs[threadIdx.x >> 6] = s; rn[threadIdx.x >> 6] = neg; }
+  int parity = 0;
+  if (mode == 2) {
+    for (int w = threadIdx.x; w < (r + 31) / 32; w += blockDim.x) seen[w] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int* piv = fidx + f.piv;
+      int cycles = 0;
+      for (int i = 0; i < r; ++i) {
+        if (seen[i >> 5] & (1u << (i & 31))) continue;
+        ++cycles;
+        for (int j = i; !(seen[j >> 5] & (1u << (j & 31))); j = piv[j]) seen[j >> 5] |= 1u << (j & 31);
+      }
+      parity = (r - cycles) & 1;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double st = 0;
+    int nt = 0;
+    for (int q = 0; q < (int)(blockDim.x / 64); ++q) { st += rs[q]; nt += rn[q]; }
+    out[3 * (int64_t)blockIdx.x] = st;
+    out[3 * (int64_t)blockIdx.x + 1] = nt;
+    out[3 * (int64_t)blockIdx.x + 2] = parity;
+  }
+}
+
+template <typename S>
+void launch_logdet(const FacRec* recs, int nrec, const S* fval, const int* fidx, int mode,
+                   double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_logdet<S>, dim3(nrec), dim3(256), 0, st, recs, fval, fidx, mode, out);
+}
+
 // ------------------------------------------- FP64 on-the-fly A X product --
 constexpr double kTwoPiM = 6.283185307179586476925286766559;
 
@@ -861,6 +921,8 @@ void launch_gather_points(const double* x, const double* nu, const double* w, co
   template void launch_perm<float, S>(bool, void*, int64_t, S*, int64_t, int, int,          \
                                       const int*, hipStream_t);                               \
   template void launch_pack<S>(bool, const FacRec*, int, S*, hipStream_t);                    \
+  template void launch_logdet<S>(const FacRec*, int, const S*, const int*, int, double*,      \
+                                 hipStream_t);                                                \
   template void launch_solve<S>(int, bool, const FacRec*, int, const S*, const int*, S*,      \
                                 int64_t, int, int, int, size_t, S*, int64_t, hipStream_t);
 RSKX_SWEEP_INST(double)

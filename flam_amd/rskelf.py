@@ -439,6 +439,26 @@ def rskelf_mv(F, X, trans="n"):
     return _sweep(L.lib().rskx_rskelf_mv, F, X, trans)
 
 
+def rskelf_logdet(F):
+    """log det of the factored matrix, imaginary part in [0, 2 pi)
+    (rskelf/rskelf_logdet.m:11)."""
+    re, im = ctypes.c_double(), ctypes.c_double()
+    torch.cuda.synchronize()
+    _check(L.lib().rskx_rskelf_logdet(F.handle, ctypes.byref(re), ctypes.byref(im)))
+    return complex(re.value, im.value)
+
+
+def rskelf_cholmv(F, X, trans="n"):
+    """C X (or C' X) for the generalized Cholesky factor of F = C C', symm
+    'p' (rskelf/rskelf_cholmv.m:15)."""
+    return _sweep(L.lib().rskx_rskelf_cholmv, F, X, trans)
+
+
+def rskelf_cholsv(F, X, trans="n"):
+    """C \\ X (or C' \\ X), symm 'p' (rskelf/rskelf_cholsv.m:15)."""
+    return _sweep(L.lib().rskx_rskelf_cholsv, F, X, trans)
+
+
 def kernel_matvec(A, x, X):
     """FP64 Y = A X with the kernel evaluated on the fly (no compression)."""
     x = _dev(x)

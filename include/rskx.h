@@ -161,6 +161,24 @@ int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char tr
 int rskx_rskelf_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
                    int dtype);
 
+/* LD = rskelf_logdet(F)                         -- rskelf/rskelf_logdet.m:11
+ * log det of the factored matrix (of its factored part, for a partial
+ * factorization) as *re + i *im with 0 <= *im < 2 pi: 2 sum log diag(L)
+ * for 'p', sum log D for 'h', sum log diag(U) + log detperm(p) for 'n';
+ * each negative pivot or odd row permutation adds i pi.  Host outputs. */
+int rskx_rskelf_logdet(const rskx_factor* F, double* re, double* im);
+
+/* Y = rskelf_cholmv(F,X,trans), Y = rskelf_cholsv(F,X,trans)
+ *                                 -- rskelf/rskelf_cholmv.m:15, rskelf_cholsv.m:15
+ * The generalized Cholesky factor C of F = C C' (symm 'p' only): C X, C' X
+ * ('t' = 'c' for real data), C \ X, C' \ X, in place on device X as
+ * rskx_rskelf_sv.  Error FLAM:rskelf_cholmv:invalidSymm (cholsv) for
+ * other symmetries. */
+int rskx_rskelf_cholmv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
+                       int dtype);
+int rskx_rskelf_cholsv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
+                       int dtype);
+
 /* The upward (mode 1) or downward (mode 2) sweep alone, or both (mode 3),
  * of rskelf_sv / rskelf_mv: the sweeps' mode argument
  * (rskelf/sv/rskelf_sv_nn.m:3, rskelf/mv/rskelf_mv_nn.m:3), as

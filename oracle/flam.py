@@ -560,6 +560,74 @@ def rskelf_mv(F, X, trans="n", mode=3):
     return Xw[:, 0] if vec else Xw
 
 
+def _logdet_ldl(D):
+    """log det of the block-diagonal D of an L D L' factorization; follows
+    core/logdet_ldl.m: 1 x 1 blocks by their diagonal, 2 x 2 blocks by
+    their determinant (complex logarithm)."""
+    D = np.asarray(D)
+    n = D.shape[0]
+    off = np.zeros(n, dtype=bool)
+    off[:-1] |= D[np.arange(n - 1), np.arange(1, n)] != 0
+    off[1:] |= D[np.arange(1, n), np.arange(n - 1)] != 0
+    ld = np.sum(np.log(np.diag(D)[~off].astype(complex)))
+    idx = np.flatnonzero(off)
+    for i in range(0, len(idx), 2):
+        ld += np.log(complex(np.linalg.det(D[np.ix_(idx[i:i + 2], idx[i:i + 2])])))
+    return ld
+
+
+def _detperm(p):
+    """Sign of a permutation vector (compat/detperm.m): (-1)^(n - cycles)."""
+    p = np.asarray(p)
+    seen = np.zeros(len(p), dtype=bool)
+    cycles = 0
+    for i in range(len(p)):
+        if not seen[i]:
+            cycles += 1
+            j = i
+            while not seen[j]:
+                seen[j] = True
+                j = p[j]
+    return -1.0 if (len(p) - cycles) % 2 else 1.0
+
+
+def rskelf_logdet(F):
+    """log det of the factored matrix with 0 <= imag < 2 pi; follows
+    rskelf/rskelf_logdet.m:11-30."""
+    ld = 0j
+    for f in F["factors"]:
+        if F["symm"] == "p":
+            ld += 2 * np.sum(np.log(np.diag(f["L"]).astype(complex)))
+        elif F["symm"] == "h":
+            ld += _logdet_ldl(f["U"])
+        else:
+            ld += np.sum(np.log(np.diag(f["U"]).astype(complex))) + np.log(complex(_detperm(f["p"])))
+    return complex(ld.real, math.fmod(ld.imag, 2 * math.pi) % (2 * math.pi))
+
+
+def rskelf_cholmv(F, X, trans="n"):
+    """Apply the generalized Cholesky factor C of F = C C' ('p' only);
+    follows rskelf/rskelf_cholmv.m:15-52: trans 'n' is the downward half of
+    rskelf_mv, 'c' the upward half."""
+    if F["symm"] != "p":
+        raise FlamError("FLAM:rskelf_cholmv:invalidSymm", "Symmetry parameter must be 'P'.")
+    trans = chktrans(trans)
+    if trans == "t":
+        return np.conj(rskelf_cholmv(F, np.conj(X), "c"))
+    return rskelf_mv(F, X, "n", 2 if trans == "n" else 1)
+
+
+def rskelf_cholsv(F, X, trans="n"):
+    """Apply C^{-1} ('p' only); follows rskelf/rskelf_cholsv.m:15-52: trans
+    'n' is the upward half of rskelf_sv, 'c' the downward half."""
+    if F["symm"] != "p":
+        raise FlamError("FLAM:rskelf_cholsv:invalidSymm", "Symmetry parameter must be 'P'.")
+    trans = chktrans(trans)
+    if trans == "t":
+        return np.conj(rskelf_cholsv(F, np.conj(X), "c"))
+    return rskelf_sv(F, X, "n", 1 if trans == "n" else 2)
+
+
 # --------------------------------------------------------------------------
 # harness
 # --------------------------------------------------------------------------

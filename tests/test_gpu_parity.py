@@ -338,3 +338,63 @@ def test_symm_h_fp32_refined():
     R = B - g.circulant_matvec(X)
     rel = (torch.linalg.vector_norm(R, dim=0) / torch.linalg.vector_norm(B, dim=0)).max().item()
     assert rel <= 1e-10 and info["iters"] <= 8
+
+
+# ------------------------------------------------- logdet / cholmv / cholsv --
+@pytest.mark.parametrize("symm,kind,prec", [("p", "circle", "fp64"), ("h", "circle", "fp64"),
+                                            ("n", "circle", "fp64"), ("h", "ie_circle", "fp64"),
+                                            ("n", "ie_circle", "fp64"), ("p", "circle", "fp32")])
+def test_logdet_vs_closed_form_and_oracle(symm, kind, prec):
+    """rskelf_logdet (rskelf_logdet.m) against log det(alpha I + beta 1 1')
+    and against the oracle's factor; FP32 factors to FP32 accuracy."""
+    flam_amd, geometry = _dev()
+    N, occ, tol = 4096, 64, 1e-12
+    print(f"logdet {symm} {kind} {prec} N={N}", flush=True)
+    g = geometry.circle(N) if kind == "circle" else geometry.ie_circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, dict(symm=symm, precision=prec))
+    ld = flam_amd.rskelf_logdet(F)
+    alpha, beta = (0.5 - 1.0 / N, 0.5 / N) if kind == "circle" else (-0.5, -0.5 / N)
+    ref = (N - 1) * np.log(complex(alpha)) + np.log(complex(alpha + N * beta))
+    rtol = 1e-10 if prec == "fp64" else 1e-4
+    assert abs(ld.real - ref.real) <= rtol * abs(ref.real), (ld, ref)
+    assert abs(ld.imag - (ref.imag % (2 * math.pi))) < 1e-9
+    if prec == "fp64":
+        pr = opr.CurveDL(N, kind)
+        Fo = ofl.rskelf(pr.afun, pr.x, occ, tol, pr.pxyfun, dict(symm=symm, order="level"))
+        ldo = ofl.rskelf_logdet(Fo)
+        assert abs(ld.real - ldo.real) <= 1e-9 * abs(ref.real)
+        assert abs(np.exp(1j * ld.imag) - np.exp(1j * ldo.imag)) < 1e-8     # phases mod 2 pi
+
+
+def test_cholmv_cholsv():
+    """C C' = F for symm 'p' (rskelf_cholmv.m, rskelf_cholsv.m): C (C' x) =
+    rskelf_mv(x), cholsv inverts cholmv for 'n' and 'c', the halves agree
+    with the oracle's, ||C' x||^2 = x' A x, and symm 'n' is refused."""
+    flam_amd, geometry = _dev()
+    N = 4096
+    print(f"cholmv/cholsv circle N={N}", flush=True)
+    g = geometry.circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, 64, 1e-12, g.proxy, dict(symm="p"))
+    x = torch.from_numpy(np.random.default_rng(7).standard_normal((N, 3))).cuda()
+    y = flam_amd.rskelf_cholmv(F, flam_amd.rskelf_cholmv(F, x, "c"), "n")
+    assert torch.allclose(y, flam_amd.rskelf_mv(F, x), rtol=0, atol=1e-13)
+    for t in ("n", "c"):
+        z = flam_amd.rskelf_cholsv(F, flam_amd.rskelf_cholmv(F, x, t), t)
+        assert torch.allclose(z, x, rtol=0, atol=1e-11)
+    assert torch.equal(flam_amd.rskelf_cholmv(F, x, "t"), flam_amd.rskelf_cholmv(F, x, "c"))
+    cx = flam_amd.rskelf_cholmv(F, x[:, 0], "c")
+    ax = g.circulant_matvec(x[:, 0])
+    assert abs((cx @ cx).item() - (x[:, 0] @ ax).item()) <= 1e-10 * (x[:, 0] @ ax).item()
+    # C depends on the ID's skeleton choices (ties may break differently from
+    # the oracle's), so compare what does not: ||C' x||^2 and ||C \ x||^2
+    # against x' F x and x' F^{-1} x of the oracle's factor
+    pr = opr.CurveDL(N, "circle")
+    Fo = ofl.rskelf(pr.afun, pr.x, 64, 1e-12, pr.pxyfun, dict(symm="p", order="level"))
+    xc = x[:, 0].cpu().numpy()
+    cs = flam_amd.rskelf_cholsv(F, x[:, 0], "n")
+    assert abs((cx @ cx).item() - xc @ ofl.rskelf_mv(Fo, xc)) <= 1e-10 * (cx @ cx).item()
+    assert abs((cs @ cs).item() - xc @ ofl.rskelf_sv(Fo, xc)) <= 1e-10 * (cs @ cs).item()
+    Fn = flam_amd.rskelf(g.kernel, g.x, 64, 1e-12, g.proxy, dict(symm="n"))
+    with pytest.raises(flam_amd.FlamError) as e:
+        flam_amd.rskelf_cholmv(Fn, x)
+    assert e.value.ident == "FLAM:rskelf_cholmv:invalidSymm"

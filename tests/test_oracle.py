@@ -381,3 +381,52 @@ def test_sphere_rskelf_vs_dense_solve():
     tab = flam.rank_table(F)
     assert tab[-1][3] == 0 and tab[0][2] == pr.N
     assert sum(r[2] - r[3] for r in tab) == pr.N
+
+
+# ------------------------------------------------- logdet / cholmv / cholsv --
+def _circle_logdet(N, kind):
+    """log det of alpha I + beta 1 1' = (N - 1) log alpha + log(alpha + N beta)."""
+    alpha, beta = (0.5 - 1.0 / N, 0.5 / N) if kind == "circle" else (-0.5, -0.5 / N)
+    return (N - 1) * np.log(complex(alpha)) + np.log(complex(alpha + N * beta))
+
+
+@pytest.mark.parametrize("symm,kind", [("p", "circle"), ("h", "circle"), ("n", "circle"),
+                                       ("h", "ie_circle"), ("n", "ie_circle")])
+def test_logdet_closed_form(symm, kind):
+    """rskelf_logdet against det(alpha I + beta 1 1') in closed form; for the
+    negative definite ie_circle matrix the imaginary part is N pi mod 2 pi."""
+    N = 1024
+    pr = problems.CurveDL(N, kind)
+    F = flam.rskelf(pr.afun, pr.x, 64, 1e-12, pr.pxyfun, dict(symm=symm, order="level"))
+    ld = flam.rskelf_logdet(F)
+    ref = _circle_logdet(N, kind)
+    assert abs(ld.real - ref.real) < 1e-9 * abs(ref.real)
+    assert abs(ld.imag - math.fmod(ref.imag, 2 * math.pi) % (2 * math.pi)) < 1e-9
+    # and the dense determinant of the ellipse ('n', permutation signs count)
+    if symm == "n" and kind == "circle":
+        pe = problems.CurveDL(512, "ellipse", b=0.5)
+        Fe = flam.rskelf(pe.afun, pe.x, 32, 1e-12, pe.pxyfun, dict(symm="n", order="level"))
+        sign, lad = np.linalg.slogdet(pe.dense())
+        lde = flam.rskelf_logdet(Fe)
+        assert abs(lde.real - lad) < 1e-9 * abs(lad)
+        assert abs(np.exp(1j * lde.imag) - sign) < 1e-9
+
+
+def test_cholmv_cholsv_identities():
+    """C C' = F: cholmv(cholmv(x, 'c'), 'n') = rskelf_mv(x); cholsv inverts
+    cholmv; 't' is the conjugate of 'c'; both refuse symm 'n'."""
+    N = 1024
+    pr = problems.CurveDL(N, "circle")
+    F = flam.rskelf(pr.afun, pr.x, 64, 1e-12, pr.pxyfun, dict(symm="p", order="level"))
+    x = np.random.default_rng(0).standard_normal((N, 2))
+    y = flam.rskelf_cholmv(F, flam.rskelf_cholmv(F, x, "c"), "n")
+    assert np.allclose(y, flam.rskelf_mv(F, x), atol=1e-13)
+    assert np.allclose(flam.rskelf_cholsv(F, flam.rskelf_cholmv(F, x, "n"), "n"), x, atol=1e-12)
+    assert np.allclose(flam.rskelf_cholsv(F, flam.rskelf_cholmv(F, x, "c"), "c"), x, atol=1e-12)
+    assert np.allclose(flam.rskelf_cholmv(F, x, "t"), flam.rskelf_cholmv(F, x, "c"))
+    # ||C' x||^2 = x' A x for the SPD matrix
+    cx = flam.rskelf_cholmv(F, x[:, 0], "c")
+    assert np.isclose(cx @ cx, x[:, 0] @ pr.matvec(x[:, 0]), rtol=1e-10)
+    Fn = flam.rskelf(pr.afun, pr.x, 64, 1e-12, pr.pxyfun, dict(symm="n", order="level"))
+    with pytest.raises(flam.FlamError):
+        flam.rskelf_cholmv(Fn, x)
