@@ -80,7 +80,7 @@ void launch_iota(int*, int64_t, hipStream_t);
 int cgs_max_vectors();
 int cgs_blocks(int64_t N);
 void launch_cgs(const double*, int64_t, int64_t, int, double*, int64_t, int64_t, int, double*,
-                double*, double*, hipStream_t);
+                double*, double*, double*, hipStream_t);
 void launch_tree_split(bool, const TreeChunk*, int, int, int64_t, double*, int*, double*, int*,
                        int*, hipStream_t);
 
@@ -198,10 +198,58 @@ struct PinnedRing {
 };
 static PinnedRing g_pin;
 
+// Device-to-host readbacks land in a pinned bounce area too: a copy into
+// pageable memory waits for the stream on its own, so three of them cost
+// three round trips, while copies into pinned memory are queued and the
+// next sync_stream moves every one to its host destination after a single
+// wait.  A destination may be read only after that synchronization.
+struct PinnedDown {
+  char* p = nullptr;
+  size_t cap = 0, top = 0;
+  struct Pending {
+    void* dst;
+    size_t off, bytes;
+  };
+  std::vector<Pending> pending;
+};
+static PinnedDown g_down;
+
 static int sync_stream() {
   RSKX_HIP(hipStreamSynchronize(g_stream));
   g_pin.top = 0;
+  for (const auto& q : g_down.pending) std::memcpy(q.dst, g_down.p + q.off, q.bytes);
+  g_down.pending.clear();
+  g_down.top = 0;
   return RSKX_OK;
+}
+
+static int download_queue(void* dst, const void* src, size_t bytes) {
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (g_down.top + need > g_down.cap) {
+    RSKX_TRY(sync_stream());                 // earlier readbacks reach their destinations
+    if (need > g_down.cap) {
+      const size_t cap = std::max(need, std::max(g_down.cap * 2, (size_t)1 << 20));
+      if (g_down.p) RSKX_HIP(hipHostFree(g_down.p));
+      g_down.p = nullptr;
+      g_down.cap = 0;
+      RSKX_HIP(hipHostMalloc((void**)&g_down.p, cap, hipHostMallocDefault));
+      g_down.cap = cap;
+    }
+  }
+  RSKX_HIP(hipMemcpyAsync(g_down.p + g_down.top, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  g_down.pending.push_back({dst, g_down.top, bytes});
+  g_down.top += need;
+  return RSKX_OK;
+}
+
+static int download(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return RSKX_OK;
+  const int rc = download_queue(dst, src, bytes);
+  if (rc != RSKX_OK) {                       // the caller returns: forget its destinations
+    g_down.pending.clear();
+    g_down.top = 0;
+  }
+  return rc;
 }
 
 template <typename T>
@@ -228,18 +276,21 @@ static int upload(DevBuf& b, const std::vector<T>& v) {
 
 // -------------------------------------------------------------- profile --
 struct KStat { double ms = 0; int64_t launches = 0; int64_t bytes = 0; double flops = 0; };
+// every kernel class (profile 1) or only the elimination class "factor"
+// (profile 2, so the other launches run without event markers between them)
 struct Prof {
   bool on = false;
+  bool elim_only = false;
   std::map<std::string, KStat> st;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  bool wants(const char* name) const { return on && (!elim_only || !strcmp(name, "factor")); }
   void begin(const char* name, hipEvent_t& a) {
-    if (!on) return;
+    if (!wants(name)) return;
     hipEventCreate(&a);
     hipEventRecord(a, g_stream);
-    (void)name;
   }
   void end(const char* name, hipEvent_t a, int64_t bytes = 0, double flops = 0) {
-    if (!on) return;
+    if (!wants(name)) return;
     hipEvent_t b;
     hipEventCreate(&b);
     hipEventRecord(b, g_stream);
@@ -262,7 +313,7 @@ struct Prof {
     pending.clear();
   }
   void host(const char* name, double ms) {
-    if (!on) return;
+    if (!wants(name)) return;
     auto& s = st[name];
     s.ms += ms;
     s.launches++;
@@ -336,7 +387,7 @@ int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* 
   launch_tree_init(x, d, N, ws.poff.as<int64_t>(), nprob, xs, ord, ws.tpart.as<double>(), nblk,
                    g_stream);
   std::vector<double> part((size_t)nprob * nblk * 6);
-  RSKX_HIP(hipMemcpyAsync(part.data(), ws.tpart.p, part.size() * 8, hipMemcpyDeviceToHost, g_stream));
+  RSKX_TRY(download(part.data(), ws.tpart.p, part.size() * 8));
   RSKX_TRY(sync_stream());
   t = HostTree();
   t.d = d;
@@ -402,8 +453,7 @@ int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* 
     launch_tree_split(false, ws.tchunks.as<TreeChunk>(), nch, d, N, xs, ord,
                       ws.txs.as<double>(), ws.tord.as<int>(), ws.tcounts.as<int>(), g_stream);
     counts.resize((size_t)nch * 8);
-    RSKX_HIP(hipMemcpyAsync(counts.data(), ws.tcounts.p, counts.size() * sizeof(int),
-                            hipMemcpyDeviceToHost, g_stream));
+    RSKX_TRY(download(counts.data(), ws.tcounts.p, counts.size() * sizeof(int)));
     RSKX_TRY(sync_stream());
     // child boxes and each chunk's destinations (stable within a child)
     int c0 = 0;
@@ -531,6 +581,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   Workspace& ws = g_ws;
   Prof& prof = F->prof;
   prof.on = o.profile != 0;
+  prof.elim_only = o.profile == 2;
   auto t0 = clk::now();
 
   // ---- tree (device partition, host box bookkeeping)
@@ -723,7 +774,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     if (!exact) {
       for (int64_t ib = 0; ib < nb; ++ib) hmn[ib] = hb[ib].mc;
     } else {
-      RSKX_HIP(hipMemcpyAsync(hmn.data(), ws.mnbr.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
+      RSKX_TRY(download(hmn.data(), ws.mnbr.p, nb * sizeof(int)));
       RSKX_TRY(sync_stream());
     }
     std::vector<int> idg, idr[2], idq[kGramClasses], idall;
@@ -832,9 +883,9 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     hk.resize(nb);
     hit.resize(nb);
     std::vector<int> hm(nb);
-    RSKX_HIP(hipMemcpyAsync(hk.data(), ws.kout.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipMemcpyAsync(hit.data(), ws.iter.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
-    RSKX_HIP(hipMemcpyAsync(hm.data(), ws.mrows.p, nb * sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    RSKX_TRY(download(hk.data(), ws.kout.p, nb * sizeof(int)));
+    RSKX_TRY(download(hit.data(), ws.iter.p, nb * sizeof(int)));
+    RSKX_TRY(download(hm.data(), ws.mrows.p, nb * sizeof(int)));
     RSKX_TRY(sync_stream());
     RSKX_TRY(elim_status());                  // the previous level's elimination
     prof.flush();
@@ -892,7 +943,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         O.nl_off = -1;
       }
     }
-    prof.st["id"].flops += id_flops;
+    if (prof.wants("id")) prof.st["id"].flops += id_flops;
     F->lval.emplace_back();
     F->lidx.emplace_back();
     RSKX_TRY(F->lval.back().ensure((need_v + 1) * sizeof(S)));
@@ -1134,8 +1185,7 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
   launch_perm<U, S>(false, X, ldx, Y, N, nrhs, ncol, F->order.as<int>(), g_stream);
   prof.end("permute", ev, (int64_t)N * nrhs * (sizeof(S) + sizeof(U)) + N * 4);
   RSKX_HIP(hipGetLastError());
-  RSKX_TRY(sync_stream());
-  prof.flush();
+  prof.flush();                              // waits only when profiling
   return RSKX_OK;
 }
 
@@ -1194,6 +1244,9 @@ int rskx_opts_default(rskx_opts* o) {
 const char* rskx_last_error(void) { return g_err.c_str(); }
 
 int rskx_set_stream(void* s) {
+  // the asynchronous calls leave work queued on the old stream that still
+  // uses the library's workspaces: drain it before switching
+  if ((hipStream_t)s != g_stream) RSKX_TRY(sync_stream());
   g_stream = (hipStream_t)s;
   return RSKX_OK;
 }
@@ -1493,7 +1546,6 @@ int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, 
   launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, nullptr, 1,
                 g_stream);
   RSKX_HIP(hipGetLastError());
-  RSKX_TRY(sync_stream());
   return RSKX_OK;
 }
 
@@ -1506,10 +1558,11 @@ int rskx_cgs2(const double* V, int64_t ldv, int64_t ldr, int nv, double* w, int6
   if (N < 1 || nrhs < 1 || ldv < N || ldw < N || ldr < (int64_t)nv * ldv)
     return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad sizes or leading dimensions");
   const int nblk = cgs_blocks(N);
-  RSKX_TRY(g_ws.cgs.ensure((size_t)nrhs * nblk * (nv + 1) * sizeof(double)));
+  RSKX_TRY(g_ws.cgs.ensure(((size_t)nrhs * nblk * (nv + 1) + (size_t)nrhs * nv) * sizeof(double)));
   double* partial = g_ws.cgs.as<double>();
-  launch_cgs(V, ldv, ldr, nv, w, ldw, N, nrhs, partial, partial + (size_t)nrhs * nblk * nv, hout,
-             g_stream);
+  double* normpart = partial + (size_t)nrhs * nblk * nv;
+  launch_cgs(V, ldv, ldr, nv, w, ldw, N, nrhs, partial, normpart, normpart + (size_t)nrhs * nblk,
+             hout, g_stream);
   RSKX_HIP(hipGetLastError());
   return RSKX_OK;
 }
@@ -1531,7 +1584,6 @@ int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x,
   launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs,
                 doff.as<int64_t>(), nprob, g_stream);
   RSKX_HIP(hipGetLastError());
-  RSKX_TRY(sync_stream());
   return RSKX_OK;
 }
 

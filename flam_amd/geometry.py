@@ -41,6 +41,32 @@ class Curve:
         c[0] = self.diag[0]
         return c
 
+    def closed_form_matvec(self, X):
+        """A X with the unit circle's closed form A = a I + b 1 1': the
+        double-layer entries off the diagonal are analytically all equal.
+        a and b come from the first column evaluated entry by entry (b the
+        mean of its off-diagonal entries); their rounding spread (differences
+        of nearby points) bounds how far this is from the entry-wise matrix,
+        ~1e-11 in norm at N = 2^20, so it serves as the cheap inner operator
+        of the refinement while true residuals use circulant_matvec.  One
+        pass over X instead of two FFTs."""
+        if getattr(self, "_ab", None) is None:
+            c = self.column0()
+            b = c[1:].mean()
+            dev = (c[1:] - b).abs().sum()
+            if not bool(dev <= 1e-8 * c[0].abs()):
+                raise ValueError("closed_form_matvec: off-diagonal entries are not constant "
+                                 "(not the unit circle); use circulant_matvec")
+            self._ab = ((c[0] - b).item(), b.item())
+        a, b = self._ab
+        vec = X.dim() == 1
+        Xt = X.reshape(self.N, -1).to(torch.float64).t()    # (nrhs, N), rows contiguous
+        # row sums in two stages (math.gcd(N, 1024) contiguous pieces per row)
+        # so the reduction spreads over many blocks even for few columns
+        s = Xt.reshape(Xt.shape[0], math.gcd(self.N, 1024), -1).sum(2).sum(1, keepdim=True)
+        Yt = torch.add(s.mul_(b), Xt, alpha=a)
+        return Yt.reshape(-1) if vec else Yt.t()
+
     def circulant_matvec(self, X):
         """Exact FP64 A X for rotation-invariant curves (the circle): A is
         circulant, so A X = ifft(fft(A(:,1)) .* fft(X)) (ie_circle.m:47-48),

@@ -16,9 +16,11 @@ is one batched product.  Classical Gram-Schmidt is applied twice per step,
 by the library's fused kernels (rskx_cgs2): each pass reads the basis and
 the new vector once, with no temporaries.
 """
+import numpy as np
 import torch
 
 from . import _lib as L
+from .rskelf import Factor, rskelf_sv_
 
 
 def _cgs2(V, nv, w):
@@ -52,25 +54,31 @@ def _seg_norms(W, segments):
     return torch.stack([_norms(W[:, a:b]) for a, b in segments], dim=1)
 
 
-def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None):
+def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None, inner_matvec=None):
     """Solve A X = B.  matvec, precond: callables on (N, nrhs) FP64 CUDA
-    tensors.  segments: optional row ranges [(a, b), ...] of independent
+    tensors; precond may also be a Factor, whose solve then runs in place
+    on the stored preconditioned direction.  The small least-squares
+    problems are kept in QR form by Givens rotations, one new rotation per
+    step, on the host.  segments: optional row ranges [(a, b), ...] of independent
     problems stacked in one system (a block-diagonal A); then every
-    problem's own relative residual must reach tol.  Returns (X, info) with
-    info = dict(iters, outer, relres)."""
+    problem's own relative residual must reach tol.  inner_matvec: an
+    optional cheaper operator close to A for the Arnoldi steps (inexact
+    Krylov); convergence is still decided on true residuals B - A X taken
+    with matvec after each cycle, so the result meets tol for matvec's A.
+    Returns (X, info) with info = dict(iters, outer, relres)."""
     vec = B.dim() == 1
     B2 = B.reshape(B.shape[0], -1).to(torch.float64)
     N, nrhs = B2.shape
     Bt = B2.t().contiguous()                       # (nrhs, N)
     bnorm = _norms(Bt)
     bnorm = torch.where(bnorm > 0, bnorm, torch.ones_like(bnorm))
-    bnorm_h = bnorm.cpu()
+    bnorm_h = bnorm.cpu().numpy()
     if segments is not None:
         bseg = _seg_norms(Bt, segments)
         bseg = torch.where(bseg > 0, bseg, torch.ones_like(bseg))
         # the Arnoldi estimate bounds the stacked residual, which bounds
         # every block's: stop it at the smallest block's share
-        inner_ref = bseg.min(dim=1).values.cpu()
+        inner_ref = bseg.min(dim=1).values.cpu().numpy()
     else:
         inner_ref = bnorm_h
     Xt = torch.zeros_like(Bt)
@@ -93,27 +101,46 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None):
         if bool((relres <= tol).all()) or outer == maxit:
             break
         V[:, 0, :] = R / rn[:, None]
-        H = torch.zeros(nrhs, restart + 1, restart, dtype=torch.float64)
-        g = torch.zeros(nrhs, restart + 1, dtype=torch.float64)
-        g[:, 0] = rn.cpu()
-        y = None
+        Rh = np.zeros((nrhs, restart, restart))                # rotated H, upper triangular
+        cs = np.zeros((nrhs, restart))
+        sn = np.zeros((nrhs, restart))
+        g = np.zeros((nrhs, restart + 1))
+        g[:, 0] = rn.cpu().numpy()
         j = 0
         for j in range(restart):
-            Z[:, j, :] = op(precond, V[:, j, :])
-            w = op(matvec, Z[:, j, :])
+            if isinstance(precond, Factor):
+                Z[:, j, :].copy_(V[:, j, :])
+                rskelf_sv_(precond, Z[:, j, :])
+            else:
+                Z[:, j, :] = op(precond, V[:, j, :])
+            w = op(inner_matvec or matvec, Z[:, j, :])
             hdev = _cgs2(V, j + 1, w)                          # (nrhs, j + 2): h, ||w||
             hn = hdev[:, -1]
             V[:, j + 1, :] = w / torch.where(hn > 0, hn, torch.ones_like(hn))[:, None]
-            hcol = hdev.cpu()
-            H[:, : j + 2, j] = hcol
+            h = hdev.cpu().numpy()
             iters += 1
-            Hj = H[:, : j + 2, : j + 1]
-            y = torch.linalg.lstsq(Hj, g[:, : j + 2, None]).solution[..., 0]
-            est = torch.linalg.vector_norm(g[:, : j + 2] - (Hj @ y[..., None])[..., 0], dim=1)
-            if bool((est / inner_ref <= 0.5 * tol).all()) or bool((hcol[:, -1] == 0).all()):
+            for i in range(j):                                 # earlier rotations
+                t = cs[:, i] * h[:, i] + sn[:, i] * h[:, i + 1]
+                h[:, i + 1] = -sn[:, i] * h[:, i] + cs[:, i] * h[:, i + 1]
+                h[:, i] = t
+            r = np.hypot(h[:, j], h[:, j + 1])
+            safe = np.where(r > 0, r, 1.0)
+            cs[:, j] = np.where(r > 0, h[:, j] / safe, 1.0)
+            sn[:, j] = np.where(r > 0, h[:, j + 1] / safe, 0.0)
+            Rh[:, : j + 1, j] = h[:, : j + 1]
+            Rh[:, j, j] = r
+            g[:, j + 1] = -sn[:, j] * g[:, j]
+            g[:, j] = cs[:, j] * g[:, j]
+            est = np.abs(g[:, j + 1])                          # ||g - H y|| of the LS solution
+            if bool((est / inner_ref <= 0.5 * tol).all()) or bool((h[:, j + 1] == 0).all()):
                 break
-        yd = y.to(B2.device)[:, :, None]                       # (nrhs, j+1, 1)
-        Xt = Xt + (Z[:, : j + 1, :] * yd).sum(1)
+        y = np.zeros((nrhs, j + 1))                            # back substitution
+        for i in range(j, -1, -1):
+            d = Rh[:, i, i]
+            y[:, i] = np.where(d != 0, (g[:, i] - (Rh[:, i, i + 1: j + 1] * y[:, i + 1:]).sum(1))
+                               / np.where(d != 0, d, 1.0), 0.0)
+        yd = torch.from_numpy(y).to(B2.device)[:, None, :]     # (nrhs, 1, j+1)
+        Xt = Xt + torch.bmm(yd, Z[:, : j + 1, :])[:, 0, :]
     info = dict(iters=iters, outer=outer, relres=relres.tolist())
     X = Xt.t()
     return (X.reshape(-1) if vec else X), info

@@ -34,6 +34,13 @@ def _check(rc):
         raise FlamError(rc, L.last_error())
 
 
+def _on_torch_stream():
+    """Run the library on torch's current stream, so its work is ordered
+    after the torch work that made its inputs and before what reads its
+    outputs (the sweeps and products return once their work is queued)."""
+    _check(L.lib().rskx_set_stream(torch.cuda.current_stream().cuda_stream))
+
+
 def _dev(t, dtype=torch.float64):
     if not isinstance(t, torch.Tensor):
         t = torch.as_tensor(t, dtype=dtype)
@@ -243,7 +250,7 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     k = A._struct()
     p = pxyfun._struct() if pxyfun is not None else L.Proxy(L.RSKX_PROXY_NONE, 0, 1.5, 0.0)
     h = ctypes.c_void_p()
-    torch.cuda.synchronize()
+    _on_torch_stream()
     _check(L.lib().rskx_rskelf(xc.data_ptr(), d, N, int(occ), float(rank_or_tol),
                                ctypes.byref(k), ctypes.byref(p), ctypes.byref(o), ctypes.byref(h)))
     return Factor(h.value, N, keep=(A, xc))
@@ -316,7 +323,7 @@ def rskelf_batch(batch, occ, rank_or_tol, opts=None):
             st.pts = structs[0].pts
     pxs = (L.Proxy * nprob)(*structs)
     h = ctypes.c_void_p()
-    torch.cuda.synchronize()
+    _on_torch_stream()
     _check(L.lib().rskx_rskelf_batch(nprob, offs, xc.data_ptr(), d, int(occ), float(rank_or_tol),
                                      ctypes.byref(k), pxs, ctypes.byref(o), ctypes.byref(h)))
     return Factor(h.value, N, keep=(batch.kernel, xc))
@@ -335,7 +342,7 @@ def kernel_matvec_batch(batch, X):
     k = batch.kernel._struct()
     nprob = len(batch.offsets) - 1
     offs = (ctypes.c_int64 * (nprob + 1))(*batch.offsets)
-    torch.cuda.synchronize()
+    _on_torch_stream()
     _check(L.lib().rskx_kernel_matvec_batch(nprob, offs, xc.data_ptr(), d, ctypes.byref(k),
                                             Xc.data_ptr(), N, Yc.data_ptr(), N, Xc.shape[0]))
     Y = Yc.t()
@@ -357,7 +364,7 @@ def _sweep(fn, F, X, trans):
     Xc = X2.t().clone(memory_format=torch.contiguous_format)
     dt = L.RSKX_F64 if Xc.dtype == torch.float64 else L.RSKX_F32
     t = str(trans).encode()[:1] if trans else b"n"
-    torch.cuda.synchronize()
+    _on_torch_stream()
     _check(fn(F.handle, Xc.data_ptr(), F.N, Xc.shape[0], t, dt))
     Y = Xc.t()
     return Y.reshape(-1) if vec else Y
@@ -368,7 +375,7 @@ def _sweep_mode(fn, F, X, trans, mode):
     X2 = X if X.dim() == 2 else X.reshape(-1, 1)
     Xc = X2.t().clone(memory_format=torch.contiguous_format)
     dt = L.RSKX_F64 if Xc.dtype == torch.float64 else L.RSKX_F32
-    torch.cuda.synchronize()
+    _on_torch_stream()
     _check(fn(F.handle, Xc.data_ptr(), F.N, Xc.shape[0], str(trans).encode()[:1], dt, mode))
     return Xc.t()
 
@@ -434,6 +441,21 @@ def rskelf_sv(F, X, trans="n"):
     return _sweep(L.lib().rskx_rskelf_sv, F, X, trans)
 
 
+def rskelf_sv_(F, Xt, trans="n"):
+    """In-place F^{-1} on a column-major N x nrhs block given as its
+    (nrhs, N) row view Xt: unit stride along N, rows any stride >= N (a
+    slice of a larger array; the C ABI's leading dimension).  Returns Xt."""
+    if (Xt.dim() != 2 or not Xt.is_cuda or Xt.shape[1] != F.N or Xt.stride(1) != 1
+            or Xt.stride(0) < F.N or Xt.dtype not in (torch.float64, torch.float32)):
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: Xt must be an (nrhs, N) CUDA view "
+                                        "with unit stride along N")
+    dt = L.RSKX_F64 if Xt.dtype == torch.float64 else L.RSKX_F32
+    _on_torch_stream()
+    _check(L.lib().rskx_rskelf_sv(F.handle, Xt.data_ptr(), Xt.stride(0), Xt.shape[0],
+                                  str(trans).encode()[:1], dt))
+    return Xt
+
+
 def rskelf_mv(F, X, trans="n"):
     """Apply F (rskelf/rskelf_mv.m:13-36)."""
     return _sweep(L.lib().rskx_rskelf_mv, F, X, trans)
@@ -443,7 +465,7 @@ def rskelf_logdet(F):
     """log det of the factored matrix, imaginary part in [0, 2 pi)
     (rskelf/rskelf_logdet.m:11)."""
     re, im = ctypes.c_double(), ctypes.c_double()
-    torch.cuda.synchronize()
+    _on_torch_stream()
     _check(L.lib().rskx_rskelf_logdet(F.handle, ctypes.byref(re), ctypes.byref(im)))
     return complex(re.value, im.value)
 
@@ -470,7 +492,7 @@ def kernel_matvec(A, x, X):
     Xc = X2.t().contiguous()
     Yc = torch.empty_like(Xc)
     k = A._struct()
-    torch.cuda.synchronize()
+    _on_torch_stream()
     _check(L.lib().rskx_kernel_matvec(xc.data_ptr(), d, N, ctypes.byref(k), Xc.data_ptr(), N,
                                       Yc.data_ptr(), N, Xc.shape[0]))
     Y = Yc.t()

@@ -10,10 +10,17 @@
  *     ownership.  Matrices are column-major, like MATLAB arrays.
  *   - Indices handed back to the host (rskx_export_factor) are 0-based;
  *     the gateway adds 1 for MATLAB.
- *   - Every call is synchronous on the library's stream (rskx_set_stream,
- *     default: the null stream) and returns RSKX_OK (0) or a negative
- *     code; rskx_last_error() then gives a message that starts with the
- *     reference's error identifier ("FLAM:chksymm:invalidSymm: ...").
+ *   - Work runs on the library's stream (rskx_set_stream, default: the
+ *     null stream).  The factorizations and every call that hands data
+ *     back to the host return when their work is done; the sweeps
+ *     (rskx_rskelf_sv, _mv, _cholmv, _cholsv), the on-the-fly products
+ *     (rskx_kernel_matvec, _batch) and rskx_cgs2 only queue theirs: the
+ *     results are ready for later work on that stream (synchronize it
+ *     before reading them on the host).  rskx_set_stream first drains the
+ *     old stream when the stream changes.
+ *   - Every call returns RSKX_OK (0) or a negative code; rskx_last_error()
+ *     then gives a message that starts with the reference's error
+ *     identifier ("FLAM:chksymm:invalidSymm: ...").
  *   - A factor is device-resident behind an opaque handle; one handle must
  *     not be used from two threads at once.
  */
@@ -92,10 +99,12 @@ typedef struct {
   const double* ext;    /* HOST d x 2 root extent, or NULL (from the data)  */
   double Tmax;          /* ID interpolation bound, default 2                */
   double rrqr_iter;     /* max RRQR iterations, default Inf                 */
-  char symm;            /* 'n' (LU) or 'p' (Cholesky); 's','h' -> UNSUP     */
+  char symm;            /* 'n' (LU), 'p' (Cholesky), 'h' (LDL'); 's' ->   */
+                        /* UNSUP                                         */
   int verb;             /* 1: print the per-level table to stdout           */
   int precision;        /* RSKX_PREC_*                                      */
-  int profile;          /* 1: time each kernel class with HIP events        */
+  int profile;          /* 1: time each kernel class with HIP events;       */
+                        /* 2: time only the elimination class ("factor")  */
   int stop;             /* factor at most this many levels, leaves first
                            (rskelf.m:128,172: numeric opts.stop); < 0: all   */
 } rskx_opts;

@@ -103,6 +103,17 @@ def test_python_mirror_rejects_without_gpu():
     assert hasattr(flam_amd, "rskelf") and hasattr(flam_amd, "rskelf_sv")
 
 
+def test_inplace_solve_checks_its_view():
+    """rskelf_sv_ refuses a view it cannot hand to the C ABI as a
+    column-major block (host memory, wrong length, non-unit stride)."""
+    import torch
+    from flam_amd.rskelf import Factor, FlamError, rskelf_sv_
+    F = Factor(None, 10, None)
+    for bad in (torch.zeros(2, 10), torch.zeros(2, 9), torch.zeros(10, 2).t()):
+        with pytest.raises(FlamError):
+            rskelf_sv_(F, bad)
+
+
 def _batch_call(L, offsets, proxies, ext=False):
     o = L.Opts()
     L.lib().rskx_opts_default(ctypes.byref(o))
