@@ -108,7 +108,8 @@ def roofline_of(F, F_full, precision):
     step F when that step timed it (the elimination class), else from
     F_full."""
     kt = F_full.kernel_times()
-    name, st = max(((k, v) for k, v in kt.items() if k != "tree"), key=lambda kv: kv[1]["ms"])
+    name, st = max(((k, v) for k, v in kt.items() if k != "tree" and not k.startswith("h:")),
+                   key=lambda kv: kv[1]["ms"])
     st = F.kernel_times().get(name, st)
     avg_ms = st["ms"] / max(st["launches"], 1)
     if st["flops"] > 0:

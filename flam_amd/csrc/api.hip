@@ -369,7 +369,7 @@ namespace {
 int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* poff, int occ,
                    int lvlmax, const double* ext, HostTree& t, DevBuf& order) {
   Workspace& ws = g_ws;
-  constexpr int kChunk = 8192;
+  constexpr int kChunk = 2048;           // points per split workgroup
   constexpr int kMaxLevels = 64;
   RSKX_TRY(ws.xs.ensure((size_t)d * N * sizeof(double)));
   RSKX_TRY(ws.txs.ensure((size_t)d * N * sizeof(double)));
@@ -393,7 +393,8 @@ int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* 
   t.d = d;
   t.nprob = nprob;
   std::vector<double> l((size_t)nprob * d);     // each tree's current box size
-  std::vector<std::vector<int>> chld(nprob);
+  t.cfirst.assign(nprob, -1);
+  t.ccnt.assign(nprob, 0);
   for (int j = 0; j < nprob; ++j) {
     double lo[3], hi[3];
     for (int i = 0; i < d; ++i) {
@@ -490,8 +491,9 @@ int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* 
         cb += tot[u];
         t.prnt.push_back(p);
         t.prob.push_back(j);
-        chld.emplace_back();
-        chld[p].push_back(t.nbox() - 1);
+        t.cfirst.push_back(-1);
+        t.ccnt.push_back(0);
+        if (t.ccnt[p]++ == 0) t.cfirst[p] = t.nbox() - 1;
       }
       t.cnt[p] = 0;
       c0 = c1;
@@ -501,13 +503,13 @@ int build_tree_dev(const double* x, int d, int64_t N, int nprob, const int64_t* 
                       ws.txs.as<double>(), ws.tord.as<int>(), ws.tcounts.as<int>(), g_stream);
     // the parent level's split status is now final: its neighbour lists
     // are built while the device scatters this level's points
-    append_level_neighbours(t, chld, nlvl - 1);
+    append_level_neighbours(t, nlvl - 1);
     ++nlvl;
     t.lvp.push_back(t.nbox());
   }
   RSKX_HIP(hipGetLastError());
   t.nlvl = nlvl;
-  finish_tree(t, chld);
+  finish_tree(t);
   return RSKX_OK;
 }
 
@@ -668,6 +670,13 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     if (o.stop >= 0 && F->nlvl_done >= o.stop) break;
     lv_last = lv;
     auto tl = clk::now();
+    // host time per phase of the level ("h:" classes, full profiling only)
+    auto th = tl;
+    auto hlap = [&](const char* name) {
+      const auto now = clk::now();
+      prof.host(name, std::chrono::duration<double, std::milli>(now - th).count());
+      th = now;
+    };
     const int lvl_ref = lv + 1;   // the reference's level number
     const int64_t b0 = t.lvp[lv], nb = t.lvp[lv + 1] - b0;
     const int64_t nrem1 = nrem;
@@ -750,6 +759,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     RSKX_TRY(ws.perm.ensure(std::max<int64_t>(off_perm, 1) * sizeof(int)));
     RSKX_TRY(ws.T.ensure(std::max<int64_t>(off_T, 1) * sizeof(double)));
     RSKX_TRY(ws.W.ensure(std::max<int64_t>(off_W, 1) * sizeof(S)));
+    hlap("h:segments");
 
     hipEvent_t ev;
     prof.begin("lists", ev);
@@ -880,6 +890,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     }
     prof.end("id", ev);
     RSKX_HIP(hipGetLastError());
+    hlap("h:routing");
     hk.resize(nb);
     hit.resize(nb);
     std::vector<int> hm(nb);
@@ -887,6 +898,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     RSKX_TRY(download(hit.data(), ws.iter.p, nb * sizeof(int)));
     RSKX_TRY(download(hm.data(), ws.mrows.p, nb * sizeof(int)));
     RSKX_TRY(sync_stream());
+    hlap("h:wait");
     RSKX_TRY(elim_status());                  // the previous level's elimination
     prof.flush();
     for (int64_t ib = 0; ib < nb; ++ib)
@@ -1019,6 +1031,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     }
     prof.end("factor", ev, 0, el_flops);
     RSKX_HIP(hipGetLastError());
+    hlap("h:layout");
     // the failure flag comes back with the next synchronization; the host
     // meanwhile lays out the next level
     RSKX_HIP(hipMemcpyAsync(ws.hflag, ws.flag.p, sizeof(int), hipMemcpyDeviceToHost, g_stream));
@@ -1080,6 +1093,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     const int64_t nblk = pblk[lv] + nb;
     F->table.push_back({lvl_ref, nblk, nrem1, nrem});
     F->nlvl_done++;
+    hlap("h:commit");
     if (o.verb)
       printf("%3d | %6lld | %8lld | %8lld | %8.2f | %8.2f | %10.2e\n", lvl_ref, (long long)nblk,
              (long long)nrem1, (long long)nrem, (double)nrem1 / nblk, (double)nrem / nblk,

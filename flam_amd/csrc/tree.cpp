@@ -92,7 +92,7 @@ class HostPool {
   bool stop_ = false;
 };
 
-void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& chld, int lv) {
+void append_level_neighbours(HostTree& t, int lv) {
   // neighbours of the boxes of level lv (hypoct.m:141-170): siblings,
   // adjacent non-split boxes among the parent's neighbours, and adjacent
   // children of the parent's neighbours.  Boxes are numbered level by
@@ -111,7 +111,7 @@ void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& c
     const double* c = &t.ctr[(size_t)d * i];
     const double* lc = &t.bsize[(size_t)d * i];
     const int p = t.prnt[i];
-    for (int j : chld[p])
+    for (int j = t.cfirst[p]; j < t.cfirst[p] + t.ccnt[p]; ++j)
       if (j != i) out.push_back(j);
     const int pn0 = t.nbor_off[p], pn1 = t.nbor_off[p + 1];
     for (int e = pn0; e < pn1; ++e) {
@@ -126,7 +126,7 @@ void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& c
     }
     for (int e = pn0; e < pn1; ++e) {
       const int j = t.nbor[e];
-      for (int q2 : chld[j]) {
+      for (int q2 = t.cfirst[j]; q2 < t.cfirst[j] + t.ccnt[j]; ++q2) {
         const double* qc = &t.ctr[(size_t)d * q2];
         double dmax = -INFINITY;
         for (int u = 0; u < d; ++u) dmax = std::max(dmax, matlab_round(std::fabs(c[u] - qc[u]) / lc[u]));
@@ -140,6 +140,8 @@ void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& c
   pool.parallel_for(nb, parts, [&](int w, int64_t lo, int64_t hi) {
     std::vector<int>& out = lists[w];
     std::vector<int>& cnt = counts[w];
+    out.reserve((size_t)(hi - lo) * (t.d == 3 ? 32 : 8));
+    cnt.reserve((size_t)(hi - lo));
     for (int64_t q = lo; q < hi; ++q) {
       const size_t before = out.size();
       box_list(b0 + q, out);
@@ -156,14 +158,14 @@ void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& c
   }
 }
 
-void finish_tree(HostTree& t, const std::vector<std::vector<int>>& chld) {
+void finish_tree(HostTree& t) {
   const int nbox = t.nbox();
   t.chld_off.assign(nbox + 1, 0);
-  for (int b = 0; b < nbox; ++b) t.chld_off[b + 1] = t.chld_off[b] + (int)chld[b].size();
-  t.chld.clear();
-  t.chld.reserve(t.chld_off[nbox]);
-  for (int b = 0; b < nbox; ++b) t.chld.insert(t.chld.end(), chld[b].begin(), chld[b].end());
-  for (int lv = 0; lv < t.nlvl; ++lv) append_level_neighbours(t, chld, lv);
+  for (int b = 0; b < nbox; ++b) t.chld_off[b + 1] = t.chld_off[b] + t.ccnt[b];
+  t.chld.resize(t.chld_off[nbox]);
+  for (int b = 0; b < nbox; ++b)
+    for (int c = 0; c < t.ccnt[b]; ++c) t.chld[t.chld_off[b] + c] = t.cfirst[b] + c;
+  for (int lv = 0; lv < t.nlvl; ++lv) append_level_neighbours(t, lv);
 }
 
 }  // namespace rskx

@@ -30,6 +30,9 @@ struct HostTree {
   std::vector<double> ctr;        // d x nbox centres
   std::vector<int> prnt;          // parent box, -1 for the root
   std::vector<int> chld_off, chld;  // CSR children
+  // while building: a split box's children are consecutive boxes
+  // [cfirst, cfirst + ccnt) (ccnt 0: not split)
+  std::vector<int> cfirst, ccnt;
   std::vector<int> nbor_off, nbor;  // CSR neighbours
   std::vector<int64_t> beg, cnt;  // leaf point range in `order` (cnt 0: not a leaf)
   std::vector<int64_t> order;     // tree order -> original point index
@@ -40,13 +43,13 @@ struct HostTree {
 };
 
 // Neighbour lists of the boxes of level lv (hypoct.m:141-170), appended to
-// t.nbor / t.nbor_off; needs the lists of every earlier level and the final
-// split status (cnt) of the levels above lv.
-void append_level_neighbours(HostTree& t, const std::vector<std::vector<int>>& chld, int lv);
+// t.nbor / t.nbor_off; needs the lists of every earlier level, the final
+// split status (cnt) of the levels above lv and the child ranges.
+void append_level_neighbours(HostTree& t, int lv);
 
 // Children in CSR form and the neighbour lists of the levels not yet
-// appended, given the boxes (ctr, prnt, beg/cnt, lvp, bsize, nlvl) and each
-// box's children.
-void finish_tree(HostTree& t, const std::vector<std::vector<int>>& chld);
+// appended, given the boxes (ctr, prnt, beg/cnt, lvp, bsize, nlvl) and the
+// child ranges (cfirst, ccnt).
+void finish_tree(HostTree& t);
 
 }  // namespace rskx
