@@ -283,16 +283,29 @@ struct Prof {
   bool elim_only = false;
   std::map<std::string, KStat> st;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> spare;           // events reused after a flush
+  Prof() = default;
+  Prof(const Prof&) = delete;
+  Prof& operator=(const Prof&) = delete;
+  ~Prof() {
+    for (auto& p : pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
+    for (hipEvent_t e : spare) hipEventDestroy(e);
+  }
+  hipEvent_t event() {
+    hipEvent_t e = nullptr;
+    if (!spare.empty()) { e = spare.back(); spare.pop_back(); }
+    else hipEventCreate(&e);
+    return e;
+  }
   bool wants(const char* name) const { return on && (!elim_only || !strcmp(name, "factor")); }
   void begin(const char* name, hipEvent_t& a) {
     if (!wants(name)) return;
-    hipEventCreate(&a);
+    a = event();
     hipEventRecord(a, g_stream);
   }
   void end(const char* name, hipEvent_t a, int64_t bytes = 0, double flops = 0) {
     if (!wants(name)) return;
-    hipEvent_t b;
-    hipEventCreate(&b);
+    hipEvent_t b = event();
     hipEventRecord(b, g_stream);
     pending.push_back({name, {a, b}});
     auto& s = st[name];
@@ -307,8 +320,8 @@ struct Prof {
       float ms = 0;
       hipEventElapsedTime(&ms, p.second.first, p.second.second);
       st[p.first].ms += ms;
-      hipEventDestroy(p.second.first);
-      hipEventDestroy(p.second.second);
+      spare.push_back(p.second.first);
+      spare.push_back(p.second.second);
     }
     pending.clear();
   }

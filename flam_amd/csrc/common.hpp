@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 
 namespace rskx {
 
@@ -120,5 +122,24 @@ struct TreeChunk {
   double ctr[3];          // that box's centre
   int64_t off[8];         // scatter: first destination of each child code
 };
+
+// Raise a kernel's dynamic shared-memory limit to `bytes` (above 64 KB),
+// calling into HIP only when the kernel has not yet been given that much:
+// the per-level launches of a factorization and the sweeps would otherwise
+// repeat the same attribute call every launch.
+inline void ensure_dyn_lds(const void* kern, size_t bytes) {
+  if (bytes <= 65536) return;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  static std::mutex mu;
+  static std::unordered_map<const void*, size_t> set[64];   // per device
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& cur = set[dev & 63][kern];
+  if (cur >= bytes) return;
+  if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
+    cur = bytes;
+  else
+    (void)hipGetLastError();
+}
 
 }  // namespace rskx

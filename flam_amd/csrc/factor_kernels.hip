@@ -1341,9 +1341,7 @@ void launch_gram_id(int cls, const int* ids, const LvlBox* boxes, const int* poo
 #define RSKX_GID(NTV, TPTV)                                                                      \
   {                                                                                              \
     auto kern = k_gram_id<NTV, TPTV>;                                                            \
-    if (shmem > 65536)                                                                           \
-      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,    \
-                                (int)shmem);                                                     \
+    ensure_dyn_lds((const void*)kern, shmem);                                                     \
     hipLaunchKernelGGL(kern, dim3(nb), dim3(NTV), shmem, st, ids, boxes, pool, nbrar, mnbr,      \
                        mrows, Tar, permar, P, L, prm, kout);                                     \
   }
@@ -1373,9 +1371,7 @@ void launch_id_reg(int cls, const int* ids, const LvlBox* boxes, const int* pool
 #define RSKX_IDR(RAV, CBV)                                                                       \
   {                                                                                              \
     auto kern = k_id_reg<S, RAV, CBV>;                                                           \
-    if (shmem > 65536)                                                                           \
-      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,    \
-                                (int)shmem);                                                     \
+    ensure_dyn_lds((const void*)kern, shmem);                                                     \
     hipLaunchKernelGGL(kern, dim3(nb), dim3(1024), shmem, st, ids, boxes, pool, nbrar, mnbr,     \
                        mrows, Kar, permar, P, L, prm, kout);                                     \
   }
@@ -1413,9 +1409,7 @@ void launch_id(int mode, const int* ids, const LvlBox* boxes, const int* mrows, 
                int* permar, double* Tar, const IdParams& prm, int* kout, int* iterout, int nb,
                size_t shmem, double* vglob, int64_t vstride, hipStream_t st) {
   auto kern = mode == 0 ? k_id<S, 0> : mode == 1 ? k_id<S, 1> : k_id<S, 2>;
-  if (shmem > 65536)
-    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)shmem);
+  ensure_dyn_lds((const void*)kern, shmem);
   hipLaunchKernelGGL(kern, dim3(nb), dim3(256), shmem, st, ids, boxes, mrows, Kar, permar, Tar, prm,
                      kout, iterout, vglob, vstride);
 }
@@ -1996,9 +1990,7 @@ void launch_elim_reg(bool lu, bool ldl, int nw, size_t shmem, const int* ids, in
   {                                                                                             \
     auto kern = ldl ? k_elim_reg<S, false, TAV, TBV, NWV, true>                                  \
                     : k_elim_reg<S, LUV, TAV, TBV, NWV, false>;                                  \
-    if (shmem > 65536)                                                                          \
-      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                                (int)shmem);                                                    \
+    ensure_dyn_lds((const void*)kern, shmem);                                                    \
     hipLaunchKernelGGL(kern, dim3(nb), dim3(64 * NWV), shmem, st, ids, boxes, outs, csegs, pool, \
                        permar, Tar, Mprev, Mcur, fval, fidx, P, fail);                          \
   }
@@ -2024,9 +2016,7 @@ void launch_elim(bool lu, bool ldl, int mode, size_t shmem, const int* ids, int 
 #define RSKX_ELIM(LUV, SHV)                                                                   \
   {                                                                                           \
     auto kern = ldl ? k_elim<S, false, SHV, true> : k_elim<S, LUV, SHV, false>;               \
-    if (shmem > 65536)                                                                        \
-      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                (int)shmem);                                                  \
+    ensure_dyn_lds((const void*)kern, shmem);                                                  \
     hipLaunchKernelGGL(kern, dim3(nb), dim3(1024), shmem, st, ids, boxes, outs, csegs, pool,  \
                        permar, Tar, Mprev, Mcur, War, fval, fidx, P, fail);                   \
   }

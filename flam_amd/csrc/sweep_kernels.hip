@@ -332,9 +332,7 @@ static void launch_prog(bool up, const FacRec* recs, int nrec, const S* fval, co
                         int64_t gstride, hipStream_t st) {
   auto kern = gws ? (up ? k_sweep<S, PROG, true, true> : k_sweep<S, PROG, false, true>)
                   : (up ? k_sweep<S, PROG, true, false> : k_sweep<S, PROG, false, false>);
-  if (shmem > 65536)
-    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)shmem);
+  ensure_dyn_lds((const void*)kern, shmem);
   hipLaunchKernelGGL(kern, dim3(nrec), dim3(256), shmem, st, recs, fval, fidx, X, ldx, nrhs, kmax,
                      rmax, bcap, gws, gstride);
 }
@@ -632,9 +630,7 @@ void launch_solve(int prog, bool up, const FacRec* recs, int nrec, const S* fval
   {                                                                                             \
     auto kern = gws ? (up ? k_solve<S, P, true, true> : k_solve<S, P, false, true>)            \
                     : (up ? k_solve<S, P, true, false> : k_solve<S, P, false, false>);          \
-    if (shmem > 65536)                                                                          \
-      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                                (int)shmem);                                                    \
+    ensure_dyn_lds((const void*)kern, shmem);                                                    \
     hipLaunchKernelGGL(kern, dim3(nrec), dim3(kSolveThreads), shmem, st, recs, fval, fidx, X,    \
                        ldx, nrhs, kmax, rmax, gws, gstride);                                    \
   }

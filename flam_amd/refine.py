@@ -20,7 +20,20 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .rskelf import Factor, rskelf_sv_
+from .rskelf import Factor, rskelf_mv, rskelf_sv_
+
+
+def factor_operator(F):
+    """A X through a factor's forward sweep (rskelf_mv, rskelf/rskelf_mv.m:13).
+
+    An FP64 factor built at a tight tolerance reproduces A to about that
+    tolerance at the cost of one sweep, so it serves as the accurate
+    operator of the refinement where A has no fast exact product (the
+    reference's demos use an ifmm for that, rskelf/test/ie_sphere.m:99-125;
+    this is the other accurate operator the library can offer).  The
+    attained residual then holds for this operator; the caller confirms it
+    against the exact product."""
+    return lambda V: rskelf_mv(F, V)
 
 
 def _cgs2(V, nv, w):

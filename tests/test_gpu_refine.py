@@ -82,3 +82,33 @@ def test_inplace_solve_on_strided_rows():
     ref = flam_amd.rskelf_sv(F, Z0[:, 1, :].t())            # (N, 3)
     assert torch.equal(Z[:, 1, :], ref.t())
     assert torch.equal(Z[:, 0, :], Z0[:, 0, :]) and torch.equal(Z[:, 2:, :], Z0[:, 2:, :])
+
+
+def test_factor_operator_refines_against_exact_product():
+    """A tol-1e-12 FP64 factor's forward sweep as the refinement operator
+    (no fast exact product for ellipses): it agrees with the exact
+    on-the-fly product to ~1e-12, and FGMRES on it with an FP32
+    preconditioner reaches a true residual (exact product) <= 1e-10 in
+    every problem of a batch."""
+    import flam_amd
+    from flam_amd import geometry
+    from flam_amd.refine import factor_operator, fgmres
+    probs = [geometry.ellipse(n, 1.0, b) for n, b in [(3000, 0.3), (9000, 0.8), (20000, 0.5)]]
+    print("factor operator: 3 ellipses N=3000+9000+20000", flush=True)
+    batch = flam_amd.stack_problems(probs)
+    F_op = flam_amd.rskelf_batch(batch, 128, 1e-12, dict(symm="n"))
+    op = factor_operator(F_op)
+    exact = lambda V: flam_amd.kernel_matvec_batch(batch, V)
+    X = torch.from_numpy(np.random.default_rng(10).standard_normal((batch.N, 3))).cuda()
+    Y1, Y2 = op(X), exact(X)
+    rel = (torch.linalg.vector_norm(Y1 - Y2, dim=0) / torch.linalg.vector_norm(Y2, dim=0)).max()
+    print(f"operator vs exact product: {rel.item():.2e}", flush=True)
+    assert rel.item() <= 1e-11
+    F = flam_amd.rskelf_batch(batch, 128, 1e-6, dict(symm="n", precision="fp32"))
+    segs = list(zip(batch.offsets[:-1], batch.offsets[1:]))
+    Xs, info = fgmres(op, X, F, tol=1e-10, segments=segs)
+    R = X - exact(Xs)
+    worst = max((torch.linalg.vector_norm(R[a:b], dim=0) /
+                 torch.linalg.vector_norm(X[a:b], dim=0)).max().item() for a, b in segs)
+    print(f"fgmres iters {info['iters']}, true residual {worst:.2e}", flush=True)
+    assert worst <= 1e-10
