@@ -1918,30 +1918,36 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
       S x[kDiagBlock];
 #pragma unroll
       for (int t = 0; t < kDiagBlock; ++t) x[t] = (S)0;
-      const S* Lb = staged ? stg + (size_t)b * kBB : fLU + j0 + (int64_t)j0 * r;
-      const int ldb = staged ? kDiagBlock : r;
-      if (!upper) {
+      // column c of the block's inverse by substitution; inlined once for
+      // the LDS copy and once for the factor in global memory, so that
+      // each call site's loads know their address space (a pointer that
+      // may be either compiles to flat loads)
+      auto invert = [&](const S* Lb, int ldb) __attribute__((always_inline)) {
+        if (!upper) {
 #pragma unroll
-        for (int t = 0; t < kDiagBlock; ++t) {
-          if (t < bs) {
-            S v = (t == c) ? (S)1 : (S)0;
+          for (int t = 0; t < kDiagBlock; ++t) {
+            if (t < bs) {
+              S v = (t == c) ? (S)1 : (S)0;
 #pragma unroll
-            for (int u = 0; u < t; ++u) v -= Lb[t + u * ldb] * x[u];
-            x[t] = (LU || LDL) ? v : v / Lb[t + t * ldb];
+              for (int u = 0; u < t; ++u) v -= Lb[t + u * ldb] * x[u];
+              x[t] = (LU || LDL) ? v : v / Lb[t + t * ldb];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int tt = kDiagBlock - 1; tt >= 0; --tt) {
+            if (tt < bs) {
+              S v = (tt == c) ? (S)1 : (S)0;
+#pragma unroll
+              for (int u = tt + 1; u < kDiagBlock; ++u)
+                if (u < bs) v -= Lb[tt + u * ldb] * x[u];
+              x[tt] = v / Lb[tt + tt * ldb];
+            }
           }
         }
-      } else {
-#pragma unroll
-        for (int tt = kDiagBlock - 1; tt >= 0; --tt) {
-          if (tt < bs) {
-            S v = (tt == c) ? (S)1 : (S)0;
-#pragma unroll
-            for (int u = tt + 1; u < kDiagBlock; ++u)
-              if (u < bs) v -= Lb[tt + u * ldb] * x[u];
-            x[tt] = v / Lb[tt + tt * ldb];
-          }
-        }
-      }
+      };
+      if (staged) invert(stg + (size_t)b * kBB, kDiagBlock);
+      else invert(fLU + j0 + (int64_t)j0 * r, r);
       if (c < kDiagBlock) {
         S* dst = fval + (upper ? O.Ui : O.Li) + (int64_t)b * kDiagBlock * kDiagBlock;
         S* dstt = fval + (upper ? O.Uti : O.Lti) + (int64_t)b * kDiagBlock * kDiagBlock;
