@@ -652,7 +652,9 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   RSKX_TRY(ws.flag.ensure(64));
   RSKX_HIP(hipMemsetAsync(ws.flag.p, 0, 64, g_stream));
 
-  const bool lu = (o.symm == 'n');
+  // 's' (rskelf.m:210,220-227): LU like 'n' but one-sided compression;
+  // with real data T.' = T' and its sweeps are the 'n' sweeps
+  const bool lu = (o.symm == 'n' || o.symm == 's');
   const bool two = (o.symm == 'n');
   const bool ldl = (o.symm == 'h');
   IdParams prm;
@@ -1298,8 +1300,6 @@ static int check_factor_args(const double* x, int d, int64_t N, int occ, double 
   char s = (char)tolower(o.symm);
   if (s != 'n' && s != 's' && s != 'h' && s != 'p')
     return fail(RSKX_ERR_ARG, "FLAM:chksymm:invalidSymm: Symmetry parameter must be one of 'N', 'S', 'H', or 'P'.");
-  if (s == 's')
-    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:symm: symm 's' (complex symmetric) is not built on the device (use 'n', 'h' or 'p')");
   o.symm = s;
   if (occ < 0) return fail(RSKX_ERR_ARG, "FLAM:hypoct:invalidOcc: Leaf occupancy must be nonnegative.");
   if (o.lvlmax < 0) return fail(RSKX_ERR_ARG, "FLAM:hypoct:invalidLvlmax: Maximum tree depth must be at least 1.");
@@ -1495,7 +1495,7 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
   if (!v->sk) return RSKX_OK;
   const int flv = F->level_of(i);
   const int64_t k = f.k, r = f.r;
-  const bool lu = F->symm == 'n';
+  const bool lu = F->symm == 'n' || F->symm == 's';
   const int es = F->prec == RSKX_PREC_FP64 ? 8 : 4;
   auto getv = [&](int64_t off, int64_t cnt, std::vector<double>& dst) -> int {
     dst.resize(cnt);

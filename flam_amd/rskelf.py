@@ -174,7 +174,7 @@ class Factor:
         v = L.FactorView()
         _check(L.lib().rskx_export_factor(self._h, i, ctypes.byref(v)))
         k, r = v.k, v.r
-        lu = self.info()["symm"] == "n"
+        lu = self.info()["symm"] in ("n", "s")
         arr = dict(sk=np.zeros(k, np.int64), rd=np.zeros(r, np.int64),
                    T=np.zeros((r, k)), L=np.zeros((r, r)), U=np.zeros((r, r)),
                    p=np.zeros(r, np.int64), E=np.zeros((r, k)), F=np.zeros((k, r)))

@@ -99,8 +99,8 @@ typedef struct {
   const double* ext;    /* HOST d x 2 root extent, or NULL (from the data)  */
   double Tmax;          /* ID interpolation bound, default 2                */
   double rrqr_iter;     /* max RRQR iterations, default Inf                 */
-  char symm;            /* 'n' (LU), 'p' (Cholesky), 'h' (LDL'); 's' ->   */
-                        /* UNSUP                                         */
+  char symm;            /* 'n' (LU), 's' (LU, one-sided compression: a    */
+                        /* symmetric matrix), 'p' (Cholesky), 'h' (LDL') */
   int verb;             /* 1: print the per-level table to stdout           */
   int precision;        /* RSKX_PREC_*                                      */
   int profile;          /* 1: time each kernel class with HIP events;       */

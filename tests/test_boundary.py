@@ -79,8 +79,7 @@ def test_argument_errors_carry_reference_ids(kw, ident):
     assert msg.startswith(ident)
 
 
-@pytest.mark.parametrize("kw", [dict(symm=b"s"), dict(d=3),
-                                dict(pkind=0), dict(tol=5.0)])
+@pytest.mark.parametrize("kw", [dict(d=3), dict(pkind=0), dict(tol=5.0)])
 def test_unbuilt_options_are_refused(kw):
     L = _lib()
     rc, msg = _call(L, **kw)

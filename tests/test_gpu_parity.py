@@ -398,3 +398,36 @@ def test_cholmv_cholsv():
     with pytest.raises(flam_amd.FlamError) as e:
         flam_amd.rskelf_cholmv(Fn, x)
     assert e.value.ident == "FLAM:rskelf_cholmv:invalidSymm"
+
+
+# ------------------------------------------------------------ symm 's' --
+def test_symm_s_real_symmetric():
+    """symm 's' on the real symmetric benchmark circle (rskelf.m:210,220-227,
+    sv/rskelf_sv_sn.m, sv/rskelf_sv_sc.m): one-sided compression (the
+    oracle's 's' rank table), LU with partial pivoting; solve against the
+    closed form, apply against the exact circulant product, 't' / 'c'
+    solves of the symmetric matrix, exported factors with a permutation,
+    and boxes beyond the register kernel."""
+    flam_amd, geometry = _dev()
+    N, occ, tol = 4096, 64, 1e-12
+    print(f"symm s: circle N={N} occ={occ} tol={tol}", flush=True)
+    g = geometry.circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, dict(symm="s"))
+    pr = opr.CurveDL(N, "circle")
+    Fo = ofl.rskelf(pr.afun, pr.x, occ, tol, pr.pxyfun, dict(symm="s", order="level"))
+    _cmp_tables(F.level_table(), ofl.rank_table(Fo))
+    assert F.info()["symm"] == "s"
+    b = np.random.default_rng(11).standard_normal(N)
+    bt = torch.from_numpy(b).cuda()
+    xe = pr.closed_form_inverse_apply(b)
+    for t in ("n", "t", "c"):
+        x = flam_amd.rskelf_sv(F, bt, t).cpu().numpy()
+        assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-10, t
+    y = flam_amd.rskelf_mv(F, torch.from_numpy(xe).cuda()).cpu().numpy()
+    assert np.linalg.norm(y - b) / np.linalg.norm(b) < 1e-10
+    f = F.export(0)
+    assert f["p"] is not None and sorted(f["p"].tolist()) == list(range(len(f["rd"])))
+    assert np.allclose(np.diag(f["L"]), 1.0) and np.allclose(np.triu(f["L"], 1), 0.0)
+    F2 = flam_amd.rskelf(g.kernel, g.x, 300, tol, g.proxy, dict(symm="s"))
+    x2 = flam_amd.rskelf_sv(F2, bt).cpu().numpy()
+    assert np.linalg.norm(x2 - xe) / np.linalg.norm(xe) < 1e-10
