@@ -252,6 +252,57 @@ static int download(void* dst, const void* src, size_t bytes) {
   return rc;
 }
 
+// Side streams for the size classes of one phase.  A level's boxes go to
+// a few kernels by size, independent of each other; on one stream each
+// class's launch waits for the last workgroups of the previous one, which
+// leaves CUs idle.  The classes of the ID and of the elimination run
+// concurrently on two extra non-blocking streams (so the legacy null
+// stream's implicit ordering does not serialize them; default priority),
+// forked from and joined back into g_stream with events.
+struct SideStreams {
+  int dev = -1;
+  hipStream_t s[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+};
+static SideStreams g_side;
+
+static int side_init() {
+  int dev = 0;
+  RSKX_HIP(hipGetDevice(&dev));
+  if (g_side.dev == dev) return RSKX_OK;
+  if (g_side.dev >= 0) {                        // another device: new streams there
+    for (int i = 0; i < 2; ++i) {
+      hipStreamDestroy(g_side.s[i]);
+      hipEventDestroy(g_side.join[i]);
+    }
+    hipEventDestroy(g_side.fork);
+    g_side.dev = -1;
+  }
+  for (int i = 0; i < 2; ++i) {
+    RSKX_HIP(hipStreamCreateWithFlags(&g_side.s[i], hipStreamNonBlocking));
+    RSKX_HIP(hipEventCreateWithFlags(&g_side.join[i], hipEventDisableTiming));
+  }
+  RSKX_HIP(hipEventCreateWithFlags(&g_side.fork, hipEventDisableTiming));
+  g_side.dev = dev;
+  return RSKX_OK;
+}
+
+// the side streams wait for everything queued on g_stream so far
+static int side_fork() {
+  RSKX_HIP(hipEventRecord(g_side.fork, g_stream));
+  for (int i = 0; i < 2; ++i) RSKX_HIP(hipStreamWaitEvent(g_side.s[i], g_side.fork, 0));
+  return RSKX_OK;
+}
+
+// g_stream waits for everything queued on the side streams so far
+static int side_join() {
+  for (int i = 0; i < 2; ++i) {
+    RSKX_HIP(hipEventRecord(g_side.join[i], g_side.s[i]));
+    RSKX_HIP(hipStreamWaitEvent(g_stream, g_side.join[i], 0));
+  }
+  return RSKX_OK;
+}
+
 template <typename T>
 static int upload(DevBuf& b, const std::vector<T>& v) {
   const size_t bytes = v.size() * sizeof(T);
@@ -651,6 +702,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   };
   RSKX_TRY(ws.flag.ensure(64));
   RSKX_HIP(hipMemsetAsync(ws.flag.p, 0, 64, g_stream));
+  RSKX_TRY(side_init());
 
   // 's' (rskelf.m:210,220-227): LU like 'n' but one-sided compression;
   // with real data T.' = T' and its sweeps are the 'n' sweeps
@@ -886,15 +938,24 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     }
     const int* qids = dids + ng + nr0 + nr1;
     {
+      // the Gram classes run concurrently (largest boxes on a side stream,
+      // the smallest on the other, the middle class on g_stream); the
+      // tails below read every class's output, so they follow the join
+      int used = 0;
+      for (int c = 0; c < kGramClasses; ++c) used += idq[c].empty() ? 0 : 1;
+      const bool conc = used > 1;
+      if (conc) RSKX_TRY(side_fork());
       int off = 0;
       for (int c = kGramClasses - 1; c >= 0; --c) {     // largest boxes first
         const int cnt = (int)idq[c].size();
         if (!cnt) continue;
+        hipStream_t st = !conc ? g_stream : c == 2 ? g_side.s[0] : c == 0 ? g_side.s[1] : g_stream;
         launch_gram_id(c, qids + off, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
                        ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.T.as<double>(), ws.perm.as<int>(),
-                       P, L, prm, ws.kout.as<int>(), cnt, shm_q[c], g_stream);
+                       P, L, prm, ws.kout.as<int>(), cnt, shm_q[c], st);
         off += cnt;
       }
+      if (conc) RSKX_TRY(side_join());
     }
     if (nq) {
       int nmx = 0;
@@ -1024,6 +1085,13 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     RSKX_TRY(upload(ws.ids, allids));
     prof.begin("factor", ev);
     {
+      // the classes run concurrently: 16-wave boxes on one side stream,
+      // 8-wave boxes on g_stream, the shared-memory / global classes of
+      // k_elim on the other side stream; joined before anything reads them
+      const int nclass = (creg[1].empty() ? 0 : 1) + (creg[0].empty() ? 0 : 1) +
+                         ((cls[0].empty() && cls[1].empty() && cls[2].empty()) ? 0 : 1);
+      const bool conc = nclass > 1;
+      if (conc) RSKX_TRY(side_fork());
       int off = 0;
       for (int c = 1; c >= 0; --c) {           // 16-wave boxes first, then 8-wave
         if (creg[c].empty()) continue;
@@ -1031,7 +1099,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
                            (int)creg[c].size(), ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(),
                            ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(),
                            ws.T.as<double>(), ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), fval, fidx,
-                           P, ws.flag.as<int>(), g_stream);
+                           P, ws.flag.as<int>(), conc && c == 1 ? g_side.s[0] : g_stream);
         off += (int)creg[c].size();
       }
       for (int c = 2; c >= 0; --c) {
@@ -1040,9 +1108,10 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
                        (int)cls[c].size(), ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(),
                        ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(), ws.T.as<double>(),
                        ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), ws.W.as<S>(), fval,
-                       fidx, P, ws.flag.as<int>(), g_stream);
+                       fidx, P, ws.flag.as<int>(), conc ? g_side.s[1] : g_stream);
         off += (int)cls[c].size();
       }
+      if (conc) RSKX_TRY(side_join());
     }
     prof.end("factor", ev, 0, el_flops);
     RSKX_HIP(hipGetLastError());
