@@ -21,8 +21,10 @@
  *   - Every call returns RSKX_OK (0) or a negative code; rskx_last_error()
  *     then gives a message that starts with the reference's error
  *     identifier ("FLAM:chksymm:invalidSymm: ...").
- *   - A factor is device-resident behind an opaque handle; one handle must
- *     not be used from two threads at once.
+ *   - A factor is device-resident behind an opaque handle.  The library's
+ *     scratch (workspaces, staging buffers, the stream) is shared by all
+ *     factors, so calls must come from one thread at a time, like the
+ *     single-threaded MATLAB functions they replace.
  */
 #ifndef RSKX_H
 #define RSKX_H
