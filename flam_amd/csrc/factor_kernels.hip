@@ -1127,6 +1127,9 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
 // On exit the third n x n slice of the box's T scratch holds R's leading k
 // rows by pivot position (leading dimension n), perm the pivot order and
 // kout the rank; k_id<S, 2> then forms T and refines the ID.
+// rows of K staged in shared memory per accumulation step of k_gram_id
+constexpr int kGramRows = 16;
+
 template <int NT, int TPT>
 __global__ void __launch_bounds__(NT)
 k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
@@ -1134,7 +1137,7 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
           const int* __restrict__ mnbr, int* __restrict__ mrows, double* __restrict__ Tar,
           int* __restrict__ permar, PointsDev P, LevelParams L, IdParams prm,
           int* __restrict__ kout) {
-  constexpr int RC = 16;             // rows of K per chunk
+  constexpr int RC = kGramRows;      // rows of K per chunk
   constexpr int NW = NT / 64;
   extern __shared__ double sh[];
   const int b = ids[blockIdx.x];
@@ -1359,7 +1362,7 @@ int gram_id_class(int n) {
 }
 size_t gram_id_shmem(int cls, int n, int mn, int m, int p) {
   const int np = (n + 3) / 4 * 4, nw = (cls == 0 ? 256 : 512) / 64;
-  return (size_t)(16 * np + 3 * p + 5 * n + 5 * mn + 2 * np + 2 * nw) * sizeof(double) +
+  return (size_t)(kGramRows * np + 3 * p + 5 * n + 5 * mn + 2 * np + 2 * nw) * sizeof(double) +
          (size_t)n * sizeof(int) + 64;
 }
 
