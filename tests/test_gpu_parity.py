@@ -431,3 +431,31 @@ def test_symm_s_real_symmetric():
     F2 = flam_amd.rskelf(g.kernel, g.x, 300, tol, g.proxy, dict(symm="s"))
     x2 = flam_amd.rskelf_sv(F2, bt).cpu().numpy()
     assert np.linalg.norm(x2 - xe) / np.linalg.norm(xe) < 1e-10
+
+
+# ------------------------------------------------------------- streams --
+def test_work_on_a_non_default_torch_stream():
+    """The library runs on torch's current stream (its sweeps only queue
+    their work): a factorization and solves issued inside a side stream
+    give the default stream's results bit for bit, with torch work on the
+    same stream reading them without a host synchronization, and switching
+    back drains the side stream first."""
+    flam_amd, geometry = _dev()
+    N = 4096
+    print(f"streams: circle N={N} occ=64 tol=1e-10", flush=True)
+    g = geometry.circle(N)
+    B = torch.from_numpy(np.random.default_rng(12).standard_normal((N, 3))).cuda()
+    F0 = flam_amd.rskelf(g.kernel, g.x, 64, 1e-10, g.proxy, dict(symm="p"))
+    X0 = flam_amd.rskelf_sv(F0, B)
+    Y0 = flam_amd.rskelf_mv(F0, X0)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        F1 = flam_amd.rskelf(g.kernel, g.x, 64, 1e-10, g.proxy, dict(symm="p"))
+        X1 = flam_amd.rskelf_sv(F1, B)
+        Y1 = flam_amd.rskelf_mv(F1, X1 * 1.0)       # torch work between two library calls
+        D1 = (Y1 - B).abs().max()
+    torch.cuda.current_stream().wait_stream(s)
+    X2 = flam_amd.rskelf_sv(F1, B)                   # back on the default stream
+    assert torch.equal(X1, X0) and torch.equal(Y1, Y0) and torch.equal(X2, X0)
+    assert D1.item() <= 1e-8 * B.abs().max().item()
