@@ -75,6 +75,7 @@ SIGNATURES = {
     "rskx_partial_info": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64),
                                   POINTER(c_int64), POINTER(c_int64), POINTER(c_double)]),
     "rskx_free": (None, [c_void_p]),
+    "rskx_release_cache": (c_int, []),
     "rskx_last_error": (c_char_p, []),
     "rskx_set_stream": (c_int, [c_void_p]),
     "rskx_factor_info": (c_int, [c_void_p, POINTER(Info)]),

@@ -96,7 +96,11 @@ static hipStream_t g_stream = nullptr;
 // (hipMalloc of large blocks and hipFree, which synchronizes the device,
 // would otherwise sit inside every factorization).  A request takes the
 // smallest cached block of at least its size and at most 4/3 of it; the
-// cache is trimmed, largest blocks first, above kCacheBytes.
+// cache is trimmed, largest blocks first, above kCacheBytes.  When
+// hipMalloc fails, every idle cached block is freed and the allocation
+// tried once more, so the cache never makes a factorization fail that
+// would fit without it; rskx_release_cache returns the cache and the
+// workspaces to the device on request.
 struct BlockCache {
   std::multimap<size_t, void*> free;
   size_t bytes = 0;
@@ -104,6 +108,17 @@ struct BlockCache {
 };
 static BlockCache g_blocks;
 constexpr size_t kCacheBytes = size_t(48) << 30;
+
+// hipFree every idle cached block, largest first
+static void block_trim_all() {
+  std::lock_guard<std::mutex> lk(g_blocks.mu);
+  while (!g_blocks.free.empty()) {
+    auto it = std::prev(g_blocks.free.end());
+    (void)hipFree(it->second);
+    g_blocks.bytes -= it->first;
+    g_blocks.free.erase(it);
+  }
+}
 
 static void* block_get(size_t want, size_t* got) {
   {
@@ -120,7 +135,15 @@ static void* block_get(size_t want, size_t* got) {
   void* q = nullptr;
   if (hipMalloc(&q, want) != hipSuccess) {
     (void)hipGetLastError();
-    return nullptr;
+    // cached blocks may hold the memory this request needs: release them
+    // (no kernel uses an idle block; hipFree waits for the device) and retry
+    if (g_blocks.bytes == 0) return nullptr;
+    (void)hipStreamSynchronize(g_stream);
+    block_trim_all();
+    if (hipMalloc(&q, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
   }
   *got = want;
   return q;
@@ -695,7 +718,8 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     if (ws.hflag[0] & 1)
       return fail(RSKX_ERR_NUM, "FLAM:rskx:chol: matrix is not positive definite (symm 'p')");
     if (ws.hflag[0] & 4)
-      return fail(RSKX_ERR_NUM, "FLAM:rskx:ldl: zero pivot in L D L' (symm 'h' factors without pivoting)");
+      return fail(RSKX_ERR_NUM, "FLAM:rskx:ldl: zero pivot or element growth above 1/sqrt(eps) in L D L' "
+                                "(symm 'h' factors without Bunch-Kaufman pivots)");
     if (ws.hflag[0] & 2)
       return fail(RSKX_ERR_NUM, "FLAM:rskx:lu: exactly singular redundant block");
     return RSKX_OK;
@@ -1346,6 +1370,20 @@ int rskx_set_stream(void* s) {
   // uses the library's workspaces: drain it before switching
   if ((hipStream_t)s != g_stream) RSKX_TRY(sync_stream());
   g_stream = (hipStream_t)s;
+  return RSKX_OK;
+}
+
+int rskx_release_cache(void) {
+  RSKX_TRY(sync_stream());
+  Workspace& ws = g_ws;
+  for (DevBuf* b : {&ws.boxes, &ws.nsegs, &ws.csegs, &ws.outs, &ws.ids, &ws.pool, &ws.nbr,
+                    &ws.mnbr, &ws.mrows, &ws.kout, &ws.iter, &ws.K, &ws.perm, &ws.T, &ws.W,
+                    &ws.M[0], &ws.M[1], &ws.flag, &ws.V, &ws.lrecs, &ws.sweep, &ws.cgs, &ws.pts,
+                    &ws.xw, &ws.txs, &ws.tord, &ws.tpart, &ws.tchunks, &ws.tcounts, &ws.xs,
+                    &ws.poff})
+    b->release();
+  block_trim_all();
+  RSKX_HIP(hipGetLastError());
   return RSKX_OK;
 }
 

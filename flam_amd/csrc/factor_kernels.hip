@@ -745,8 +745,9 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
     }
   } else {
     // right-looking Cholesky ('p'), or L D L' without pivoting ('h': unit L,
-    // the pivots to D; rskelf.m:229 takes Bunch-Kaufman pivots, which on the
-    // definite matrices this path is for are the diagonal in order)
+    // the pivots to D; rskelf.m:229 takes Bunch-Kaufman pivots, which may
+    // interchange rows even on definite matrices; a zero pivot or element
+    // growth in L is reported instead of pivoting)
     S* fD = LDL ? fval + O.D : nullptr;
     for (int s = 0; s < r; ++s) {
       if (tid == 0) {
@@ -772,6 +773,13 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
       __syncthreads();
       for (int i = s + 1 + tid; i < r; i += nt) Krr[i + s * r] *= s_d;
       __syncthreads();
+    }
+    if (LDL) {                     // element growth of the unit L (as k_elim_reg)
+      const S gmax = (S)(1.0 / sqrt(Eps<S>::v));
+      bool bad = false;
+      for (int e = tid; e < r * r; e += nt)
+        if (e % r > e / r && !(fabs(Krr[e]) <= gmax)) bad = true;
+      if (__any(bad) && lane == 0) atomicOr(fail, 4);
     }
   }
   // inverted 32 x 32 diagonal blocks of L (unit for LU) and U, one wave
@@ -1637,9 +1645,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
 #pragma unroll
           for (int ta = 0; ta < TA; ++ta) if (ta == (sc >> 6)) dv = pc[c][ta];
           dv = lane_bcast(dv, sc & 63);
-          // 'p': Cholesky pivot sqrt(d); 'h': unit L, d to D (no pivoting;
-          // rskelf.m:229's Bunch-Kaufman pivots are the diagonal in order on
-          // the definite matrices this path is for)
+          // 'p': Cholesky pivot sqrt(d); 'h': unit L, d to D without
+          // pivoting (rskelf.m:229 takes Bunch-Kaufman pivots, which may
+          // interchange rows even on definite matrices; a zero pivot or
+          // element growth in L is reported instead of pivoting)
           if (LDL) {
             if (!(dv != (S)0)) { if (lane == 0) atomicOr(fail, 4); dv = (S)1; }
             if (lane == 0) { fval[O.D + sc] = dv; s_dpan[p & 1][c] = dv; }
@@ -1704,6 +1713,21 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
           Z[ta][tb] = z;
         }
       }
+    }
+    if (LDL) {
+      // L D L' without pivoting is stable while L stays bounded: flag
+      // entries of the unit L above 1/sqrt(eps) (a pivot tiny against its
+      // column) as a failure rather than return an inaccurate factor
+      const S gmax = (S)(1.0 / sqrt(Eps<S>::v));
+      bool bad = false;
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+          const int i = lane + 64 * ta, j = colj(tb);
+          if (j < r && i > j && i < r && !(fabs(Z[ta][tb]) <= gmax)) bad = true;
+        }
+      if (__any(bad) && lane == 0) atomicOr(fail, 4);
     }
   }
 

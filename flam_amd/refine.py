@@ -113,7 +113,10 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None, in
             relres = (rn / bnorm).cpu()
         if bool((relres <= tol).all()) or outer == maxit:
             break
-        V[:, 0, :] = R / rn[:, None]
+        # a column whose residual is exactly zero (a zero right-hand side, or
+        # one already solved) keeps a zero direction: its g, and so its
+        # update y, stay zero while the other columns iterate
+        V[:, 0, :] = R / torch.where(rn > 0, rn, torch.ones_like(rn))[:, None]
         Rh = np.zeros((nrhs, restart, restart))                # rotated H, upper triangular
         cs = np.zeros((nrhs, restart))
         sn = np.zeros((nrhs, restart))

@@ -497,3 +497,9 @@ def kernel_matvec(A, x, X):
                                       Yc.data_ptr(), N, Xc.shape[0]))
     Y = Yc.t()
     return Y.reshape(-1) if vec else Y
+
+
+def release_cache():
+    """Give the library's idle device memory (workspaces and the cache of
+    freed factor blocks) back to the device; live factors are untouched."""
+    _check(L.lib().rskx_release_cache())

@@ -210,6 +210,15 @@ int rskx_partial_info(const rskx_factor* F, int64_t* nsk, int64_t* nnz, int64_t*
                       int64_t* J, double* V);
 
 void rskx_free(rskx_factor* F);
+
+/* Return the library's idle device memory to the device: the workspaces
+ * reused across calls and the cache of device blocks released by freed
+ * factors (up to 48 GiB is kept for reuse, so that factorizations do not
+ * call hipMalloc / hipFree, which synchronizes the device).  Live factors
+ * are untouched.  The library also empties the cache by itself when an
+ * allocation fails.  No counterpart in the reference: MATLAB frees F's
+ * memory when the variable is cleared. */
+int rskx_release_cache(void);
 const char* rskx_last_error(void);
 int rskx_set_stream(void* hip_stream);
 

@@ -273,3 +273,54 @@ class SphereSL:
 
     def matvec(self, X):
         return self.dense() @ np.asarray(X)
+
+
+# --------------------------------------------------------------------------
+# PDE known-answer checks (the demos' end-to-end accuracy tests)
+# --------------------------------------------------------------------------
+def laplace2d_pde_error(x, nu, h, solve, ratio=1.0, seed=0):
+    """Interior Dirichlet problem with a known harmonic solution, as
+    rskelf/test/ie_circle.m:65-81 and ie_ellipse.m:83-99 run it: 16 point
+    charges on the curve (ratio cos t, sin t) scaled by 2 (outside) make
+    the boundary data B = G(x, src) q with G = -log r / 2pi; the density
+    X = solve(B) of the double-layer equation gives the field at 16 targets
+    scaled by 0.5 (inside) as (dG/dn_y w) X, compared with G(trg, src) q.
+    The charges are seeded here (the demos draw an unseeded rand).
+    Returns the relative error of the 16 field values."""
+    m = 16
+    th = np.arange(1, m + 1) * 2.0 * math.pi / m
+    src = 2.0 * np.stack([ratio * np.cos(th), np.sin(th)])
+    trg = 0.5 * np.stack([ratio * np.cos(th), np.sin(th)])
+    q = np.random.default_rng(seed).random(m)
+
+    def g(a, b):
+        dx = a[0][:, None] - b[0][None, :]
+        dy = a[1][:, None] - b[1][None, :]
+        return -np.log(np.sqrt(dx * dx + dy * dy)) / (2.0 * math.pi)
+
+    B = g(x, src) @ q
+    X = np.asarray(solve(B))
+    Y = (CurveDL.dl(trg, x, nu) * h[None, :]) @ X
+    Z = g(trg, src) @ q
+    return float(np.linalg.norm(Z - Y) / np.linalg.norm(Z))
+
+
+def sphere_pde_error(x, area, solve, seed=0):
+    """The known-answer check of rskelf/test/ie_sphere.m:127-145 adapted to
+    the first-kind single layer: 16 seeded unit charges on the sphere of
+    radius 2 (outside) give boundary data B = G(x, src) q with
+    G = 1 / (4 pi r); the density X = solve(B) of S X = B gives the field
+    at 16 seeded points on the sphere of radius 0.5 (inside) as
+    (G(trg, x) area) X, compared with G(trg, src) q.  One-point quadrature
+    makes this converge at first order in the mesh width."""
+    rng = np.random.default_rng(seed)
+    src = rng.standard_normal((3, 16))
+    src = 2.0 * src / np.linalg.norm(src, axis=0)
+    trg = rng.standard_normal((3, 16))
+    trg = 0.5 * trg / np.linalg.norm(trg, axis=0)
+    q = rng.random(16)
+    B = SphereSL.sl(x, src) @ q
+    X = np.asarray(solve(B))
+    Y = (SphereSL.sl(trg, x) * area[None, :]) @ X
+    Z = SphereSL.sl(trg, src) @ q
+    return float(np.linalg.norm(Z - Y) / np.linalg.norm(Z))

@@ -194,20 +194,26 @@ def test_chol_failure_reported():
 
 
 def test_readme_configuration_rank_table():
-    """The README demo (ie_circle, N=16384, occ=64, tol=1e-12, symm 'h'
-    there; 'n' here, which compresses the same blocks of this symmetric
-    matrix) on the device: block counts exact, ranks within 5% of
-    README.md:99-106."""
+    """The README demo exactly as README.md:95-111 ran it (ie_circle,
+    N=16384, occ=64, tol=1e-12, symm 'h', the ie_circle.m:31 default) on
+    the device: block counts exact, ranks within 6% of README.md:99-106,
+    and the demo's PDE known-answer error at the README's level
+    (README.md:111: 1.35e-12)."""
     import json
     import os
     flam_amd, geometry = _dev()
     with open(os.path.join(os.path.dirname(__file__), "golden", "readme_ie_circle.json")) as f:
         gold = json.load(f)
     g = geometry.ie_circle(16384)
-    F = flam_amd.rskelf(g.kernel, g.x, 64, 1e-12, g.proxy, dict(symm="n"))
+    F = flam_amd.rskelf(g.kernel, g.x, 64, 1e-12, g.proxy, dict(symm="h"))
     for (lvl, nblk, s, e), (rl, rn, rs, re) in zip(F.level_table(), gold["rows"]):
         assert (lvl, nblk) == (rl, rn)
         assert abs(e - re) <= max(2, 0.06 * re)
+    x, nu, w = (t.cpu().numpy() for t in (g.x, g.nu, g.w))
+    err = opr.laplace2d_pde_error(
+        x, nu, w, lambda B: flam_amd.rskelf_sv(F, torch.from_numpy(B).cuda()).cpu().numpy())
+    print(f"readme configuration: pde error {err:.3e}", flush=True)
+    assert err < 1e-11
 
 
 def test_sweeps_leave_input_untouched():
@@ -459,3 +465,78 @@ def test_work_on_a_non_default_torch_stream():
     X2 = flam_amd.rskelf_sv(F1, B)                   # back on the default stream
     assert torch.equal(X1, X0) and torch.equal(Y1, Y0) and torch.equal(X2, X0)
     assert D1.item() <= 1e-8 * B.abs().max().item()
+
+
+def test_release_cache_between_factorizations():
+    """rskx_release_cache returns the workspaces and cached blocks; a live
+    factor keeps working and a new factorization allocates afresh."""
+    flam_amd, geometry = _dev()
+    N = 4096
+    print(f"release cache: circle N={N}", flush=True)
+    g = geometry.circle(N)
+    F1 = flam_amd.rskelf(g.kernel, g.x, 64, 1e-10, g.proxy, dict(symm="p"))
+    F2 = flam_amd.rskelf(g.kernel, g.x, 64, 1e-10, g.proxy, dict(symm="p"))
+    b = torch.from_numpy(np.random.default_rng(0).standard_normal(N)).cuda()
+    x1 = flam_amd.rskelf_sv(F1, b)
+    del F2
+    flam_amd.release_cache()
+    x1b = flam_amd.rskelf_sv(F1, b)
+    assert torch.equal(x1, x1b)
+    F3 = flam_amd.rskelf(g.kernel, g.x, 64, 1e-10, g.proxy, dict(symm="p"))
+    assert torch.equal(flam_amd.rskelf_sv(F3, b), x1)
+
+
+def _indefinite_circle(geometry, N):
+    """The benchmark circle's constant off-diagonal b = 1/(2N) with a chosen
+    diagonal: +-1 alternating and A_00 = 1e-12 b, an indefinite matrix whose
+    first pivot is tiny against its column (rskelf.m:229's Bunch-Kaufman
+    pivoting takes a 2 x 2 pivot there)."""
+    g = geometry.circle(N)
+    d = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
+    d[0] = 1e-12 * 0.5 / N
+    g.diag = torch.from_numpy(d).cuda()
+    A = opr.CurveDL(N, "circle").dense()
+    A[np.arange(N), np.arange(N)] = d
+    return g, A
+
+
+@pytest.mark.parametrize("N", [64, 300])
+def test_symm_h_growth_reported(N):
+    """symm 'h' factors without pivoting: on a matrix whose unpivoted L D L'
+    grows (one box, so K(rd,rd) is the whole matrix), the register (N = 64)
+    and global-scratch (N = 300) eliminations report FLAM:rskx:ldl instead of
+    returning an inaccurate factor; 'n' (LU with partial pivoting) solves
+    the same matrix to the dense answer."""
+    flam_amd, geometry = _dev()
+    print(f"symm h growth: indefinite circle N={N} (one box)", flush=True)
+    g, A = _indefinite_circle(geometry, N)
+    with pytest.raises(flam_amd.FlamError) as e:
+        flam_amd.rskelf(g.kernel, g.x, N, 1e-12, g.proxy, dict(symm="h"))
+    assert e.value.ident == "FLAM:rskx:ldl"
+    F = flam_amd.rskelf(g.kernel, g.x, N, 1e-12, g.proxy, dict(symm="n"))
+    b = np.random.default_rng(5).standard_normal(N)
+    x = flam_amd.rskelf_sv(F, torch.from_numpy(b).cuda()).cpu().numpy()
+    xe = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xe) <= 1e-10 * np.linalg.norm(xe)
+
+
+def test_large_root_global_paths():
+    """One box of 2600 points (occ = N): the root's dense LU runs in the
+    elimination's global-scratch class and both sweeps in their global
+    workspace slices (rows beyond shared memory); solve and apply against
+    the closed form and the exact product."""
+    flam_amd, geometry = _dev()
+    N = 2600
+    print(f"large root: circle N={N} occ={N} symm n fp64", flush=True)
+    g = geometry.circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, N, 1e-12, g.proxy, dict(symm="n"))
+    assert F.level_table() == [(1, 1, N, 0)]
+    pr = opr.CurveDL(N, "circle")
+    B = np.random.default_rng(6).standard_normal((N, 3))
+    X = flam_amd.rskelf_sv(F, torch.from_numpy(B).cuda()).cpu().numpy()
+    Xe = pr.closed_form_inverse_apply(B)
+    assert np.linalg.norm(X - Xe) <= 1e-11 * np.linalg.norm(Xe)
+    Y = flam_amd.rskelf_mv(F, torch.from_numpy(Xe).cuda()).cpu().numpy()
+    assert np.linalg.norm(Y - B) <= 1e-11 * np.linalg.norm(B)
+    Xc = flam_amd.rskelf_sv(F, torch.from_numpy(B).cuda(), "c").cpu().numpy()
+    assert np.linalg.norm(Xc - Xe) <= 1e-11 * np.linalg.norm(Xe)   # symmetric matrix

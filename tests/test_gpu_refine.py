@@ -1,5 +1,7 @@
 """The refinement's fused orthogonalization (rskx_cgs2) against a plain
 FP64 torch restatement of classical Gram-Schmidt applied twice."""
+import math
+
 import numpy as np
 import pytest
 
@@ -112,3 +114,25 @@ def test_factor_operator_refines_against_exact_product():
                  torch.linalg.vector_norm(X[a:b], dim=0)).max().item() for a, b in segs)
     print(f"fgmres iters {info['iters']}, true residual {worst:.2e}", flush=True)
     assert worst <= 1e-10
+
+
+def test_fgmres_zero_column():
+    """A zero right-hand side among random ones comes back exactly zero with
+    a finite residual (MATLAB's gmres returns x = 0 for b = 0) while the
+    other columns converge as usual."""
+    import flam_amd
+    from flam_amd import geometry
+    from flam_amd.refine import fgmres
+    N = 8192
+    print(f"fgmres with a zero column, circle N={N}", flush=True)
+    g = geometry.circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, 128, 1e-6, g.proxy, dict(symm="p", precision="fp32"))
+    B = torch.from_numpy(np.random.default_rng(4).standard_normal((N, 3))).cuda()
+    B[:, 1] = 0
+    X, info = fgmres(g.circulant_matvec, B, F, tol=1e-10)
+    assert bool(torch.isfinite(X).all())
+    assert X[:, 1].abs().max().item() == 0.0
+    assert all(math.isfinite(r) for r in info["relres"])
+    R = B - g.circulant_matvec(X)
+    for c in (0, 2):
+        assert (torch.linalg.vector_norm(R[:, c]) / torch.linalg.vector_norm(B[:, c])).item() <= 1e-10

@@ -7,8 +7,10 @@ points on a sphere of one box size (rskelf/test/ie_sphere.m:36-40,
 
 Tolerances: the ID's rank decisions may break ties differently, so the
 per-level skeleton totals must agree within 2%; solves must agree with the
-dense FP64 LU to the accuracy the factor tolerance allows (the oracle's own
-error times 10, and at most 1e-5 at tol 1e-6).
+dense FP64 LU to the accuracy the factor tolerance allows: the oracle's own
+error times 10, and at most 1000 tol (the first-kind matrix's condition
+number, which grows with N, amplifies the factor's tolerance; the oracle's
+own error is the binding bound).
 """
 import numpy as np
 import pytest
@@ -60,7 +62,7 @@ def test_sphere_fp64_vs_oracle_and_dense(nface, occ, tol):
     err = np.linalg.norm(x - xe) / np.linalg.norm(xe)
     err_o = np.linalg.norm(xo - xe) / np.linalg.norm(xe)
     print(f"solve error: device {err:.2e}, oracle {err_o:.2e}", flush=True)
-    assert err <= max(10 * err_o, 1e-12) and err <= 10 * tol * 100
+    assert err <= max(10 * err_o, 1e-12) and err <= 1000 * tol
     y = flam_amd.rskelf_mv(F, torch.from_numpy(xe).cuda()).cpu().numpy()
     assert np.linalg.norm(y - b) / np.linalg.norm(b) <= max(10 * err_o, 1e-12)
     # adjoint sweeps against the transposed dense solve

@@ -430,3 +430,91 @@ def test_cholmv_cholsv_identities():
     Fn = flam.rskelf(pr.afun, pr.x, 64, 1e-12, pr.pxyfun, dict(symm="n", order="level"))
     with pytest.raises(flam.FlamError):
         flam.rskelf_cholmv(Fn, x)
+
+
+# ------------------------------------------------- matrix definitions -----
+@pytest.mark.parametrize("kind,kw", [("ie_circle", {}), ("ellipse", dict(a=2.0, b=1.0)),
+                                     ("ellipse", dict(a=1.0, b=0.3))])
+def test_double_layer_gauss_identity(kind, kw):
+    """Gauss's lemma: the double layer of a constant density is -1/2 on a
+    smooth closed curve, so with the jump term -1/2 the interior Dirichlet
+    matrix of ie_circle.m / ie_ellipse.m (trapezoidal weights, curvature
+    diagonal -1/2 (1 + h kappa / 2 pi)) has every row sum -1, to spectral
+    accuracy.  A flipped normal gives row sums near 0, a wrong curvature
+    term misses by h kappa / 2 pi ~ 1e-3."""
+    for N in (512, 2048):
+        pr = problems.CurveDL(N, kind, **kw)
+        assert np.abs(pr.dense().sum(axis=1) + 1.0).max() < 1e-12
+    bad = problems.CurveDL(512, kind, **kw)
+    bad.nu = -bad.nu
+    assert np.abs(bad.dense().sum(axis=1) + 1.0).max() > 0.5
+    if kind == "ellipse":                           # ie_circle.m:102 has no kappa term
+        bad = problems.CurveDL(512, kind, **kw)
+        bad.kappa = -bad.kappa
+        assert np.abs(bad.dense().sum(axis=1) + 1.0).max() > 1e-4
+
+
+@pytest.mark.parametrize("kind,kw,ratio,symm", [("ie_circle", {}, 1.0, "h"),
+                                                ("ellipse", dict(a=2.0, b=1.0), 2.0, "n")])
+def test_pde_known_answer_curves(kind, kw, ratio, symm):
+    """ie_circle.m:65-81 / ie_ellipse.m:83-99: the field of exterior charges
+    recovered inside from the solved double-layer density, through a dense
+    solve and through the oracle's factor (README.md:111 reports 1.35e-12
+    at N = 16384).  A sign error in the kernel or the jump term gives O(1)."""
+    N = 2048
+    pr = problems.CurveDL(N, kind, **kw)
+    A = pr.dense()
+    assert problems.laplace2d_pde_error(pr.x, pr.nu, pr.h, lambda b: np.linalg.solve(A, b),
+                                        ratio) < 1e-13
+    F = flam.rskelf(pr.afun, pr.x, 64, 1e-12, pr.pxyfun, dict(symm=symm))
+    assert problems.laplace2d_pde_error(pr.x, pr.nu, pr.h, lambda b: flam.rskelf_sv(F, b),
+                                        ratio) < 1e-11
+    flipped = -A + np.diag(2 * np.diag(A))           # the exterior-jump sign
+    assert problems.laplace2d_pde_error(pr.x, pr.nu, pr.h,
+                                        lambda b: np.linalg.solve(flipped, b), ratio) > 1e-2
+
+
+def test_sphere_single_layer_identities():
+    """The single layer of a unit density on the unit sphere is 1 on the
+    sphere (int dS_y / (4 pi |x - y|) = R), so the matrix's row sums tend to
+    1 at first order in the mesh width (one-point quadrature, self-panel
+    integral on the diagonal); the known-answer field of exterior charges
+    (the ie_sphere.m:127-145 check for this kernel) converges at the same
+    rate.  A missing or mis-scaled diagonal (~0.03 at 1280 triangles) or a
+    wrong weight fails both."""
+    errs, pde = [], []
+    for nf in (1280, 5120):
+        pr = problems.SphereSL(nf)
+        A = pr.dense()
+        errs.append(np.abs(A.sum(axis=1) - 1.0).max())
+        pde.append(problems.sphere_pde_error(pr.x, pr.area, lambda b: np.linalg.solve(A, b)))
+        if nf == 1280:
+            Ad = A - np.diag(np.diag(A))
+            assert np.abs(Ad.sum(axis=1) - 1.0).max() > 5 * errs[0]
+            F = flam.rskelf(pr.afun, pr.x, 64, 1e-6, pr.pxyfun, dict(symm="n", order="level"))
+            assert abs(problems.sphere_pde_error(pr.x, pr.area, lambda b: flam.rskelf_sv(F, b))
+                       - pde[0]) < 1e-5
+    assert errs[0] < 5e-3 and errs[1] < 2e-3 and errs[0] / errs[1] > 1.8
+    assert pde[0] < 4e-3 and pde[1] < 2e-3 and pde[0] / pde[1] > 1.7
+
+
+def test_symm_h_bunch_kaufman_on_indefinite():
+    """rskelf.m:229's ldl pivots (Bunch-Kaufman): on an indefinite matrix
+    whose first pivot is tiny against its column the oracle's 'h' still
+    solves to the dense answer (the device, which factors without
+    pivoting, reports FLAM:rskx:ldl on the same matrix instead)."""
+    N = 64
+    pr = problems.CurveDL(N, "circle")
+    d = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
+    d[0] = 1e-12 * 0.5 / N
+    A = pr.dense()
+    A[np.arange(N), np.arange(N)] = d
+    afun = lambda I, J: A[np.ix_(np.asarray(I, dtype=np.int64), np.asarray(J, dtype=np.int64))]
+    F = flam.rskelf(afun, pr.x, N, 1e-12, pr.pxyfun, dict(symm="h"))
+    b = np.random.default_rng(5).standard_normal(N)
+    xe = np.linalg.solve(A, b)
+    assert np.linalg.norm(flam.rskelf_sv(F, b) - xe) <= 1e-10 * np.linalg.norm(xe)
+    # unpivoted L D L' of the same matrix: L grows like 1 / A_00
+    L = np.eye(N)
+    L[1:, 0] = A[1:, 0] / A[0, 0]
+    assert np.abs(L).max() > 1e8
