@@ -67,6 +67,7 @@ SIGNATURES = {
                                   POINTER(c_void_p)]),
     "rskx_rskelf_sv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
     "rskx_rskelf_mv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
+    "rskx_rskelf_sv_noalloc": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
     "rskx_rskelf_logdet": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
     "rskx_rskelf_cholmv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),
     "rskx_rskelf_cholsv": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int]),

@@ -1217,19 +1217,75 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   return RSKX_OK;
 }
 
+// Device bytes of a sweep's workspaces: the work copy of X in tree order
+// (N x ncol, padded to 256 bytes) and the largest per-level global slice
+// set of the levels whose rows do not fit shared memory (k_solve /
+// k_sweep's gws, same sizes as sweep_impl takes them).
+template <typename S>
+static void sweep_ws_bytes(const rskx_factor* F, int ncol, bool solve, int mode, size_t* ybytes,
+                           size_t* gbytes) {
+  constexpr int kRhs = 8;
+  *ybytes = ((size_t)F->N * ncol * sizeof(S) + 255) & ~(size_t)255;
+  *gbytes = 0;
+  const int prog = (F->symm == 'p' || F->symm == 'h') ? (solve ? 2 : 5) : (solve ? 0 : 3);
+  for (int lv = 0; lv + 1 < (int)F->lvp.size(); ++lv) {
+    const int64_t nf = F->lvp[lv + 1] - F->lvp[lv];
+    if (!nf || !(mode & 3)) continue;
+    const int kmx = F->lev_kmax[lv], rmx = F->lev_rmax[lv];
+    size_t g = 0;
+    if (solve) {
+      const size_t shs = solve_shmem(prog, kmx, rmx, sizeof(S));
+      if (shs > 160 * 1024) g = (shs / sizeof(S) + 63) / 64 * 64 * nf * sizeof(S);
+    } else {
+      const size_t xb = ((size_t)(kmx + 2 * rmx) * kRhs + (size_t)kRhs * 256) * sizeof(S);
+      const size_t want = (size_t)std::max(rmx * rmx, kmx * rmx) * sizeof(S);
+      const size_t room = xb < 76 * 1024 ? 76 * 1024 - xb : 0;
+      if (xb + std::min(want, room) > 160 * 1024) g = (xb / sizeof(S) + 63) / 64 * 64 * nf * sizeof(S);
+    }
+    *gbytes = std::max(*gbytes, g);
+  }
+}
+
+// profile state of calls that must not create events (no profiling)
+static Prof g_noprof;
+
+// ext: when not NULL, the caller's workspace of ext_cap bytes (at least
+// what sweep_ws_bytes reports) holds the work copy and the global slices,
+// and nothing is allocated and no event is recorded
 template <typename S, typename U>
 int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans, bool solve,
-               int mode) {
+               int mode, void* ext = nullptr, size_t ext_cap = 0) {
   rskx_factor* F = const_cast<rskx_factor*>(Fc);
   const int64_t N = F->N;
   if (nrhs <= 0 || N == 0) return RSKX_OK;
   // work copy in tree order, padded with zero columns to a multiple of 8
   constexpr int kRhs = 8;
   const int ncol = (nrhs + kRhs - 1) / kRhs * kRhs;
-  RSKX_TRY(g_ws.xw.ensure((size_t)N * ncol * sizeof(S)));
-  S* Y = g_ws.xw.as<S>();
+  S* Y = nullptr;
+  size_t ybytes = 0, gbytes = 0;
+  if (ext) {
+    sweep_ws_bytes<S>(F, ncol, solve, mode, &ybytes, &gbytes);
+    if (ybytes + gbytes > ext_cap)
+      return fail(RSKX_ERR_MEM, "FLAM:rskx:scratch: this sweep needs " + std::to_string(ybytes + gbytes) +
+                                    " bytes of work space; the allocation-free entry point holds " +
+                                    std::to_string(ext_cap));
+    Y = reinterpret_cast<S*>(ext);
+  } else {
+    RSKX_TRY(g_ws.xw.ensure((size_t)N * ncol * sizeof(S)));
+    Y = g_ws.xw.as<S>();
+  }
+  // a global slice set for the rows of one level's factors
+  auto slices = [&](size_t bytes, S** out) -> int {
+    if (ext) {
+      *out = reinterpret_cast<S*>(reinterpret_cast<char*>(ext) + ybytes);
+      return RSKX_OK;
+    }
+    RSKX_TRY(g_ws.sweep.ensure(bytes));
+    *out = g_ws.sweep.as<S>();
+    return RSKX_OK;
+  };
   hipEvent_t ev;
-  Prof& prof = F->prof;
+  Prof& prof = ext ? g_noprof : F->prof;
   prof.begin("permute", ev);
   launch_perm<U, S>(true, X, ldx, Y, N, nrhs, ncol, F->order.as<int>(), g_stream);
   prof.end("permute", ev, (int64_t)N * nrhs * sizeof(U) + N * ncol * sizeof(S) + N * 4);
@@ -1259,8 +1315,7 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
         int64_t gstride = 0;
         if (shs > 160 * 1024) {
           gstride = (int64_t)((shs / sizeof(S) + 63) / 64 * 64);
-          RSKX_TRY(g_ws.sweep.ensure((size_t)gstride * (f1 - f0) * sizeof(S)));
-          gws = g_ws.sweep.as<S>();
+          RSKX_TRY(slices((size_t)gstride * (f1 - f0) * sizeof(S), &gws));
           shs = 0;
         }
         int64_t bytes = 0;
@@ -1285,8 +1340,7 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
       int64_t gstride = 0;
       if (shm > 160 * 1024) {                 // as the solve: a global workspace slice
         gstride = (int64_t)((xb / sizeof(S) + 63) / 64 * 64);
-        RSKX_TRY(g_ws.sweep.ensure((size_t)gstride * (f1 - f0) * sizeof(S)));
-        gws = g_ws.sweep.as<S>();
+        RSKX_TRY(slices((size_t)gstride * (f1 - f0) * sizeof(S), &gws));
         bcap = 0;
         shm = 0;
       }
@@ -1343,6 +1397,30 @@ static int logdet_impl(const rskx_factor* F, double* re, double* im) {
   // 'p': 2 sum log diag(L); each negative pivot or odd permutation adds i pi
   *re = mode == 0 ? 2 * lr : lr;
   *im = (nneg & 1) ? 3.141592653589793238462643383279502884 : 0.0;
+  return RSKX_OK;
+}
+
+// ----------------------------------------------- allocation-free solve --
+// The work rows of rskx_rskelf_sv_noalloc: a file-scope device array that
+// every GPU the library runs on holds until the process ends.  Its name is
+// a host-side stand-in in host code; the device address comes from
+// hipGetSymbolAddress, once per device, and only that address is passed on.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  RSKX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:scratch: device index beyond " + std::to_string(kMaxDevices));
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    RSKX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
   return RSKX_OK;
 }
 
@@ -1567,6 +1645,25 @@ int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char tr
 
 int rskx_rskelf_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype) {
   return sweep_entry(F, X, ldx, nrhs, trans, dtype, false);
+}
+
+int rskx_rskelf_sv_noalloc(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
+                           int dtype) {
+  if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
+  char t = (char)tolower(trans);
+  if (t != 'n' && t != 't' && t != 'c')
+    return fail(RSKX_ERR_ARG, "FLAM:chktrans:invalidTrans: Transpose parameter must be one of 'N', 'T', or 'C'.");
+  if (t == 't') t = 'c';
+  if (nrhs < 0 || ldx < F->N) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad nrhs or leading dimension");
+  if (nrhs > 0 && !X) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: X is NULL");
+  if (dtype != RSKX_F64 && dtype != RSKX_F32) return fail(RSKX_ERR_ARG, "FLAM:rskx:dtype: unknown element type");
+  void* ws = nullptr;
+  RSKX_TRY(scratch_address(&ws));
+  if (F->prec == RSKX_PREC_FP64)
+    return dtype == RSKX_F64 ? sweep_impl<double, double>(F, X, ldx, nrhs, t, true, 3, ws, kScratchBytes)
+                             : sweep_impl<double, float>(F, X, ldx, nrhs, t, true, 3, ws, kScratchBytes);
+  return dtype == RSKX_F64 ? sweep_impl<float, double>(F, X, ldx, nrhs, t, true, 3, ws, kScratchBytes)
+                           : sweep_impl<float, float>(F, X, ldx, nrhs, t, true, 3, ws, kScratchBytes);
 }
 
 void rskx_free(rskx_factor* F) { delete F; }

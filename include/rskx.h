@@ -172,6 +172,21 @@ int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char tr
 int rskx_rskelf_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
                    int dtype);
 
+/* rskelf_sv without allocating: no counterpart in the reference (MATLAB
+ * allocates freely).  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same result as rskx_rskelf_sv, with the work copy of X
+ * and the global row slices of large factors in a static 64 MiB device
+ * array of the library instead of its workspaces; no hipMalloc, hipFree or
+ * event is created.  The sweep needs N * ceil(nrhs / 8) * 8 elements of
+ * the factor's precision (plus slices for factors beyond shared memory);
+ * a larger sweep returns RSKX_ERR_MEM (FLAM:rskx:scratch) without touching
+ * X.  Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (another thread, or work still queued on another stream) race.
+ * Asynchronous on the library stream, like rskx_rskelf_sv. */
+int rskx_rskelf_sv_noalloc(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
+                           int dtype);
+
 /* LD = rskelf_logdet(F)                         -- rskelf/rskelf_logdet.m:11
  * log det of the factored matrix (of its factored part, for a partial
  * factorization) as *re + i *im with 0 <= *im < 2 pi: 2 sum log diag(L)
