@@ -20,17 +20,21 @@ RSKX_PREC_FP64 = 0
 RSKX_PREC_FP32 = 1
 RSKX_F64 = 0
 RSKX_F32 = 1
+RSKX_Z64 = 2
+RSKX_Z32 = 3
 RSKX_KERNEL_LAPLACE2D_DL = 1
 RSKX_KERNEL_LAPLACE3D_SL = 2
+RSKX_KERNEL_HELMHOLTZ2D_CFIE = 3
 RSKX_PROXY_NONE = 0
 RSKX_PROXY_RING_SL = 1
 RSKX_PROXY_RING_DL = 2
 RSKX_PROXY_POINTS_SL = 3
+RSKX_PROXY_RING_CFIE = 4
 
 
 class Kernel(ctypes.Structure):
     _fields_ = [("kind", c_int), ("nu", c_void_p), ("w", c_void_p), ("diag", c_void_p),
-                ("scale", c_double)]
+                ("scale", c_double), ("wavenumber", c_double), ("eta", c_double)]
 
 
 class Proxy(ctypes.Structure):

@@ -22,6 +22,7 @@
 #include "common.hpp"
 #include "rskx.h"
 #include "tree.hpp"
+#include "zcomplex.hpp"
 
 namespace rskx {
 
@@ -83,6 +84,36 @@ void launch_cgs(const double*, int64_t, int64_t, int, double*, int64_t, int64_t,
                 double*, double*, double*, hipStream_t);
 void launch_tree_split(bool, const TreeChunk*, int, int, int64_t, double*, int*, double*, int*,
                        int*, hipStream_t);
+
+// complex (Helmholtz) path: zfactor_kernels.hip, zsweep_kernels.hip
+template <typename S>
+void launch_zassemble(const int*, const LvlBox*, const int*, const int*, const int*, int*, S*,
+                      const PointsDev&, const LevelParams&, int, hipStream_t);
+template <typename S>
+void launch_zid(const int*, const LvlBox*, const int*, S*, int*, double*, const IdParams&, int*,
+                int*, int, size_t, zd*, int64_t, hipStream_t);
+template <typename S>
+void launch_zelim(bool, bool, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*,
+                  int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
+                  int*, hipStream_t);
+size_t zelim_elems(int k, int r);
+void launch_zgather_diag(const double*, const int*, int64_t, double*, double*, hipStream_t);
+template <typename S>
+void launch_zsweep(int, bool, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int,
+                   size_t, hipStream_t);
+template <typename U, typename S>
+void launch_zperm(bool, void*, int64_t, S*, int64_t, int, int, const int*, bool, hipStream_t);
+size_t zsweep_shmem(int kmax, int rmax, int nc);
+void launch_zmatvec(const double*, int64_t, const double*, const double*, const double*, double,
+                    double, const double*, int64_t, double*, int64_t, int, hipStream_t);
+
+// proxies on the ring of a combined-field box of longest side l: the ring
+// holds the box's outgoing field, whose angular bandwidth is about
+// k l / sqrt 2 plus a margin for the tolerance, so the count grows with k l
+inline int cfie_proxy_count(int pmin, double k, double l) {
+  const int p = 8 * (int)std::ceil((1.5 * k * l + 48.0) / 8.0);
+  return std::max(pmin, p);
+}
 
 // --------------------------------------------------------------- errors --
 static thread_local std::string g_err;
@@ -418,6 +449,7 @@ struct rskx_factor {
   int nprob = 1;                          // independent problems (rskx_rskelf_batch)
   char symm = 'n';
   int prec = RSKX_PREC_FP64;
+  bool cplx = false;                      // complex factor (RSKX_KERNEL_HELMHOLTZ2D_CFIE)
   int nlvl = 0, nlvl_done = 0;
   std::vector<int64_t> lvp;               // cumulative factor counts per processed level
   std::vector<int> lev_kmax, lev_rmax;
@@ -686,11 +718,11 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   }
 
   // ---- points in tree order (SoA)
-  RSKX_TRY(ws.pts.ensure((size_t)N * (2 * d + 2) * sizeof(double)));
+  constexpr bool CPLX = is_cplx<S>::v;
+  RSKX_TRY(ws.pts.ensure((size_t)N * (2 * d + (CPLX ? 3 : 2)) * sizeof(double)));
   double* pts = ws.pts.as<double>();
   // gather points, normals, weights and diagonal into tree order on the device
   launch_gather_points(x, A.nu, A.w, A.diag, F->order.as<int>(), N, d, pts, g_stream);
-  RSKX_HIP(hipGetLastError());
   PointsDev P{};
   for (int i = 0; i < 3; ++i) {
     P.x[i] = i < d ? pts + (size_t)i * N : nullptr;
@@ -701,6 +733,14 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
   P.d = d;
   P.kind = A.kind;
   P.scale = A.scale;
+  if (CPLX) {            // the complex diagonal (interleaved) in two real rows
+    P.diag_im = pts + (size_t)(2 * d + 2) * N;
+    launch_zgather_diag(A.diag, F->order.as<int>(), N, pts + (size_t)(2 * d + 1) * N,
+                        pts + (size_t)(2 * d + 2) * N, g_stream);
+    P.k = A.wavenumber;
+    P.eta = A.eta;
+  }
+  RSKX_HIP(hipGetLastError());
 
   // ---- pool of point lists: leaves own their tree-order ranges
   int nbox = t.nbox();
@@ -778,7 +818,15 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     const bool use_proxy = lvl_ref > 2 && PX.kind != RSKX_PROXY_NONE;
     // point proxies of an unsymmetric matrix add the incoming rows
     L.np = use_proxy ? PX.p : 0;
-    L.p = L.np * (PX.kind == RSKX_PROXY_POINTS_SL && two ? 2 : 1);
+    if (use_proxy && PX.kind == RSKX_PROXY_RING_CFIE) {
+      double lmax = 0;
+      for (int64_t b = t.lvp[lv]; b < t.lvp[lv + 1]; ++b)
+        for (int i = 0; i < d; ++i) lmax = std::max(lmax, t.bsize[(size_t)d * b + i]);
+      L.np = cfie_proxy_count(PX.p, A.wavenumber, lmax);
+    }
+    // point and combined-field proxies of an unsymmetric matrix add the
+    // incoming rows
+    L.p = L.np * ((PX.kind == RSKX_PROXY_POINTS_SL || PX.kind == RSKX_PROXY_RING_CFIE) && two ? 2 : 1);
     L.filter = use_proxy ? 1 : 0;
     L.two_sided = two ? 1 : 0;
 
@@ -824,7 +872,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       B.nbr_off = off_nbr;             off_nbr += mc;
       B.K_off = off_K;                 off_K += (int64_t)B.mcap * n;
       B.perm_off = off_perm;           off_perm += n;
-      B.T_off = off_T;                 off_T += 3 * (int64_t)n * n;
+      B.T_off = off_T;                 off_T += (CPLX ? 6 : 3) * (int64_t)n * n;
       // elimination scratch of the global class: 2 n^2 elements, then a
       // 6 n double point cache (12 n elements keep the next box 8-byte aligned)
       B.W_off = off_W;                 off_W += 2 * (int64_t)n * n + 12 * (int64_t)n;
@@ -857,13 +905,47 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     launch_lists(ws.boxes.as<LvlBox>(), ws.nsegs.as<Seg>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
                  ws.nbr.as<int>(), ws.mnbr.as<int>(), P, L, (int)nb, g_stream);
     prof.end("lists", ev);
+    const bool gram = !CPLX && sizeof(S) == 4 && prm.tol >= 1e-7;
+    if constexpr (CPLX) {
+      // complex: every box assembles K in global memory and runs the
+      // complex QRCP ID (k_zid)
+      std::vector<int> ida;
+      int nmx = 0;
+      int64_t mvmax = 0, Kbytes = 0;
+      for (int64_t ib = 0; ib < nb; ++ib) {
+        ida.push_back((int)ib);
+        nmx = std::max(nmx, hb[ib].n);
+        mvmax = std::max<int64_t>(mvmax, std::max(hb[ib].mcap, 3 * hb[ib].n));
+        Kbytes += (int64_t)hb[ib].mcap * hb[ib].n * (int64_t)sizeof(S);
+      }
+      RSKX_TRY(upload(ws.ids, ida));
+      prof.begin("assemble", ev);
+      launch_zassemble<S>(ws.ids.as<int>(), ws.boxes.as<LvlBox>(), ws.pool.as<int>(),
+                          ws.nbr.as<int>(), ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P,
+                          L, (int)nb, g_stream);
+      prof.end("assemble", ev, Kbytes);
+      size_t shz = (size_t)(2 * nmx + 2 * mvmax + 64) * sizeof(double);
+      zd* vglob = nullptr;
+      if (shz > 160 * 1024) {               // Householder vectors in global memory
+        shz = (size_t)(2 * nmx + 64) * sizeof(double);
+        RSKX_TRY(ws.V.ensure((size_t)nb * mvmax * sizeof(zd)));
+        vglob = ws.V.as<zd>();
+      }
+      if (shz > 160 * 1024)
+        return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box too large for the complex ID kernel (n=" +
+                                         std::to_string(nmx) + ")");
+      prof.begin("id", ev);
+      launch_zid<S>(ws.ids.as<int>(), ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
+                    ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
+                    (int)nb, shz, vglob, mvmax, g_stream);
+      prof.end("id", ev);
+    } else {
     // ID: boxes whose K fits a register tile assemble and factor it in
     // registers (k_id_reg); the rest assemble K in global memory first
     // FP32 factorizations at tol >= 1e-7 take the Gram route (k_gram_id),
     // whose shared memory is sized by the unfiltered neighbour count, so
     // the host need not wait for k_lists; the QR routes read back the
     // filtered counts to choose register classes
-    const bool gram = sizeof(S) == 4 && prm.tol >= 1e-7;
     std::vector<int> hmn(nb);
     bool exact = !gram;
     for (int64_t ib = 0; ib < nb && !exact; ++ib) {
@@ -989,6 +1071,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
                    nq, (size_t)(3 * nmx + 32) * sizeof(double), nullptr, 0, g_stream);
     }
     prof.end("id", ev);
+    }
     RSKX_HIP(hipGetLastError());
     hlap("h:routing");
     hk.resize(nb);
@@ -1020,9 +1103,26 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       const int64_t k = O.k, r = O.r, m = hm[ib], n = B.n;
       if (gram && n > 0 && m > 0 && gram_id_class((int)n) >= 0)   // G = K'K, pivoted Cholesky, T
         id_flops += (double)m * n * (n + 1) + 2.0 * (double)n * n * k + (double)k * k * r;
-      else                                                  // truncated QRCP, T
-        id_flops += 4.0 * m * n * k - 2.0 * k * k * (m + n) + 4.0 / 3.0 * k * k * k + (double)k * k * r;
-      if (r > 0) {
+      else                                                  // truncated QRCP, T (x4 complex)
+        id_flops += (CPLX ? 4.0 : 1.0) * (4.0 * m * n * k - 2.0 * k * k * (m + n) +
+                                         4.0 / 3.0 * k * k * k + (double)k * k * r);
+      if (r > 0 && CPLX) {                // complex: T, LU, E, G and the pivots only
+        O.fac = (int)(nfac++);
+        O.T = need_v; need_v += k * r;
+        O.LU = need_v; need_v += r * r;
+        O.E = need_v; need_v += k * r;
+        O.G = need_v; need_v += r * k;
+        O.Li = O.Ui = O.Tt = O.Et = O.Gt = O.LUt = O.Lti = O.Uti = -1;
+        O.sk = need_i; need_i += k;
+        O.rd = need_i; need_i += r;
+        O.piv = need_i; need_i += r;
+        O.D = -1;
+        O.M_off = offM; offM += k * k;
+        O.nl_off = pool_top + off_slf + off_nl; off_nl += k;
+        kmx = std::max<int>(kmx, (int)k);
+        rmx = std::max<int>(rmx, (int)r);
+        el_flops += 4.0 * (4.0 * n * k * r + 2.0 / 3.0 * r * r * r + 2.0 * k * r * r + 2.0 * k * k * r);
+      } else if (r > 0) {
         O.fac = (int)(nfac++);
         O.T = need_v; need_v += k * r;
         O.LU = need_v; need_v += r * r;
@@ -1066,6 +1166,37 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     RSKX_TRY(ws.M[mcur].ensure(std::max<int64_t>(offM, 1) * sizeof(S)));
     RSKX_TRY(ws.M[mpar].ensure(sizeof(S)));
     RSKX_TRY(upload(ws.outs, ho));
+    // k_elim_reg writes the transposed copies itself; the complex sweeps
+    // read the factor as stored (nothing to pack)
+    std::vector<char> regbox(nb, CPLX ? 1 : 0);
+    if constexpr (CPLX) {
+      // complex: boxes whose blocks and point cache fit shared memory, and
+      // the rest (global scratch; also the no-compression copies)
+      std::vector<int> zc[2];
+      size_t shz = 0;
+      for (int64_t ib = 0; ib < nb; ++ib) {
+        const BoxOut& O = ho[ib];
+        const size_t by = O.r > 0 ? zelim_elems(O.k, O.r) * sizeof(S) + 8 * sizeof(double) * (size_t)hb[ib].n : 0;
+        if (O.r > 0 && by <= 158 * 1024) { zc[0].push_back((int)ib); shz = std::max(shz, by); }
+        else zc[1].push_back((int)ib);
+      }
+      std::vector<int> zall(zc[0]);
+      zall.insert(zall.end(), zc[1].begin(), zc[1].end());
+      RSKX_TRY(upload(ws.ids, zall));
+      prof.begin("factor", ev);
+      const bool sym = o.symm == 's';
+      if (!zc[0].empty())
+        launch_zelim<S>(sym, true, shz, ws.ids.as<int>(), (int)zc[0].size(), ws.boxes.as<LvlBox>(),
+                        ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(),
+                        ws.T.as<double>(), ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), ws.W.as<S>(), fval,
+                        fidx, P, ws.flag.as<int>(), g_stream);
+      if (!zc[1].empty())
+        launch_zelim<S>(sym, false, 0, ws.ids.as<int>() + zc[0].size(), (int)zc[1].size(),
+                        ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(),
+                        ws.pool.as<int>(), ws.perm.as<int>(), ws.T.as<double>(), ws.M[mpar].as<S>(),
+                        ws.M[mcur].as<S>(), ws.W.as<S>(), fval, fidx, P, ws.flag.as<int>(), g_stream);
+      prof.end("factor", ev, 0, el_flops);
+    } else {
     // elimination: boxes whose blocks fit two per CU in shared memory, one
     // per CU, or neither (global scratch; also the no-compression copies)
     const size_t kHalf = 78 * 1024, kFull = 158 * 1024;   // 1024-thread workgroups
@@ -1073,7 +1204,6 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // workgroups (two per CU, n <= 192 in FP32, 128 in FP64) when the box
     // and its shared memory allow, else 16-wave workgroups
     std::vector<int> cls[3], creg[2];
-    std::vector<char> regbox(nb, 0);           // k_elim_reg also writes the transposed copies
     size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0};
     const int nreg8 = elim_reg_nmax(sizeof(S), 8), nreg16 = elim_reg_nmax(sizeof(S), 16);
     for (int64_t ib = 0; ib < nb; ++ib) {
@@ -1138,6 +1268,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       if (conc) RSKX_TRY(side_join());
     }
     prof.end("factor", ev, 0, el_flops);
+    }
     RSKX_HIP(hipGetLastError());
     hlap("h:layout");
     // the failure flag comes back with the next synchronization; the host
@@ -1171,7 +1302,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     }
     // transposed copies for the solve sweeps; runs while the host lays
     // out the next level
-    {
+    if constexpr (!CPLX) {
       const int64_t f0 = F->lvp.back(), nl = (int64_t)F->recs.size() - f0;
       bool any = false;
       for (int64_t f = f0; f < f0 + nl; ++f) any = any || !F->recs[f].packed;
@@ -1208,7 +1339,9 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
              std::chrono::duration<double>(clk::now() - tl).count());
   }
   if (o.verb) printf("%s\n", std::string(69, '-').c_str());
-  if (nrem > 0) RSKX_TRY(partial_assembly<S>(t, bs, lv_last, ws.M[mpar], F));
+  if constexpr (!CPLX) {
+    if (nrem > 0) RSKX_TRY(partial_assembly<S>(t, bs, lv_last, ws.M[mpar], F));
+  }
   RSKX_TRY(upload(F->d_recs, F->recs));
   RSKX_HIP(hipGetLastError());
   RSKX_TRY(sync_stream());
@@ -1365,6 +1498,65 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
   return RSKX_OK;
 }
 
+// Complex sweeps (rskelf_sv.m / rskelf_mv.m with sv/*_nn, _nc, _sn, _sc and
+// the mv counterparts): trans 't' is 'c' applied to conj(X), conjugated
+// back (rskelf_sv.m:22, rskelf_mv.m:22).  Columns go through the level
+// kernels in groups whose rows fit shared memory (at most 8).
+template <typename S, typename U>
+int zsweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans, bool solve,
+                int mode) {
+  rskx_factor* F = const_cast<rskx_factor*>(Fc);
+  const int64_t N = F->N;
+  if (nrhs <= 0 || N == 0) return RSKX_OK;
+  const bool cj = trans == 't';
+  const char tr = cj ? 'c' : trans;
+  RSKX_TRY(g_ws.xw.ensure((size_t)N * nrhs * sizeof(S)));
+  S* Y = g_ws.xw.as<S>();
+  hipEvent_t ev;
+  Prof& prof = F->prof;
+  prof.begin("permute", ev);
+  launch_zperm<U, S>(true, X, ldx, Y, N, nrhs, nrhs, F->order.as<int>(), cj, g_stream);
+  prof.end("permute", ev, (int64_t)N * nrhs * (sizeof(S) + sizeof(U)) + N * 4);
+  const int prog = solve ? (tr == 'n' ? 0 : 1) : (tr == 'n' ? 2 : 3);
+  const bool sym = F->symm == 's';
+  const int nl = (int)F->lvp.size() - 1;
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool up = pass == 0;
+    if (!(mode & (up ? 1 : 2))) continue;
+    for (int q = 0; q < nl; ++q) {
+      const int lv = up ? q : nl - 1 - q;
+      const int64_t f0 = F->lvp[lv], f1 = F->lvp[lv + 1];
+      if (f1 == f0) continue;
+      const int kmx = F->lev_kmax[lv], rmx = F->lev_rmax[lv];
+      int nc = std::min(nrhs, 8);
+      while (nc > 1 && zsweep_shmem(kmx, rmx, nc) > 160 * 1024) nc = (nc + 1) / 2;
+      const size_t shm = zsweep_shmem(kmx, rmx, nc);
+      if (shm > 160 * 1024)
+        return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: a complex factor too large for the sweep's shared memory (k=" +
+                                        std::to_string(kmx) + ", r=" + std::to_string(rmx) + ")");
+      int64_t bytes = 0;
+      for (int64_t f = f0; f < f1; ++f) {
+        const FacRec& fr = F->recs[f];
+        const int64_t kr = (int64_t)fr.k * fr.r, rr = (int64_t)fr.r * fr.r;
+        bytes += (int64_t)sizeof(S) * (2 * kr + (rr + fr.r) / 2 + 2 * (int64_t)(fr.k + fr.r) * nrhs) +
+                 4 * (fr.k + fr.r);
+      }
+      prof.begin("sweep", ev);
+      for (int c0 = 0; c0 < nrhs; c0 += nc)
+        launch_zsweep<S>(prog, sym, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0),
+                         F->lval[lv].as<S>(), F->lidx[lv].as<int>(), Y, N, c0,
+                         std::min(nc, nrhs - c0), shm, g_stream);
+      prof.end("sweep", ev, bytes);
+    }
+  }
+  prof.begin("permute", ev);
+  launch_zperm<U, S>(false, X, ldx, Y, N, nrhs, nrhs, F->order.as<int>(), cj, g_stream);
+  prof.end("permute", ev, (int64_t)N * nrhs * (sizeof(S) + sizeof(U)) + N * 4);
+  RSKX_HIP(hipGetLastError());
+  prof.flush();
+  return RSKX_OK;
+}
+
 }  // namespace
 
 template <typename S>
@@ -1472,8 +1664,14 @@ static int check_kernel(const rskx_kernel* A, int d) {
     return RSKX_OK;
   }
   if (A && A->kind == RSKX_KERNEL_LAPLACE3D_SL && d == 3) return RSKX_OK;
-  return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: built kernels are the 2D Laplace double layer (d = 2) "
-                              "and the 3D Laplace single layer (d = 3)");
+  if (A && A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE && d == 2) {
+    if (!A->nu) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: the combined field needs normals");
+    if (!(A->wavenumber > 0) || !std::isfinite(A->eta))
+      return fail(RSKX_ERR_ARG, "FLAM:rskx:kernel: the combined field needs a wavenumber > 0 and a finite eta");
+    return RSKX_OK;
+  }
+  return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: built kernels are the 2D Laplace double layer (d = 2), "
+                              "the 3D Laplace single layer (d = 3) and the 2D Helmholtz combined field (d = 2)");
 }
 
 // argument checks shared by rskx_rskelf and rskx_rskelf_batch
@@ -1494,6 +1692,13 @@ static int check_factor_args(const double* x, int d, int64_t N, int occ, double 
   if (rank_or_tol >= 1 && std::isfinite(o.Tmax) && o.rrqr_iter > 0)
     return fail(RSKX_ERR_UNSUP, "FLAM:rskx:rank: fixed-rank ID with RRQR refinement is not built (pass a tolerance < 1, or Tmax = Inf)");
   RSKX_TRY(check_kernel(A, d));
+  if (A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE) {
+    if (o.symm != 'n' && o.symm != 's')
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:symm: complex factors are built for symm 'n' and 's' "
+                                  "('h' and 'p' need a Hermitian matrix)");
+    if (o.stop >= 0)
+      return fail(RSKX_ERR_UNSUP, "FLAM:rskx:stop: partial factorization is built for real data");
+  }
   if (!x || !A->w || !A->diag) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: point or kernel array is NULL");
   if (N < 1) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: N must be positive");
   if (N > (int64_t)1 << 30) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: N above 2^30");
@@ -1504,7 +1709,7 @@ static int check_factor_args(const double* x, int d, int64_t N, int occ, double 
 
 static int check_proxy(const rskx_proxy* pxy, int d) {
   if (!pxy || (pxy->kind != RSKX_PROXY_RING_SL && pxy->kind != RSKX_PROXY_RING_DL &&
-               pxy->kind != RSKX_PROXY_POINTS_SL))
+               pxy->kind != RSKX_PROXY_POINTS_SL && pxy->kind != RSKX_PROXY_RING_CFIE))
     return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: naive global compression (no pxyfun) is not built; pass a proxy surface");
   if (pxy->p < 1 || !(pxy->radius > 0)) return fail(RSKX_ERR_ARG, "FLAM:rskx:proxy: invalid proxy surface");
   if (pxy->kind == RSKX_PROXY_POINTS_SL) {
@@ -1525,9 +1730,22 @@ static int run_factor(const double* x, int d, int64_t N, int nprob, const int64_
   F->nprob = nprob;
   F->symm = o.symm;
   F->prec = o.precision;
-  int rc = o.precision == RSKX_PREC_FP64
-               ? factor_impl<double>(x, d, N, nprob, poff, occ, rank_or_tol, *A, px, pw, o, F)
-               : factor_impl<float>(x, d, N, nprob, poff, occ, rank_or_tol, *A, px, pw, o, F);
+  const bool cfie = A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE;
+  if (cfie != (px.kind == RSKX_PROXY_RING_CFIE)) {
+    delete F;
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: the combined field takes RSKX_PROXY_RING_CFIE, "
+                                "and that ring only the combined field");
+  }
+  F->cplx = cfie;
+  int rc;
+  if (cfie)
+    rc = o.precision == RSKX_PREC_FP64
+             ? factor_impl<zd>(x, d, N, nprob, poff, occ, rank_or_tol, *A, px, pw, o, F)
+             : factor_impl<zf>(x, d, N, nprob, poff, occ, rank_or_tol, *A, px, pw, o, F);
+  else
+    rc = o.precision == RSKX_PREC_FP64
+             ? factor_impl<double>(x, d, N, nprob, poff, occ, rank_or_tol, *A, px, pw, o, F)
+             : factor_impl<float>(x, d, N, nprob, poff, occ, rank_or_tol, *A, px, pw, o, F);
   if (rc != RSKX_OK) { delete F; return rc; }
   *out = F;
   return RSKX_OK;
@@ -1577,9 +1795,18 @@ static int sweep_entry(const rskx_factor* F, void* X, int64_t ldx, int nrhs, cha
   char t = (char)tolower(trans);
   if (t != 'n' && t != 't' && t != 'c')
     return fail(RSKX_ERR_ARG, "FLAM:chktrans:invalidTrans: Transpose parameter must be one of 'N', 'T', or 'C'.");
-  if (t == 't') t = 'c';   // real data: transpose = conjugate transpose (rskelf_sv.m:22)
   if (nrhs < 0 || ldx < F->N) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad nrhs or leading dimension");
   if (nrhs > 0 && !X) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: X is NULL");
+  if (F->cplx) {
+    if (dtype != RSKX_Z64 && dtype != RSKX_Z32)
+      return fail(RSKX_ERR_ARG, "FLAM:rskx:dtype: a complex factor takes complex right-hand sides (RSKX_Z64, RSKX_Z32)");
+    if (F->prec == RSKX_PREC_FP64)
+      return dtype == RSKX_Z64 ? zsweep_impl<zd, zd>(F, X, ldx, nrhs, t, solve, mode)
+                               : zsweep_impl<zd, zf>(F, X, ldx, nrhs, t, solve, mode);
+    return dtype == RSKX_Z64 ? zsweep_impl<zf, zd>(F, X, ldx, nrhs, t, solve, mode)
+                             : zsweep_impl<zf, zf>(F, X, ldx, nrhs, t, solve, mode);
+  }
+  if (t == 't') t = 'c';   // real data: transpose = conjugate transpose (rskelf_sv.m:22)
   if (dtype != RSKX_F64 && dtype != RSKX_F32) return fail(RSKX_ERR_ARG, "FLAM:rskx:dtype: unknown element type");
   if (F->prec == RSKX_PREC_FP64)
     return dtype == RSKX_F64 ? sweep_impl<double, double>(F, X, ldx, nrhs, t, solve, mode)
@@ -1612,6 +1839,7 @@ int rskx_partial_info(const rskx_factor* F, int64_t* nsk, int64_t* nnz, int64_t*
 
 int rskx_rskelf_logdet(const rskx_factor* F, double* re, double* im) {
   if (!F || !re || !im) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  if (F->cplx) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:logdet: built for real factors");
   return F->prec == RSKX_PREC_FP64 ? logdet_impl<double>(F, re, im) : logdet_impl<float>(F, re, im);
 }
 
@@ -1657,6 +1885,8 @@ int rskx_rskelf_sv_noalloc(const rskx_factor* F, void* X, int64_t ldx, int nrhs,
   if (nrhs < 0 || ldx < F->N) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad nrhs or leading dimension");
   if (nrhs > 0 && !X) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: X is NULL");
   if (dtype != RSKX_F64 && dtype != RSKX_F32) return fail(RSKX_ERR_ARG, "FLAM:rskx:dtype: unknown element type");
+  if (F->cplx)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:dtype: the allocation-free solve is built for real factors");
   void* ws = nullptr;
   RSKX_TRY(scratch_address(&ws));
   if (F->prec == RSKX_PREC_FP64)
@@ -1675,7 +1905,7 @@ int rskx_factor_info(const rskx_factor* F, rskx_info* info) {
   info->nlvl = F->nlvl;
   info->nlvl_done = F->nlvl_done;
   info->nfactor = (int64_t)F->recs.size();
-  int es = F->prec == RSKX_PREC_FP64 ? 8 : 4;
+  int es = (F->prec == RSKX_PREC_FP64 ? 8 : 4) * (F->cplx ? 2 : 1);
   info->bytes = F->fval_used * es + F->fidx_used * 4 + (int64_t)F->recs.size() * sizeof(FacRec) + F->N * 4;
   info->precision = F->prec;
   info->symm = F->symm;
@@ -1700,6 +1930,56 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
   const int flv = F->level_of(i);
   const int64_t k = f.k, r = f.r;
   const bool lu = F->symm == 'n' || F->symm == 's';
+  if ((int64_t)F->h_order.size() != F->N) {
+    std::vector<int> o32(F->N);
+    RSKX_HIP(hipMemcpy(o32.data(), F->order.p, F->N * sizeof(int), hipMemcpyDeviceToHost));
+    F->h_order.assign(o32.begin(), o32.end());
+  }
+  if (F->cplx) {
+    // complex factor: every matrix of the view holds interleaved (re, im)
+    // pairs, twice the element counts of the real view
+    const bool d64 = F->prec == RSKX_PREC_FP64;
+    auto getz = [&](int64_t off, int64_t cnt, std::vector<double>& dst) -> int {
+      dst.assign(2 * cnt, 0.0);
+      if (!cnt) return RSKX_OK;
+      if (d64) {
+        RSKX_HIP(hipMemcpy(dst.data(), F->lval[flv].as<char>() + off * 16, cnt * 16, hipMemcpyDeviceToHost));
+      } else {
+        std::vector<float> raw(2 * cnt);
+        RSKX_HIP(hipMemcpy(raw.data(), F->lval[flv].as<char>() + off * 8, cnt * 8, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < 2 * cnt; ++q) dst[q] = raw[q];
+      }
+      return RSKX_OK;
+    };
+    std::vector<double> T, LU, E, G;
+    std::vector<int> sk(k), rd(r), pv(r);
+    RSKX_TRY(getz(f.T, k * r, T));
+    RSKX_TRY(getz(f.LU, r * r, LU));
+    RSKX_TRY(getz(f.E, k * r, E));
+    RSKX_TRY(getz(f.G, r * k, G));
+    if (k) RSKX_HIP(hipMemcpy(sk.data(), F->lidx[flv].as<int>() + f.sk, k * 4, hipMemcpyDeviceToHost));
+    if (r) RSKX_HIP(hipMemcpy(rd.data(), F->lidx[flv].as<int>() + f.rd, r * 4, hipMemcpyDeviceToHost));
+    if (r) RSKX_HIP(hipMemcpy(pv.data(), F->lidx[flv].as<int>() + f.piv, r * 4, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < k; ++q) v->sk[q] = F->h_order[sk[q]];
+    for (int64_t q = 0; q < r; ++q) v->rd[q] = F->h_order[rd[q]];
+    if (v->T) std::copy(T.begin(), T.end(), v->T);
+    if (v->E) std::copy(E.begin(), E.end(), v->E);
+    if (v->F) std::copy(G.begin(), G.end(), v->F);
+    for (int64_t c = 0; c < r; ++c)
+      for (int64_t a = 0; a < r; ++a) {
+        const int64_t e = 2 * (a + c * r);
+        if (v->L) {
+          v->L[e] = a > c ? LU[e] : (a == c ? 1.0 : 0.0);
+          v->L[e + 1] = a > c ? LU[e + 1] : 0.0;
+        }
+        if (v->U) {
+          v->U[e] = a <= c ? LU[e] : 0.0;
+          v->U[e + 1] = a <= c ? LU[e + 1] : 0.0;
+        }
+      }
+    if (v->p) for (int64_t q = 0; q < r; ++q) v->p[q] = pv[q];
+    return RSKX_OK;
+  }
   const int es = F->prec == RSKX_PREC_FP64 ? 8 : 4;
   auto getv = [&](int64_t off, int64_t cnt, std::vector<double>& dst) -> int {
     dst.resize(cnt);
@@ -1774,8 +2054,11 @@ int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, 
   RSKX_TRY(check_kernel(A, d));
   if (!x || !X || !Y) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL array");
   if (ldx < N || ldy < N || nrhs < 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad leading dimension");
-  launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, nullptr, 1,
-                g_stream);
+  if (A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE)   // X, Y interleaved complex, ld in complex elements
+    launch_zmatvec(x, N, A->nu, A->w, A->diag, A->wavenumber, A->eta, X, ldx, Y, ldy, nrhs, g_stream);
+  else
+    launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, nullptr, 1,
+                  g_stream);
   RSKX_HIP(hipGetLastError());
   return RSKX_OK;
 }
@@ -1802,6 +2085,8 @@ int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x,
                              const rskx_kernel* A, const double* X, int64_t ldx, double* Y,
                              int64_t ldy, int nrhs) {
   RSKX_TRY(check_kernel(A, d));
+  if (A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: the batched product is built for the real kernels");
   if (nprob < 1 || !offsets || offsets[0] != 0)
     return fail(RSKX_ERR_ARG, "FLAM:rskx:batch: need nprob >= 1 and offsets starting at 0");
   for (int j = 0; j < nprob; ++j)

@@ -28,10 +28,12 @@ struct PointsDev {
   const double* x[3];     // coordinates
   const double* nu[3];    // unit normals
   const double* w;        // quadrature weights
-  const double* diag;     // diagonal entries
+  const double* diag;     // diagonal entries (real parts for a complex kernel)
+  const double* diag_im;  // imaginary parts of the diagonal (complex kernels)
   int d;
   int kind;               // RSKX_KERNEL_*
   double scale;           // off-diagonal multiplier
+  double k, eta;          // wavenumber and coupling of RSKX_KERNEL_HELMHOLTZ2D_CFIE
 };
 
 // proxy / filter parameters of one level (box sizes and proxy weights are

@@ -15,7 +15,7 @@ import math
 
 import torch
 
-from .rskelf import LaplaceDL2D, LaplaceSL3D, ProxyPoints, ProxyRing
+from .rskelf import HelmholtzCFIE2D, LaplaceDL2D, LaplaceSL3D, ProxyPoints, ProxyRing
 
 
 class Curve:
@@ -199,3 +199,75 @@ def sphere(nface, nproxy=512, seed=0, device="cuda"):
     sel = torch.from_numpy(np.random.default_rng(seed).permutation(Pv.shape[0])[:nproxy])
     proxy = ProxyPoints(Pv[sel].t().contiguous().to(device), radius=1.5, weight=4 * math.pi / N)
     return Surface(C.t().contiguous().to(device), area.to(device), diag.to(device), 1.0, proxy)
+
+
+# ---------------------------------------------------- Helmholtz CFIE curves --
+EULER_GAMMA = 0.57721566490153286061
+
+
+class HelmholtzCurve:
+    """A closed curve with the exterior Dirichlet combined-field matrix
+    (HelmholtzCFIE2D) and its combined-field proxy ring."""
+
+    def __init__(self, kind, x, nu, w, diag, k, eta, nproxy):
+        self.kind, self.x, self.nu, self.w, self.diag = kind, x, nu, w, diag
+        self.k, self.eta = k, eta
+        self.N = x.shape[1]
+        self.proxy = ProxyRing(p=nproxy, radius=1.5, kind="cfie", weight=float(w.mean()))
+
+    @property
+    def kernel(self):
+        return HelmholtzCFIE2D(self.nu, self.w, self.diag, self.k, self.eta)
+
+    def matvec(self, X):
+        """Exact FP64 complex A X, evaluated on the fly (O(N^2))."""
+        from .rskelf import kernel_matvec
+        return kernel_matvec(self.kernel, self.x, X)
+
+    def circulant_matvec(self, X):
+        """Exact A X on the unit circle, where A is circulant: one complex
+        FFT per column with the first column's symbol (as ie_circle.m:47-48
+        does for the real matrix); the first column is one product with e_1."""
+        if self.kind != "circle":
+            raise ValueError("circulant_matvec: the matrix is circulant only on the circle")
+        if getattr(self, "_symbol", None) is None:
+            e1 = torch.zeros(self.N, dtype=torch.complex128, device=self.x.device)
+            e1[0] = 1
+            self._symbol = torch.fft.fft(self.matvec(e1))
+        vec = X.dim() == 1
+        Xt = X.reshape(self.N, -1).to(torch.complex128).t()
+        Y = torch.fft.ifft(self._symbol * torch.fft.fft(Xt, dim=1), dim=1).t()
+        return Y.reshape(-1) if vec else Y
+
+
+def helmholtz_curve(N, kind="star", k=200.0, eta=None, nproxy=64, device="cuda"):
+    """Exterior Dirichlet Helmholtz combined field A = 1/2 I + D - i eta S on
+    the unit circle or the star r(t) = 1 + 0.3 cos 5t, t_i = 2 pi i / N,
+    i = 1..N, eta = k by default: arc-length weights h_i; diagonal 1/2
+    - h_i kappa_i / 4 pi (the double layer's limit) - i eta s_i with the
+    single layer's corrected-trapezoidal self weight s_i = h_i ((i/4) -
+    (log(k h_i / 4 pi) + gamma) / 2 pi) (Kapur and Rokhlin's weight
+    h log(h / 2 pi) for the log|t| singularity; third-order convergence);
+    outward normals."""
+    eta = float(k if eta is None else eta)
+    t = _theta(N, device)
+    if kind == "circle":
+        r = torch.ones_like(t)
+        rp = torch.zeros_like(t)
+        rpp = torch.zeros_like(t)
+    elif kind == "star":
+        r, rp, rpp = 1 + 0.3 * torch.cos(5 * t), -1.5 * torch.sin(5 * t), -7.5 * torch.cos(5 * t)
+    else:
+        raise ValueError(kind)
+    c, s = torch.cos(t), torch.sin(t)
+    x = torch.stack([r * c, r * s])
+    xp, yp = rp * c - r * s, rp * s + r * c
+    xpp, ypp = rpp * c - 2 * rp * s - r * c, rpp * s + 2 * rp * c - r * s
+    speed = torch.sqrt(xp * xp + yp * yp)
+    nu = torch.stack([yp, -xp]) / speed
+    kappa = (xp * ypp - yp * xpp) / speed ** 3
+    h = (2 * math.pi / N) * speed
+    re = 0.5 - h * kappa / (4 * math.pi) + eta * h / 4
+    im = eta * h * (torch.log(k * h / (4 * math.pi)) + EULER_GAMMA) / (2 * math.pi)
+    diag = torch.complex(re, im)
+    return HelmholtzCurve(kind, x, nu, h, diag, float(k), eta, nproxy)

@@ -49,6 +49,20 @@ def _cgs2(V, nv, w):
     return hout
 
 
+def _cgs2_torch(V, nv, w):
+    """The same orthogonalization for complex vectors, with torch's batched
+    products (the fused kernel is real): h = V' w twice, w -= V h, and the
+    new norm; returns (nrhs, nv + 1) with the norm last."""
+    Vj = V[:, :nv, :].contiguous()
+    Vc = Vj.conj().resolve_conj()                               # (nrhs, nv, N)
+    h1 = torch.bmm(Vc, w.unsqueeze(2)).squeeze(2)               # V' w
+    w -= torch.bmm(h1.unsqueeze(1), Vj).squeeze(1)
+    h2 = torch.bmm(Vc, w.unsqueeze(2)).squeeze(2)
+    w -= torch.bmm(h2.unsqueeze(1), Vj).squeeze(1)
+    nrm = _norms(w)
+    return torch.cat([h1 + h2, nrm.to(w.dtype).unsqueeze(1)], dim=1)
+
+
 def _rowsum(P):
     """Sum over the last dimension in two stages (1024 partial sums per
     row first), so a few very long rows still spread over the whole GPU."""
@@ -59,6 +73,8 @@ def _rowsum(P):
 
 
 def _norms(W):
+    if W.is_complex():
+        return torch.sqrt(_rowsum(W.real * W.real + W.imag * W.imag))
     return torch.sqrt(_rowsum(W * W))
 
 
@@ -80,7 +96,9 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None, in
     with matvec after each cycle, so the result meets tol for matvec's A.
     Returns (X, info) with info = dict(iters, outer, relres)."""
     vec = B.dim() == 1
-    B2 = B.reshape(B.shape[0], -1).to(torch.float64)
+    cplx = B.is_complex() or getattr(precond, "cplx", False)
+    dt = torch.complex128 if cplx else torch.float64
+    B2 = B.reshape(B.shape[0], -1).to(dt)
     N, nrhs = B2.shape
     Bt = B2.t().contiguous()                       # (nrhs, N)
     bnorm = _norms(Bt)
@@ -95,8 +113,9 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None, in
     else:
         inner_ref = bnorm_h
     Xt = torch.zeros_like(Bt)
-    V = torch.empty(nrhs, restart + 1, N, dtype=torch.float64, device=B2.device)
-    Z = torch.empty(nrhs, restart, N, dtype=torch.float64, device=B2.device)
+    V = torch.empty(nrhs, restart + 1, N, dtype=dt, device=B2.device)
+    Z = torch.empty(nrhs, restart, N, dtype=dt, device=B2.device)
+    hdt = np.complex128 if cplx else np.float64
 
     def op(f, Yt):                                 # (nrhs, N) -> (nrhs, N)
         return f(Yt.t()).t().contiguous()
@@ -117,10 +136,10 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None, in
         # one already solved) keeps a zero direction: its g, and so its
         # update y, stay zero while the other columns iterate
         V[:, 0, :] = R / torch.where(rn > 0, rn, torch.ones_like(rn))[:, None]
-        Rh = np.zeros((nrhs, restart, restart))                # rotated H, upper triangular
+        Rh = np.zeros((nrhs, restart, restart), hdt)           # rotated H, upper triangular
         cs = np.zeros((nrhs, restart))
-        sn = np.zeros((nrhs, restart))
-        g = np.zeros((nrhs, restart + 1))
+        sn = np.zeros((nrhs, restart), hdt)
+        g = np.zeros((nrhs, restart + 1), hdt)
         g[:, 0] = rn.cpu().numpy()
         j = 0
         for j in range(restart):
@@ -130,27 +149,48 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None, in
             else:
                 Z[:, j, :] = op(precond, V[:, j, :])
             w = op(inner_matvec or matvec, Z[:, j, :])
-            hdev = _cgs2(V, j + 1, w)                          # (nrhs, j + 2): h, ||w||
+            # (nrhs, j + 2): h, ||w||
+            hdev = _cgs2_torch(V, j + 1, w) if cplx else _cgs2(V, j + 1, w)
             hn = hdev[:, -1]
-            V[:, j + 1, :] = w / torch.where(hn > 0, hn, torch.ones_like(hn))[:, None]
+            V[:, j + 1, :] = w / torch.where(hn.real > 0, hn, torch.ones_like(hn))[:, None]
             h = hdev.cpu().numpy()
             iters += 1
-            for i in range(j):                                 # earlier rotations
-                t = cs[:, i] * h[:, i] + sn[:, i] * h[:, i + 1]
-                h[:, i + 1] = -sn[:, i] * h[:, i] + cs[:, i] * h[:, i + 1]
-                h[:, i] = t
-            r = np.hypot(h[:, j], h[:, j + 1])
-            safe = np.where(r > 0, r, 1.0)
-            cs[:, j] = np.where(r > 0, h[:, j] / safe, 1.0)
-            sn[:, j] = np.where(r > 0, h[:, j + 1] / safe, 0.0)
-            Rh[:, : j + 1, j] = h[:, : j + 1]
-            Rh[:, j, j] = r
-            g[:, j + 1] = -sn[:, j] * g[:, j]
-            g[:, j] = cs[:, j] * g[:, j]
+            if cplx:
+                # complex Givens rotations (zrotg): c real, s complex,
+                # [c s; -conj(s) c] [a; b] = [a / |a| r; 0]
+                for i in range(j):
+                    t = cs[:, i] * h[:, i] + sn[:, i] * h[:, i + 1]
+                    h[:, i + 1] = -np.conj(sn[:, i]) * h[:, i] + cs[:, i] * h[:, i + 1]
+                    h[:, i] = t
+                a, b = h[:, j], h[:, j + 1]
+                aa = np.abs(a)
+                r = np.sqrt(aa ** 2 + np.abs(b) ** 2)
+                ph = np.where(aa > 0, a / np.where(aa > 0, aa, 1.0), 1.0)
+                safe = np.where(r > 0, r, 1.0)
+                cs[:, j] = np.where(r > 0, aa / safe, 1.0)
+                sn[:, j] = np.where(r > 0, ph * np.conj(b) / safe, 0.0)
+                Rh[:, : j + 1, j] = h[:, : j + 1]
+                Rh[:, j, j] = ph * r
+                g[:, j + 1] = -np.conj(sn[:, j]) * g[:, j]
+                g[:, j] = cs[:, j] * g[:, j]
+                h = np.abs(h)                              # only |h_{j+1}| is read below
+            else:
+                for i in range(j):                             # earlier rotations
+                    t = cs[:, i] * h[:, i] + sn[:, i] * h[:, i + 1]
+                    h[:, i + 1] = -sn[:, i] * h[:, i] + cs[:, i] * h[:, i + 1]
+                    h[:, i] = t
+                r = np.hypot(h[:, j], h[:, j + 1])
+                safe = np.where(r > 0, r, 1.0)
+                cs[:, j] = np.where(r > 0, h[:, j] / safe, 1.0)
+                sn[:, j] = np.where(r > 0, h[:, j + 1] / safe, 0.0)
+                Rh[:, : j + 1, j] = h[:, : j + 1]
+                Rh[:, j, j] = r
+                g[:, j + 1] = -sn[:, j] * g[:, j]
+                g[:, j] = cs[:, j] * g[:, j]
             est = np.abs(g[:, j + 1])                          # ||g - H y|| of the LS solution
             if bool((est / inner_ref <= 0.5 * tol).all()) or bool((h[:, j + 1] == 0).all()):
                 break
-        y = np.zeros((nrhs, j + 1))                            # back substitution
+        y = np.zeros((nrhs, j + 1), hdt)                       # back substitution
         for i in range(j, -1, -1):
             d = Rh[:, i, i]
             y[:, i] = np.where(d != 0, (g[:, i] - (Rh[:, i, i + 1: j + 1] * y[:, i + 1:]).sum(1))

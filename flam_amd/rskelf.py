@@ -49,6 +49,21 @@ def _dev(t, dtype=torch.float64):
     return t.to(dtype).contiguous()
 
 
+_RDT = {torch.float64: L.RSKX_F64, torch.float32: L.RSKX_F32,
+        torch.complex128: L.RSKX_Z64, torch.complex64: L.RSKX_Z32}
+
+
+def _rhs_dtype(F, X):
+    """The element type a sweep of F takes X in: a complex factor takes
+    complex right-hand sides (real ones are promoted), a real factor real
+    ones; FP32 / complex-FP32 inputs keep their width."""
+    if getattr(F, "cplx", False) or X.is_complex():
+        if not getattr(F, "cplx", False):
+            raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:dtype: complex right-hand sides need a complex factor")
+        return torch.complex64 if X.dtype == torch.complex64 else torch.complex128
+    return X.dtype if X.dtype in (torch.float64, torch.float32) else torch.float64
+
+
 @dataclass
 class LaplaceDL2D:
     """A_ij = scale w_j ((x_i - x_j).nu_j) / (2 pi |x_i - x_j|^2), A_ii = diag_i.
@@ -91,6 +106,30 @@ class LaplaceSL3D:
 
 
 @dataclass
+class HelmholtzCFIE2D:
+    """Exterior Dirichlet Helmholtz combined field, complex:
+    A_ij = w_j (dG/dn_j(x_i, x_j) - i eta G(x_i, x_j)), G = (i/4) H0^(1)(k r)
+    (the kernel of rskelf/test/ie_square3.m:44,107-110), A_ii = diag_i
+    (complex).  nu: (2, N) unit normals, w: (N,) weights, diag: (N,) complex
+    diagonal; k the wavenumber, eta the single layer's coupling (k by
+    default).  Pairs with ProxyRing(kind='cfie')."""
+    nu: torch.Tensor
+    w: torch.Tensor
+    diag: torch.Tensor
+    k: float
+    eta: float = None
+
+    def _struct(self):
+        self.nu = _dev(self.nu).reshape(2, -1)
+        self.w = _dev(self.w).reshape(-1)
+        self.diag = _dev(self.diag, torch.complex128).reshape(-1)
+        self._nu_cm = self.nu.t().contiguous()
+        eta = float(self.k if self.eta is None else self.eta)
+        return L.Kernel(L.RSKX_KERNEL_HELMHOLTZ2D_CFIE, self._nu_cm.data_ptr(), self.w.data_ptr(),
+                        self.diag.data_ptr(), 1.0, float(self.k), eta)
+
+
+@dataclass
 class ProxyPoints:
     """pxyfun of rskelf/test/ie_sphere.m:174-187: proxy points pts (3, p) in
     box-size units about the box centre (pxy = pts l + ctr), single-layer
@@ -117,14 +156,18 @@ class ProxyRing:
     """pxyfun of the demos: p points on a ring of radius `radius` box sizes
     around the box centre; neighbours inside the ring are kept explicitly.
     kind 'sl': single layer times `weight` (ie_circle.m:106-115);
-    kind 'dl': double layer of the box points (ie_ellipse.m:126-132)."""
+    kind 'dl': double layer of the box points (ie_ellipse.m:126-132);
+    kind 'cfie': the combined-field ring of HelmholtzCFIE2D, at least p
+    points and more as k times the box size grows, incoming rows weighted
+    by `weight` (the mean quadrature weight)."""
     p: int = 64
     radius: float = 1.5
     kind: str = "sl"
     weight: float = 0.0
 
     def _struct(self):
-        kind = {"sl": L.RSKX_PROXY_RING_SL, "dl": L.RSKX_PROXY_RING_DL}[self.kind]
+        kind = {"sl": L.RSKX_PROXY_RING_SL, "dl": L.RSKX_PROXY_RING_DL,
+                "cfie": L.RSKX_PROXY_RING_CFIE}[self.kind]
         return L.Proxy(kind, int(self.p), float(self.radius), float(self.weight), None)
 
 
@@ -134,10 +177,11 @@ _PREC = {"fp64": L.RSKX_PREC_FP64, "fp32": L.RSKX_PREC_FP32}
 class Factor:
     """Device-resident factorization (opaque handle of the C ABI)."""
 
-    def __init__(self, handle, N, keep):
+    def __init__(self, handle, N, keep, cplx=False):
         self._h = handle
         self.N = N
         self._keep = keep
+        self.cplx = cplx
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -175,9 +219,10 @@ class Factor:
         _check(L.lib().rskx_export_factor(self._h, i, ctypes.byref(v)))
         k, r = v.k, v.r
         lu = self.info()["symm"] in ("n", "s")
+        dt = np.complex128 if self.cplx else np.float64   # complex: interleaved (re, im)
         arr = dict(sk=np.zeros(k, np.int64), rd=np.zeros(r, np.int64),
-                   T=np.zeros((r, k)), L=np.zeros((r, r)), U=np.zeros((r, r)),
-                   p=np.zeros(r, np.int64), E=np.zeros((r, k)), F=np.zeros((k, r)))
+                   T=np.zeros((r, k), dt), L=np.zeros((r, r), dt), U=np.zeros((r, r), dt),
+                   p=np.zeros(r, np.int64), E=np.zeros((r, k), dt), F=np.zeros((k, r), dt))
 
         def ptr(a, t):
             return a.ctypes.data_as(ctypes.POINTER(t))
@@ -253,7 +298,7 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     _on_torch_stream()
     _check(L.lib().rskx_rskelf(xc.data_ptr(), d, N, int(occ), float(rank_or_tol),
                                ctypes.byref(k), ctypes.byref(p), ctypes.byref(o), ctypes.byref(h)))
-    return Factor(h.value, N, keep=(A, xc))
+    return Factor(h.value, N, keep=(A, xc), cplx=isinstance(A, HelmholtzCFIE2D))
 
 
 @dataclass
@@ -353,8 +398,7 @@ def _sweep(fn, F, X, trans):
     X = X if isinstance(X, torch.Tensor) else torch.as_tensor(X)
     if not X.is_cuda:
         X = X.to("cuda")
-    if X.dtype not in (torch.float64, torch.float32):
-        X = X.to(torch.float64)
+    X = X.to(_rhs_dtype(F, X))
     vec = X.dim() == 1
     X2 = X.reshape(-1, 1) if vec else X
     if X2.shape[0] != F.N:
@@ -362,7 +406,7 @@ def _sweep(fn, F, X, trans):
     # column-major N x nrhs; always a fresh copy (a (1, N) view of a vector
     # is already "contiguous" and would alias the caller's storage)
     Xc = X2.t().clone(memory_format=torch.contiguous_format)
-    dt = L.RSKX_F64 if Xc.dtype == torch.float64 else L.RSKX_F32
+    dt = _RDT[Xc.dtype]
     t = str(trans).encode()[:1] if trans else b"n"
     _on_torch_stream()
     _check(fn(F.handle, Xc.data_ptr(), F.N, Xc.shape[0], t, dt))
@@ -374,7 +418,7 @@ def _sweep_mode(fn, F, X, trans, mode):
     """One sweep (mode 1 upward, 2 downward) on a fresh column-major copy."""
     X2 = X if X.dim() == 2 else X.reshape(-1, 1)
     Xc = X2.t().clone(memory_format=torch.contiguous_format)
-    dt = L.RSKX_F64 if Xc.dtype == torch.float64 else L.RSKX_F32
+    dt = _RDT[Xc.dtype]
     _on_torch_stream()
     _check(fn(F.handle, Xc.data_ptr(), F.N, Xc.shape[0], str(trans).encode()[:1], dt, mode))
     return Xc.t()
@@ -446,10 +490,11 @@ def rskelf_sv_(F, Xt, trans="n"):
     (nrhs, N) row view Xt: unit stride along N, rows any stride >= N (a
     slice of a larger array; the C ABI's leading dimension).  Returns Xt."""
     if (Xt.dim() != 2 or not Xt.is_cuda or Xt.shape[1] != F.N or Xt.stride(1) != 1
-            or Xt.stride(0) < F.N or Xt.dtype not in (torch.float64, torch.float32)):
+            or Xt.stride(0) < F.N or Xt.dtype not in _RDT
+            or Xt.is_complex() != getattr(F, "cplx", False)):
         raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: Xt must be an (nrhs, N) CUDA view "
-                                        "with unit stride along N")
-    dt = L.RSKX_F64 if Xt.dtype == torch.float64 else L.RSKX_F32
+                                        "with unit stride along N, complex for a complex factor")
+    dt = _RDT[Xt.dtype]
     _on_torch_stream()
     _check(L.lib().rskx_rskelf_sv(F.handle, Xt.data_ptr(), Xt.stride(0), Xt.shape[0],
                                   str(trans).encode()[:1], dt))
@@ -486,7 +531,7 @@ def kernel_matvec(A, x, X):
     x = _dev(x)
     d, N = x.shape
     xc = x.t().contiguous()
-    X = _dev(X)
+    X = _dev(X, torch.complex128 if isinstance(A, HelmholtzCFIE2D) else torch.float64)
     vec = X.dim() == 1
     X2 = X.reshape(-1, 1) if vec else X
     Xc = X2.t().contiguous()

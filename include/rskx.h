@@ -50,6 +50,8 @@ extern "C" {
 /* element type of sweep right-hand sides */
 #define RSKX_F64 0
 #define RSKX_F32 1
+#define RSKX_Z64 2            /* complex double, interleaved (re, im)        */
+#define RSKX_Z32 3            /* complex float, interleaved                  */
 
 /* built-in matrices (the reference passes a MATLAB function handle A(I,J);
  * a handle cannot run on the GPU, so the kernel is described by data) */
@@ -64,12 +66,22 @@ extern "C" {
  * the single-layer kernel of rskelf/test/ie_sphere.m:149-160 with
  * one-point quadrature (w = triangle areas) and the self-panel integral on
  * the diagonal.  d = 3; nu is not read (may be NULL). */
+#define RSKX_KERNEL_HELMHOLTZ2D_CFIE 3
+/*   exterior Dirichlet Helmholtz combined field, complex:
+ *   A_ij = w_j (dG/dn_j(x_i, x_j) - i eta G(x_i, x_j)), i != j,
+ *   G(x, y) = (i/4) H0^(1)(k |x - y|) (the kernel of
+ *   rskelf/test/ie_square3.m:44,107-110), dG/dn_y = (i k / 4) H1^(1)(k r)
+ *   ((x - y) . n_y) / r;  A_ii = diag_i, complex: diag holds 2 N doubles
+ *   (re, im interleaved).  d = 2; scale is not read.  The factor is complex
+ *   (FP32 precision: complex float storage), symm 'n' or 's'. */
 typedef struct {
   int kind;             /* RSKX_KERNEL_*                                    */
   const double* nu;     /* d x N unit normals (device)                      */
   const double* w;      /* N quadrature weights (device)                    */
-  const double* diag;   /* N diagonal entries (device)                      */
+  const double* diag;   /* N diagonal entries (device); 2 N for complex    */
   double scale;         /* off-diagonal multiplier                          */
+  double wavenumber;    /* HELMHOLTZ2D_CFIE: k > 0                          */
+  double eta;           /* HELMHOLTZ2D_CFIE: coupling of the single layer   */
 } rskx_kernel;
 
 /* proxy surfaces (the reference passes pxyfun(x,slf,nbr,l,ctr)) */
@@ -80,6 +92,13 @@ typedef struct {
                                  pxy = pts * l + ctr; rows w_j G(pxy, x_j) and,
                                  for symm 'n', weight * G(x_j, pxy) (incoming);
                                  d = 3 */
+#define RSKX_PROXY_RING_CFIE 4 /* ring for HELMHOLTZ2D_CFIE (the pxyfun pattern
+                                 of ie_square3.m:132-143): p_l = max(p,
+                                 8 ceil((1.5 k l + 48) / 8)) points at radius
+                                 box sizes (l: the box's longest side); rows
+                                 A(pxy, x_j) and, for symm 'n', the incoming
+                                 conj(weight (dG/dn_q - i eta G)(x_j, pxy)) with
+                                 the ring's normal n_q; d = 2 */
 typedef struct {
   int kind;             /* RSKX_PROXY_*                                     */
   int p;                /* number of proxy points (64 in the demos)         */

@@ -324,3 +324,177 @@ def sphere_pde_error(x, area, solve, seed=0):
     Y = (SphereSL.sl(trg, x) * area[None, :]) @ X
     Z = SphereSL.sl(trg, src) @ q
     return float(np.linalg.norm(Z - Y) / np.linalg.norm(Z))
+
+
+# --------------------------------------------------------------------------
+# 2D Helmholtz combined-field integral equation (complex)
+# --------------------------------------------------------------------------
+EULER_GAMMA = 0.57721566490153286061
+
+
+class CurveCFIE:
+    """Exterior Dirichlet Helmholtz problem on a closed 2D curve by the
+    combined-field integral equation (Brakhage-Werner): u = D[s] - i eta S[s]
+    with G(x, y) = (i/4) H0^(1)(k |x - y|), whose exterior trace gives
+        A = 1/2 I + D - i eta S,
+        A_ij = h_j (dG/dn_j(x_i, x_j) - i eta G(x_i, x_j)),  i != j,
+    dG/dn_y(x, y) = (i k / 4) H1^(1)(k r) ((x - y) . n_y) / r, on the
+    trapezoidal rule in the curve parameter t_i = 2 pi i / N, i = 1..N, with
+    arc-length weights h_j.  The kernel follows rskelf/test/ie_square3.m:44,
+    107-110 (Kfun = 0.25i besselh(0, 1, k r)); the combined layer and the
+    curves are the builder's (the reference has no curve CFIE demo).
+
+    Diagonal: 1/2 (jump) + h_i (-kappa_i / 4 pi) (the double layer's
+    continuous limit, as ie_ellipse.m:120-122 takes it) - i eta s_i with
+    s_i = h_i ((i/4) - (log(k h_i / 4 pi) + gamma) / 2 pi): the single layer's
+    smooth part at r = 0 plus the corrected-trapezoidal weight h log(h / 2 pi)
+    of its log|t| singularity (the zeta'(0) term of the Euler-Maclaurin
+    expansion for a punctured periodic trapezoidal rule; Kapur and Rokhlin,
+    SIAM J. Numer. Anal. 34 (1997)), which makes the rule converge at third
+    order in h.
+
+    kind 'circle': the unit circle (then A is circulant and complex
+    symmetric); 'star': r(t) = 1 + 0.3 cos 5t.
+
+    Proxies (pxyfun): a ring at 1.5 box sizes (as ie_circle.m:35,107) of
+    p_l = max(p, 8 ceil((1.5 k l + 48) / 8)) points, l the box's longest
+    side: a box's outgoing field on the ring has angular bandwidth about
+    k l / sqrt 2 + O(log 1/tol), so the count grows with k l.  Rows: the
+    outgoing interactions A(pxy, slf) (the matrix kernel with the proxy
+    point as target) and, for symm 'n', the incoming ones as the conjugate
+    of a combined-field source on the ring, w (dG/dn_q - i eta G)(x_j, y_q)
+    with the ring's outward normal n_q and w the mean weight (the interior
+    field of far sources is a combined-field potential on the ring for
+    every k, so no interior resonance of the ring can drop a direction);
+    neighbours kept within 1.5 box sizes (ie_circle.m:114)."""
+
+    def __init__(self, N, kind="star", k=20.0, eta=None, nproxy=64, symm="n"):
+        self.N, self.kind, self.k = N, kind, float(k)
+        self.eta = float(k if eta is None else eta)
+        self.nproxy = nproxy
+        self.two_sided = symm == "n"
+        t = np.arange(1, N + 1) * 2.0 * math.pi / N
+        if kind == "circle":
+            r, rp, rpp = np.ones(N), np.zeros(N), np.zeros(N)
+        elif kind == "star":
+            r, rp, rpp = 1 + 0.3 * np.cos(5 * t), -1.5 * np.sin(5 * t), -7.5 * np.cos(5 * t)
+        else:
+            raise ValueError(kind)
+        c, s = np.cos(t), np.sin(t)
+        self.x = np.stack([r * c, r * s])
+        xp, yp = rp * c - r * s, rp * s + r * c
+        xpp, ypp = rpp * c - 2 * rp * s - r * c, rpp * s + 2 * rp * c - r * s
+        speed = np.sqrt(xp ** 2 + yp ** 2)
+        self.nu = np.stack([yp, -xp]) / speed
+        self.kappa = (xp * ypp - yp * xpp) / speed ** 3
+        self.h = 2.0 * math.pi / N * speed
+        self.pw = float(self.h.mean())
+        h, kk = self.h, self.k
+        ssl = h * (0.25j - (np.log(kk * h / (4.0 * math.pi)) + EULER_GAMMA) / (2.0 * math.pi))
+        self.diag = 0.5 - h * self.kappa / (4.0 * math.pi) - 1j * self.eta * ssl
+
+    @classmethod
+    def from_arrays(cls, x, nu, h, diag, k, eta, pw, nproxy=64, symm="n"):
+        """The same problem on given points, normals, weights and diagonal."""
+        self = cls.__new__(cls)
+        self.x, self.nu, self.h = (np.asarray(a, dtype=float) for a in (x, nu, h))
+        self.diag = np.asarray(diag, dtype=complex)
+        self.N = self.x.shape[1]
+        self.k, self.eta, self.pw, self.nproxy = float(k), float(eta), float(pw), nproxy
+        self.two_sided = symm == "n"
+        self.kind = "given"
+        return self
+
+    # -- kernels -----------------------------------------------------------
+    def g(self, xt, ys):
+        from scipy.special import hankel1
+        dx = xt[0][:, None] - ys[0][None, :]
+        dy = xt[1][:, None] - ys[1][None, :]
+        with np.errstate(all="ignore"):
+            return 0.25j * hankel1(0, self.k * np.sqrt(dx * dx + dy * dy))
+
+    def dg(self, xt, ys, nus):
+        from scipy.special import hankel1
+        dx = xt[0][:, None] - ys[0][None, :]
+        dy = xt[1][:, None] - ys[1][None, :]
+        r = np.sqrt(dx * dx + dy * dy)
+        with np.errstate(all="ignore"):
+            return (0.25j * self.k * hankel1(1, self.k * r) *
+                    (dx * nus[0][None, :] + dy * nus[1][None, :]) / r)
+
+    def cfie(self, xt, ys, nus, ws):
+        """h_j (dG/dn_j - i eta G)(x_t, y_j): the matrix kernel."""
+        return (self.dg(xt, ys, nus) - 1j * self.eta * self.g(xt, ys)) * ws[None, :]
+
+    # -- callbacks ---------------------------------------------------------
+    def afun(self, I, J):
+        I = np.asarray(I, dtype=np.int64)
+        J = np.asarray(J, dtype=np.int64)
+        A = self.cfie(self.x[:, I], self.x[:, J], self.nu[:, J], self.h[J])
+        same = I[:, None] == J[None, :]
+        return np.where(same, self.diag[J][None, :], A)
+
+    def nproxy_at(self, l):
+        return max(self.nproxy, 8 * int(math.ceil((1.5 * self.k * float(np.max(l)) + 48) / 8)))
+
+    def proxy_ring(self, l, ctr):
+        p = self.nproxy_at(l)
+        ph = np.arange(1, p + 1) * 2.0 * math.pi / p
+        unit = 1.5 * np.stack([np.cos(ph), np.sin(ph)])
+        pxy = unit * l[:, None] + ctr[:, None]
+        nq = unit / l[:, None]                    # normal of the (possibly elliptic) ring
+        nq = nq / np.sqrt((nq ** 2).sum(axis=0))
+        return pxy, nq
+
+    def pxyfun(self, x, slf, nbr, l, ctr):
+        pxy, nq = self.proxy_ring(l, ctr)
+        Kout = self.cfie(pxy, x[:, slf], self.nu[:, slf], self.h[slf])
+        if self.two_sided:
+            Kin = self.cfie(x[:, slf], pxy, nq, np.full(pxy.shape[1], self.pw)).T.conj()
+            Kpxy = np.concatenate([Kout, Kin])
+        else:
+            Kpxy = Kout
+        keep = np.sum(((x[:, nbr] - ctr[:, None]) / l[:, None]) ** 2, axis=0) < 1.5 ** 2
+        return Kpxy, nbr[keep]
+
+    # -- exact operators ---------------------------------------------------
+    def dense(self):
+        idx = np.arange(self.N)
+        return self.afun(idx, idx)
+
+    def matvec(self, X):
+        X = np.asarray(X)
+        if self.kind == "circle":                 # circulant (ie_circle.m:47-48 pattern)
+            col = self.afun(np.arange(self.N), np.arange(1))[:, 0]
+            Y = np.fft.ifft(np.fft.fft(col)[:, None] * np.fft.fft(X.reshape(self.N, -1), axis=0),
+                            axis=0)
+            return Y.reshape(X.shape)
+        return self.dense() @ X
+
+    def circle_eigenvalue(self, m):
+        """lambda_m of the continuous exterior-trace operator on the unit
+        circle, A e^{imt} = lambda_m e^{imt}: with the addition theorem
+        G = (i/4) sum_n J_n(k|y|) H_n(k|x|) e^{in(t_x - t_y)} (|x| > |y|),
+        D e^{imt} -> (i pi k / 2) J_m'(k) H_m(k) and S e^{imt} ->
+        (i pi / 2) J_m(k) H_m(k) on the circle, so
+        lambda_m = (i pi k / 2) J_m'(k) H_m(k) + (eta pi / 2) J_m(k) H_m(k)."""
+        from scipy.special import hankel1, jv, jvp
+        k = self.k
+        return ((0.5j * math.pi * k) * jvp(m, k) * hankel1(m, k) +
+                (0.5 * self.eta * math.pi) * jv(m, k) * hankel1(m, k))
+
+
+def helmholtz_pde_error(pr, solve, x0=(0.1, 0.2), radius=2.5):
+    """Exterior Dirichlet known answer: the radiating field G(x, x0) of a
+    source inside the curve is the boundary data; the density X = solve(B)
+    gives the field at 16 points on a circle of `radius` (outside) as the
+    combined-field potential, compared with G(trg, x0) (the known-answer
+    pattern of rskelf/test/ie_circle.m:65-81 for this exterior problem)."""
+    src = np.asarray(x0, dtype=float).reshape(2, 1)
+    B = pr.g(pr.x, src)[:, 0]
+    X = np.asarray(solve(B))
+    ang = np.arange(16) * 2.0 * math.pi / 16
+    trg = radius * np.stack([np.cos(ang), np.sin(ang)])
+    Y = pr.cfie(trg, pr.x, pr.nu, pr.h) @ X
+    Z = pr.g(trg, src)[:, 0]
+    return float(np.linalg.norm(Y - Z) / np.linalg.norm(Z))

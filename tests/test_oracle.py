@@ -518,3 +518,65 @@ def test_symm_h_bunch_kaufman_on_indefinite():
     L = np.eye(N)
     L[1:, 0] = A[1:, 0] / A[0, 0]
     assert np.abs(L).max() > 1e8
+
+
+# ------------------------------------------------- Helmholtz CFIE (complex) --
+def test_cfie_circle_eigenvalues():
+    """On the unit circle the combined-field matrix is circulant and complex
+    symmetric, and e^{imt} is an eigenvector with the continuous
+    operator's eigenvalue (iπk/2) J_m'(k) H_m(k) + (ηπ/2) J_m(k) H_m(k) up to
+    the discretization error, which falls at third order in h (corrected
+    trapezoidal rule).  This pins the kernel (scipy's hankel1), the jump
+    term, the normal's direction and the sign of -i eta S."""
+    errs = []
+    for N in (256, 1024):
+        pr = problems.CurveCFIE(N, "circle", k=10.0)
+        A = pr.dense()
+        assert np.abs(A - A.T).max() < 1e-13
+        t = np.arange(1, N + 1) * 2 * np.pi / N
+        e = 0.0
+        for m in range(-25, 26):
+            v = np.exp(1j * m * t)
+            lam = pr.circle_eigenvalue(m)
+            e = max(e, np.linalg.norm(A @ v - lam * v) / (np.linalg.norm(v) * abs(lam)))
+        errs.append(e)
+    assert errs[1] < 5e-5 and errs[0] / errs[1] > 30
+    bad = problems.CurveCFIE(1024, "circle", k=10.0)
+    bad.nu = -bad.nu
+    v = np.exp(1j * 3 * np.arange(1, 1025) * 2 * np.pi / 1024)
+    lam = pr.circle_eigenvalue(3)
+    assert np.linalg.norm(bad.dense() @ v - lam * v) / (np.linalg.norm(v) * abs(lam)) > 0.1
+
+
+def test_cfie_star_pde_known_answer():
+    """Exterior known answer on the star curve r = 1 + 0.3 cos 5t: the field
+    of a point source inside, recovered outside from the solved density,
+    with third-order convergence; through a dense solve and the oracle's
+    factor (tol 1e-10)."""
+    errs = []
+    for N in (512, 2048):
+        pr = problems.CurveCFIE(N, "star", k=10.0)
+        A = pr.dense()
+        errs.append(problems.helmholtz_pde_error(pr, lambda b: np.linalg.solve(A, b)))
+    assert errs[1] < 1e-6 and errs[0] / errs[1] > 30
+    F = flam.rskelf(pr.afun, pr.x, 64, 1e-10, pr.pxyfun, dict(symm="n", order="level"))
+    assert abs(problems.helmholtz_pde_error(pr, lambda b: flam.rskelf_sv(F, b)) - errs[1]) < 1e-8
+
+
+@pytest.mark.parametrize("kind,symm", [("star", "n"), ("circle", "s")])
+def test_cfie_rskelf_vs_dense(kind, symm):
+    """Complex factorization ('n': LU after two-sided compression with
+    conjugate-transposed incoming rows; 's': one-sided compression and
+    T.' updates, rskelf.m:220-221, for the complex symmetric circle):
+    solves for trans 'n', 'c', 't' and the apply against the dense matrix."""
+    N, tol = 2048, 1e-9
+    pr = problems.CurveCFIE(N, kind, k=20.0, symm=symm)
+    F = flam.rskelf(pr.afun, pr.x, 64, tol, pr.pxyfun, dict(symm=symm, order="level"))
+    A = pr.dense()
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    for trans, M in (("n", A), ("c", A.conj().T), ("t", A.T)):
+        xe = np.linalg.solve(M, b)
+        assert np.linalg.norm(flam.rskelf_sv(F, b, trans) - xe) < 100 * tol * np.linalg.norm(xe)
+        ye = M @ b
+        assert np.linalg.norm(flam.rskelf_mv(F, b, trans) - ye) < 100 * tol * np.linalg.norm(ye)
