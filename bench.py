@@ -41,9 +41,13 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-N", type=int, default=1 << 20)
-    ap.add_argument("--workload", choices=["circle", "ellipses", "sphere"], default="circle",
+    ap.add_argument("--workload", choices=["circle", "ellipses", "sphere", "helmholtz"],
+                    default="circle",
                     help="circle: the metric's configuration (default); ellipses: 96 uneven "
-                         "ellipses in one batched factorization; sphere: 3D single layer")
+                         "ellipses in one batched factorization; sphere: 3D single layer; "
+                         "helmholtz: complex combined field on the star curve")
+    ap.add_argument("--hN", type=int, default=262144, help="helmholtz: points on the star")
+    ap.add_argument("--k", type=float, default=200.0, help="helmholtz: wavenumber")
     ap.add_argument("--nprob", type=int, default=96, help="ellipses: number of problems")
     ap.add_argument("--nface", type=int, default=20480, help="sphere: triangles")
     ap.add_argument("--exact-op", action="store_true",
@@ -64,7 +68,9 @@ def run_path(flam_amd, g, B, args, precision, steps, warmup, log):
     times, infos, F = [], [], None
     for it in range(warmup + steps + 1):
         full = it == warmup + steps
-        opts = dict(symm="p", precision=precision, profile=1 if full else 2)
+        # both legs take the same ID route (the Gram-matrix ID at tol >= 1e-7),
+        # so the ratio measures precision, not a change of algorithm
+        opts = dict(symm="p", precision=precision, profile=1 if full else 2, id_route="gram")
         if full:
             F_timed, F = F, None
         if F is not None:
@@ -91,21 +97,22 @@ def run_path(flam_amd, g, B, args, precision, steps, warmup, log):
     return times, infos, F_timed, F
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of a kernel class from the calibrated rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary kept under profiles/ (None if absent)."""
+def measured_traffic(kernel, workload="circle_N1048576_fp32"):
+    """HBM bytes per launch of a kernel class on one workload from the
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept under
+    profiles/ (None if that workload has none)."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md"))):
         with open(path) as fh:
             for line in fh:
                 if line.startswith("traffic_json:"):
                     rec = json.loads(line.split(":", 1)[1])
-                    if rec.get("kernel") == kernel:
+                    if rec.get("kernel") == kernel and rec.get("workload") == workload:
                         return rec["bytes_per_launch"], os.path.basename(path)
     return None, None
 
 
-def roofline_of(F, F_full, precision):
+def roofline_of(F, F_full, precision, workload=None):
     """Dominant kernel class by time in the fully profiled step F_full, with
     algorithmic flops / bytes; its launch times come from the last timed
     step F when that step timed it (the elimination class), else from
@@ -118,49 +125,44 @@ def roofline_of(F, F_full, precision):
     if st["flops"] > 0:
         ach = st["flops"] / st["launches"] / (avg_ms * 1e-3)
         peak = PEAK["fp64"] if name == "id" or precision == "fp64" else PEAK["fp32"]
-        traffic, src = measured_traffic(name) if precision == "fp32" else (None, None)
+        traffic, src = measured_traffic(name, workload) if workload else (None, None)
         return dict(kernel=name, bound="mfma", achieved=ach / 1e12, peak=peak / 1e12,
                     unit="TFLOP/s", frac=ach / peak, traffic=traffic, traffic_source=src,
                     avg_launch_ms=avg_ms, launches=st["launches"]), kt
     ach = st["bytes"] / st["launches"] / (avg_ms * 1e-3)
-    traffic, src = measured_traffic(name) if precision == "fp32" else (None, None)
+    traffic, src = measured_traffic(name, workload) if workload else (None, None)
     return dict(kernel=name, bound="hbm", achieved=ach / 1e9, peak=PEAK["hbm"] / 1e9,
                 unit="GB/s", frac=ach / PEAK["hbm"], traffic=traffic, traffic_source=src,
                 avg_launch_ms=avg_ms, launches=st["launches"]), kt
 
 
-def cpu_baseline(N, occ, tol, nrhs, target, log):
-    """The FP64 CPU oracle on a bounded sample (N points, one thread)."""
+def cpu_baseline(N, occ, tol, nrhs, target, log, runs=3):
+    """The FP64 CPU oracle on the host's cores: the restated factorization,
+    then the same flexible GMRES (restated in NumPy) to the same target on
+    the exact FFT product; median of `runs` runs."""
     import numpy as np
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:
-        threadpool_limits = None
     from oracle import flam as ofl, problems as opr
-    ctx = threadpool_limits(1) if threadpool_limits else None
-    if ctx:
-        ctx.__enter__()
     try:
-        pr = opr.CurveDL(N, "circle")
-        B = np.random.default_rng(0).standard_normal((N, nrhs))
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except ImportError:
+        threads = os.cpu_count() or 1
+    pr = opr.CurveDL(N, "circle")
+    B = np.random.default_rng(0).standard_normal((N, nrhs))
+    times = []
+    for _ in range(runs):
         t0 = time.perf_counter()
         Fo = ofl.rskelf(pr.afun, pr.x, occ, tol, pr.pxyfun, dict(symm="p", order="level"))
-        X = ofl.rskelf_sv(Fo, B)
-        for _ in range(4):                      # same refinement rule, FP64 matvec
-            R = B - pr.matvec(X)
-            rel = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
-            if rel.max() <= target:
-                break
-            X = X + ofl.rskelf_sv(Fo, R)
-        dt = time.perf_counter() - t0
-    finally:
-        if ctx:
-            ctx.__exit__(None, None, None)
-    log(f"cpu oracle N={N}: {dt:.2f} s, relres {rel.max():.2e}")
-    return dict(value=N / dt, unit="points/s", cores=1, kind="port",
-                sample=f"circle N={N}, occ={occ}, tol={tol:g}, symm p, {nrhs} RHS, "
-                       f"FP64 NumPy/SciPy restatement of rskelf (level order), one thread, "
-                       f"{dt:.1f} s")
+        X, iters, rel = ofl.fgmres(pr.matvec, B, lambda V: ofl.rskelf_sv(Fo, V), tol=target)
+        times.append(time.perf_counter() - t0)
+        log(f"cpu oracle N={N}: {times[-1]:.2f} s, fgmres iters {iters}, relres {rel.max():.2e}")
+        del Fo
+    dt = float(np.median(times))
+    return dict(value=N / dt, unit="points/s", cores=int(threads), kind="port",
+                sample=f"circle N={N}, occ={occ}, tol={tol:g}, symm p, {nrhs} RHS, FP64 "
+                       f"NumPy/SciPy restatement of rskelf (level order) + the same FGMRES to "
+                       f"{target:g}, BLAS threads {threads} of {os.cpu_count()} host cores, "
+                       f"median of {runs} runs {dt:.1f} s")
 
 
 def other_workload(args, log):
@@ -200,6 +202,16 @@ def other_workload(args, log):
         desc = (f"{args.nprob} ellipses (a=1, b~U[0.3,1], N log-uniform [2048,131072], seed 0), "
                 f"N total {N}, one batched factorization, occ={occ}, tol={tol:g}, symm n, "
                 f"{args.nrhs} RHS, FP32 factor + FP64 FGMRES to {target:g} per problem")
+    elif args.workload == "helmholtz":
+        g = geometry.helmholtz_curve(args.hN, "star", k=args.k)
+        N, occ, tol, target, symm = g.N, 256, 1e-6, 1e-8, "n"
+        segs = None
+        factor = lambda o: flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, o)
+        mv = g.matvec
+        exact = mv
+        desc = (f"2D Helmholtz combined field (eta = k) on the star r = 1 + 0.3 cos 5t, N={N}, "
+                f"k={args.k:g}, occ={occ}, tol={tol:g}, symm n, {args.nrhs} complex RHS, complex64 "
+                f"factor + complex FP64 FGMRES (exact on-the-fly product) to {target:g}")
     else:
         g = geometry.sphere(args.nface)
         N, occ, tol, target, symm = g.N, 256, 1e-5, 1e-9, "n"
@@ -212,7 +224,10 @@ def other_workload(args, log):
     if args.workload == "ellipses" and not args.exact_op:
         desc += "; FGMRES on a tol-1e-12 FP64 factor's rskelf_mv, residuals with the exact product"
     log(f"workload: {desc}")
-    B = torch.from_numpy(np.random.default_rng(0).standard_normal((N, args.nrhs))).cuda()
+    rng = np.random.default_rng(0)
+    B = torch.from_numpy(rng.standard_normal((N, args.nrhs))).cuda()
+    if args.workload == "helmholtz":
+        B = torch.complex(B, torch.from_numpy(rng.standard_normal((N, args.nrhs))).cuda())
 
     def exact_relres(X):
         """max over problems and columns of ||B - A X|| / ||B|| with the
@@ -243,7 +258,7 @@ def other_workload(args, log):
                 f"(exact product {rel:.2e})")
             if args.warmup <= it < args.warmup + args.steps:
                 times.append((t2 - t0, 1e3 * (t1 - t0), 1e3 * (t2 - t1), info["iters"], rel))
-        roof, kt = roofline_of(F_timed, F, prec)
+        roof, kt = roofline_of(F_timed, F, prec, None)
         res[prec] = dict(value=N / float(np.mean([t[0] for t in times])),
                          ms_per_step=1e3 * float(np.mean([t[0] for t in times])),
                          factor_ms=float(np.mean([t[1] for t in times])),
@@ -252,10 +267,11 @@ def other_workload(args, log):
                          levels=F.level_table(), roofline=roof,
                          kernels={k: dict(ms=v["ms"], launches=v["launches"]) for k, v in kt.items()})
         del F, F_timed
+    dtype = "c64+c128" if args.workload == "helmholtz" else "f32+f64"
     out = {"metric": f"{args.workload} factor+solve points/sec at the target residual",
            "value": res["fp32"]["value"], "unit": "points/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_step"],
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
            "data": "synthetic", "config": {"workload": desc, "N": N},
            "mixed": res["fp32"], "plain_fp64": res["fp64"],
            "mixed_over_plain": res["fp32"]["value"] / res["fp64"]["value"]}
@@ -301,7 +317,7 @@ def main():
 
     t_mix, i_mix, F_mix, Ff_mix = run_path(flam_amd, g, B, args, "fp32", args.steps, args.warmup,
                                            log)
-    roof, kt = roofline_of(F_mix, Ff_mix, "fp32")
+    roof, kt = roofline_of(F_mix, Ff_mix, "fp32", f"circle_N{N}_fp32")
     del Ff_mix
     info_mix = F_mix.info()
     table = F_mix.level_table()
@@ -309,7 +325,7 @@ def main():
     if not args.no_plain:
         t_pl, i_pl, F_pl, Ff_pl = run_path(flam_amd, g, B, args, "fp64", args.steps, args.warmup,
                                            log)
-        roof64, kt64 = roofline_of(F_pl, Ff_pl, "fp64")
+        roof64, kt64 = roofline_of(F_pl, Ff_pl, "fp64", f"circle_N{N}_fp64")
         del Ff_pl
         plain = dict(value=N / float(np.mean(t_pl)), ms_per_step=1e3 * float(np.mean(t_pl)),
                      steps=i_pl, roofline=roof64,

@@ -45,13 +45,14 @@ class Proxy(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("lvlmax", c_int), ("ext", POINTER(c_double)), ("Tmax", c_double),
                 ("rrqr_iter", c_double), ("symm", c_char), ("verb", c_int),
-                ("precision", c_int), ("profile", c_int), ("stop", c_int)]
+                ("precision", c_int), ("profile", c_int), ("stop", c_int), ("id_gram", c_int),
+                ("rank_model", c_void_p), ("rank_scale", c_double), ("record_boxes", c_int)]
 
 
 class Info(ctypes.Structure):
     _fields_ = [("N", c_int64), ("d", c_int), ("nlvl", c_int), ("nlvl_done", c_int),
                 ("nfactor", c_int64), ("bytes", c_int64), ("precision", c_int),
-                ("symm", c_char)]
+                ("symm", c_char), ("id_scratch", c_int64), ("rank_fallbacks", c_int64)]
 
 
 class FactorView(ctypes.Structure):
@@ -90,6 +91,16 @@ SIGNATURES = {
                                   POINTER(c_int64), POINTER(c_int64), POINTER(c_double), c_int]),
     "rskx_kernel_matvec": (c_int, [c_void_p, c_int, c_int64, POINTER(Kernel), c_void_p, c_int64,
                                    c_void_p, c_int64, c_int]),
+    "rskx_rank_model_create": (c_int, [ctypes.c_uint64, POINTER(c_void_p)]),
+    "rskx_rank_model_free": (None, [c_void_p]),
+    "rskx_rank_model_nparams": (c_int, []),
+    "rskx_rank_model_params": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "rskx_rank_model_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                     POINTER(c_double)]),
+    "rskx_rank_model_train": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int64,
+                                      c_double, POINTER(c_double)]),
+    "rskx_rank_model_predict": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "rskx_box_features": (c_int, [c_void_p, POINTER(c_int64), c_void_p, c_void_p]),
     "rskx_cgs2": (c_int, [c_void_p, c_int64, c_int64, c_int, c_void_p, c_int64, c_int64, c_int,
                           c_void_p]),
     "rskx_kernel_matvec_batch": (c_int, [c_int, POINTER(c_int64), c_void_p, c_int,

@@ -107,6 +107,17 @@ size_t zsweep_shmem(int kmax, int rmax, int nc);
 void launch_zmatvec(const double*, int64_t, const double*, const double*, const double*, double,
                     double, const double*, int64_t, double*, int64_t, int, hipStream_t);
 
+// rank model (mlp_kernels.hip)
+int mlp_nparams();
+int mlp_grad_blocks(int64_t n);
+void launch_mlp_forward(const double*, const double*, int64_t, double*, hipStream_t);
+void launch_mlp_grad(const double*, const double*, const double*, int64_t, double*, double*,
+                     double*, double*, hipStream_t);
+void launch_mlp_adam(double*, const double*, double*, double*, double, double, double, double, int,
+                     hipStream_t);
+void launch_box_moments(const int64_t*, const int64_t*, int, const double*, const double*,
+                        const double*, double*, hipStream_t);
+
 // proxies on the ring of a combined-field box of longest side l: the ring
 // holds the box's outgoing field, whose angular bandwidth is about
 // k l / sqrt 2 plus a margin for the tolerance, so the count grows with k l
@@ -442,6 +453,15 @@ struct Prof {
 
 using namespace rskx;
 
+// -------------------------------------------------------- rank model --
+struct rskx_rank_model {
+  DevBuf params, m, v, g, part, lpart, loss;
+  int step = 0;
+  ~rskx_rank_model() {
+    for (DevBuf* b : {&params, &m, &v, &g, &part, &lpart, &loss}) b->release();
+  }
+};
+
 // ---------------------------------------------------------- the factor --
 struct rskx_factor {
   int64_t N = 0;
@@ -466,6 +486,11 @@ struct rskx_factor {
   // original indices) and the sparse modification of their system
   std::vector<int64_t> Si, SI, SJ;
   std::vector<double> SV;
+  // rank model: features of every box (8 each) and the exact ranks of the
+  // boxes the factorization compressed (-1: none), when recorded
+  std::vector<double> boxfeat;
+  std::vector<int64_t> boxrank;
+  int64_t id_scratch = 0, rank_fallbacks = 0;
   int level_of(int64_t f) const {         // processed level holding factor f
     return (int)(std::upper_bound(lvp.begin(), lvp.end(), f) - lvp.begin()) - 1;
   }
@@ -694,6 +719,59 @@ int partial_assembly(const HostTree& t, const std::vector<BoxState>& bs, int lv_
   return RSKX_OK;
 }
 
+// Features of every box for the rank model (8 per box, see rskx.h): the
+// point moments of the leaves on the device, summed up the tree on the host.
+int box_features(const HostTree& t, const PointsDev& P, int d, double tol, double kwave,
+                 std::vector<double>& feat) {
+  const int nbox = t.nbox();
+  std::vector<int64_t> lb, lc;
+  std::vector<int> leaves;
+  for (int b = 0; b < nbox; ++b)
+    if (t.leaf(b) && t.cnt[b] > 0) { leaves.push_back(b); lb.push_back(t.beg[b]); lc.push_back(t.cnt[b]); }
+  static DevBuf dbeg, dcnt, dmom;
+  RSKX_TRY(upload(dbeg, lb));
+  RSKX_TRY(upload(dcnt, lc));
+  RSKX_TRY(dmom.ensure(std::max<size_t>(leaves.size(), 1) * 5 * sizeof(double)));
+  launch_box_moments(dbeg.as<int64_t>(), dcnt.as<int64_t>(), (int)leaves.size(), P.x[0], P.x[1],
+                     d > 2 ? P.x[2] : nullptr, dmom.as<double>(), g_stream);
+  std::vector<double> lm(leaves.size() * 5), mom((size_t)nbox * 5, 0.0);
+  RSKX_TRY(download(lm.data(), dmom.p, lm.size() * sizeof(double)));
+  RSKX_TRY(sync_stream());
+  for (size_t i = 0; i < leaves.size(); ++i)
+    for (int j = 0; j < 5; ++j) mom[5 * (size_t)leaves[i] + j] = lm[5 * i + j];
+  for (int b = nbox - 1; b > 0; --b)          // children follow their parents
+    if (t.prnt[b] >= 0)
+      for (int j = 0; j < 5; ++j) mom[5 * (size_t)t.prnt[b] + j] += mom[5 * (size_t)b + j];
+  feat.assign((size_t)nbox * 8, 0.0);
+  for (int b = 0; b < nbox; ++b) {
+    const double* q = &mom[5 * (size_t)b];
+    double lmax = 0, lmin = INFINITY, c2 = 0;
+    for (int i = 0; i < d; ++i) {
+      const double li = t.bsize[(size_t)d * b + i];
+      lmax = std::max(lmax, li);
+      lmin = std::min(lmin, li);
+    }
+    const double cntb = std::max(q[0], 1.0);
+    double mean[3] = {q[1] / cntb, q[2] / cntb, q[3] / cntb}, m2 = 0;
+    for (int i = 0; i < 3; ++i) m2 += mean[i] * mean[i];
+    for (int i = 0; i < d; ++i) {
+      const double dc = mean[i] - t.ctr[(size_t)d * b + i];
+      c2 += dc * dc;
+    }
+    const double var = std::max(q[4] / cntb - m2, 0.0), L = lmax > 0 ? lmax : 1.0;
+    double* f = &feat[8 * (size_t)b];
+    f[0] = (t.level_of(b) + 1) / 16.0;
+    f[1] = std::log2(cntb) / 20.0;
+    f[2] = std::log2(L) / 8.0;
+    f[3] = lmax > 0 ? lmin / lmax : 1.0;
+    f[4] = std::sqrt(var) / L;
+    f[5] = std::sqrt(c2) / L;
+    f[6] = tol > 0 ? -std::log10(tol) / 16.0 : 1.0;
+    f[7] = std::log2(1.0 + kwave * L) / 10.0;
+  }
+  return RSKX_OK;
+}
+
 template <typename S>
 int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* poff, int occ,
                 double rank_or_tol, const rskx_kernel& A, const rskx_proxy& PX,
@@ -741,6 +819,32 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     P.eta = A.eta;
   }
   RSKX_HIP(hipGetLastError());
+
+  // ---- rank model: features of every box, predicted caps of the ranks
+  const double tol_id = std::fmod(rank_or_tol, 1.0);
+  std::vector<int> kcap;                 // per box (0: no cap)
+  if (o.rank_model || o.record_boxes) {
+    RSKX_TRY(box_features(t, P, d, tol_id, A.kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE ? A.wavenumber : 0.0,
+                          F->boxfeat));
+    F->boxrank.assign(t.nbox(), -1);
+  }
+  if (o.rank_model) {
+    static DevBuf dX, dy;
+    const int nb_all = t.nbox();
+    RSKX_TRY(upload(dX, F->boxfeat));
+    RSKX_TRY(dy.ensure(std::max(nb_all, 1) * sizeof(double)));
+    launch_mlp_forward(o.rank_model->params.as<double>(), dX.as<double>(), nb_all, dy.as<double>(),
+                       g_stream);
+    std::vector<double> yh(nb_all);
+    RSKX_TRY(download(yh.data(), dy.p, nb_all * sizeof(double)));
+    RSKX_TRY(sync_stream());
+    const double sc = o.rank_scale > 0 ? o.rank_scale : 1.0;
+    kcap.assign(nb_all, 0);
+    for (int b = 0; b < nb_all; ++b) {
+      const double kh = std::exp2(10.0 * std::min(yh[b], 3.0)) - 1.0;   // predicted rank
+      kcap[b] = std::max(1, (int)std::ceil(sc * (1.25 * std::max(kh, 0.0) + 4.0)));
+    }
+  }
 
   // ---- pool of point lists: leaves own their tree-order ranges
   int nbox = t.nbox();
@@ -872,6 +976,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       B.nbr_off = off_nbr;             off_nbr += mc;
       B.K_off = off_K;                 off_K += (int64_t)B.mcap * n;
       B.perm_off = off_perm;           off_perm += n;
+      B.ldt = n;
       B.T_off = off_T;                 off_T += (CPLX ? 6 : 3) * (int64_t)n * n;
       // elimination scratch of the global class: 2 n^2 elements, then a
       // 6 n double point cache (12 n elements keep the next box 8-byte aligned)
@@ -896,7 +1001,9 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     RSKX_TRY(ws.iter.ensure(nb * sizeof(int)));
     RSKX_TRY(ws.K.ensure(std::max<int64_t>(off_K, 1) * sizeof(S)));
     RSKX_TRY(ws.perm.ensure(std::max<int64_t>(off_perm, 1) * sizeof(int)));
-    RSKX_TRY(ws.T.ensure(std::max<int64_t>(off_T, 1) * sizeof(double)));
+    // with a rank model the real ID's buffers are laid out after routing
+    const bool sized_by_model = !kcap.empty() && !CPLX;
+    if (!sized_by_model) RSKX_TRY(ws.T.ensure(std::max<int64_t>(off_T, 1) * sizeof(double)));
     RSKX_TRY(ws.W.ensure(std::max<int64_t>(off_W, 1) * sizeof(S)));
     hlap("h:segments");
 
@@ -905,7 +1012,9 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     launch_lists(ws.boxes.as<LvlBox>(), ws.nsegs.as<Seg>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
                  ws.nbr.as<int>(), ws.mnbr.as<int>(), P, L, (int)nb, g_stream);
     prof.end("lists", ev);
-    const bool gram = !CPLX && sizeof(S) == 4 && prm.tol >= 1e-7;
+    std::vector<int> hmn(nb);                  // neighbour rows the routing sized K for
+    const bool gram = !CPLX && prm.tol >= 1e-7 &&
+                      (o.id_gram > 0 || (o.id_gram == 0 && sizeof(S) == 4));
     if constexpr (CPLX) {
       // complex: every box assembles K in global memory and runs the
       // complex QRCP ID (k_zid)
@@ -946,7 +1055,6 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // whose shared memory is sized by the unfiltered neighbour count, so
     // the host need not wait for k_lists; the QR routes read back the
     // filtered counts to choose register classes
-    std::vector<int> hmn(nb);
     bool exact = !gram;
     for (int64_t ib = 0; ib < nb && !exact; ++ib) {
       const int n = hb[ib].n, mc = hb[ib].mc;
@@ -995,6 +1103,23 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         mcapmax_g = std::max(mcapmax_g, hb[ib].mcap);
       }
     }
+    if (sized_by_model) {
+      // the Gram ID keeps only its rank cap of R rows: size every Gram
+      // box's R / T buffers (3 ldt n doubles) by the predicted cap
+      std::vector<char> isg(nb, 0);
+      for (int c = 0; c < kGramClasses; ++c)
+        for (int ib : idq[c]) isg[ib] = 1;
+      off_T = 0;
+      for (int64_t ib = 0; ib < nb; ++ib) {
+        LvlBox& B = hb[ib];
+        B.ldt = isg[ib] ? std::min(B.n, kcap[b0 + ib]) : B.n;
+        B.T_off = off_T;
+        off_T += 3 * (int64_t)B.ldt * B.n;
+      }
+      RSKX_TRY(ws.T.ensure(std::max<int64_t>(off_T, 1) * sizeof(double)));
+      RSKX_TRY(upload(ws.boxes, hb));
+    }
+    F->id_scratch = std::max<int64_t>(F->id_scratch, off_T * (int64_t)sizeof(double));
     idall = idg;
     idall.insert(idall.end(), idr[0].begin(), idr[0].end());
     idall.insert(idall.end(), idr[1].begin(), idr[1].end());
@@ -1084,6 +1209,56 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     hlap("h:wait");
     RSKX_TRY(elim_status());                  // the previous level's elimination
     prof.flush();
+    if constexpr (!CPLX) {
+      // boxes whose rank exceeded their predicted cap (kout = cap + 1): the
+      // plain Gram ID again with full buffers
+      std::vector<int> redo;
+      for (int64_t ib = 0; ib < nb; ++ib)
+        if (hb[ib].ldt < hb[ib].n && hk[ib] > hb[ib].ldt) redo.push_back((int)ib);
+      if (!redo.empty()) {
+        F->rank_fallbacks += (int64_t)redo.size();
+        for (int ib : redo) {
+          hb[ib].ldt = hb[ib].n;
+          hb[ib].T_off = off_T;
+          off_T += 3 * (int64_t)hb[ib].n * hb[ib].n;
+        }
+        RSKX_TRY(ws.T.ensure(std::max<int64_t>(off_T, 1) * sizeof(double), true));
+        RSKX_TRY(upload(ws.boxes, hb));
+        std::vector<int> rid[kGramClasses];
+        size_t rsh[kGramClasses] = {0, 0, 0};
+        int nmx = 0;
+        for (int ib : redo) {
+          const int n = hb[ib].n, c = gram_id_class(n);
+          const int mb = hmn[ib] * (two ? 2 : 1) + L.p;
+          rid[c].push_back(ib);
+          rsh[c] = std::max(rsh[c], gram_id_shmem(c, n, hmn[ib], mb, L.p));
+          nmx = std::max(nmx, n);
+        }
+        std::vector<int> rall;
+        for (int c = 0; c < kGramClasses; ++c) rall.insert(rall.end(), rid[c].begin(), rid[c].end());
+        RSKX_TRY(upload(ws.ids, rall));
+        int off = 0;
+        for (int c = 0; c < kGramClasses; ++c) {
+          if (rid[c].empty()) continue;
+          launch_gram_id(c, ws.ids.as<int>() + off, ws.boxes.as<LvlBox>(), ws.pool.as<int>(),
+                         ws.nbr.as<int>(), ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.T.as<double>(),
+                         ws.perm.as<int>(), P, L, prm, ws.kout.as<int>(), (int)rid[c].size(), rsh[c],
+                         g_stream);
+          off += (int)rid[c].size();
+        }
+        launch_id<S>(2, ws.ids.as<int>(), ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
+                     ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
+                     (int)rall.size(), (size_t)(3 * nmx + 32) * sizeof(double), nullptr, 0, g_stream);
+        RSKX_HIP(hipGetLastError());
+        RSKX_TRY(download(hk.data(), ws.kout.p, nb * sizeof(int)));
+        RSKX_TRY(download(hit.data(), ws.iter.p, nb * sizeof(int)));
+        RSKX_TRY(sync_stream());
+        F->id_scratch = std::max<int64_t>(F->id_scratch, off_T * (int64_t)sizeof(double));
+      }
+    }
+    if (!F->boxrank.empty())
+      for (int64_t ib = 0; ib < nb; ++ib)
+        if (hb[ib].n > 0) F->boxrank[b0 + ib] = hk[ib];
     for (int64_t ib = 0; ib < nb; ++ib)
       if (hit[ib] < 0) {
         fprintf(stderr, "Warning: FLAM:id:maxIterCount: Maximum number of RRQR iterations reached.\n");
@@ -1630,6 +1805,10 @@ int rskx_opts_default(rskx_opts* o) {
   o->precision = RSKX_PREC_FP64;
   o->profile = 0;
   o->stop = -1;
+  o->id_gram = 0;
+  o->rank_model = nullptr;
+  o->rank_scale = 1.0;
+  o->record_boxes = 0;
   return RSKX_OK;
 }
 
@@ -1909,6 +2088,8 @@ int rskx_factor_info(const rskx_factor* F, rskx_info* info) {
   info->bytes = F->fval_used * es + F->fidx_used * 4 + (int64_t)F->recs.size() * sizeof(FacRec) + F->N * 4;
   info->precision = F->prec;
   info->symm = F->symm;
+  info->id_scratch = F->id_scratch;
+  info->rank_fallbacks = F->rank_fallbacks;
   return RSKX_OK;
 }
 
@@ -2060,6 +2241,132 @@ int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, 
     launch_matvec(A->kind, x, N, A->nu, A->w, A->diag, A->scale, X, ldx, Y, ldy, nrhs, nullptr, 1,
                   g_stream);
   RSKX_HIP(hipGetLastError());
+  return RSKX_OK;
+}
+
+int rskx_rank_model_nparams(void) { return mlp_nparams(); }
+
+int rskx_rank_model_create(uint64_t seed, rskx_rank_model** out) {
+  if (!out) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: output handle is NULL");
+  *out = nullptr;
+  rskx_rank_model* M = new rskx_rank_model();
+  const int np = mlp_nparams();
+  // Glorot-uniform weights (seeded), zero biases, in the parameter order
+  // W1 (32 x 8), b1, W2 (32 x 32), b2, w3 (32), b3
+  std::vector<double> p(np, 0.0);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto unif = [&]() {                       // splitmix64 -> [0, 1)
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const int F = RSKX_RANK_FEATURES, H = 32;
+  auto fill = [&](int off, int cnt, int fin, int fout) {
+    const double a = std::sqrt(6.0 / (fin + fout));
+    for (int i = 0; i < cnt; ++i) p[off + i] = a * (2.0 * unif() - 1.0);
+  };
+  fill(0, H * F, F, H);
+  fill(H * F + H, H * H, H, H);
+  fill(H * F + 2 * H + H * H, H, H, 1);
+  int rc = M->params.ensure(np * sizeof(double));
+  if (rc == RSKX_OK) rc = M->m.ensure(np * sizeof(double));
+  if (rc == RSKX_OK) rc = M->v.ensure(np * sizeof(double));
+  if (rc == RSKX_OK) rc = M->g.ensure(np * sizeof(double));
+  if (rc == RSKX_OK) rc = M->loss.ensure(sizeof(double));
+  if (rc != RSKX_OK) { delete M; return rc; }
+  std::vector<double> z(np, 0.0);
+  if (hipMemcpy(M->params.p, p.data(), np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(M->m.p, z.data(), np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(M->v.p, z.data(), np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    delete M;
+    return fail(RSKX_ERR_HIP, "FLAM:rskx:hip: rank model upload failed");
+  }
+  *out = M;
+  return RSKX_OK;
+}
+
+void rskx_rank_model_free(rskx_rank_model* M) {
+  if (M) (void)hipStreamSynchronize(g_stream);
+  delete M;
+}
+
+int rskx_rank_model_params(rskx_rank_model* M, double* out, const double* in) {
+  if (!M) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: model is NULL");
+  const size_t bytes = mlp_nparams() * sizeof(double);
+  RSKX_TRY(sync_stream());
+  if (out) RSKX_HIP(hipMemcpy(out, M->params.p, bytes, hipMemcpyDeviceToHost));
+  if (in) {
+    RSKX_HIP(hipMemcpy(M->params.p, in, bytes, hipMemcpyHostToDevice));
+    RSKX_HIP(hipMemset(M->m.p, 0, bytes));
+    RSKX_HIP(hipMemset(M->v.p, 0, bytes));
+    M->step = 0;
+  }
+  return RSKX_OK;
+}
+
+static int model_grad(rskx_rank_model* M, const double* X, const double* t, int64_t n, double* g) {
+  const int nblk = mlp_grad_blocks(n);
+  RSKX_TRY(M->part.ensure((size_t)nblk * mlp_nparams() * sizeof(double)));
+  RSKX_TRY(M->lpart.ensure((size_t)nblk * sizeof(double)));
+  launch_mlp_grad(M->params.as<double>(), X, t, n, M->part.as<double>(), M->lpart.as<double>(), g,
+                  M->loss.as<double>(), g_stream);
+  RSKX_HIP(hipGetLastError());
+  return RSKX_OK;
+}
+
+int rskx_rank_model_grad(rskx_rank_model* M, const double* X, const double* t, int64_t n, double* g,
+                         double* loss) {
+  if (!M || !X || !t || !g) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  if (n < 1) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: no samples");
+  RSKX_TRY(model_grad(M, X, t, n, g));
+  if (loss) {
+    RSKX_TRY(download(loss, M->loss.p, sizeof(double)));
+    RSKX_TRY(sync_stream());
+  }
+  return RSKX_OK;
+}
+
+int rskx_rank_model_train(rskx_rank_model* M, const double* X, const double* t, int64_t n,
+                          int epochs, int64_t batch, double lr, double* loss) {
+  if (!M || !X || !t) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  if (n < 1 || epochs < 0 || batch < 1 || !(lr > 0))
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad training parameters");
+  for (int e = 0; e < epochs; ++e)
+    for (int64_t b0 = 0; b0 < n; b0 += batch) {
+      const int64_t nb = std::min(batch, n - b0);
+      RSKX_TRY(model_grad(M, X + b0 * RSKX_RANK_FEATURES, t + b0, nb, M->g.as<double>()));
+      launch_mlp_adam(M->params.as<double>(), M->g.as<double>(), M->m.as<double>(),
+                      M->v.as<double>(), lr, 0.9, 0.999, 1e-8, ++M->step, g_stream);
+    }
+  RSKX_HIP(hipGetLastError());
+  if (loss) {
+    RSKX_TRY(model_grad(M, X, t, n, M->g.as<double>()));
+    RSKX_TRY(download(loss, M->loss.p, sizeof(double)));
+  }
+  RSKX_TRY(sync_stream());
+  return RSKX_OK;
+}
+
+int rskx_rank_model_predict(const rskx_rank_model* M, const double* X, int64_t n, double* y) {
+  if (!M || !X || !y) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  launch_mlp_forward(M->params.as<double>(), X, n, y, g_stream);
+  RSKX_HIP(hipGetLastError());
+  return RSKX_OK;
+}
+
+int rskx_box_features(const rskx_factor* F, int64_t* nbox, double* X, int64_t* rank) {
+  if (!F || !nbox) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  int64_t c = 0;
+  for (size_t b = 0; b < F->boxrank.size(); ++b) {
+    if (F->boxrank[b] < 0) continue;
+    if (X) std::copy(&F->boxfeat[8 * b], &F->boxfeat[8 * b] + 8, X + 8 * c);
+    if (rank) rank[c] = F->boxrank[b];
+    ++c;
+  }
+  *nbox = c;
   return RSKX_OK;
 }
 

@@ -55,12 +55,13 @@ struct LvlBox {
   int mcap;               // K row capacity (leading dimension of K)
   int nseg, seg0;         // neighbour candidate segments
   int ncs, cs0;           // child segments forming slf (own points for a leaf)
-  int pad0;
+  int ldt;                // rows (leading dimension) of the box's T / R scratch:
+                          // n, or a rank cap when a rank model sized it
   int64_t slf_off;        // pool offset of slf
   int64_t nbr_off;        // offset in the neighbour list arena
   int64_t K_off;          // offset in the K arena
   int64_t perm_off;       // offset in the ID permutation arena (n)
-  int64_t T_off;          // offset in the double T/R scratch (2 n^2)
+  int64_t T_off;          // offset in the double T/R scratch (3 ldt n)
   int64_t W_off;          // offset in the factor scratch (n^2)
   double ctr[3];
   double l[3];            // box size (its tree's size at this level)

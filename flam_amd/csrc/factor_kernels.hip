@@ -409,24 +409,31 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
   }
   }
 
-  // T = R11 \ R12 in double, leading dimension n (id.m:114)
+  // T = R11 \ R12 in double (id.m:114); T, R and the work matrix X are
+  // slices of ldt x n with leading dimension ldt = B.ldt (n, or the rank
+  // cap a rank model sized the box's buffers with)
+  const int ldt = B.ldt;
   double* T = Tar + B.T_off;
-  double* R = T + (int64_t)n * n;
-  double* X = R + (int64_t)n * n;
+  double* R = T + (int64_t)ldt * n;
+  double* X = R + (int64_t)ldt * n;
   // leading rows of R by position
   auto rv = [&](int i, int j) -> double {
-    return MODE == 2 ? X[i + (int64_t)j * n] : (double)K[i + (int64_t)j * ld];
+    return MODE == 2 ? X[i + (int64_t)j * ldt] : (double)K[i + (int64_t)j * ld];
   };
   if (!QR) {
     k = kout[b];
+    if (k > ldt) {                  // the ID ran past the buffer's rank cap: redone later
+      if (tid == 0) iterout[b] = 0;
+      return;
+    }
     tol_abs = k > 0 ? fabs(rv(0, 0)) * prm.tol : 0.0;
   }
   const int r0 = n - k;
   for (int c = tid; c < r0; c += nt) {
     for (int l = k - 1; l >= 0; --l) {
       double s = rv(l, k + c);
-      for (int q = l + 1; q < k; ++q) s -= rv(l, q) * T[q + (int64_t)c * n];
-      T[l + (int64_t)c * n] = s / rv(l, l);
+      for (int q = l + 1; q < k; ++q) s -= rv(l, q) * T[q + (int64_t)c * ldt];
+      T[l + (int64_t)c * ldt] = s / rv(l, l);
     }
   }
   int iter = 0;
@@ -437,7 +444,7 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
     __syncthreads();
     for (int e = tid; e < k * k; e += nt) {
       int i = e % k, j = e / k;
-      R[i + (int64_t)j * n] = i <= j ? rv(i, j) : 0.0;
+      R[i + (int64_t)j * ldt] = i <= j ? rv(i, j) : 0.0;
     }
     __syncthreads();
     const double f2 = prm.Tmax * prm.Tmax;
@@ -454,18 +461,18 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
       int bi = k * r;
       for (int e = tid; e < k * r; e += nt) {
         int i = e % k, c = e / k;
-        double a = T[i + (int64_t)c * n];
+        double a = T[i + (int64_t)c * ldt];
         argmax_merge(bv, bi, a * a, e);
       }
       block_argmax(bv, bi, red);
       if (bv <= f2) { conv = true; break; }
       ++iter;
       const int i = bi % k, j = bi / k;
-      if (i < k - 1) swap_skel(perm, T, n, r, R, n, k, i, k - 1);
+      if (i < k - 1) swap_skel(perm, T, ldt, r, R, ldt, k, i, k - 1);
       if (j > 0) {
         if (tid == 0) { int q = perm[k]; perm[k] = perm[k + j]; perm[k + j] = q; }
         for (int l = tid; l < k; l += nt) {
-          double a = T[l]; T[l] = T[l + (int64_t)j * n]; T[l + (int64_t)j * n] = a;
+          double a = T[l]; T[l] = T[l + (int64_t)j * ldt]; T[l + (int64_t)j * ldt] = a;
         }
         __syncthreads();
       }
@@ -474,14 +481,14 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
       // new last skeleton column in R-space: R * t
       for (int l = tid; l < k; l += nt) {
         double s = 0;
-        for (int q = 0; q < k; ++q) s += R[l + (int64_t)q * n] * t[q];
+        for (int q = 0; q < k; ++q) s += R[l + (int64_t)q * ldt] * t[q];
         u[l] = s;
       }
       __syncthreads();
       // rank-one update of T for the exchange of skeleton k-1 and redundant 0
       const double g = 1.0 / t[k - 1];
       for (int c = tid; c < r; c += nt) {
-        double* Tc = T + (int64_t)c * n;
+        double* Tc = T + (int64_t)c * ldt;
         if (c == 0) {
           for (int l = 0; l < k - 1; ++l) Tc[l] = -t[l] * g;
           Tc[k - 1] = g;
@@ -491,43 +498,43 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
           Tc[k - 1] = rho;
         }
       }
-      for (int l = tid; l < k; l += nt) R[l + (int64_t)(k - 1) * n] = u[l];
+      for (int l = tid; l < k; l += nt) R[l + (int64_t)(k - 1) * ldt] = u[l];
       if (tid == 0) { int q = perm[k - 1]; perm[k - 1] = perm[k]; perm[k] = q; }
       __syncthreads();
-      small_qr(R, k, n, v, red);
+      small_qr(R, k, ldt, v, red);
       // id.m:210-229: drop a skeleton within tol of the others' span
       double* r2 = u;
-      inv_row_norms(R, k, n, X, n, r2);
+      inv_row_norms(R, k, ldt, X, ldt, r2);
       double best = -1.0;
       int ib = k;
       for (int l = tid; l < k; l += nt) argmax_merge(best, ib, r2[l], l);
       block_argmax(best, ib, red);
       if (1.0 / sqrt(best) > tol_abs) continue;
       if (ib < k - 1) {
-        swap_skel(perm, T, n, r, R, n, k, ib, k - 1);
-        small_qr(R, k, n, v, red);
+        swap_skel(perm, T, ldt, r, R, ldt, k, ib, k - 1);
+        small_qr(R, k, ldt, v, red);
       }
       // skeleton k-1 becomes the first redundant column
       const int k1 = k - 1;
-      for (int l = tid; l < k1; l += nt) t[l] = R[l + (int64_t)k1 * n];
+      for (int l = tid; l < k1; l += nt) t[l] = R[l + (int64_t)k1 * ldt];
       __syncthreads();
       if (tid == 0) {
         for (int l = k1 - 1; l >= 0; --l) {
           double s = t[l];
-          for (int q = l + 1; q < k1; ++q) s -= R[l + (int64_t)q * n] * t[q];
-          t[l] = s / R[l + (int64_t)l * n];
+          for (int q = l + 1; q < k1; ++q) s -= R[l + (int64_t)q * ldt] * t[q];
+          t[l] = s / R[l + (int64_t)l * ldt];
         }
       }
       __syncthreads();
       for (int e = tid; e < k1 * r; e += nt) {        // shift columns right by one
         int l = e % k1, c = e / k1;
-        X[l + (int64_t)(c + 1) * n] = T[l + (int64_t)c * n] + t[l] * T[k1 + (int64_t)c * n];
+        X[l + (int64_t)(c + 1) * ldt] = T[l + (int64_t)c * ldt] + t[l] * T[k1 + (int64_t)c * ldt];
       }
       for (int l = tid; l < k1; l += nt) X[l] = t[l];
       __syncthreads();
       for (int e = tid; e < k1 * (r + 1); e += nt) {
         int l = e % k1, c = e / k1;
-        T[l + (int64_t)c * n] = X[l + (int64_t)c * n];
+        T[l + (int64_t)c * ldt] = X[l + (int64_t)c * ldt];
       }
       __syncthreads();
       k = k1;
@@ -611,7 +618,7 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
     return;
   }
   const int* q = permar + B.perm_off;
-  const double* Tg = Tar + B.T_off;  // k x r, leading dimension n
+  const double* Tg = Tar + B.T_off;  // k x r, leading dimension B.ldt
   // point data of slf in skeleton-first order, cached in shared memory, or
   // for the global-scratch class after the box's 2 n^2 matrix elements
   double* cx = SH ? smem_d : reinterpret_cast<double*>(War + B.W_off + 2 * (int64_t)n * n);
@@ -646,7 +653,7 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
     }
     return (S)v + (cm.moff[0] < 0 && cm.nc == 1 ? (S)0 : mval(cm, Mprev, q[a], q[c]));
   };
-  FOR_COLS_ROWS(a, 0, k, c, 0, r) Ts[a + c * k] = (S)Tg[a + (int64_t)c * n];
+  FOR_COLS_ROWS(a, 0, k, c, 0, r) Ts[a + c * k] = (S)Tg[a + (int64_t)c * B.ldt];
   FOR_COLS_ROWS(a, 0, k, c, 0, k) Kss[a + c * k] = kval(a, c);
   // Krs (r x k): own region for LU; for Cholesky it passes through Ksr's
   S* KrsTmp = LU ? Krs : Ksr;
@@ -1176,8 +1183,9 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   double* rny = rnx + mn;
   double* rw = rny + mn;
   int* p2c = reinterpret_cast<int*>(rw + mn);        // position -> column (n)
-  double* Rc = Tar + B.T_off + (int64_t)n * n;       // R rows by column
-  double* Rp = Rc + (int64_t)n * n;                  // R rows by position
+  const int ldt = B.ldt;                             // R rows kept (the rank cap)
+  double* Rc = Tar + B.T_off + (int64_t)ldt * n;     // R rows by column
+  double* Rp = Rc + (int64_t)ldt * n;                // R rows by position
   for (int j = tid; j < n; j += NT) {
     point_fields(P, slf[j], sx[j], sy[j], snx[j], sny[j], sw[j]);
     p2c[j] = j;
@@ -1282,6 +1290,8 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   // stopped as id.m:105-111 stops the QR
   int kmax = prm.kmax;
   if (kmax <= 0 || kmax > n) kmax = n;
+  const int kuser = kmax;
+  if (kmax > ldt) kmax = ldt;            // the box's buffers hold ldt rows of R
   const int smax = m < n ? m : n;
   double tol_abs = 0;
   int k = 0;
@@ -1318,7 +1328,7 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
     const double rinv = 1.0 / nrm;
     for (int j = tid; j < n; j += NT) {
       const double r = prow[j] * rinv;
-      Rc[s + (int64_t)j * n] = r;
+      Rc[s + (int64_t)j * ldt] = r;
       dg[j] -= r * r;
     }
 #pragma unroll
@@ -1336,10 +1346,23 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
     __syncthreads();
     if (k == kmax) break;
   }
+  if (k == ldt && ldt < kuser && k < smax) {
+    // stopped by the rank cap: if a remaining column is still above the
+    // tolerance the true rank exceeds the cap (an under-predicted rank);
+    // kout = ldt + 1 tells the host to redo this box with full buffers
+    double bv = -1.0;
+    int bi = n;
+    for (int pos = k + tid; pos < n; pos += NT) argmax_merge(bv, bi, dg[p2c[pos]], pos);
+    block_argmax(bv, bi, red);
+    if (bi < n && sqrt(fmax(bv, 0.0)) > tol_abs) {
+      if (tid == 0) kout[b] = ldt + 1;
+      return;
+    }
+  }
   // R's leading k rows by pivot position
   for (int e = tid; e < k * n; e += NT) {
     const int i = e % k, pos = e / k;
-    Rp[i + (int64_t)pos * n] = Rc[i + (int64_t)p2c[pos] * n];
+    Rp[i + (int64_t)pos * ldt] = Rc[i + (int64_t)p2c[pos] * ldt];
   }
   for (int pos = tid; pos < n; pos += NT) perm[pos] = p2c[pos];
   if (tid == 0) kout[b] = k;
@@ -1498,7 +1521,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
   }
   const int* slf = pool + B.slf_off;
   const int* q = permar + B.perm_off;           // q[0..k) skeletons, q[k..n) redundant
-  const double* Tg = Tar + B.T_off;              // k x r, leading dimension n
+  const double* Tg = Tar + B.T_off;              // k x r, leading dimension B.ldt
   // shared layout
   double* cx = smem_d;
   double* cy = cx + n;
@@ -1540,7 +1563,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
   // T, Kss, Ksr (positions: rd j -> j, sk a -> r + a)
   for (int e = tid; e < k * r; e += NT) {
     const int a = e % k, j = e / k;
-    Ts[e] = (S)Tg[a + (int64_t)j * n];
+    Ts[e] = (S)Tg[a + (int64_t)j * B.ldt];
     KX[e] = kv(r + a, j);
   }
   for (int e = tid; e < k * k; e += NT) Kss[e] = kv(r + e % k, r + e / k);

@@ -222,7 +222,7 @@ __global__ void k_zfrom_tree(const S* __restrict__ Y, U* __restrict__ X, int64_t
 // over all sources, 256 at a time staged in shared memory with their
 // right-hand-side values; each thread accumulates NC columns.
 constexpr int kZMvTile = 256;
-constexpr int kZMvCols = 4;
+constexpr int kZMvCols = 8;
 
 namespace {
 __device__ inline zd cfie_entry_mv(double k, double eta, double tx, double ty, double sx, double sy,

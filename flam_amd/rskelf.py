@@ -202,7 +202,20 @@ class Factor:
         return dict(N=inf.N, d=inf.d, nlvl=inf.nlvl, nlvl_done=inf.nlvl_done,
                     nfactor=inf.nfactor, bytes=inf.bytes,
                     precision="fp64" if inf.precision == L.RSKX_PREC_FP64 else "fp32",
-                    symm=inf.symm.decode())
+                    symm=inf.symm.decode(), id_scratch=inf.id_scratch,
+                    rank_fallbacks=inf.rank_fallbacks)
+
+    def box_features(self):
+        """(X, rank): the rank model's features (nbox x 8, numpy) and the
+        exact rank of every box the factorization compressed (recorded with
+        opts record_boxes or a rank model)."""
+        import numpy as np
+        n = ctypes.c_int64()
+        _check(L.lib().rskx_box_features(self._h, ctypes.byref(n), None, None))
+        X = np.zeros((n.value, 8))
+        k = np.zeros(n.value, np.int64)
+        _check(L.lib().rskx_box_features(self._h, ctypes.byref(n), X.ctypes.data, k.ctypes.data))
+        return X, k
 
     def level_table(self):
         """(lvl, nblk, start npts, end npts) per level, as rskelf.m:264-272 prints."""
@@ -276,6 +289,14 @@ def _opts(opts, d):
     if callable(stop):
         raise FlamError(L.RSKX_ERR_UNSUP, "FLAM:rskx:stop: only a numeric opts.stop (levels) is built")
     o.stop = -1 if stop is None or stop == math.inf else int(stop)
+    route = opts.get("id_route", "auto")
+    if route not in ("auto", "gram", "qr"):
+        raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:id_route: id_route must be 'auto', 'gram' or 'qr'")
+    o.id_gram = {"auto": 0, "gram": 1, "qr": -1}[route]
+    model = opts.get("rank_model")
+    o.rank_model = model.handle if model is not None else None
+    o.rank_scale = float(opts.get("rank_scale", 1.0))
+    o.record_boxes = int(bool(opts.get("record_boxes", False)))
     return o, ext_buf
 
 
@@ -285,7 +306,13 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     opts keys (rskelf.m:112-120): lvlmax (inf), ext (None), Tmax (2),
     rrqr_iter (inf), symm ('n'), verb (0), stop (inf: a numeric stop
     factors that many levels, rskelf.m:128,172); plus precision ('fp64' or
-    'fp32') and profile (0)."""
+    'fp32'), profile (0), id_route ('auto': the Gram-matrix ID for FP32
+    factors at tol >= 1e-7, column-pivoted QR otherwise; 'gram': the Gram
+    ID for FP64 too; 'qr': always QRCP), rank_model (a trained
+    flam_amd.rank.RankModel: the Gram ID's buffers sized by its predicted
+    ranks, boxes beyond their cap redone with full buffers), rank_scale (1)
+    and record_boxes (False: keep features and exact ranks of every box,
+    Factor.box_features())."""
     x = _dev(x)
     if x.dim() != 2:
         raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: x must be d x N")
@@ -298,7 +325,8 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     _on_torch_stream()
     _check(L.lib().rskx_rskelf(xc.data_ptr(), d, N, int(occ), float(rank_or_tol),
                                ctypes.byref(k), ctypes.byref(p), ctypes.byref(o), ctypes.byref(h)))
-    return Factor(h.value, N, keep=(A, xc), cplx=isinstance(A, HelmholtzCFIE2D))
+    return Factor(h.value, N, keep=(A, xc, opts and opts.get("rank_model")),
+                  cplx=isinstance(A, HelmholtzCFIE2D))
 
 
 @dataclass

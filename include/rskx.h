@@ -114,6 +114,10 @@ typedef struct {
                            sphere of ie_sphere.m:39-40 and :175); else NULL  */
 } rskx_proxy;
 
+/* A learned rank model (rskx_rank_model_*): a small multilayer perceptron
+ * that predicts a box's skeleton count from its features. */
+typedef struct rskx_rank_model rskx_rank_model;
+
 /* options: rskelf.m:112-128 */
 typedef struct {
   int lvlmax;           /* maximum tree depth, <= 0 means Inf               */
@@ -128,6 +132,20 @@ typedef struct {
                         /* 2: time only the elimination class ("factor")  */
   int stop;             /* factor at most this many levels, leaves first
                            (rskelf.m:128,172: numeric opts.stop); < 0: all   */
+  int id_gram;          /* ID route at tol >= 1e-7: 0 the Gram-matrix ID for
+                           FP32 factors and column-pivoted QR for FP64 (so
+                           FP64 rank tables follow the reference's QRCP
+                           step by step); 1 the Gram ID for both; -1 QRCP */
+  const rskx_rank_model* rank_model;
+                        /* non-NULL: predict every box's rank before the
+                           level loop and size the Gram ID's R / T work
+                           buffers to a cap of rank_scale (1.25 k^ + 4)
+                           rows instead of n; a box whose rank exceeds its
+                           cap is redone with full buffers (the exact
+                           pivoted ID is the judge)                        */
+  double rank_scale;    /* multiplier of the predicted caps (default 1)     */
+  int record_boxes;     /* 1: keep every box's features and exact rank for
+                           rskx_box_features (training data)               */
 } rskx_opts;
 
 typedef struct rskx_factor rskx_factor;
@@ -141,6 +159,8 @@ typedef struct {
   int64_t bytes;        /* device bytes held by the factor                  */
   int precision;        /* RSKX_PREC_*                                      */
   char symm;
+  int64_t id_scratch;   /* largest per-level work buffer of the ID (bytes)  */
+  int64_t rank_fallbacks; /* boxes redone with full buffers (rank model)    */
 } rskx_info;
 
 /* host copy of one factor, F.factors(i) of rskelf.m:249-257.  Call with
@@ -284,6 +304,37 @@ int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A,
 int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x, int d,
                              const rskx_kernel* A, const double* X, int64_t ldx, double* Y,
                              int64_t ldy, int nrhs);
+
+/* The learned rank model.  No reference counterpart: FLAM sizes every
+ * box's ID by its column count.  A network of RSKX_RANK_FEATURES inputs,
+ * two tanh layers of 32 and a linear output, trained by its own forward /
+ * backward kernels with Adam (FP64) on (features, log2(1 + rank) / 10)
+ * pairs harvested from factorizations (rskx_box_features).  Features of a
+ * box, from the tree and its points: reference level / 16, log2(points in
+ * its subtree) / 20, log2(longest side) / 8, shortest / longest side, RMS
+ * distance of its points from their mean / longest side, distance of that
+ * mean from the box centre / longest side, -log10(tol) / 16 and
+ * log2(1 + k * longest side) / 10 (k the wavenumber, 0 for Laplace).
+ * X: device n x RSKX_RANK_FEATURES row-major; t, y, g: device. */
+#define RSKX_RANK_FEATURES 8
+int rskx_rank_model_create(uint64_t seed, rskx_rank_model** out);
+void rskx_rank_model_free(rskx_rank_model* M);
+int rskx_rank_model_nparams(void);
+/* host copies of the parameter vector: get into out (if not NULL), then set
+ * from in (if not NULL); nparams doubles each */
+int rskx_rank_model_params(rskx_rank_model* M, double* out, const double* in);
+/* loss = mean (model(X_i) - t_i)^2 and its gradient g (device, nparams) */
+int rskx_rank_model_grad(rskx_rank_model* M, const double* X, const double* t, int64_t n,
+                         double* g, double* loss);
+/* Adam over minibatches of `batch` consecutive samples, `epochs` passes;
+ * *loss: the mean squared error over all samples afterwards (host) */
+int rskx_rank_model_train(rskx_rank_model* M, const double* X, const double* t, int64_t n,
+                          int epochs, int64_t batch, double lr, double* loss);
+int rskx_rank_model_predict(const rskx_rank_model* M, const double* X, int64_t n, double* y);
+/* features and exact ranks of every box a factorization compressed
+ * (opts.record_boxes = 1): call with NULL arrays for *nbox, then with host
+ * arrays of nbox * RSKX_RANK_FEATURES and nbox entries */
+int rskx_box_features(const rskx_factor* F, int64_t* nbox, double* X, int64_t* rank);
 
 /* Orthogonalization step of the flexible GMRES refinement that uses a
  * factor as preconditioner (the reference preconditions MATLAB's gmres with

@@ -656,3 +656,71 @@ def rank_table(F):
     """The reference's verbose per-level table (rskelf.m:264-272) as rows
     (lvl, nblk, start npts, end npts)."""
     return [(s["lvl"], s["nblk"], s["nrem1"], s["nrem2"]) for s in F["stats"]]
+
+
+# --------------------------------------------------------------------------
+# refinement (the CPU leg of the benchmark)
+# --------------------------------------------------------------------------
+def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8):
+    """Flexible GMRES with the factor as right preconditioner, all columns of
+    B at once, stopped on every column's true relative residual -- the
+    refinement the device runs (the reference preconditions MATLAB's gmres
+    with rskelf_sv, rskelf/test/ie_square3.m:97): Arnoldi with classical
+    Gram-Schmidt twice, Givens rotations on the Hessenberg columns, the
+    preconditioned directions kept.  Real data.  Returns (X, iters, relres)."""
+    B = np.asarray(B, dtype=float)
+    vec = B.ndim == 1
+    B2 = B.reshape(B.shape[0], -1)
+    N, nrhs = B2.shape
+    bn = np.linalg.norm(B2, axis=0)
+    bn = np.where(bn > 0, bn, 1.0)
+    X = np.zeros_like(B2)
+    iters = 0
+    rel = None
+    for outer in range(maxit + 1):
+        R = B2 - matvec(X) if outer else B2.copy()
+        rn = np.linalg.norm(R, axis=0)
+        rel = rn / bn
+        if np.all(rel <= tol) or outer == maxit:
+            break
+        V = np.zeros((restart + 1, N, nrhs))
+        Z = np.zeros((restart, N, nrhs))
+        V[0] = R / np.where(rn > 0, rn, 1.0)
+        H = np.zeros((restart + 1, restart, nrhs))
+        cs = np.zeros((restart, nrhs))
+        sn = np.zeros((restart, nrhs))
+        g = np.zeros((restart + 1, nrhs))
+        g[0] = rn
+        j = 0
+        for j in range(restart):
+            Z[j] = precond(V[j])
+            w = matvec(Z[j])
+            h = np.zeros((j + 2, nrhs))
+            for _ in range(2):                          # classical Gram-Schmidt, twice
+                c = np.einsum("inr,nr->ir", V[: j + 1], w)
+                w = w - np.einsum("inr,ir->nr", V[: j + 1], c)
+                h[: j + 1] += c
+            h[j + 1] = np.linalg.norm(w, axis=0)
+            V[j + 1] = w / np.where(h[j + 1] > 0, h[j + 1], 1.0)
+            iters += 1
+            for i in range(j):
+                t = cs[i] * h[i] + sn[i] * h[i + 1]
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1]
+                h[i] = t
+            r = np.hypot(h[j], h[j + 1])
+            safe = np.where(r > 0, r, 1.0)
+            cs[j] = np.where(r > 0, h[j] / safe, 1.0)
+            sn[j] = np.where(r > 0, h[j + 1] / safe, 0.0)
+            H[: j + 1, j] = h[: j + 1]
+            H[j, j] = r
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            if np.all(np.abs(g[j + 1]) / bn <= 0.5 * tol):
+                break
+        y = np.zeros((j + 1, nrhs))
+        for i in range(j, -1, -1):
+            d = H[i, i]
+            y[i] = np.where(d != 0, (g[i] - np.einsum("kr,kr->r", H[i, i + 1: j + 1], y[i + 1:]))
+                            / np.where(d != 0, d, 1.0), 0.0)
+        X = X + np.einsum("jnr,jr->nr", Z[: j + 1], y)
+    return (X[:, 0] if vec else X), iters, rel

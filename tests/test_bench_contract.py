@@ -30,3 +30,5 @@ def test_traffic_summary_is_read():
     # launches): the measured bytes per launch are of that order
     assert 5e7 < traffic < 1e9
     assert bench.measured_traffic("no-such-kernel") == (None, None)
+    # keyed by workload: another workload's class of the same name gets none
+    assert bench.measured_traffic("factor", "ellipses_fp32") == (None, None)

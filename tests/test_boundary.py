@@ -140,3 +140,44 @@ def test_batch_arguments_checked_before_device_use():
     assert rc == L.RSKX_ERR_UNSUP and msg.startswith("FLAM:rskx:batch")
     rc, msg = _batch_call(L, [0, 10, 20], [ring, ring], ext=True)
     assert rc == L.RSKX_ERR_UNSUP and msg.startswith("FLAM:rskx:batch")
+
+
+def _cfie_call(L, *, symm=b"n", pkind=None, k=20.0, stop=-1):
+    o = L.Opts()
+    L.lib().rskx_opts_default(ctypes.byref(o))
+    o.symm = symm
+    o.stop = stop
+    dummy = 16  # never dereferenced: validation fails first
+    kern = L.Kernel(L.RSKX_KERNEL_HELMHOLTZ2D_CFIE, dummy, dummy, dummy, 1.0, k, k)
+    p = L.Proxy(pkind if pkind is not None else L.RSKX_PROXY_RING_CFIE, 64, 1.5, 1.0)
+    h = ctypes.c_void_p()
+    rc = L.lib().rskx_rskelf(dummy, 2, 100, 64, 1e-6, ctypes.byref(kern), ctypes.byref(p),
+                             ctypes.byref(o), ctypes.byref(h))
+    return rc, L.last_error()
+
+
+@pytest.mark.parametrize("kw,code,ident", [
+    (dict(symm=b"p"), "UNSUP", "FLAM:rskx:symm"),
+    (dict(symm=b"h"), "UNSUP", "FLAM:rskx:symm"),
+    (dict(stop=2), "UNSUP", "FLAM:rskx:stop"),
+    (dict(pkind=1), "UNSUP", "FLAM:rskx:proxy"),
+    (dict(k=0.0), "ARG", "FLAM:rskx:kernel"),
+])
+def test_complex_kernel_arguments(kw, code, ident):
+    """The combined field takes symm 'n' / 's', no partial factorization,
+    its own proxy ring and a positive wavenumber; checked before any
+    device work."""
+    L = _lib()
+    rc, msg = _cfie_call(L, **kw)
+    assert rc == getattr(L, "RSKX_ERR_" + code) and msg.startswith(ident), msg
+
+
+def test_complex_rhs_needs_complex_factor():
+    import torch
+    from flam_amd.rskelf import Factor, FlamError, _rhs_dtype
+    F = Factor(None, 10, None, cplx=False)
+    with pytest.raises(FlamError):
+        _rhs_dtype(F, torch.zeros(10, dtype=torch.complex128))
+    Fz = Factor(None, 10, None, cplx=True)
+    assert _rhs_dtype(Fz, torch.zeros(10)) == torch.complex128
+    assert _rhs_dtype(Fz, torch.zeros(10, dtype=torch.complex64)) == torch.complex64

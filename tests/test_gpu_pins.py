@@ -168,3 +168,25 @@ def test_benchmark_configuration_iterations():
     rel = (torch.linalg.vector_norm(R, dim=0) / torch.linalg.vector_norm(B, dim=0)).max().item()
     print(f"iters {info['iters']} relres {rel:.3e}", flush=True)
     assert info["iters"] == 4 and rel <= 1e-10
+
+
+def test_fp64_gram_route_matches_qr_route():
+    """id_route 'gram' (the Gram-matrix ID, FP64 arithmetic throughout)
+    on an FP64 factor at tol 1e-6: per-level ranks within 2% of the
+    column-pivoted QR route's and of the oracle's, and both solve to the
+    closed form's accuracy at that tolerance."""
+    flam_amd, geometry = _dev()
+    N, occ, tol = 1 << 16, 256, 1e-6
+    print(f"fp64 gram vs qr route: circle N={N} occ={occ} tol={tol}", flush=True)
+    g = geometry.circle(N)
+    Fg = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, dict(symm="p", id_route="gram"))
+    Fq = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, dict(symm="p", id_route="qr"))
+    pr = opr.CurveDL(N, "circle")
+    Fo = ofl.rskelf(pr.afun, pr.x, occ, tol, pr.pxyfun, dict(symm="p", order="level"))
+    _cmp_tables(Fg.level_table(), Fq.level_table(), 0.02)
+    _cmp_tables(Fg.level_table(), ofl.rank_table(Fo), 0.02)
+    B = np.random.default_rng(0).standard_normal((N, 2))
+    Xe = pr.closed_form_inverse_apply(B)
+    for F in (Fg, Fq):
+        X = _np(flam_amd.rskelf_sv(F, torch.from_numpy(B).cuda()))
+        assert np.linalg.norm(X - Xe) / np.linalg.norm(Xe) < 1e-5

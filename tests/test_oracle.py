@@ -580,3 +580,36 @@ def test_cfie_rskelf_vs_dense(kind, symm):
         assert np.linalg.norm(flam.rskelf_sv(F, b, trans) - xe) < 100 * tol * np.linalg.norm(xe)
         ye = M @ b
         assert np.linalg.norm(flam.rskelf_mv(F, b, trans) - ye) < 100 * tol * np.linalg.norm(ye)
+
+
+def test_fgmres_restatement():
+    """The CPU FGMRES (the benchmark's CPU leg) with a loose factor as the
+    preconditioner reaches the requested true residual on every column and
+    agrees with the closed-form solution."""
+    N = 4096
+    pr = problems.CurveDL(N, "circle")
+    F = flam.rskelf(pr.afun, pr.x, 64, 1e-4, pr.pxyfun, dict(symm="p", order="level"))
+    B = np.random.default_rng(2).standard_normal((N, 3))
+    X, iters, rel = flam.fgmres(pr.matvec, B, lambda V: flam.rskelf_sv(F, V), tol=1e-12)
+    assert np.all(rel <= 1e-12) and iters <= 8
+    Xe = pr.closed_form_inverse_apply(B)
+    assert np.linalg.norm(X - Xe) <= 1e-10 * np.linalg.norm(Xe)
+
+
+def test_mlp_restatement_gradient_by_finite_differences():
+    """The rank model's CPU restatement: back-propagated gradient against
+    central differences of the loss (the device gradient is checked against
+    this restatement on the GPU)."""
+    from oracle import mlp
+    rng = np.random.default_rng(0)
+    p = 0.3 * rng.standard_normal(mlp.NPARAMS)
+    X = rng.standard_normal((40, mlp.F))
+    t = rng.standard_normal(40)
+    loss, g = mlp.loss_grad(p, X, t)
+    h = 1e-6
+    idx = rng.choice(mlp.NPARAMS, 60, replace=False).tolist() + [mlp.NPARAMS - 1, 0]
+    for i in idx:
+        e = np.zeros(mlp.NPARAMS)
+        e[i] = h
+        fd = (mlp.loss_grad(p + e, X, t)[0] - mlp.loss_grad(p - e, X, t)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 * max(1.0, abs(g[i]))
