@@ -86,11 +86,14 @@ SIGNATURES = {
     "rskx_set_stream": (c_int, [c_void_p]),
     "rskx_factor_info": (c_int, [c_void_p, POINTER(Info)]),
     "rskx_level_table": (c_int, [c_void_p, POINTER(c_int64), c_int]),
+    "rskx_factor_lvp": (c_int, [c_void_p, POINTER(c_int64), c_int]),
     "rskx_export_factor": (c_int, [c_void_p, c_int64, POINTER(FactorView)]),
     "rskx_kernel_times": (c_int, [c_void_p, POINTER(c_char_p), POINTER(c_double),
                                   POINTER(c_int64), POINTER(c_int64), POINTER(c_double), c_int]),
     "rskx_kernel_matvec": (c_int, [c_void_p, c_int, c_int64, POINTER(Kernel), c_void_p, c_int64,
                                    c_void_p, c_int64, c_int]),
+    "rskx_import": (c_int, [c_int64, c_char, c_int, c_int, POINTER(c_int64), c_void_p,
+                            POINTER(c_void_p)]),
     "rskx_rank_model_create": (c_int, [ctypes.c_uint64, POINTER(c_void_p)]),
     "rskx_rank_model_free": (None, [c_void_p]),
     "rskx_rank_model_nparams": (c_int, []),

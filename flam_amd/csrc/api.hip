@@ -2093,6 +2093,13 @@ int rskx_factor_info(const rskx_factor* F, rskx_info* info) {
   return RSKX_OK;
 }
 
+int rskx_factor_lvp(const rskx_factor* F, int64_t* lvp, int maxn) {
+  if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
+  const int n = (int)F->lvp.size();
+  for (int i = 0; i < n && i < maxn && lvp; ++i) lvp[i] = F->lvp[i];
+  return n;
+}
+
 int rskx_level_table(const rskx_factor* F, int64_t* rows, int maxrows) {
   if (!F) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: factor is NULL");
   int n = (int)F->table.size();
@@ -2210,6 +2217,180 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
     if (v->F) std::copy(G.begin(), G.end(), v->F);
     if (v->p) for (int64_t q = 0; q < r; ++q) v->p[q] = pv[q];
   }
+  return RSKX_OK;
+}
+
+// Import of a factor in the reference's layout (rskelf.m:156-158,249-257,263):
+// the factors' values go to one value / index block per level, laid out as
+// the device factorization lays them out; the inverted 32 x 32 diagonal
+// blocks of L and U are formed on the host and the transposed copies by
+// k_pack, so the sweeps run on an imported factor unchanged.  The tree
+// order is the identity (points stay in the caller's order).
+extern "C++" {
+template <typename S>
+static int import_impl(int64_t N, char symm, int nlvl, const int64_t* lvp,
+                       const rskx_factor_view* fv, rskx_factor* F) {
+  const bool lu = symm == 'n' || symm == 's', ldl = symm == 'h';
+  const int64_t nf = lvp[nlvl];
+  {
+    std::vector<int> iota(N);
+    for (int64_t i = 0; i < N; ++i) iota[i] = (int)i;
+    RSKX_TRY(F->order.ensure(std::max<int64_t>(N, 1) * sizeof(int)));
+    RSKX_HIP(hipMemcpy(F->order.p, iota.data(), N * sizeof(int), hipMemcpyHostToDevice));
+  }
+  int64_t nrem = N;
+  F->lvp.assign(1, 0);
+  for (int lv = 0; lv < nlvl; ++lv) {
+    const int64_t f0 = lvp[lv], f1 = lvp[lv + 1];
+    std::vector<S> val;
+    std::vector<int> idx;
+    int kmx = 0, rmx = 0;
+    const int64_t nrem1 = nrem;
+    for (int64_t f = f0; f < f1; ++f) {
+      const rskx_factor_view& v = fv[f];
+      const int64_t k = v.k, r = v.r, nbk = (r + kDiagBlock - 1) / kDiagBlock;
+      FacRec fr{};
+      fr.k = (int)k; fr.r = (int)r; fr.packed = 0; fr.D = -1;
+      auto put = [&](const double* a, int64_t cnt) {
+        const int64_t o = (int64_t)val.size();
+        for (int64_t q = 0; q < cnt; ++q) val.push_back(a ? (S)a[q] : (S)0);
+        return o;
+      };
+      fr.T = put(v.T, k * r);
+      // packed LU: L strictly below the diagonal, U on and above ('n');
+      // L with its diagonal ('p'); unit L ('h', D apart)
+      std::vector<double> LU((size_t)r * r, 0.0);
+      for (int64_t c = 0; c < r; ++c)
+        for (int64_t a = 0; a < r; ++a) {
+          const double l = v.L ? v.L[a + c * r] : 0.0, u = v.U ? v.U[a + c * r] : 0.0;
+          LU[a + c * r] = lu ? (a > c ? l : u) : (a > c || (!ldl && a == c) ? l : (a == c ? 1.0 : 0.0));
+        }
+      fr.LU = put(LU.data(), r * r);
+      fr.E = put(v.E, k * r);
+      fr.G = lu ? put(v.F, r * k) : -1;
+      // inverted diagonal blocks: unit lower for 'n' and 'h'
+      auto inv_blocks = [&](bool upper) {
+        std::vector<double> out((size_t)nbk * kDiagBlock * kDiagBlock, 0.0);
+        for (int64_t b = 0; b < nbk; ++b) {
+          const int64_t j0 = b * kDiagBlock, bs = std::min<int64_t>(kDiagBlock, r - j0);
+          double* X = &out[(size_t)b * kDiagBlock * kDiagBlock];
+          for (int64_t c = 0; c < bs; ++c) {
+            if (!upper) {
+              for (int64_t t = 0; t < bs; ++t) {
+                double x = t == c ? 1.0 : 0.0;
+                for (int64_t u = 0; u < t; ++u) x -= LU[(j0 + t) + (j0 + u) * r] * X[u + c * kDiagBlock];
+                X[t + c * kDiagBlock] = (lu || ldl) ? x : x / LU[(j0 + t) + (j0 + t) * r];
+              }
+            } else {
+              for (int64_t t = bs - 1; t >= 0; --t) {
+                double x = t == c ? 1.0 : 0.0;
+                for (int64_t u = t + 1; u < bs; ++u) x -= LU[(j0 + t) + (j0 + u) * r] * X[u + c * kDiagBlock];
+                X[t + c * kDiagBlock] = x / LU[(j0 + t) + (j0 + t) * r];
+              }
+            }
+          }
+        }
+        return put(out.data(), (int64_t)out.size());
+      };
+      fr.Li = inv_blocks(false);
+      fr.Ui = lu ? inv_blocks(true) : -1;
+      fr.Tt = put(nullptr, k * r);
+      fr.Et = put(nullptr, k * r);
+      fr.Gt = lu ? put(nullptr, r * k) : -1;
+      fr.LUt = put(nullptr, r * r);
+      fr.Lti = put(nullptr, nbk * kDiagBlock * kDiagBlock);
+      fr.Uti = lu ? put(nullptr, nbk * kDiagBlock * kDiagBlock) : -1;
+      if (ldl) {                       // U = D (rskelf.m:231), diagonal
+        std::vector<double> D(r);
+        for (int64_t a = 0; a < r; ++a) D[a] = v.U[a + a * r];
+        fr.D = put(D.data(), r);
+      }
+      fr.sk = (int64_t)idx.size();
+      for (int64_t q = 0; q < k; ++q) idx.push_back((int)v.sk[q]);
+      fr.rd = (int64_t)idx.size();
+      for (int64_t q = 0; q < r; ++q) idx.push_back((int)v.rd[q]);
+      fr.piv = (int64_t)idx.size();
+      if (lu) for (int64_t q = 0; q < r; ++q) idx.push_back((int)v.p[q]);
+      F->recs.push_back(fr);
+      kmx = std::max<int>(kmx, (int)k);
+      rmx = std::max<int>(rmx, (int)r);
+      nrem -= r;
+    }
+    F->lval.emplace_back();
+    F->lidx.emplace_back();
+    RSKX_TRY(F->lval.back().ensure((val.size() + 1) * sizeof(S)));
+    RSKX_TRY(F->lidx.back().ensure((idx.size() + 1) * sizeof(int)));
+    if (!val.empty())
+      RSKX_HIP(hipMemcpy(F->lval.back().p, val.data(), val.size() * sizeof(S), hipMemcpyHostToDevice));
+    if (!idx.empty())
+      RSKX_HIP(hipMemcpy(F->lidx.back().p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (f1 > f0) {
+      DevBuf recs;
+      RSKX_TRY(recs.ensure((f1 - f0) * sizeof(FacRec)));
+      RSKX_HIP(hipMemcpy(recs.p, F->recs.data() + f0, (f1 - f0) * sizeof(FacRec), hipMemcpyHostToDevice));
+      launch_pack<S>(lu, recs.as<FacRec>(), (int)(f1 - f0), F->lval.back().as<S>(), g_stream);
+      RSKX_HIP(hipGetLastError());
+      RSKX_TRY(sync_stream());
+      recs.release();
+    }
+    F->fval_used += (int64_t)val.size();
+    F->fidx_used += (int64_t)idx.size();
+    F->lvp.push_back(f1);
+    F->lev_kmax.push_back(kmx);
+    F->lev_rmax.push_back(rmx);
+    F->table.push_back({nlvl - lv, f1 - f0, nrem1, nrem});
+    F->nlvl_done++;
+  }
+  for (FacRec& fr : F->recs) fr.packed = 1;
+  RSKX_TRY(upload(F->d_recs, F->recs));
+  RSKX_TRY(sync_stream());
+  (void)nf;
+  return RSKX_OK;
+}
+}  // extern "C++"
+
+int rskx_import(int64_t N, char symm, int precision, int nlvl, const int64_t* lvp,
+                const rskx_factor_view* factors, rskx_factor** out) {
+  if (!out || !lvp || (!factors && nlvl > 0 && lvp[nlvl] > 0))
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  *out = nullptr;
+  const char sy = (char)tolower(symm);
+  if (sy != 'n' && sy != 's' && sy != 'h' && sy != 'p')
+    return fail(RSKX_ERR_ARG, "FLAM:chksymm:invalidSymm: Symmetry parameter must be one of 'N', 'S', 'H', or 'P'.");
+  if (precision != RSKX_PREC_FP64 && precision != RSKX_PREC_FP32)
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:precision: unknown precision");
+  if (N < 1 || nlvl < 0 || lvp[0] != 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad N or lvp");
+  for (int l = 0; l < nlvl; ++l)
+    if (lvp[l + 1] < lvp[l]) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: lvp must be nondecreasing");
+  for (int64_t f = 0; f < lvp[nlvl]; ++f) {
+    const rskx_factor_view& v = factors[f];
+    if (v.k < 0 || v.r < 1 || (v.k && !v.sk) || !v.rd || !v.L || (v.k && (!v.T || !v.E)))
+      return fail(RSKX_ERR_ARG, "FLAM:rskx:import: factor " + std::to_string(f) + " is incomplete");
+    if ((sy == 'n' || sy == 's') && (!v.U || !v.p || (v.k && !v.F)))
+      return fail(RSKX_ERR_ARG, "FLAM:rskx:import: factor " + std::to_string(f) + " lacks U, p or F");
+    if (sy == 'h') {
+      if (!v.U) return fail(RSKX_ERR_ARG, "FLAM:rskx:import: symm 'h' needs D in U");
+      for (int64_t a = 0; a < v.r; ++a)
+        for (int64_t c = 0; c < v.r; ++c)
+          if (a != c && v.U[a + c * v.r] != 0)
+            return fail(RSKX_ERR_UNSUP, "FLAM:rskx:import: 2 x 2 pivot blocks of D (Bunch-Kaufman) are not built");
+    }
+    for (int64_t q = 0; q < v.k; ++q)
+      if (v.sk[q] < 0 || v.sk[q] >= N) return fail(RSKX_ERR_ARG, "FLAM:rskx:import: sk out of range");
+    for (int64_t q = 0; q < v.r; ++q)
+      if (v.rd[q] < 0 || v.rd[q] >= N || ((sy == 'n' || sy == 's') && (v.p[q] < 0 || v.p[q] >= v.r)))
+        return fail(RSKX_ERR_ARG, "FLAM:rskx:import: rd or p out of range");
+  }
+  rskx_factor* F = new rskx_factor();
+  F->N = N;
+  F->d = 0;
+  F->symm = sy;
+  F->prec = precision;
+  F->nlvl = nlvl;
+  const int rc = precision == RSKX_PREC_FP64 ? import_impl<double>(N, sy, nlvl, lvp, factors, F)
+                                             : import_impl<float>(N, sy, nlvl, lvp, factors, F);
+  if (rc != RSKX_OK) { delete F; return rc; }
+  *out = F;
   return RSKX_OK;
 }
 

@@ -576,3 +576,49 @@ def release_cache():
     """Give the library's idle device memory (workspaces and the cache of
     freed factor blocks) back to the device; live factors are untouched."""
     _check(L.lib().rskx_release_cache())
+
+
+def rskelf_import(F, N=None, precision="fp64"):
+    """A device factor from one in the reference's layout (F.factors,
+    F.lvp, F.symm of rskelf.m:156-158,249-257,263) -- e.g. the oracle's,
+    or a factor exported with Factor.export: F is a dict with 'factors'
+    (dicts sk, rd, T, L, U, p, E, F; 0-based indices), 'lvp' and 'symm'
+    (and 'N').  The inverse of Factor.export (rskx_import)."""
+    import numpy as np
+    facs = F["factors"]
+    lvp = np.ascontiguousarray(F["lvp"], dtype=np.int64)
+    symm = str(F["symm"]).lower()
+    N = int(F.get("N", N) if N is None else N)
+    keep = []
+    views = (L.FactorView * max(len(facs), 1))()
+
+    def arr(a, dt):
+        if a is None:
+            return None
+        a = np.asfortranarray(np.asarray(a, dtype=dt))
+        keep.append(a)
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64 if dt == np.int64 else ctypes.c_double))
+    for i, f in enumerate(facs):
+        v = views[i]
+        v.k, v.r = len(f["sk"]), len(f["rd"])
+        v.sk, v.rd = arr(f["sk"], np.int64), arr(f["rd"], np.int64)
+        v.p = arr(f.get("p"), np.int64)
+        for key in ("T", "L", "U", "E", "F"):
+            setattr(v, key, arr(f.get(key), np.float64))
+    h = ctypes.c_void_p()
+    _check(L.lib().rskx_import(N, symm.encode()[:1], _PREC[precision], len(lvp) - 1,
+                               lvp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                               ctypes.cast(views, ctypes.c_void_p), ctypes.byref(h)))
+    return Factor(h.value, N, keep=None)
+
+
+def export_all(F):
+    """The whole factor in the reference's layout: dict(N, symm, lvp,
+    factors) with every F.factors(i) from Factor.export and F.lvp
+    (cumulative factor counts per level, leaves first)."""
+    info = F.info()
+    n = L.lib().rskx_factor_lvp(F.handle, None, 0)
+    lvp = (ctypes.c_int64 * max(n, 1))()
+    L.lib().rskx_factor_lvp(F.handle, lvp, n)
+    return dict(N=info["N"], symm=info["symm"], lvp=list(lvp[:n]),
+                factors=[F.export(i) for i in range(info["nfactor"])])

@@ -283,8 +283,26 @@ int rskx_factor_info(const rskx_factor* F, rskx_info* info);
  * Returns the number of rows. */
 int rskx_level_table(const rskx_factor* F, int64_t* rows, int maxrows);
 
+/* F.lvp (rskelf.m:263): the cumulative factor counts per factored level,
+ * leaves first, nlvl_done + 1 host entries.  Returns the count. */
+int rskx_factor_lvp(const rskx_factor* F, int64_t* lvp, int maxn);
+
 /* Host copy of F.factors(i) (0-based i), indices in the caller's order. */
 int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* view);
+
+/* F from the reference's layout: the inverse of rskx_export_factor, for a
+ * factor computed elsewhere (a MATLAB F through the gateway, which
+ * subtracts 1 from its indices; the oracle's factors in the tests).
+ * factors[f] holds F.factors(f+1) as rskx_export_factor writes it (host
+ * arrays; indices 0-based in the caller's point order; T, L, U, E, F
+ * column-major doubles; symm 'h': U = D, diagonal only); lvp (host, nlvl+1
+ * entries) the cumulative factor counts of F.lvp, leaves first.  The
+ * factor is copied to the device in the given precision; the sweeps and
+ * rskx_export_factor work on it as on a factor of rskx_rskelf (the level
+ * table lists each level's factor count and the points it eliminates).
+ * Real data only. */
+int rskx_import(int64_t N, char symm, int precision, int nlvl, const int64_t* lvp,
+                const rskx_factor_view* factors, rskx_factor** out);
 
 /* Accumulated HIP-event time (ms) and launch count per kernel class of the
  * last rskx_rskelf / sweep calls when opts.profile was set.  names[i] is a

@@ -540,3 +540,43 @@ def test_large_root_global_paths():
     assert np.linalg.norm(Y - B) <= 1e-11 * np.linalg.norm(B)
     Xc = flam_amd.rskelf_sv(F, torch.from_numpy(B).cuda(), "c").cpu().numpy()
     assert np.linalg.norm(Xc - Xe) <= 1e-11 * np.linalg.norm(Xe)   # symmetric matrix
+
+
+@pytest.mark.parametrize("symm", ["n", "p", "h"])
+def test_import_export_round_trip(symm):
+    """rskx_import of the reference's F layout: the device factor exported
+    with rskx_export_factor and imported again gives the same sweeps (the
+    inverted diagonal blocks recomputed on the host: agreement to 1e-13)."""
+    flam_amd, geometry = _dev()
+    N = 4096
+    print(f"import/export round trip: circle N={N} symm {symm}", flush=True)
+    g = geometry.circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, 64, 1e-10, g.proxy, dict(symm=symm))
+    Fi = flam_amd.rskelf_import(flam_amd.export_all(F))
+    assert Fi.info()["nfactor"] == F.info()["nfactor"]
+    B = torch.from_numpy(np.random.default_rng(7).standard_normal((N, 3))).cuda()
+    for fn in (flam_amd.rskelf_sv, flam_amd.rskelf_mv):
+        for trans in ("n", "c"):
+            a, b = fn(F, B, trans), fn(Fi, B, trans)
+            assert (a - b).abs().max().item() <= 1e-13 * a.abs().max().item(), (fn, trans)
+
+
+@pytest.mark.parametrize("kind,symm", [("circle", "p"), ("ellipse", "n")])
+def test_oracle_factor_imported_to_the_device(kind, symm):
+    """The oracle's factor (the CPU restatement of rskelf.m) imported into
+    the device: the device sweeps on it reproduce the oracle's own sweeps
+    (sv_*.m, mv_*.m) to 1e-12, independently of the device factorization."""
+    flam_amd, geometry = _dev()
+    N = 2048
+    print(f"oracle factor on the device: {kind} N={N} symm {symm}", flush=True)
+    pr = opr.CurveDL(N, kind, a=1.0, b=0.6) if kind == "ellipse" else opr.CurveDL(N, kind)
+    Fo = ofl.rskelf(pr.afun, pr.x, 64, 1e-10, pr.pxyfun, dict(symm=symm, order="level"))
+    Fd = flam_amd.rskelf_import(dict(Fo, N=N))
+    B = np.random.default_rng(8).standard_normal((N, 2))
+    for trans in ("n", "c", "t"):
+        xs = flam_amd.rskelf_sv(Fd, torch.from_numpy(B).cuda(), trans).cpu().numpy()
+        xo = ofl.rskelf_sv(Fo, B, trans)
+        assert np.linalg.norm(xs - xo) <= 1e-12 * np.linalg.norm(xo), trans
+        ys = flam_amd.rskelf_mv(Fd, torch.from_numpy(B).cuda(), trans).cpu().numpy()
+        yo = ofl.rskelf_mv(Fo, B, trans)
+        assert np.linalg.norm(ys - yo) <= 1e-12 * np.linalg.norm(yo), trans
