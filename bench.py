@@ -311,6 +311,11 @@ def main():
     N = args.N
     log(f"workload: circle N={N} occ={args.occ} tol={args.tol:g} nrhs={args.nrhs} "
         f"target {args.target:g}")
+    # launch shapes chosen by the autotuner for this workload (flam_amd/tune.py)
+    tuned = os.path.join(ROOT, "flam_amd", "tuning", f"circle_N{N}.json")
+    if os.path.exists(tuned):
+        from flam_amd import tune
+        log(f"tuning table {os.path.basename(tuned)}: {tune.load(tuned)['table']}")
     g = geometry.circle(N)
     B = torch.from_numpy(np.random.default_rng(0).standard_normal((N, args.nrhs))).cuda()
     torch.cuda.synchronize()

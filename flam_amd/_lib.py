@@ -92,6 +92,11 @@ SIGNATURES = {
                                   POINTER(c_int64), POINTER(c_int64), POINTER(c_double), c_int]),
     "rskx_kernel_matvec": (c_int, [c_void_p, c_int, c_int64, POINTER(Kernel), c_void_p, c_int64,
                                    c_void_p, c_int64, c_int]),
+    "rskx_sv_graph": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_char, c_int, POINTER(c_void_p)]),
+    "rskx_graph_launch": (c_int, [c_void_p]),
+    "rskx_graph_free": (None, [c_void_p]),
+    "rskx_set_tuning": (c_int, [c_char_p, c_double]),
+    "rskx_get_tuning": (c_int, [c_char_p, POINTER(c_double)]),
     "rskx_import": (c_int, [c_int64, c_char, c_int, c_int, POINTER(c_int64), c_void_p,
                             POINTER(c_void_p)]),
     "rskx_rank_model_create": (c_int, [ctypes.c_uint64, POINTER(c_void_p)]),

@@ -126,6 +126,34 @@ inline int cfie_proxy_count(int pmin, double k, double l) {
   return std::max(pmin, p);
 }
 
+// ------------------------------------------------------------- tuning --
+// Launch-shape choices the autotuner (flam_amd/tune.py) sets by measuring
+// (rskx_set_tuning): which register shape and Gram-ID class a box of a
+// given size goes to, the shared-memory budgets that route boxes between
+// kernels, and whether a phase's kernel classes run concurrently.  The
+// defaults are the values measured best on the benchmark's workload.
+struct Tuning {
+  int elim8_nmax = 1 << 30;        // largest box of the 8-wave register elimination (capped
+                                   // at what its tile holds)
+  int elim8_lds = 80 * 1024;       // its shared-memory budget (two workgroups per CU)
+  int gram_n0 = 1 << 30;           // largest box of Gram-ID class 0 / 1 (capped at the
+  int gram_n1 = 1 << 30;           // classes' register tiles)
+  int concurrent = 1;              // size classes of one phase on side streams
+  int sweep_stage = 76 * 1024;     // shared bytes of a sweep workgroup's row/stage buffers
+  int id_lds = 160 * 1024;         // shared bytes above which the QR ID keeps its Householder
+                                   // vectors in global memory
+  int id_reg = 1;                  // register QRCP classes for the boxes that fit them
+};
+static Tuning g_tune;
+
+// the Gram-ID class of a box of n columns under the tuned class limits
+static int gram_class_tuned(int n) {
+  const int c = gram_id_class(n);
+  if (c == 0 && n > g_tune.gram_n0) return gram_id_nmax(1) >= n ? 1 : 2;
+  if (c == 1 && n > g_tune.gram_n1) return 2;
+  return c;
+}
+
 // --------------------------------------------------------------- errors --
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
@@ -1059,7 +1087,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     for (int64_t ib = 0; ib < nb && !exact; ++ib) {
       const int n = hb[ib].n, mc = hb[ib].mc;
       if (n > 0 && gram_id_class(n) >= 0 &&
-          gram_id_shmem(gram_id_class(n), n, mc, mc * (two ? 2 : 1) + L.p, L.p) > 160 * 1024)
+          gram_id_shmem(gram_class_tuned(n), n, mc, mc * (two ? 2 : 1) + L.p, L.p) > 160 * 1024)
         exact = true;                        // the bound is too loose: use the filtered counts
     }
     if (!exact) {
@@ -1076,7 +1104,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
       if (gram && n > 0 && mb > 0 && gram_id_class(n) >= 0) {
-        const int c = gram_id_class(n);
+        const int c = gram_class_tuned(n);
         const size_t by = gram_id_shmem(c, n, hmn[ib], mb, L.p);
         if (by <= 160 * 1024) {
           idq[c].push_back((int)ib);
@@ -1094,7 +1122,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       const size_t by = (size_t)8 * 1024 * sizeof(S) +
                         (size_t)(3 * L.p + 7 * n + 5 * hmn[ib] + mb + 32) * sizeof(double) +
                         (size_t)2 * n * sizeof(int) + 64;
-      if (cls >= 0 && n > 0 && mb > 0 && by <= 160 * 1024) {
+      if (cls >= 0 && g_tune.id_reg && n > 0 && mb > 0 && by <= 160 * 1024) {
         idr[cls].push_back((int)ib);
         shm_r[cls] = std::max(shm_r[cls], by);
       } else {
@@ -1141,7 +1169,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       size_t shg = (size_t)(2 * nmax_g + std::max(nmax_g, mcapmax_g) + 32) * sizeof(double);
       double* vglob = nullptr;             // Householder vectors too long for shared memory
       const int64_t vstride = std::max(nmax_g, mcapmax_g);
-      if (shg > 160 * 1024) {
+      if (shg > (size_t)g_tune.id_lds) {
         shg = (size_t)(3 * nmax_g + 32) * sizeof(double);
         RSKX_TRY(ws.V.ensure((size_t)ng * vstride * sizeof(double)));
         vglob = ws.V.as<double>();
@@ -1174,7 +1202,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       // tails below read every class's output, so they follow the join
       int used = 0;
       for (int c = 0; c < kGramClasses; ++c) used += idq[c].empty() ? 0 : 1;
-      const bool conc = used > 1;
+      const bool conc = used > 1 && g_tune.concurrent;
       if (conc) RSKX_TRY(side_fork());
       int off = 0;
       for (int c = kGramClasses - 1; c >= 0; --c) {     // largest boxes first
@@ -1228,7 +1256,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         size_t rsh[kGramClasses] = {0, 0, 0};
         int nmx = 0;
         for (int ib : redo) {
-          const int n = hb[ib].n, c = gram_id_class(n);
+          const int n = hb[ib].n, c = gram_class_tuned(n);
           const int mb = hmn[ib] * (two ? 2 : 1) + L.p;
           rid[c].push_back(ib);
           rsh[c] = std::max(rsh[c], gram_id_shmem(c, n, hmn[ib], mb, L.p));
@@ -1380,13 +1408,14 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // and its shared memory allow, else 16-wave workgroups
     std::vector<int> cls[3], creg[2];
     size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0};
-    const int nreg8 = elim_reg_nmax(sizeof(S), 8), nreg16 = elim_reg_nmax(sizeof(S), 16);
+    const int nreg8 = std::min(elim_reg_nmax(sizeof(S), 8), g_tune.elim8_nmax),
+              nreg16 = elim_reg_nmax(sizeof(S), 16);
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
       const int n = hb[ib].n;
       if (O.r > 0 && n <= nreg8) {
         const size_t by = elim_reg_shmem(n, O.k, O.r, sizeof(S), 8, lu);
-        if (by <= 80 * 1024) {
+        if (by <= (size_t)g_tune.elim8_lds) {
           creg[0].push_back((int)ib);
           regbox[ib] = 1;
           shm_reg[0] = std::max(shm_reg[0], by);
@@ -1419,7 +1448,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       // k_elim on the other side stream; joined before anything reads them
       const int nclass = (creg[1].empty() ? 0 : 1) + (creg[0].empty() ? 0 : 1) +
                          ((cls[0].empty() && cls[1].empty() && cls[2].empty()) ? 0 : 1);
-      const bool conc = nclass > 1;
+      const bool conc = nclass > 1 && g_tune.concurrent;
       if (conc) RSKX_TRY(side_fork());
       int off = 0;
       for (int c = 1; c >= 0; --c) {           // 16-wave boxes first, then 8-wave
@@ -1547,7 +1576,8 @@ static void sweep_ws_bytes(const rskx_factor* F, int ncol, bool solve, int mode,
     } else {
       const size_t xb = ((size_t)(kmx + 2 * rmx) * kRhs + (size_t)kRhs * 256) * sizeof(S);
       const size_t want = (size_t)std::max(rmx * rmx, kmx * rmx) * sizeof(S);
-      const size_t room = xb < 76 * 1024 ? 76 * 1024 - xb : 0;
+      const size_t stage = (size_t)g_tune.sweep_stage;
+      const size_t room = xb < stage ? stage - xb : 0;
       if (xb + std::min(want, room) > 160 * 1024) g = (xb / sizeof(S) + 63) / 64 * 64 * nf * sizeof(S);
     }
     *gbytes = std::max(*gbytes, g);
@@ -1641,7 +1671,8 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
       }
       const size_t xb = ((size_t)(kmx + 2 * rmx) * kRhs + (size_t)kRhs * 256) * sizeof(S);
       const size_t want = (size_t)std::max(rmx * rmx, kmx * rmx) * sizeof(S);
-      const size_t room = xb < 76 * 1024 ? 76 * 1024 - xb : 0;
+      const size_t stage = (size_t)g_tune.sweep_stage;
+      const size_t room = xb < stage ? stage - xb : 0;
       int bcap = (int)(std::min(want, room) / sizeof(S));
       size_t shm = xb + (size_t)bcap * sizeof(S);
       S* gws = nullptr;
@@ -1791,6 +1822,25 @@ static int scratch_address(void** out) {
   return RSKX_OK;
 }
 
+// ------------------------------------------------------- captured solve --
+// rskelf_sv captured once into a HIP graph for a fixed X and replayed: the
+// solve of an FGMRES step is the same launch sequence every time, so the
+// host issues one graph launch instead of 2 + 2 nlvl kernel launches.  The
+// graph owns its work rows (no allocation, no event, no synchronization
+// inside the captured region).
+struct rskx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  DevBuf ws;
+  ~rskx_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    ws.release();
+  }
+};
+
 // ================================================================ C ABI ==
 extern "C" {
 
@@ -1819,6 +1869,39 @@ int rskx_set_stream(void* s) {
   // uses the library's workspaces: drain it before switching
   if ((hipStream_t)s != g_stream) RSKX_TRY(sync_stream());
   g_stream = (hipStream_t)s;
+  return RSKX_OK;
+}
+
+static int* tuning_slot(const char* name) {
+  if (!name) return nullptr;
+  const std::string s(name);
+  if (s == "elim8_nmax") return &g_tune.elim8_nmax;
+  if (s == "elim8_lds") return &g_tune.elim8_lds;
+  if (s == "gram_n0") return &g_tune.gram_n0;
+  if (s == "gram_n1") return &g_tune.gram_n1;
+  if (s == "concurrent") return &g_tune.concurrent;
+  if (s == "sweep_stage") return &g_tune.sweep_stage;
+  if (s == "id_lds") return &g_tune.id_lds;
+  if (s == "id_reg") return &g_tune.id_reg;
+  return nullptr;
+}
+
+int rskx_set_tuning(const char* name, double value) {
+  int* slot = tuning_slot(name);
+  if (!slot) return fail(RSKX_ERR_ARG, std::string("FLAM:rskx:tuning: unknown parameter ") + (name ? name : "NULL"));
+  if (!(value >= 0) || value > 2147483647.0) return fail(RSKX_ERR_ARG, "FLAM:rskx:tuning: value out of range");
+  const std::string s(name);
+  if ((s == "elim8_lds" || s == "sweep_stage" || s == "id_lds") && (value < 4096 || value > 160 * 1024))
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:tuning: shared-memory budgets lie in [4096, 163840] bytes");
+  RSKX_TRY(sync_stream());                 // queued work keeps the values it was launched with
+  *slot = (int)value;
+  return RSKX_OK;
+}
+
+int rskx_get_tuning(const char* name, double* value) {
+  int* slot = tuning_slot(name);
+  if (!slot || !value) return fail(RSKX_ERR_ARG, "FLAM:rskx:tuning: unknown parameter or NULL value");
+  *value = *slot;
   return RSKX_OK;
 }
 
@@ -2052,6 +2135,79 @@ int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char tr
 
 int rskx_rskelf_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype) {
   return sweep_entry(F, X, ldx, nrhs, trans, dtype, false);
+}
+
+int rskx_sv_graph(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
+                  rskx_graph** out) {
+  if (!F || !out) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL argument");
+  *out = nullptr;
+  char t = (char)tolower(trans);
+  if (t != 'n' && t != 't' && t != 'c')
+    return fail(RSKX_ERR_ARG, "FLAM:chktrans:invalidTrans: Transpose parameter must be one of 'N', 'T', or 'C'.");
+  if (t == 't') t = 'c';
+  if (nrhs < 1 || ldx < F->N || !X) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad X, nrhs or leading dimension");
+  if (F->cplx) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:graph: built for real factors");
+  if (dtype != RSKX_F64 && dtype != RSKX_F32) return fail(RSKX_ERR_ARG, "FLAM:rskx:dtype: unknown element type");
+  const int ncol = (nrhs + 7) / 8 * 8;
+  size_t yb = 0, gb = 0;
+  if (F->prec == RSKX_PREC_FP64) sweep_ws_bytes<double>(F, ncol, true, 3, &yb, &gb);
+  else sweep_ws_bytes<float>(F, ncol, true, 3, &yb, &gb);
+  rskx_graph* G = new rskx_graph();
+  auto run = [&](void* Xp) -> int {
+    if (F->prec == RSKX_PREC_FP64)
+      return dtype == RSKX_F64 ? sweep_impl<double, double>(F, Xp, ldx, nrhs, t, true, 3, G->ws.p, yb + gb)
+                               : sweep_impl<double, float>(F, Xp, ldx, nrhs, t, true, 3, G->ws.p, yb + gb);
+    return dtype == RSKX_F64 ? sweep_impl<float, double>(F, Xp, ldx, nrhs, t, true, 3, G->ws.p, yb + gb)
+                             : sweep_impl<float, float>(F, Xp, ldx, nrhs, t, true, 3, G->ws.p, yb + gb);
+  };
+  int rc = G->ws.ensure(std::max<size_t>(yb + gb, 256));
+  // one uncaptured run on a zeroed copy of X's shape (X untouched) sets the
+  // kernels' shared-memory attributes before capture
+  if (rc == RSKX_OK) {
+    DevBuf tmp;
+    const size_t xb = (size_t)ldx * nrhs * (dtype == RSKX_F64 ? 8 : 4);
+    rc = tmp.ensure(xb);
+    if (rc == RSKX_OK && hipMemsetAsync(tmp.p, 0, xb, g_stream) != hipSuccess)
+      rc = fail(RSKX_ERR_HIP, "FLAM:rskx:hip: hipMemsetAsync failed");
+    if (rc == RSKX_OK) rc = run(tmp.p);
+    if (rc == RSKX_OK) rc = sync_stream();
+    tmp.release();
+  }
+  if (rc == RSKX_OK && hipStreamCreateWithFlags(&G->cap, hipStreamNonBlocking) != hipSuccess)
+    rc = fail(RSKX_ERR_HIP, "FLAM:rskx:hip: capture stream");
+  if (rc == RSKX_OK) {
+    const hipStream_t keep = g_stream;
+    g_stream = G->cap;
+    if (hipStreamBeginCapture(G->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      rc = fail(RSKX_ERR_HIP, "FLAM:rskx:hip: hipStreamBeginCapture failed");
+    } else {
+      const int rr = run(X);
+      const hipError_t e = hipStreamEndCapture(G->cap, &G->graph);
+      if (rr != RSKX_OK) rc = rr;
+      else if (e != hipSuccess) rc = fail(RSKX_ERR_HIP, "FLAM:rskx:hip: hipStreamEndCapture failed");
+    }
+    g_stream = keep;
+  }
+  if (rc == RSKX_OK && hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = fail(RSKX_ERR_HIP, "FLAM:rskx:hip: hipGraphInstantiate failed");
+  if (rc != RSKX_OK) {
+    (void)hipGetLastError();
+    delete G;
+    return rc;
+  }
+  *out = G;
+  return RSKX_OK;
+}
+
+int rskx_graph_launch(rskx_graph* G) {
+  if (!G) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: graph is NULL");
+  RSKX_HIP(hipGraphLaunch(G->exec, g_stream));
+  return RSKX_OK;
+}
+
+void rskx_graph_free(rskx_graph* G) {
+  if (G) (void)hipStreamSynchronize(g_stream);
+  delete G;
 }
 
 int rskx_rskelf_sv_noalloc(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,

@@ -86,7 +86,8 @@ def _seg_norms(W, segments):
 def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None, inner_matvec=None):
     """Solve A X = B.  matvec, precond: callables on (N, nrhs) FP64 CUDA
     tensors; precond may also be a Factor, whose solve then runs in place
-    on the stored preconditioned direction.  The small least-squares
+    on the stored preconditioned direction, or a graph.SolveGraph (the
+    same solve replayed from a HIP graph).  The small least-squares
     problems are kept in QR form by Givens rotations, one new rotation per
     step, on the host.  segments: optional row ranges [(a, b), ...] of independent
     problems stacked in one system (a block-diagonal A); then every
@@ -146,6 +147,9 @@ def fgmres(matvec, B, precond, tol=1e-10, restart=16, maxit=8, segments=None, in
             if isinstance(precond, Factor):
                 Z[:, j, :].copy_(V[:, j, :])
                 rskelf_sv_(precond, Z[:, j, :])
+            elif hasattr(precond, "solve_"):             # a captured solve (graph.SolveGraph)
+                Z[:, j, :].copy_(V[:, j, :])
+                precond.solve_(Z[:, j, :])
             else:
                 Z[:, j, :] = op(precond, V[:, j, :])
             w = op(inner_matvec or matvec, Z[:, j, :])

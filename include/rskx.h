@@ -211,6 +211,31 @@ int rskx_rskelf_sv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char tr
 int rskx_rskelf_mv(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans,
                    int dtype);
 
+/* rskelf_sv captured in a HIP graph: the solve of F on the fixed device
+ * block X (N x nrhs, leading dimension ldx) is recorded once
+ * (rskx_sv_graph, which runs it once on its own rows first and leaves X
+ * untouched) and replayed in place on X by rskx_graph_launch, on the
+ * library stream, as often as needed (an FGMRES preconditioner: same
+ * launches every step).  The graph owns its work rows; F must outlive it.
+ * No reference counterpart (MATLAB has no graphs).  Real factors. */
+typedef struct rskx_graph rskx_graph;
+int rskx_sv_graph(const rskx_factor* F, void* X, int64_t ldx, int nrhs, char trans, int dtype,
+                  rskx_graph** out);
+int rskx_graph_launch(rskx_graph* G);
+void rskx_graph_free(rskx_graph* G);
+
+/* Launch-shape parameters chosen by measurement (flam_amd/tune.py): the
+ * largest box of the 8-wave register elimination ("elim8_nmax") and its
+ * shared-memory budget ("elim8_lds"), the largest box of Gram-ID classes 0
+ * and 1 ("gram_n0", "gram_n1"), concurrent size classes ("concurrent"),
+ * the sweep's row / stage buffer budget ("sweep_stage"), the shared
+ * bytes above which the QR ID keeps Householder vectors in global memory
+ * ("id_lds") and the register QRCP classes on or off ("id_reg").  Values
+ * apply to later calls; results do not depend on them (only which kernel
+ * shape runs). */
+int rskx_set_tuning(const char* name, double value);
+int rskx_get_tuning(const char* name, double* value);
+
 /* rskelf_sv without allocating: no counterpart in the reference (MATLAB
  * allocates freely).  For callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -181,3 +181,13 @@ def test_complex_rhs_needs_complex_factor():
     Fz = Factor(None, 10, None, cplx=True)
     assert _rhs_dtype(Fz, torch.zeros(10)) == torch.complex128
     assert _rhs_dtype(Fz, torch.zeros(10, dtype=torch.complex64)) == torch.complex64
+
+
+def test_tuning_parameters_checked():
+    L = _lib()
+    lib = L.lib()
+    assert lib.rskx_set_tuning(b"no_such_knob", 1.0) == L.RSKX_ERR_ARG
+    assert L.last_error().startswith("FLAM:rskx:tuning")
+    assert lib.rskx_set_tuning(b"elim8_lds", 200 * 1024) == L.RSKX_ERR_ARG
+    v = ctypes.c_double()
+    assert lib.rskx_get_tuning(b"concurrent", ctypes.byref(v)) == 0 and v.value == 1

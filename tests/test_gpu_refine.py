@@ -136,3 +136,74 @@ def test_fgmres_zero_column():
     R = B - g.circulant_matvec(X)
     for c in (0, 2):
         assert (torch.linalg.vector_norm(R[:, c]) / torch.linalg.vector_norm(B[:, c])).item() <= 1e-10
+
+
+def test_graph_captured_solve():
+    """rskelf_sv captured in a HIP graph: every replay equals the eager
+    solve bit for bit, with eager work and synchronizations between the
+    replays, and FGMRES preconditioned by the graph takes the same iterates
+    as with the factor itself."""
+    import flam_amd
+    from flam_amd import geometry
+    from flam_amd.graph import SolveGraph
+    from flam_amd.refine import fgmres
+    N = 1 << 16
+    print(f"graph-captured solve: circle N={N} fp32", flush=True)
+    g = geometry.circle(N)
+    F = flam_amd.rskelf(g.kernel, g.x, 256, 1e-6, g.proxy, dict(symm="p", precision="fp32"))
+    G = SolveGraph(F, 8)
+    rng = np.random.default_rng(9)
+    for rep in range(4):
+        B = torch.from_numpy(rng.standard_normal((8, N))).cuda()
+        eager = flam_amd.rskelf_sv(F, B.t()).t()
+        flam_amd.rskelf_mv(F, B.t())                  # other eager work in between
+        torch.cuda.synchronize()
+        Xt = B.clone()
+        G.solve_(Xt)
+        assert torch.equal(Xt, eager), rep
+    B = torch.from_numpy(rng.standard_normal((N, 8))).cuda()
+    X1, i1 = fgmres(g.circulant_matvec, B, F, tol=1e-10, inner_matvec=g.closed_form_matvec)
+    X2, i2 = fgmres(g.circulant_matvec, B, G, tol=1e-10, inner_matvec=g.closed_form_matvec)
+    assert i1["iters"] == i2["iters"] and torch.equal(X1, X2)
+
+
+def test_tuned_shapes_give_the_same_factor():
+    """Launch-shape parameters change which kernel shape a box runs on, not
+    the result: the level table is unchanged; concurrent classes off gives
+    the same factor bit for bit, the 8-wave register shape narrowed sends
+    boxes to the 16-wave shape (same arithmetic up to FP32 rounding), and a
+    small QR shared-memory budget forces the QR ID's Householder vectors to
+    global memory (FP64 QRCP route)."""
+    import flam_amd
+    from flam_amd import geometry, tune
+    N = 1 << 17
+    print(f"tuning parameters: circle N={N}", flush=True)
+    g = geometry.circle(N)
+    B = torch.from_numpy(np.random.default_rng(3).standard_normal((N, 2))).cuda()
+    o32 = dict(symm="p", precision="fp32")
+    F0 = flam_amd.rskelf(g.kernel, g.x, 256, 1e-6, g.proxy, o32)
+    X0 = flam_amd.rskelf_sv(F0, B)
+    try:
+        tune.set_tuning("concurrent", 0)
+        F1 = flam_amd.rskelf(g.kernel, g.x, 256, 1e-6, g.proxy, o32)
+        assert F1.level_table() == F0.level_table()
+        assert torch.equal(flam_amd.rskelf_sv(F1, B), X0)
+        tune.set_tuning("concurrent", 1)
+        tune.set_tuning("elim8_nmax", 128)
+        F2 = flam_amd.rskelf(g.kernel, g.x, 256, 1e-6, g.proxy, o32)
+        assert F2.level_table() == F0.level_table()
+        X2 = flam_amd.rskelf_sv(F2, B)
+        assert (X2 - X0).abs().max().item() <= 1e-5 * X0.abs().max().item()
+        tune.set_tuning("elim8_nmax", 1 << 30)
+        o64 = dict(symm="n", id_route="qr")
+        Fq = flam_amd.rskelf(g.kernel, g.x, 128, 1e-10, g.proxy, o64)
+        tune.set_tuning("id_reg", 0)               # every QR box to the global kernel
+        tune.set_tuning("id_lds", 4096)            # ... with its vectors in global memory
+        Fg = flam_amd.rskelf(g.kernel, g.x, 128, 1e-10, g.proxy, o64)
+        assert Fg.level_table() == Fq.level_table()
+        Xq, Xg = flam_amd.rskelf_sv(Fq, B), flam_amd.rskelf_sv(Fg, B)
+        assert (Xg - Xq).abs().max().item() <= 1e-12 * Xq.abs().max().item()
+    finally:
+        for k, v in (("concurrent", 1), ("elim8_nmax", 1 << 30), ("id_lds", 160 * 1024),
+                     ("id_reg", 1)):
+            tune.set_tuning(k, v)
