@@ -25,21 +25,31 @@ RSKX_Z32 = 3
 RSKX_KERNEL_LAPLACE2D_DL = 1
 RSKX_KERNEL_LAPLACE3D_SL = 2
 RSKX_KERNEL_HELMHOLTZ2D_CFIE = 3
+RSKX_KERNEL_CALLBACK = 4
 RSKX_PROXY_NONE = 0
 RSKX_PROXY_RING_SL = 1
 RSKX_PROXY_RING_DL = 2
 RSKX_PROXY_POINTS_SL = 3
 RSKX_PROXY_RING_CFIE = 4
+RSKX_PROXY_CALLBACK = 5
+
+# host callbacks of RSKX_KERNEL_CALLBACK / RSKX_PROXY_CALLBACK
+ENTRIES_FN = ctypes.CFUNCTYPE(c_int, c_void_p, POINTER(c_int64), c_int64, POINTER(c_int64), c_int64,
+                              POINTER(c_double))
+PROXY_FN = ctypes.CFUNCTYPE(c_int, c_void_p, POINTER(c_int64), c_int64, POINTER(c_int64),
+                            POINTER(c_int64), POINTER(c_double), POINTER(c_double),
+                            POINTER(c_double), c_int64, POINTER(c_int64))
 
 
 class Kernel(ctypes.Structure):
     _fields_ = [("kind", c_int), ("nu", c_void_p), ("w", c_void_p), ("diag", c_void_p),
-                ("scale", c_double), ("wavenumber", c_double), ("eta", c_double)]
+                ("scale", c_double), ("wavenumber", c_double), ("eta", c_double),
+                ("entries", ENTRIES_FN), ("ctx", c_void_p)]
 
 
 class Proxy(ctypes.Structure):
     _fields_ = [("kind", c_int), ("p", c_int), ("radius", c_double), ("weight", c_double),
-                ("pts", c_void_p)]
+                ("pts", c_void_p), ("fn", PROXY_FN), ("ctx", c_void_p)]
 
 
 class Opts(ctypes.Structure):

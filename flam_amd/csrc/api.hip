@@ -273,6 +273,7 @@ struct Workspace {
   DevBuf sweep;                 // sweep work rows of factors too large for shared memory
   DevBuf cgs;                   // Gram-Schmidt partial sums
   DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs, poff;
+  DevBuf Ablk;                  // host-callback matrices: A(slf,slf) of the level's boxes
   int* hflag = nullptr;
 };
 static Workspace g_ws;
@@ -800,6 +801,113 @@ int box_features(const HostTree& t, const PointsDev& P, int d, double tol, doubl
   return RSKX_OK;
 }
 
+// One level of a host-callback factorization (RSKX_KERNEL_CALLBACK): read
+// back each box's slf list and its neighbour candidates (k_lists ran without
+// a filter), call pxyfun for the filtered neighbours and proxy rows
+// (rskelf.m:205, levels > 2), evaluate K = [A(nbr,slf); A(slf,nbr)' (symm
+// 'n'); Kpxy] (rskelf.m:210-212) and A(slf,slf) (rskelf.m:219) through the
+// entries callback, and upload them: K into the K arena for the QRCP route
+// (its rows in mrows, leading dimension LvlBox::mcap), A(slf,slf) into
+// Workspace::Ablk for the elimination.  hmn receives the kept neighbours.
+template <typename S>
+static int callback_level(const rskx_kernel& A, const rskx_proxy& PX, int d, const HostTree& t,
+                          int64_t b0, int64_t nb, bool use_proxy, bool two, int64_t pool_top,
+                          int64_t nslf_all, int64_t nnbr_all, const std::vector<int>& ord,
+                          std::vector<LvlBox>& hb, std::vector<int>& hmn) {
+  Workspace& ws = g_ws;
+  std::vector<int> slf_all(std::max<int64_t>(nslf_all, 1)), nbr_all(std::max<int64_t>(nnbr_all, 1));
+  std::vector<int> mnb(nb);
+  if (nslf_all)
+    RSKX_TRY(download(slf_all.data(), ws.pool.as<int>() + pool_top, nslf_all * sizeof(int)));
+  if (nnbr_all) RSKX_TRY(download(nbr_all.data(), ws.nbr.p, nnbr_all * sizeof(int)));
+  RSKX_TRY(download(mnb.data(), ws.mnbr.p, nb * sizeof(int)));
+  RSKX_TRY(sync_stream());
+  std::vector<int64_t> I, J;
+  std::vector<double> blk, Kh, Kp, ent;
+  std::vector<S> Kall;
+  std::vector<double> Aall;
+  std::vector<int> mrow(nb, 0);
+  int64_t maxp = std::max(PX.p, 0);      // proxy-row buffer, grown on request
+  int64_t offK = 0, offA = 0;
+  auto bad = [](const char* what, int rc) {
+    return fail(RSKX_ERR_ARG, std::string("FLAM:rskx:callback: ") + what + " returned " + std::to_string(rc));
+  };
+  for (int64_t ib = 0; ib < nb; ++ib) {
+    LvlBox& B = hb[ib];
+    const int n = B.n;
+    const int b = (int)(b0 + ib);
+    B.A_off = offA;
+    if (n == 0) {
+      B.mcap = 1;
+      B.K_off = offK;
+      offK += 0;
+      hmn[ib] = 0;
+      continue;
+    }
+    I.resize(n);
+    for (int i = 0; i < n; ++i) I[i] = ord[slf_all[B.slf_off - pool_top + i]];
+    int64_t nn = mnb[ib];
+    J.resize(std::max<int64_t>(nn, 1));
+    for (int64_t j = 0; j < nn; ++j) J[j] = ord[nbr_all[B.nbr_off + j]];
+    int64_t rows = 0;
+    if (use_proxy) {
+      // a callback that needs more rows than the buffer holds says so in
+      // *rows and is called once more with a buffer of that many rows
+      const int64_t nn0 = nn;
+      for (int attempt = 0; attempt < 2; ++attempt) {
+        Kp.assign((size_t)std::max<int64_t>(maxp, 1) * n, 0.0);
+        nn = nn0;
+        for (int64_t j = 0; j < nn0; ++j) J[j] = ord[nbr_all[B.nbr_off + j]];
+        rows = 0;
+        const int rc = PX.fn(PX.ctx, I.data(), n, J.data(), &nn, &t.bsize[(size_t)d * b],
+                             &t.ctr[(size_t)d * b], Kp.data(), maxp, &rows);
+        if (rc) return bad("pxyfun", rc);
+        if (rows <= maxp || attempt == 1) break;
+        maxp = rows;
+      }
+      if (nn < 0 || nn > nn0 || rows < 0 || rows > maxp)
+        return fail(RSKX_ERR_ARG, "FLAM:rskx:callback: pxyfun returned " + std::to_string(rows) +
+                                      " proxy rows (buffer " + std::to_string(maxp) + ") and " +
+                                      std::to_string(nn) + " of " + std::to_string(nn0) + " neighbours");
+    }
+    const int64_t m = nn * (two ? 2 : 1) + rows;
+    hmn[ib] = (int)nn;
+    mrow[ib] = (int)m;
+    B.mcap = (int)std::max<int64_t>(m, 1);
+    B.K_off = offK;
+    Kall.resize(offK + (int64_t)B.mcap * n);
+    S* K = Kall.data() + offK;
+    if (nn) {
+      ent.resize((size_t)nn * n);
+      int rc = A.entries(A.ctx, J.data(), nn, I.data(), n, ent.data());      // A(nbr,slf)
+      if (rc) return bad("A(I,J)", rc);
+      for (int c = 0; c < n; ++c)
+        for (int64_t i = 0; i < nn; ++i) K[i + (int64_t)c * m] = (S)ent[i + (size_t)c * nn];
+      if (two) {
+        rc = A.entries(A.ctx, I.data(), n, J.data(), nn, ent.data());         // A(slf,nbr)'
+        if (rc) return bad("A(I,J)", rc);
+        for (int c = 0; c < n; ++c)
+          for (int64_t i = 0; i < nn; ++i) K[nn + i + (int64_t)c * m] = (S)ent[c + (size_t)i * n];
+      }
+    }
+    for (int c = 0; c < n; ++c)
+      for (int64_t i = 0; i < rows; ++i)
+        K[m - rows + i + (int64_t)c * m] = (S)Kp[i + (size_t)c * maxp];
+    offK += (int64_t)B.mcap * n;
+    Aall.resize(offA + (int64_t)n * n);
+    int rc = A.entries(A.ctx, I.data(), n, I.data(), n, Aall.data() + offA);   // A(slf,slf)
+    if (rc) return bad("A(I,J)", rc);
+    offA += (int64_t)n * n;
+  }
+  if (Kall.empty()) Kall.assign(1, (S)0);
+  if (Aall.empty()) Aall.assign(1, 0.0);
+  RSKX_TRY(upload(ws.K, Kall));
+  RSKX_TRY(upload(ws.Ablk, Aall));
+  RSKX_TRY(upload(ws.mrows, mrow));
+  RSKX_TRY(upload(ws.boxes, hb));
+  return RSKX_OK;
+}
+
 template <typename S>
 int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* poff, int occ,
                 double rank_or_tol, const rskx_kernel& A, const rskx_proxy& PX,
@@ -847,6 +955,14 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     P.eta = A.eta;
   }
   RSKX_HIP(hipGetLastError());
+  // host-callback matrix: caller indices of the tree order, for A(I,J) and pxyfun
+  const bool cb = A.kind == RSKX_KERNEL_CALLBACK;
+  std::vector<int> cb_ord;
+  if (cb) {
+    cb_ord.resize(N);
+    RSKX_TRY(download(cb_ord.data(), F->order.p, N * sizeof(int)));
+    RSKX_TRY(sync_stream());
+  }
 
   // ---- rank model: features of every box, predicted caps of the ranks
   const double tol_id = std::fmod(rank_or_tol, 1.0);
@@ -961,6 +1077,10 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     L.p = L.np * ((PX.kind == RSKX_PROXY_POINTS_SL || PX.kind == RSKX_PROXY_RING_CFIE) && two ? 2 : 1);
     L.filter = use_proxy ? 1 : 0;
     L.two_sided = two ? 1 : 0;
+    if (cb) {              // pxyfun filters and forms the proxy rows on the host
+      L.np = L.p = 0;
+      L.filter = 0;
+    }
 
     hb.assign(nb, LvlBox{});
     hn.clear();
@@ -1041,8 +1161,15 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
                  ws.nbr.as<int>(), ws.mnbr.as<int>(), P, L, (int)nb, g_stream);
     prof.end("lists", ev);
     std::vector<int> hmn(nb);                  // neighbour rows the routing sized K for
-    const bool gram = !CPLX && prm.tol >= 1e-7 &&
+    const bool gram = !cb && !CPLX && prm.tol >= 1e-7 &&
                       (o.id_gram > 0 || (o.id_gram == 0 && sizeof(S) == 4));
+    if constexpr (!CPLX) {
+      if (cb) {
+        RSKX_TRY(callback_level<S>(A, PX, d, t, b0, nb, use_proxy, two, pool_top, off_slf, off_nbr,
+                                   cb_ord, hb, hmn));
+        P.blk = ws.Ablk.as<double>();
+      }
+    }
     if constexpr (CPLX) {
       // complex: every box assembles K in global memory and runs the
       // complex QRCP ID (k_zid)
@@ -1083,14 +1210,16 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // whose shared memory is sized by the unfiltered neighbour count, so
     // the host need not wait for k_lists; the QR routes read back the
     // filtered counts to choose register classes
-    bool exact = !gram;
-    for (int64_t ib = 0; ib < nb && !exact; ++ib) {
+    bool exact = !gram && !cb;
+    for (int64_t ib = 0; ib < nb && gram && !exact; ++ib) {
       const int n = hb[ib].n, mc = hb[ib].mc;
       if (n > 0 && gram_id_class(n) >= 0 &&
           gram_id_shmem(gram_class_tuned(n), n, mc, mc * (two ? 2 : 1) + L.p, L.p) > 160 * 1024)
         exact = true;                        // the bound is too loose: use the filtered counts
     }
-    if (!exact) {
+    if (cb) {
+      // hmn holds the neighbours pxyfun kept
+    } else if (!exact) {
       for (int64_t ib = 0; ib < nb; ++ib) hmn[ib] = hb[ib].mc;
     } else {
       RSKX_TRY(download(hmn.data(), ws.mnbr.p, nb * sizeof(int)));
@@ -1122,7 +1251,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       const size_t by = (size_t)8 * 1024 * sizeof(S) +
                         (size_t)(3 * L.p + 7 * n + 5 * hmn[ib] + mb + 32) * sizeof(double) +
                         (size_t)2 * n * sizeof(int) + 64;
-      if (cls >= 0 && g_tune.id_reg && n > 0 && mb > 0 && by <= 160 * 1024) {
+      if (!cb && cls >= 0 && g_tune.id_reg && n > 0 && mb > 0 && by <= 160 * 1024) {
         idr[cls].push_back((int)ib);
         shm_r[cls] = std::max(shm_r[cls], by);
       } else {
@@ -1160,7 +1289,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     int64_t Kbytes = 0;
     for (int ib : idg) Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
     prof.begin("assemble", ev);
-    if (ng)
+    if (ng && !cb)                         // callbacks uploaded K already
       launch_assemble<S>(dids, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
                          ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng, g_stream);
     prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
@@ -1926,6 +2055,10 @@ static int check_kernel(const rskx_kernel* A, int d) {
     return RSKX_OK;
   }
   if (A && A->kind == RSKX_KERNEL_LAPLACE3D_SL && d == 3) return RSKX_OK;
+  if (A && A->kind == RSKX_KERNEL_CALLBACK && (d == 2 || d == 3)) {
+    if (!A->entries) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: the callback kernel needs its entries function");
+    return RSKX_OK;
+  }
   if (A && A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE && d == 2) {
     if (!A->nu) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: the combined field needs normals");
     if (!(A->wavenumber > 0) || !std::isfinite(A->eta))
@@ -1933,7 +2066,8 @@ static int check_kernel(const rskx_kernel* A, int d) {
     return RSKX_OK;
   }
   return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: built kernels are the 2D Laplace double layer (d = 2), "
-                              "the 3D Laplace single layer (d = 3) and the 2D Helmholtz combined field (d = 2)");
+                              "the 3D Laplace single layer (d = 3), the 2D Helmholtz combined field (d = 2) "
+                              "and host callbacks (d = 2, 3)");
 }
 
 // argument checks shared by rskx_rskelf and rskx_rskelf_batch
@@ -1961,7 +2095,8 @@ static int check_factor_args(const double* x, int d, int64_t N, int occ, double 
     if (o.stop >= 0)
       return fail(RSKX_ERR_UNSUP, "FLAM:rskx:stop: partial factorization is built for real data");
   }
-  if (!x || !A->w || !A->diag) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: point or kernel array is NULL");
+  if (!x || (A->kind != RSKX_KERNEL_CALLBACK && (!A->w || !A->diag)))
+    return fail(RSKX_ERR_ARG, "FLAM:rskx:null: point or kernel array is NULL");
   if (N < 1) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: N must be positive");
   if (N > (int64_t)1 << 30) return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: N above 2^30");
   if (o.precision != RSKX_PREC_FP64 && o.precision != RSKX_PREC_FP32)
@@ -1970,6 +2105,11 @@ static int check_factor_args(const double* x, int d, int64_t N, int occ, double 
 }
 
 static int check_proxy(const rskx_proxy* pxy, int d) {
+  if (pxy && pxy->kind == RSKX_PROXY_CALLBACK) {
+    if (!pxy->fn) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: the callback proxy needs its function");
+    if (pxy->p < 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:proxy: p (the most proxy rows) must be >= 0");
+    return RSKX_OK;
+  }
   if (!pxy || (pxy->kind != RSKX_PROXY_RING_SL && pxy->kind != RSKX_PROXY_RING_DL &&
                pxy->kind != RSKX_PROXY_POINTS_SL && pxy->kind != RSKX_PROXY_RING_CFIE))
     return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: naive global compression (no pxyfun) is not built; pass a proxy surface");
@@ -1997,6 +2137,16 @@ static int run_factor(const double* x, int d, int64_t N, int nprob, const int64_
     delete F;
     return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: the combined field takes RSKX_PROXY_RING_CFIE, "
                                 "and that ring only the combined field");
+  }
+  const bool cb = A->kind == RSKX_KERNEL_CALLBACK;
+  if (cb != (px.kind == RSKX_PROXY_CALLBACK)) {
+    delete F;
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:proxy: a callback matrix takes RSKX_PROXY_CALLBACK, "
+                                "and that proxy only a callback matrix");
+  }
+  if (cb && nprob > 1) {
+    delete F;
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:batch: callback matrices are factored one problem at a time");
   }
   F->cplx = cfie;
   int rc;
@@ -2570,6 +2720,8 @@ int rskx_kernel_times(const rskx_factor* F, const char** names, double* ms, int6
 int rskx_kernel_matvec(const double* x, int d, int64_t N, const rskx_kernel* A, const double* X,
                        int64_t ldx, double* Y, int64_t ldy, int nrhs) {
   RSKX_TRY(check_kernel(A, d));
+  if (A->kind == RSKX_KERNEL_CALLBACK)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: no device product of a host-callback matrix");
   if (!x || !X || !Y) return fail(RSKX_ERR_ARG, "FLAM:rskx:null: NULL array");
   if (ldx < N || ldy < N || nrhs < 0) return fail(RSKX_ERR_ARG, "FLAM:rskx:size: bad leading dimension");
   if (A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE)   // X, Y interleaved complex, ld in complex elements
@@ -2729,8 +2881,8 @@ int rskx_kernel_matvec_batch(int nprob, const int64_t* offsets, const double* x,
                              const rskx_kernel* A, const double* X, int64_t ldx, double* Y,
                              int64_t ldy, int nrhs) {
   RSKX_TRY(check_kernel(A, d));
-  if (A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE)
-    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: the batched product is built for the real kernels");
+  if (A->kind == RSKX_KERNEL_HELMHOLTZ2D_CFIE || A->kind == RSKX_KERNEL_CALLBACK)
+    return fail(RSKX_ERR_UNSUP, "FLAM:rskx:kernel: the batched product is built for the real built-in kernels");
   if (nprob < 1 || !offsets || offsets[0] != 0)
     return fail(RSKX_ERR_ARG, "FLAM:rskx:batch: need nprob >= 1 and offsets starting at 0");
   for (int j = 0; j < nprob; ++j)

@@ -34,6 +34,8 @@ struct PointsDev {
   int kind;               // RSKX_KERNEL_*
   double scale;           // off-diagonal multiplier
   double k, eta;          // wavenumber and coupling of RSKX_KERNEL_HELMHOLTZ2D_CFIE
+  const double* blk;      // RSKX_KERNEL_CALLBACK: every box's A(slf,slf) (n x n,
+                          // slf order) at LvlBox::A_off, evaluated by the host
 };
 
 // proxy / filter parameters of one level (box sizes and proxy weights are
@@ -63,6 +65,7 @@ struct LvlBox {
   int64_t perm_off;       // offset in the ID permutation arena (n)
   int64_t T_off;          // offset in the double T/R scratch (3 ldt n)
   int64_t W_off;          // offset in the factor scratch (n^2)
+  int64_t A_off;          // RSKX_KERNEL_CALLBACK: offset of A(slf,slf) in PointsDev::blk
   double ctr[3];
   double l[3];            // box size (its tree's size at this level)
   double pw;              // single-layer proxy multiplier of its problem

@@ -645,7 +645,8 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
   for (int j = (j0) + wid; j < (j1); j += nw) for (int i = (i0) + lane; i < (i1); i += kWave)
   auto kval = [&](int a, int c) -> S {
     double v;
-    if (a == c) v = cdg[c];
+    if (P.kind == RSKX_KERNEL_CALLBACK) v = P.blk[B.A_off + q[a] + (int64_t)q[c] * n];
+    else if (a == c) v = cdg[c];
     else if (P.kind == RSKX_KERNEL_LAPLACE3D_SL) v = sl3_entry(P, cx[a], cy[a], cnx[a], cx[c], cy[c], cnx[c], cw[c]);
     else {
       double dx = cx[a] - cx[c], dy = cy[a] - cy[c];
@@ -1552,6 +1553,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
   __syncthreads();
   const bool leafbox = cm.nc == 1 && cm.moff[0] < 0;
   auto aval = [&](int p1, int p2) -> double {    // A entry between Z positions
+    if (P.kind == RSKX_KERNEL_CALLBACK) return P.blk[B.A_off + qp[p1] + (int64_t)qp[p2] * n];
     if (p1 == p2) return cdg[p2];
     if (P.kind == RSKX_KERNEL_LAPLACE3D_SL)
       return sl3_entry(P, cx[p1], cy[p1], cnx[p1], cx[p2], cy[p2], cnx[p2], cw[p2]);

@@ -897,8 +897,8 @@ __global__ void k_gather_points(const double* __restrict__ x, const double* __re
       pts[i * N + p] = x[d * q + i];
       pts[(d + i) * N + p] = nu ? nu[d * q + i] : 0.0;   // kernels without normals
     }
-    pts[2 * d * N + p] = w[q];
-    pts[(2 * d + 1) * N + p] = diag[q];
+    pts[2 * d * N + p] = w ? w[q] : 0.0;             // host-callback matrices carry
+    pts[(2 * d + 1) * N + p] = diag ? diag[q] : 0.0; // neither weights nor diagonal
   }
 }
 

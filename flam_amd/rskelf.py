@@ -4,9 +4,13 @@
     Y = rskelf_sv(F, X, trans='n')                                # rskelf/rskelf_sv.m:13
     Y = rskelf_mv(F, X, trans='n')                                # rskelf/rskelf_mv.m:13
 
-Differences forced by the device: A is a kernel description (LaplaceDL2D)
-rather than a MATLAB function handle, pxyfun a ProxyRing rather than a
-callback, and all arrays are torch tensors on the GPU.  Argument errors
+A is a kernel description (LaplaceDL2D, ...) and pxyfun a proxy
+description (ProxyRing, ...) for the matrices the device evaluates itself;
+any other real matrix is given as the reference gives it, by Python
+callables A(I, J) and pxyfun(x, slf, nbr, l, ctr) (0-based indices), which
+the library calls on the host for every box (a slow path: the compression
+and elimination still run on the device).  Device arrays are torch
+tensors on the GPU.  Argument errors
 raise FlamError carrying the reference's identifiers (FLAM:chksymm:...,
 FLAM:id:..., FLAM:hypoct:...).  Like MATLAB, rskelf_sv / rskelf_mv return
 a new array and leave X untouched.
@@ -300,6 +304,67 @@ def _opts(opts, d):
     return o, ext_buf
 
 
+class _Callbacks:
+    """ctypes trampolines for a matrix given by Python callables: A(I, J)
+    returns the (len(I), len(J)) block, pxyfun(x, slf, nbr, l, ctr) returns
+    (Kpxy, kept neighbours) as rskelf.m:205 calls it (x: the d x N host
+    points).  An exception inside a callback aborts the factorization and
+    is raised again by rskelf."""
+
+    def __init__(self, A, pxyfun, xh, prows):
+        import numpy as np
+        self.np = np
+        self.A, self.pxyfun, self.xh = A, pxyfun, xh
+        self.exc = None
+        self.pending = None                 # a proxy block too large for the buffer
+        self.entries = L.ENTRIES_FN(self._entries)
+        self.proxy_fn = L.PROXY_FN(self._proxy)
+        self.prows = int(prows)
+
+    def _entries(self, ctx, I, nI, J, nJ, out):
+        np = self.np
+        try:
+            Ii = np.ctypeslib.as_array(I, shape=(nI,)).copy() if nI else np.zeros(0, np.int64)
+            Jj = np.ctypeslib.as_array(J, shape=(nJ,)).copy() if nJ else np.zeros(0, np.int64)
+            blk = np.asarray(self.A(Ii, Jj), dtype=np.float64).reshape(nI, nJ)
+            if nI and nJ:
+                np.ctypeslib.as_array(out, shape=(nJ, nI))[...] = blk.T
+            return 0
+        except BaseException as e:          # noqa: BLE001 - raised again after the call
+            self.exc = self.exc or e
+            return 1
+
+    def _proxy(self, ctx, slf, nslf, nbr, nnbr, l, ctr, Kpxy, maxrows, rows):
+        np = self.np
+        try:
+            d = self.xh.shape[0]
+            s_ = np.ctypeslib.as_array(slf, shape=(nslf,)).copy()
+            n0 = nnbr[0]
+            nb = np.ctypeslib.as_array(nbr, shape=(n0,)).copy() if n0 else np.zeros(0, np.int64)
+            lv = np.ctypeslib.as_array(l, shape=(d,)).copy()
+            cv = np.ctypeslib.as_array(ctr, shape=(d,)).copy()
+            K, kept = self.pxyfun(self.xh, s_, nb, lv, cv)
+            K = np.asarray(K, dtype=np.float64).reshape(-1, nslf)
+            kept = np.asarray(kept, dtype=np.int64).reshape(-1)
+            rows[0] = K.shape[0]
+            if K.shape[0] > maxrows:        # called again with a larger buffer
+                return 0
+            if len(kept):
+                np.ctypeslib.as_array(nbr, shape=(n0,))[:len(kept)] = kept
+            nnbr[0] = len(kept)
+            if K.shape[0] and nslf:
+                np.ctypeslib.as_array(Kpxy, shape=(nslf, maxrows))[:, :K.shape[0]] = K.T
+            return 0
+        except BaseException as e:          # noqa: BLE001
+            self.exc = self.exc or e
+            return 1
+
+    def structs(self):
+        k = L.Kernel(L.RSKX_KERNEL_CALLBACK, None, None, None, 1.0, 0.0, 0.0, self.entries, None)
+        p = L.Proxy(L.RSKX_PROXY_CALLBACK, self.prows, 1.5, 0.0, None, self.proxy_fn, None)
+        return k, p
+
+
 def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     """F = rskelf(A, x, occ, rank_or_tol, pxyfun, opts) on the GPU.
 
@@ -312,19 +377,40 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     flam_amd.rank.RankModel: the Gram ID's buffers sized by its predicted
     ranks, boxes beyond their cap redone with full buffers), rank_scale (1)
     and record_boxes (False: keep features and exact ranks of every box,
-    Factor.box_features())."""
+    Factor.box_features()).
+
+    A and pxyfun may instead be Python callables, as the reference takes
+    them: A(I, J) -> the (len(I), len(J)) block of the real matrix,
+    pxyfun(x, slf, nbr, l, ctr) -> (Kpxy, kept neighbours), with 0-based
+    index arrays and x the d x N host points; opts proxy_rows (64) is the
+    first size of the Kpxy buffer (grown on request).  The library calls
+    them on the host for every box (rskelf.m:205,210-219) and runs the
+    compression and elimination on the device."""
+    cbs = None
+    if not hasattr(A, "_struct"):
+        if not callable(A) or not callable(pxyfun):
+            raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:kernel: A must be a kernel description, or "
+                                            "A(I, J) and pxyfun both callables")
+        xh = x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else __import__("numpy").asarray(x)
+        cbs = _Callbacks(A, pxyfun, xh.astype("float64"), (opts or {}).get("proxy_rows", 64))
     x = _dev(x)
     if x.dim() != 2:
         raise FlamError(L.RSKX_ERR_ARG, "FLAM:rskx:size: x must be d x N")
     d, N = x.shape
     xc = x.t().contiguous()                # column-major d x N
     o, ext_buf = _opts(opts, d)
-    k = A._struct()
-    p = pxyfun._struct() if pxyfun is not None else L.Proxy(L.RSKX_PROXY_NONE, 0, 1.5, 0.0)
+    if cbs is not None:
+        k, p = cbs.structs()
+    else:
+        k = A._struct()
+        p = pxyfun._struct() if pxyfun is not None else L.Proxy(L.RSKX_PROXY_NONE, 0, 1.5, 0.0)
     h = ctypes.c_void_p()
     _on_torch_stream()
-    _check(L.lib().rskx_rskelf(xc.data_ptr(), d, N, int(occ), float(rank_or_tol),
-                               ctypes.byref(k), ctypes.byref(p), ctypes.byref(o), ctypes.byref(h)))
+    rc = L.lib().rskx_rskelf(xc.data_ptr(), d, N, int(occ), float(rank_or_tol),
+                             ctypes.byref(k), ctypes.byref(p), ctypes.byref(o), ctypes.byref(h))
+    if cbs is not None and cbs.exc is not None:
+        raise cbs.exc
+    _check(rc)
     return Factor(h.value, N, keep=(A, xc, opts and opts.get("rank_model")),
                   cplx=isinstance(A, HelmholtzCFIE2D))
 

@@ -74,6 +74,22 @@ extern "C" {
  *   ((x - y) . n_y) / r;  A_ii = diag_i, complex: diag holds 2 N doubles
  *   (re, im interleaved).  d = 2; scale is not read.  The factor is complex
  *   (FP32 precision: complex float storage), symm 'n' or 's'. */
+#define RSKX_KERNEL_CALLBACK 4
+/*   any real matrix, its entries evaluated on the host by `entries` (the
+ *   reference's A(I,J) handle, called where rskelf.m:210-219 evaluates
+ *   full(A(nbr,slf)), full(A(slf,nbr)) and full(A(slf,slf))).  The tree is
+ *   built on the device from x as for the built-in kernels; every box's K
+ *   and A(slf,slf) are evaluated on the host, uploaded, and compressed and
+ *   eliminated by the same device kernels (the QRCP route).  nu, w, diag
+ *   are not read; the proxy must be RSKX_PROXY_CALLBACK; d = 2 or 3; real
+ *   data, one problem per factorization.  A slow path for
+ *   matrices no built-in kernel describes: the callbacks run one box at a
+ *   time on the calling thread. */
+/* Fill out (nI x nJ, column-major, leading dimension nI) with A(I,J) at
+ * 0-based caller point indices.  Return 0, or nonzero to abort the
+ * factorization (it then fails with FLAM:rskx:callback). */
+typedef int (*rskx_entries_fn)(void* ctx, const int64_t* I, int64_t nI, const int64_t* J,
+                               int64_t nJ, double* out);
 typedef struct {
   int kind;             /* RSKX_KERNEL_*                                    */
   const double* nu;     /* d x N unit normals (device)                      */
@@ -82,6 +98,8 @@ typedef struct {
   double scale;         /* off-diagonal multiplier                          */
   double wavenumber;    /* HELMHOLTZ2D_CFIE: k > 0                          */
   double eta;           /* HELMHOLTZ2D_CFIE: coupling of the single layer   */
+  rskx_entries_fn entries;  /* CALLBACK: A(I,J) on the host               */
+  void* ctx;            /* CALLBACK: passed to entries unchanged            */
 } rskx_kernel;
 
 /* proxy surfaces (the reference passes pxyfun(x,slf,nbr,l,ctr)) */
@@ -99,6 +117,22 @@ typedef struct {
                                  A(pxy, x_j) and, for symm 'n', the incoming
                                  conj(weight (dG/dn_q - i eta G)(x_j, pxy)) with
                                  the ring's normal n_q; d = 2 */
+#define RSKX_PROXY_CALLBACK 5 /* [Kpxy,nbr] = pxyfun(x,slf,nbr,l,ctr) on the host
+                                 (rskelf.m:205, levels > 2), for
+                                 RSKX_KERNEL_CALLBACK; p = the first size of
+                                 the Kpxy buffer in rows (grown on request) */
+/* The proxy callback: given a box's points slf (nslf 0-based caller
+ * indices), its neighbour candidates nbr (*nnbr entries: the current
+ * skeletons of the neighbouring boxes, rskelf.m:186), the box size l and
+ * centre ctr (d entries each), write the proxy rows Kpxy (*rows x nslf,
+ * column-major, leading dimension maxrows) and compact nbr in place to the
+ * neighbours kept (their count in *nnbr, in their given order).  A callback
+ * with more than maxrows rows sets *rows to its count and returns 0
+ * without writing Kpxy: it is called once more, for the same box, with
+ * maxrows = *rows and the original nbr.  Return 0, or nonzero to abort. */
+typedef int (*rskx_proxy_fn)(void* ctx, const int64_t* slf, int64_t nslf, int64_t* nbr,
+                             int64_t* nnbr, const double* l, const double* ctr, double* Kpxy,
+                             int64_t maxrows, int64_t* rows);
 typedef struct {
   int kind;             /* RSKX_PROXY_*                                     */
   int p;                /* number of proxy points (64 in the demos)         */
@@ -112,6 +146,8 @@ typedef struct {
   const double* pts;    /* POINTS_SL: 3 x p points (device, column-major) in
                            box-size units about the box centre (the unit
                            sphere of ie_sphere.m:39-40 and :175); else NULL  */
+  rskx_proxy_fn fn;     /* CALLBACK: pxyfun on the host                     */
+  void* ctx;            /* CALLBACK: passed to fn unchanged                 */
 } rskx_proxy;
 
 /* A learned rank model (rskx_rank_model_*): a small multilayer perceptron

@@ -191,3 +191,81 @@ def test_tuning_parameters_checked():
     assert lib.rskx_set_tuning(b"elim8_lds", 200 * 1024) == L.RSKX_ERR_ARG
     v = ctypes.c_double()
     assert lib.rskx_get_tuning(b"concurrent", ctypes.byref(v)) == 0 and v.value == 1
+
+
+def _cb_call(L, *, kind=None, pkind=None, entries=True, fn=True, d=2, p=8):
+    o = L.Opts()
+    L.lib().rskx_opts_default(ctypes.byref(o))
+    dummy = 16  # never dereferenced: validation fails first
+    ent = L.ENTRIES_FN(lambda *a: 0) if entries else L.ENTRIES_FN()
+    pfn = L.PROXY_FN(lambda *a: 0) if fn else L.PROXY_FN()
+    kern = L.Kernel(kind if kind is not None else L.RSKX_KERNEL_CALLBACK, None, None, None, 1.0,
+                    0.0, 0.0, ent, None)
+    px = L.Proxy(pkind if pkind is not None else L.RSKX_PROXY_CALLBACK, p, 1.5, 0.0, None, pfn, None)
+    h = ctypes.c_void_p()
+    rc = L.lib().rskx_rskelf(dummy, d, 100, 64, 1e-6, ctypes.byref(kern), ctypes.byref(px),
+                             ctypes.byref(o), ctypes.byref(h))
+    return rc, L.last_error()
+
+
+@pytest.mark.parametrize("kw,code,ident", [
+    (dict(entries=False), "ARG", "FLAM:rskx:null"),
+    (dict(fn=False), "ARG", "FLAM:rskx:null"),
+    (dict(p=-1), "ARG", "FLAM:rskx:proxy"),
+    (dict(d=1), "UNSUP", "FLAM:rskx:kernel"),
+    (dict(pkind=1), "UNSUP", "FLAM:rskx:proxy"),           # callback matrix, device ring
+])
+def test_callback_kernel_arguments(kw, code, ident):
+    """A host-callback matrix needs its entries function, a callback proxy
+    with its function, and d = 2 or 3; checked before any device work."""
+    L = _lib()
+    rc, msg = _cb_call(L, **kw)
+    assert rc == getattr(L, "RSKX_ERR_" + code) and msg.startswith(ident), msg
+
+
+def test_callback_product_and_batch_refused():
+    L = _lib()
+    dummy = 16
+    kern = L.Kernel(L.RSKX_KERNEL_CALLBACK, None, None, None, 1.0, 0.0, 0.0,
+                    L.ENTRIES_FN(lambda *a: 0), None)
+    rc = L.lib().rskx_kernel_matvec(dummy, 2, 10, ctypes.byref(kern), dummy, 10, dummy, 10, 1)
+    assert rc == L.RSKX_ERR_UNSUP and L.last_error().startswith("FLAM:rskx:kernel")
+
+
+def test_callback_trampolines_on_the_host():
+    """The Python trampolines write A(I, J) column-major and the proxy
+    block with its leading dimension, compact the kept neighbours in place,
+    ask for a larger buffer when Kpxy has more rows, and hold a raised
+    exception for rskelf to raise again."""
+    import numpy as np
+    from flam_amd.rskelf import _Callbacks
+    L = _lib()
+    M = np.arange(100.0).reshape(10, 10)
+    xh = np.vstack([np.arange(10.0), np.zeros(10)])
+
+    def pxy(x, slf, nbr, l, ctr):
+        assert x is xh and l.shape == (2,) and ctr.shape == (2,)
+        return M[:3, slf] + l[0], nbr[nbr % 2 == 0]
+
+    cb = _Callbacks(lambda I, J: M[np.ix_(I, J)], pxy, xh, 2)
+    I = (ctypes.c_int64 * 3)(1, 4, 7)
+    J = (ctypes.c_int64 * 2)(0, 9)
+    out = (ctypes.c_double * 6)()
+    assert cb.entries(None, I, 3, J, 2, out) == 0
+    assert list(out) == [M[1, 0], M[4, 0], M[7, 0], M[1, 9], M[4, 9], M[7, 9]]
+    slf = (ctypes.c_int64 * 2)(2, 5)
+    nbr = (ctypes.c_int64 * 4)(1, 2, 3, 4)
+    nn, rows = ctypes.c_int64(4), ctypes.c_int64(0)
+    lv, cv = (ctypes.c_double * 2)(0.5, 0.5), (ctypes.c_double * 2)(0, 0)
+    K = (ctypes.c_double * 4)()
+    assert cb.proxy_fn(None, slf, 2, nbr, ctypes.byref(nn), lv, cv, K, 2, ctypes.byref(rows)) == 0
+    assert rows.value == 3 and nn.value == 4                    # too many rows: nothing written
+    K = (ctypes.c_double * 6)()
+    assert cb.proxy_fn(None, slf, 2, nbr, ctypes.byref(nn), lv, cv, K, 3, ctypes.byref(rows)) == 0
+    assert rows.value == 3 and nn.value == 2 and list(nbr)[:2] == [2, 4]
+    assert list(K) == [M[0, 2] + .5, M[1, 2] + .5, M[2, 2] + .5, M[0, 5] + .5, M[1, 5] + .5, M[2, 5] + .5]
+
+    def boom(I, J):
+        raise KeyError("from A")
+    cb2 = _Callbacks(boom, pxy, xh, 2)
+    assert cb2.entries(None, I, 3, J, 2, out) == 1 and isinstance(cb2.exc, KeyError)
