@@ -147,22 +147,41 @@ def cpu_baseline(N, occ, tol, nrhs, target, log, runs=3):
         threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
     except ImportError:
         threads = os.cpu_count() or 1
-    pr = opr.CurveDL(N, "circle")
-    B = np.random.default_rng(0).standard_normal((N, nrhs))
-    times = []
-    for _ in range(runs):
+    def one(n):
+        pr = opr.CurveDL(n, "circle")
+        B = np.random.default_rng(0).standard_normal((n, nrhs))
         t0 = time.perf_counter()
         Fo = ofl.rskelf(pr.afun, pr.x, occ, tol, pr.pxyfun, dict(symm="p", order="level"))
         X, iters, rel = ofl.fgmres(pr.matvec, B, lambda V: ofl.rskelf_sv(Fo, V), tol=target)
-        times.append(time.perf_counter() - t0)
-        log(f"cpu oracle N={N}: {times[-1]:.2f} s, fgmres iters {iters}, relres {rel.max():.2e}")
-        del Fo
+        dt = time.perf_counter() - t0
+        log(f"cpu oracle N={n}: {dt:.2f} s, fgmres iters {iters}, relres {rel.max():.2e}")
+        return dt
+
+    # the oracle's per-box BLAS calls are small, so all BLAS threads can be
+    # slower than one: a probe at N/4 picks the faster setting
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        threadpool_limits = None
+    use = threads
+    if threadpool_limits is not None and threads > 1:
+        with threadpool_limits(limits=1):
+            t1 = one(max(N // 4, 1024))
+        tall = one(max(N // 4, 1024))
+        use = 1 if t1 < tall else threads
+    times = []
+    for _ in range(runs):
+        if threadpool_limits is not None:
+            with threadpool_limits(limits=use):
+                times.append(one(N))
+        else:
+            times.append(one(N))
     dt = float(np.median(times))
-    return dict(value=N / dt, unit="points/s", cores=int(threads), kind="port",
+    return dict(value=N / dt, unit="points/s", cores=int(use), kind="port",
                 sample=f"circle N={N}, occ={occ}, tol={tol:g}, symm p, {nrhs} RHS, FP64 "
                        f"NumPy/SciPy restatement of rskelf (level order) + the same FGMRES to "
-                       f"{target:g}, BLAS threads {threads} of {os.cpu_count()} host cores, "
-                       f"median of {runs} runs {dt:.1f} s")
+                       f"{target:g}, BLAS threads {use} of {os.cpu_count()} host cores (the "
+                       f"faster of 1 and {threads} at N/4), median of {runs} runs {dt:.1f} s")
 
 
 def other_workload(args, log):
@@ -170,9 +189,9 @@ def other_workload(args, log):
     ellipses and the 3D sphere, mixed FP32 factor + FP64 refinement beside
     the plain FP64 factor.  The sphere refines on the exact on-the-fly FP64
     product.  The ellipses refine on the forward sweep of one FP64 factor
-    at tol 1e-12 (built once, outside the timed steps, as the circle's FFT
-    symbol is), since their exact product is O(N^2) per problem; the
-    reported residuals are then taken with the exact product."""
+    at tol 1e-12, built inside every timed step (refine_ms includes it,
+    operator_ms reports it), since their exact product is O(N^2) per
+    problem; the reported residuals are then taken with the exact product."""
     import numpy as np
     import torch
 
@@ -193,12 +212,11 @@ def other_workload(args, log):
         exact = mv
         if not args.exact_op:
             from flam_amd.refine import factor_operator
-            t0 = time.perf_counter()
-            F_op = flam_amd.rskelf_batch(batch, occ, 1e-12, dict(symm="n"))
-            torch.cuda.synchronize()
-            log(f"operator: FP64 factor at tol 1e-12 in {time.perf_counter() - t0:.2f} s "
-                f"(outside the timed steps)")
-            mv = factor_operator(F_op)
+
+            def make_op():
+                # the refinement operator: a tol-1e-12 FP64 factor's forward
+                # sweep, built inside every timed step
+                return factor_operator(flam_amd.rskelf_batch(batch, occ, 1e-12, dict(symm="n")))
         desc = (f"{args.nprob} ellipses (a=1, b~U[0.3,1], N log-uniform [2048,131072], seed 0), "
                 f"N total {N}, one batched factorization, occ={occ}, tol={tol:g}, symm n, "
                 f"{args.nrhs} RHS, FP32 factor + FP64 FGMRES to {target:g} per problem")
@@ -222,7 +240,10 @@ def other_workload(args, log):
         desc = (f"3D Laplace single layer on the unit sphere, {args.nface} triangles, occ={occ}, "
                 f"tol={tol:g}, symm n, {args.nrhs} RHS, FP32 factor + FP64 FGMRES to {target:g}")
     if args.workload == "ellipses" and not args.exact_op:
-        desc += "; FGMRES on a tol-1e-12 FP64 factor's rskelf_mv, residuals with the exact product"
+        desc += ("; FGMRES on a tol-1e-12 FP64 factor's rskelf_mv (built inside every step), "
+                 "residuals with the exact product")
+    else:
+        make_op = None
     log(f"workload: {desc}")
     rng = np.random.default_rng(0)
     B = torch.from_numpy(rng.standard_normal((N, args.nrhs))).cuda()
@@ -249,20 +270,25 @@ def other_workload(args, log):
             t0 = time.perf_counter()
             F = factor(dict(symm=symm, precision=prec, profile=1 if full else 2))
             t1 = time.perf_counter()
-            X, info = fgmres(mv, B, F, tol=target, segments=segs)
+            op = make_op() if make_op is not None else mv
+            t_op = time.perf_counter()
+            X, info = fgmres(op, B, F, tol=target, segments=segs)
+            del op
             torch.cuda.synchronize()
             t2 = time.perf_counter()
             rel = exact_relres(X)
-            log(f"{prec} step {it}: factor {1e3*(t1-t0):.1f} ms, refine {1e3*(t2-t1):.1f} ms, "
-                f"gmres iters {info['iters']}, relres {max(info['relres']):.2e} "
-                f"(exact product {rel:.2e})")
+            log(f"{prec} step {it}: factor {1e3*(t1-t0):.1f} ms, operator {1e3*(t_op-t1):.1f} ms, "
+                f"refine {1e3*(t2-t_op):.1f} ms, gmres iters {info['iters']}, "
+                f"relres {max(info['relres']):.2e} (exact product {rel:.2e})")
             if args.warmup <= it < args.warmup + args.steps:
-                times.append((t2 - t0, 1e3 * (t1 - t0), 1e3 * (t2 - t1), info["iters"], rel))
+                times.append((t2 - t0, 1e3 * (t1 - t0), 1e3 * (t2 - t1), info["iters"], rel,
+                              1e3 * (t_op - t1)))
         roof, kt = roofline_of(F_timed, F, prec, None)
         res[prec] = dict(value=N / float(np.mean([t[0] for t in times])),
                          ms_per_step=1e3 * float(np.mean([t[0] for t in times])),
                          factor_ms=float(np.mean([t[1] for t in times])),
                          refine_ms=float(np.mean([t[2] for t in times])),
+                         operator_ms=float(np.mean([t[5] for t in times])),
                          iters=times[-1][3], relres_exact=max(t[4] for t in times),
                          levels=F.level_table(), roofline=roof,
                          kernels={k: dict(ms=v["ms"], launches=v["launches"]) for k, v in kt.items()})
