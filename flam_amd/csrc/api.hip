@@ -51,7 +51,7 @@ size_t elim_elems(bool lu, int k, int r);
 size_t elim_reg_shmem(int n, int k, int r, size_t es, int nw, bool lu);
 int elim_reg_nmax(size_t es, int nw);
 template <typename S>
-void launch_elim_reg(bool, bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
+void launch_elim_reg(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
                      const int*, const double*, const S*, S*, S*, int*, const PointsDev&, int*,
                      hipStream_t);
 template <typename S>
@@ -1006,8 +1006,8 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     if (ws.hflag[0] & 1)
       return fail(RSKX_ERR_NUM, "FLAM:rskx:chol: matrix is not positive definite (symm 'p')");
     if (ws.hflag[0] & 4)
-      return fail(RSKX_ERR_NUM, "FLAM:rskx:ldl: zero pivot or element growth above 1/sqrt(eps) in L D L' "
-                                "(symm 'h' factors without Bunch-Kaufman pivots)");
+      return fail(RSKX_ERR_NUM, "FLAM:rskx:ldl: singular redundant block (a zero 1 x 1 or 2 x 2 pivot "
+                                "of L D L', symm 'h')");
     if (ws.hflag[0] & 2)
       return fail(RSKX_ERR_NUM, "FLAM:rskx:lu: exactly singular redundant block");
     return RSKX_OK;
@@ -1473,7 +1473,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         O.rd = need_i; need_i += r;
         O.piv = need_i; if (lu) need_i += r;
         O.D = -1;
-        if (ldl) { O.D = need_v; need_v += r; }
+        if (ldl) { O.D = need_v; need_v += 2 * r; }   // D's diagonal, then its subdiagonal
         O.M_off = offM; offM += k * k;
         O.nl_off = pool_top + off_slf + off_nl; off_nl += k;
         kmx = std::max<int>(kmx, (int)k);
@@ -1542,7 +1542,8 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
       const int n = hb[ib].n;
-      if (O.r > 0 && n <= nreg8) {
+      // 'h' (Bunch-Kaufman interchanges and 2 x 2 pivots) always takes k_elim
+      if (O.r > 0 && n <= nreg8 && !ldl) {
         const size_t by = elim_reg_shmem(n, O.k, O.r, sizeof(S), 8, lu);
         if (by <= (size_t)g_tune.elim8_lds) {
           creg[0].push_back((int)ib);
@@ -1551,7 +1552,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
           continue;
         }
       }
-      if (O.r > 0 && n <= nreg16) {
+      if (O.r > 0 && n <= nreg16 && !ldl) {
         const size_t by = elim_reg_shmem(n, O.k, O.r, sizeof(S), 16, lu);
         if (by <= 160 * 1024) {
           creg[1].push_back((int)ib);
@@ -1582,7 +1583,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       int off = 0;
       for (int c = 1; c >= 0; --c) {           // 16-wave boxes first, then 8-wave
         if (creg[c].empty()) continue;
-        launch_elim_reg<S>(lu, ldl, c == 0 ? 8 : 16, shm_reg[c], ws.ids.as<int>() + off,
+        launch_elim_reg<S>(lu, c == 0 ? 8 : 16, shm_reg[c], ws.ids.as<int>() + off,
                            (int)creg[c].size(), ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(),
                            ws.csegs.as<Seg>(), ws.pool.as<int>(), ws.perm.as<int>(),
                            ws.T.as<double>(), ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), fval, fidx,
@@ -2495,7 +2496,7 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
   RSKX_TRY(getv(f.LU, r * r, LU));
   RSKX_TRY(getv(f.E, k * r, E));
   const bool ldl = F->symm == 'h';
-  if (ldl) RSKX_TRY(getv(f.D, r, D));
+  if (ldl) RSKX_TRY(getv(f.D, 2 * r, D));
   if (lu) RSKX_TRY(getv(f.G, r * k, G));
   RSKX_TRY(geti(f.sk, k, sk));
   RSKX_TRY(geti(f.rd, r, rd));
@@ -2513,9 +2514,10 @@ int rskx_export_factor(const rskx_factor* F, int64_t i, rskx_factor_view* v) {
     for (int64_t c = 0; c < r; ++c)
       for (int64_t a = 0; a < r; ++a)
         v->L[a + c * r] = a > c ? LU[a + c * r] : (a == c ? ((lu || ldl) ? 1.0 : LU[a + c * r]) : 0.0);
-  if (ldl && v->U)                            // U = D (rskelf.m:231: U = sparse(D))
+  if (ldl && v->U)                            // U = D (rskelf.m:231: U = sparse(D)), 2 x 2 blocks
     for (int64_t c = 0; c < r; ++c)
-      for (int64_t a = 0; a < r; ++a) v->U[a + c * r] = a == c ? D[a] : 0.0;
+      for (int64_t a = 0; a < r; ++a)
+        v->U[a + c * r] = a == c ? D[a] : a == c + 1 ? D[r + c] : c == a + 1 ? D[r + a] : 0.0;
   if (lu) {
     if (v->U)
       for (int64_t c = 0; c < r; ++c)
@@ -2606,10 +2608,11 @@ static int import_impl(int64_t N, char symm, int nlvl, const int64_t* lvp,
       fr.LUt = put(nullptr, r * r);
       fr.Lti = put(nullptr, nbk * kDiagBlock * kDiagBlock);
       fr.Uti = lu ? put(nullptr, nbk * kDiagBlock * kDiagBlock) : -1;
-      if (ldl) {                       // U = D (rskelf.m:231), diagonal
-        std::vector<double> D(r);
+      if (ldl) {                       // U = D (rskelf.m:231): diagonal, then subdiagonal
+        std::vector<double> D(2 * r, 0.0);
         for (int64_t a = 0; a < r; ++a) D[a] = v.U[a + a * r];
-        fr.D = put(D.data(), r);
+        for (int64_t a = 0; a + 1 < r; ++a) D[r + a] = v.U[(a + 1) + a * r];
+        fr.D = put(D.data(), 2 * r);
       }
       fr.sk = (int64_t)idx.size();
       for (int64_t q = 0; q < k; ++q) idx.push_back((int)v.sk[q]);
@@ -2674,12 +2677,18 @@ int rskx_import(int64_t N, char symm, int precision, int nlvl, const int64_t* lv
       return fail(RSKX_ERR_ARG, "FLAM:rskx:import: factor " + std::to_string(f) + " is incomplete");
     if ((sy == 'n' || sy == 's') && (!v.U || !v.p || (v.k && !v.F)))
       return fail(RSKX_ERR_ARG, "FLAM:rskx:import: factor " + std::to_string(f) + " lacks U, p or F");
-    if (sy == 'h') {
+    if (sy == 'h') {                 // D: symmetric, 1 x 1 and 2 x 2 diagonal blocks
       if (!v.U) return fail(RSKX_ERR_ARG, "FLAM:rskx:import: symm 'h' needs D in U");
-      for (int64_t a = 0; a < v.r; ++a)
-        for (int64_t c = 0; c < v.r; ++c)
-          if (a != c && v.U[a + c * v.r] != 0)
-            return fail(RSKX_ERR_UNSUP, "FLAM:rskx:import: 2 x 2 pivot blocks of D (Bunch-Kaufman) are not built");
+      const int64_t r = v.r;
+      for (int64_t a = 0; a < r; ++a)
+        for (int64_t c = 0; c < r; ++c) {
+          const double u = v.U[a + c * r];
+          const bool sub = a == c + 1, sup = c == a + 1;
+          if (u != 0 && a != c && !sub && !sup)
+            return fail(RSKX_ERR_ARG, "FLAM:rskx:import: D (U) is not block diagonal with 1 x 1 and 2 x 2 blocks");
+          if (sub && (u != v.U[c + a * r] || (u != 0 && a + 1 < r && v.U[(a + 1) + a * r] != 0)))
+            return fail(RSKX_ERR_ARG, "FLAM:rskx:import: D (U) is not block diagonal with 1 x 1 and 2 x 2 blocks");
+        }
     }
     for (int64_t q = 0; q < v.k; ++q)
       if (v.sk[q] < 0 || v.sk[q] >= N) return fail(RSKX_ERR_ARG, "FLAM:rskx:import: sk out of range");

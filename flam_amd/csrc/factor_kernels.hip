@@ -751,29 +751,18 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
       for (int i = s + 1 + tid; i < r; i += nt) Krr[i + s * r] *= dinv;
       __syncthreads();
     }
-  } else {
-    // right-looking Cholesky ('p'), or L D L' without pivoting ('h': unit L,
-    // the pivots to D; rskelf.m:229 takes Bunch-Kaufman pivots, which may
-    // interchange rows even on definite matrices; a zero pivot or element
-    // growth in L is reported instead of pivoting)
-    S* fD = LDL ? fval + O.D : nullptr;
+  } else if (!LDL) {
+    // right-looking Cholesky ('p')
     for (int s = 0; s < r; ++s) {
       if (tid == 0) {
         S d = Krr[s + s * r];
-        if (LDL) {
-          if (!(d != (S)0)) { atomicOr(fail, 4); d = (S)1; }
-          fD[s] = d;
-          Krr[s + s * r] = (S)1;
-          s_d = (S)1 / d;
-        } else {
-          if (!(d > (S)0)) { atomicOr(fail, 1); d = (S)1; }
-          d = sqrt(d);
-          Krr[s + s * r] = d;
-          s_d = (S)1 / d;
-        }
+        if (!(d > (S)0)) { atomicOr(fail, 1); d = (S)1; }
+        d = sqrt(d);
+        Krr[s + s * r] = d;
+        s_d = (S)1 / d;
       }
       __syncthreads();
-      const S d2 = LDL ? s_d : s_d * s_d;
+      const S d2 = s_d * s_d;
       for (int j = s + 1 + wid; j < r; j += nw) {
         const S ljs = Krr[j + s * r] * d2;
         for (int i = j + lane; i < r; i += kWave) Krr[i + j * r] -= Krr[i + s * r] * ljs;
@@ -782,12 +771,109 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
       for (int i = s + 1 + tid; i < r; i += nt) Krr[i + s * r] *= s_d;
       __syncthreads();
     }
-    if (LDL) {                     // element growth of the unit L (as k_elim_reg)
-      const S gmax = (S)(1.0 / sqrt(Eps<S>::v));
-      bool bad = false;
-      for (int e = tid; e < r * r; e += nt)
-        if (e % r > e / r && !(fabs(Krr[e]) <= gmax)) bad = true;
-      if (__any(bad) && lane == 0) atomicOr(fail, 4);
+  } else {
+    // 'h': [L,U,p] = ldl(K(rd,rd),'vector'), rd = rd(p), T = T(:,p)
+    // (rskelf.m:229-231): L D L' with Bunch-Kaufman pivots, 1 x 1 and 2 x 2
+    // (the pivot rule of LAPACK's dsytf2 on the lower triangle, alpha =
+    // (1 + sqrt(17)) / 8).  A symmetric interchange swaps the computed rows
+    // of L, the trailing matrix, T's and Ksr''s columns and the rd order.
+    // D goes to fD[0..r) (diagonal) and fD[r..2r) (subdiagonal: D(s+1,s)
+    // at the first index s of a 2 x 2 block, 0 elsewhere).
+    S* fD = fval + O.D;
+    int* frd = fidx + O.rd;
+    for (int i = tid; i < r; i += nt) frd[i] = slf[q[k + i]];
+    __shared__ S s_e[3];
+    const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+    __syncthreads();
+    for (int s = 0; s < r;) {
+      double lam = -1.0;
+      int imax = r;
+      for (int i = s + 1 + tid; i < r; i += nt) argmax_merge(lam, imax, fabs((double)Krr[i + s * r]), i);
+      block_argmax(lam, imax, red);
+      lam = lam > 0 ? lam : 0.0;
+      const double absakk = fabs((double)Krr[s + s * r]);
+      int kstep = 1, kp = s;
+      if (absakk < alpha * lam) {
+        // the largest entry of row / column imax off its diagonal
+        double sig = -1.0;
+        int jm = r;
+        for (int j = s + tid; j < r; j += nt)
+          if (j != imax)
+            argmax_merge(sig, jm, fabs((double)(j < imax ? Krr[imax + j * r] : Krr[j + imax * r])), j);
+        block_argmax(sig, jm, red);
+        sig = sig > 0 ? sig : 0.0;
+        if (absakk * sig >= alpha * lam * lam) kp = s;
+        else if (fabs((double)Krr[imax + imax * r]) >= alpha * sig) kp = imax;
+        else { kstep = 2; kp = imax; }
+      }
+      if (s + kstep > r) kstep = 1;             // (cannot happen: imax > s)
+      const int p1 = s + kstep - 1, p2 = kp;
+      if (p2 != p1) {                           // symmetric interchange p1 <-> p2 (p1 < p2)
+        auto swp = [](S& a, S& b) { const S t = a; a = b; b = t; };
+        for (int c = tid; c < p1; c += nt) swp(Krr[p1 + c * r], Krr[p2 + c * r]);
+        for (int i = p2 + 1 + tid; i < r; i += nt) swp(Krr[i + p1 * r], Krr[i + p2 * r]);
+        for (int j = p1 + 1 + tid; j < p2; j += nt) swp(Krr[j + p1 * r], Krr[p2 + j * r]);
+        for (int a = tid; a < k; a += nt) {
+          swp(Ksr[a + p1 * k], Ksr[a + p2 * k]);
+          swp(Ts[a + p1 * k], Ts[a + p2 * k]);
+        }
+        if (tid == 0) {
+          swp(Krr[p1 + p1 * r], Krr[p2 + p2 * r]);
+          const int t = frd[p1]; frd[p1] = frd[p2]; frd[p2] = t;
+        }
+        __syncthreads();
+      }
+      if (kstep == 1) {
+        if (tid == 0) {
+          S d = Krr[s + s * r];
+          if (!(d != (S)0)) { atomicOr(fail, 4); d = (S)1; }
+          fD[s] = d;
+          fD[r + s] = (S)0;
+          Krr[s + s * r] = (S)1;
+          s_d = (S)1 / d;
+        }
+        __syncthreads();
+        for (int j = s + 1 + wid; j < r; j += nw) {
+          const S ljs = Krr[j + s * r] * s_d;
+          for (int i = j + lane; i < r; i += kWave) Krr[i + j * r] -= Krr[i + s * r] * ljs;
+        }
+        __syncthreads();
+        for (int i = s + 1 + tid; i < r; i += nt) Krr[i + s * r] *= s_d;
+        __syncthreads();
+      } else {
+        if (tid == 0) {
+          const S d11 = Krr[s + s * r], d21 = Krr[s + 1 + s * r], d22 = Krr[s + 1 + (s + 1) * r];
+          S det = d11 * d22 - d21 * d21;
+          if (!(det != (S)0)) { atomicOr(fail, 4); det = (S)1; }
+          fD[s] = d11;
+          fD[s + 1] = d22;
+          fD[r + s] = d21;
+          fD[r + s + 1] = (S)0;
+          s_e[0] = d22 / det;                   // D^{-1} = [e0 e1; e1 e2]
+          s_e[1] = -d21 / det;
+          s_e[2] = d11 / det;
+          Krr[s + s * r] = (S)1;
+          Krr[s + 1 + (s + 1) * r] = (S)1;
+          Krr[s + 1 + s * r] = (S)0;
+        }
+        __syncthreads();
+        const S e0 = s_e[0], e1 = s_e[1], e2 = s_e[2];
+        // trailing A(i,j) -= W(i,:) L(j,:)' with W = A(:, s:s+1), L = W D^{-1}
+        for (int j = s + 2 + wid; j < r; j += nw) {
+          const S w0 = Krr[j + s * r], w1 = Krr[j + (s + 1) * r];
+          const S l0 = w0 * e0 + w1 * e1, l1 = w0 * e1 + w1 * e2;
+          for (int i = j + lane; i < r; i += kWave)
+            Krr[i + j * r] -= Krr[i + s * r] * l0 + Krr[i + (s + 1) * r] * l1;
+        }
+        __syncthreads();
+        for (int i = s + 2 + tid; i < r; i += nt) {
+          const S w0 = Krr[i + s * r], w1 = Krr[i + (s + 1) * r];
+          Krr[i + s * r] = w0 * e0 + w1 * e1;
+          Krr[i + (s + 1) * r] = w0 * e1 + w1 * e2;
+        }
+        __syncthreads();
+      }
+      s += kstep;
     }
   }
   // inverted 32 x 32 diagonal blocks of L (unit for LU) and U, one wave
@@ -848,9 +934,19 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
       FOR_COLS_ROWS(i, j + 1, r, c, 0, k) Krs[i + c * r] -= Krr[i + j * r] * Krs[j + c * r];
     __syncthreads();
   }
-  if (LDL) {
+  if (LDL) {                       // E = (Ksr'/L')/D with D's 1 x 1 and 2 x 2 blocks
     const S* fD = fval + O.D;
-    FOR_COLS_ROWS(a, 0, k, c, 0, r) Ksr[a + c * k] /= fD[c];
+    FOR_COLS_ROWS(a, 0, k, c, 0, r) {
+      const S b = fD[r + c];
+      if (b != (S)0) {                 // block (c, c+1)
+        const S d11 = fD[c], d22 = fD[c + 1], det = d11 * d22 - b * b;
+        const S x0 = Ksr[a + c * k], x1 = Ksr[a + (c + 1) * k];
+        Ksr[a + c * k] = (x0 * d22 - x1 * b) / det;
+        Ksr[a + (c + 1) * k] = (x1 * d11 - x0 * b) / det;
+      } else if (c == 0 || fD[r + c - 1] == (S)0) {
+        Ksr[a + c * k] /= fD[c];
+      }
+    }
     __syncthreads();
   }
   // M = M(sk,sk) - E G, - E E' or - E D E' (rskelf.m:240-246)
@@ -858,9 +954,13 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
   FOR_COLS_ROWS(a, 0, k, c, 0, k) {
     S acc = 0;
     if (LU) for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * Krs[l + c * r];
-    else if (LDL) {
+    else if (LDL) {                   // E D E', D block diagonal
       const S* fD = fval + O.D;
-      for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * fD[l] * Ksr[c + l * k];
+      for (int l = 0; l < r; ++l) {
+        acc += Ksr[a + l * k] * fD[l] * Ksr[c + l * k];
+        const S b = fD[r + l];
+        if (b != (S)0) acc += b * (Ksr[a + l * k] * Ksr[c + (l + 1) * k] + Ksr[a + (l + 1) * k] * Ksr[c + l * k]);
+      }
     } else for (int l = 0; l < r; ++l) acc += Ksr[a + l * k] * Ksr[c + l * k];
     Mo[a + c * k] = mval(cm, Mprev, q[a], q[c]) - acc;
   }
@@ -883,7 +983,8 @@ __global__ void __launch_bounds__(1024) k_elim(const int* __restrict__ ids,
     fsk[a] = pt;
     pool[O.nl_off + a] = pt;
   }
-  for (int i = tid; i < r; i += nt) frd[i] = slf[q[k + i]];
+  if (!LDL)                                      // 'h' wrote rd(p) itself
+    for (int i = tid; i < r; i += nt) frd[i] = slf[q[k + i]];
 }
 
 // ------------------------------------------------------------- k_id_reg --
@@ -1480,8 +1581,9 @@ __device__ inline double lane_bcast(double v, int l) { return read_lane(v, l); }
 // so that one box's serial steps (a panel's factorization, a pivot search)
 // overlap the other's updates.  Pivoting is lazy: rows are never moved;
 // each step's pivot row is found among the rows not yet used, ties going to
-// the row LAPACK's swapped order would reach first.
-template <typename S, bool LU, int TA, int TB, int NW, bool LDL>
+// the row LAPACK's swapped order would reach first.  (symm 'h', whose
+// Bunch-Kaufman interchanges move rows and columns, runs in k_elim.)
+template <typename S, bool LU, int TA, int TB, int NW>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 1, 4))) k_elim_reg(const int* __restrict__ ids,
                                                    const LvlBox* __restrict__ boxes,
                                                    const BoxOut* __restrict__ outs,
@@ -1506,8 +1608,6 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
   __shared__ ChildMap cm;
   __shared__ int s_piv[2];
   __shared__ S s_pv[2];
-  __shared__ S s_dpan[2][4];                      // 'h': the pivots of the last two panels
-  if (tid < 8) s_dpan[tid >> 2][tid & 3] = (S)1;  // a short last panel leaves entries unset
   if (tid == 0) {
     int nc = B.ncs > 8 ? 8 : B.ncs;
     cm.nc = nc;
@@ -1670,15 +1770,9 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
 #pragma unroll
           for (int ta = 0; ta < TA; ++ta) if (ta == (sc >> 6)) dv = pc[c][ta];
           dv = lane_bcast(dv, sc & 63);
-          // 'p': Cholesky pivot sqrt(d); 'h': unit L, d to D without
-          // pivoting (rskelf.m:229 takes Bunch-Kaufman pivots, which may
-          // interchange rows even on definite matrices; a zero pivot or
-          // element growth in L is reported instead of pivoting)
-          if (LDL) {
-            if (!(dv != (S)0)) { if (lane == 0) atomicOr(fail, 4); dv = (S)1; }
-            if (lane == 0) { fval[O.D + sc] = dv; s_dpan[p & 1][c] = dv; }
-          } else if (!(dv > (S)0)) { if (lane == 0) atomicOr(fail, 1); dv = (S)1; }
-          const S pv = LDL ? (S)1 : sqrt(dv), inv = (S)1 / (LDL ? dv : pv);
+          // Cholesky pivot sqrt(d)
+          if (!(dv > (S)0)) { if (lane == 0) atomicOr(fail, 1); dv = (S)1; }
+          const S pv = sqrt(dv), inv = (S)1 / pv;
 #pragma unroll
           for (int ta = 0; ta < TA; ++ta) {
             const int i = lane + 64 * ta;
@@ -1693,7 +1787,6 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
 #pragma unroll
             for (int ta = 0; ta < TA; ++ta) if (ta == (j >> 6)) u = pc[c][ta];
             u = lane_bcast(u, j & 63);
-            if (LDL) u *= dv;                          // l_ic d_c l_jc
 #pragma unroll
             for (int ta = 0; ta < TA; ++ta) {
               const int i = lane + 64 * ta;
@@ -1726,7 +1819,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
         if (j < s1 || j >= n) continue;
         S u[NB];
 #pragma unroll
-        for (int c = 0; c < NB; ++c) u[c] = LDL ? Lp[c * n + j] * s_dpan[p & 1][c] : Lp[c * n + j];
+        for (int c = 0; c < NB; ++c) u[c] = Lp[c * n + j];
         // entries above the diagonal of Krr, the Krs' block and padding
         // rows are never read again: update without masks
 #pragma unroll
@@ -1738,21 +1831,6 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
           Z[ta][tb] = z;
         }
       }
-    }
-    if (LDL) {
-      // L D L' without pivoting is stable while L stays bounded: flag
-      // entries of the unit L above 1/sqrt(eps) (a pivot tiny against its
-      // column) as a failure rather than return an inaccurate factor
-      const S gmax = (S)(1.0 / sqrt(Eps<S>::v));
-      bool bad = false;
-#pragma unroll
-      for (int ta = 0; ta < TA; ++ta)
-#pragma unroll
-        for (int tb = 0; tb < TB; ++tb) {
-          const int i = lane + 64 * ta, j = colj(tb);
-          if (j < r && i > j && i < r && !(fabs(Z[ta][tb]) <= gmax)) bad = true;
-        }
-      if (__any(bad) && lane == 0) atomicOr(fail, 4);
     }
   }
 
@@ -1982,7 +2060,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
               S v = (t == c) ? (S)1 : (S)0;
 #pragma unroll
               for (int u = 0; u < t; ++u) v -= Lb[t + u * ldb] * x[u];
-              x[t] = (LU || LDL) ? v : v / Lb[t + t * ldb];
+              x[t] = LU ? v : v / Lb[t + t * ldb];
             }
           }
         } else {
@@ -2037,7 +2115,7 @@ size_t elim_reg_shmem(int n, int k, int r, size_t es, int nw, bool lu) {
 }
 
 template <typename S>
-void launch_elim_reg(bool lu, bool ldl, int nw, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
+void launch_elim_reg(bool lu, int nw, size_t shmem, const int* ids, int nb, const LvlBox* boxes,
                      const BoxOut* outs, const Seg* csegs, int* pool, const int* permar,
                      const double* Tar, const S* Mprev, S* Mcur, S* fval, int* fidx,
                      const PointsDev& P, int* fail, hipStream_t st) {
@@ -2046,8 +2124,7 @@ void launch_elim_reg(bool lu, bool ldl, int nw, size_t shmem, const int* ids, in
   constexpr int TA8 = F32 ? kElimTA8_32 : kElimTA8_64, TB8 = F32 ? kElimTB8_32 : kElimTB8_64;
 #define RSKX_ELIMR(LUV, TAV, TBV, NWV)                                                          \
   {                                                                                             \
-    auto kern = ldl ? k_elim_reg<S, false, TAV, TBV, NWV, true>                                  \
-                    : k_elim_reg<S, LUV, TAV, TBV, NWV, false>;                                  \
+    auto kern = k_elim_reg<S, LUV, TAV, TBV, NWV>;                                              \
     ensure_dyn_lds((const void*)kern, shmem);                                                    \
     hipLaunchKernelGGL(kern, dim3(nb), dim3(64 * NWV), shmem, st, ids, boxes, outs, csegs, pool, \
                        permar, Tar, Mprev, Mcur, fval, fidx, P, fail);                          \
@@ -2093,7 +2170,7 @@ void launch_elim(bool lu, bool ldl, int mode, size_t shmem, const int* ids, int 
   template void launch_id_reg<S>(int, const int*, const LvlBox*, const int*, const int*,        \
                                  const int*, int*, S*, int*, const PointsDev&, const LevelParams&, \
                                  const IdParams&, int*, int, size_t, hipStream_t);             \
-  template void launch_elim_reg<S>(bool, bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, \
+  template void launch_elim_reg<S>(bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, \
                                    const Seg*, int*, const int*, const double*, const S*, S*, S*, \
                                    int*, const PointsDev&, int*, hipStream_t);                   \
   template void launch_elim<S>(bool, bool, int, size_t, const int*, int, const LvlBox*,       \

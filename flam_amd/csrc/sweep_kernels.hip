@@ -304,7 +304,18 @@ __global__ void __launch_bounds__(256) k_sweep(const FacRec* __restrict__ recs,
         if (unit) {
           __syncthreads();
           const S* D = fval + f.D;
-          for (int o = threadIdx.x; o < r * kRhs; o += blockDim.x) xr[o] *= D[o % r];
+          // X(rd) = D X(rd), D with 1 x 1 and 2 x 2 blocks (subdiagonal at D + r)
+          for (int o = threadIdx.x; o < r * kRhs; o += blockDim.x) {
+            const int i = o % r, c = o / r;
+            const S b = D[r + i];
+            if (b != (S)0) {
+              const S x0 = xr[i + c * r], x1 = xr[i + 1 + c * r];
+              xr[i + c * r] = D[i] * x0 + b * x1;
+              xr[i + 1 + c * r] = b * x0 + D[i + 1] * x1;
+            } else if (i == 0 || D[r + i - 1] == (S)0) {
+              xr[o] *= D[i];
+            }
+          }
           __syncthreads();
         }
       } else {
@@ -596,7 +607,19 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(const FacRec* __restric
         if (f.D >= 0) {
           __syncthreads();
           const S* D = fval + f.D;
-          for (int o = threadIdx.x; o < r * kRhs; o += kSolveThreads) xr[o] /= D[o / kRhs];
+          // X(rd) = D \ X(rd), D with 1 x 1 and 2 x 2 blocks (subdiagonal at D + r)
+          for (int o = threadIdx.x; o < r * kRhs; o += kSolveThreads) {
+            const int i = o / kRhs, c = o - i * kRhs;
+            const S b = D[r + i];
+            if (b != (S)0) {
+              const S d0 = D[i], d1 = D[i + 1], det = d0 * d1 - b * b;
+              const S x0 = xr[i * kRhs + c], x1 = xr[(i + 1) * kRhs + c];
+              xr[i * kRhs + c] = (d1 * x0 - b * x1) / det;
+              xr[(i + 1) * kRhs + c] = (d0 * x1 - b * x0) / det;
+            } else if (i == 0 || D[r + i - 1] == (S)0) {
+              xr[o] /= D[i];
+            }
+          }
           __syncthreads();
         }
       } else {
@@ -695,7 +718,15 @@ __global__ void __launch_bounds__(256) k_logdet(const FacRec* __restrict__ recs,
   double s = 0;
   int neg = 0;
   for (int i = threadIdx.x; i < r; i += blockDim.x) {
-    const double d = mode == 1 ? (double)fval[f.D + i] : (double)fval[f.LU + i + (int64_t)i * r];
+    double d;
+    if (mode == 1) {          // D of L D L': 1 x 1 pivots and 2 x 2 blocks (core/logdet_ldl.m)
+      const double b = (double)fval[f.D + r + i];
+      if (b != 0.0) d = (double)fval[f.D + i] * (double)fval[f.D + i + 1] - b * b;
+      else if (i > 0 && (double)fval[f.D + r + i - 1] != 0.0) continue;   // second row of a block
+      else d = (double)fval[f.D + i];
+    } else {
+      d = (double)fval[f.LU + i + (int64_t)i * r];
+    }
     s += log(fabs(d));
     neg += d < 0;
   }

@@ -202,8 +202,10 @@ typedef struct {
 /* host copy of one factor, F.factors(i) of rskelf.m:249-257.  Call with
  * NULL arrays to get the sizes k = |sk|, r = |rd|, then again with
  * buffers of k, r, k*r, r*r, r*r, r, k*r, r*k elements.  Matrices are
- * column-major doubles; U and F are written only for symm 'n', p (0-based
- * pivot vector, K(rd(p),rd) = L U) only for 'n'. */
+ * column-major doubles; U and F are written for symm 'n' and 's', p (0-based
+ * pivot vector, K(rd(p),rd) = L U) for 'n' and 's'; for 'h', U is D of
+ * K(rd,rd) = L D L' (1 x 1 and 2 x 2 Bunch-Kaufman blocks, rd already in
+ * pivot order as rskelf.m:230 leaves it). */
 typedef struct {
   int64_t k, r;
   int64_t* sk; int64_t* rd;

@@ -293,7 +293,7 @@ def test_partial_factorization_stop_zero():
 def test_symm_h_ldl(kind):
     """symm 'h' (rskelf.m:228-233, sv/rskelf_sv_h.m, mv/rskelf_mv_h.m): the
     same ID as 'p' (identical rank table to the oracle's 'h'), L D L'
-    without pivoting on the SPD benchmark circle and on the demo's negative
+    with Bunch-Kaufman pivots on the SPD benchmark circle and on the demo's negative
     definite ie_circle matrix (where Cholesky fails); solve against the
     closed form, apply against the exact circulant product, exported L unit
     lower and U = D diagonal, and 't' / 'c' equal to 'n'."""
@@ -489,8 +489,8 @@ def test_release_cache_between_factorizations():
 def _indefinite_circle(geometry, N):
     """The benchmark circle's constant off-diagonal b = 1/(2N) with a chosen
     diagonal: +-1 alternating and A_00 = 1e-12 b, an indefinite matrix whose
-    first pivot is tiny against its column (rskelf.m:229's Bunch-Kaufman
-    pivoting takes a 2 x 2 pivot there)."""
+    first pivot is tiny against its column (Bunch-Kaufman interchanges it
+    with row 1 and takes a 1 x 1 pivot there)."""
     g = geometry.circle(N)
     d = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
     d[0] = 1e-12 * 0.5 / N
@@ -500,24 +500,65 @@ def _indefinite_circle(geometry, N):
     return g, A
 
 
+def _two_by_two_matrix(N):
+    """The benchmark circle's off-diagonal b = 1/(2N) with diagonal +-1,
+    except A_00 = A_11 = 1e-12 b and A_01 = A_10 = 1: column 0's largest
+    entry is A_10 = 1 against a negligible diagonal and so is row 1's, so
+    Bunch-Kaufman takes the 2 x 2 pivot {0, 1} (dsytf2's rule)."""
+    A = opr.CurveDL(N, "circle").dense()
+    d = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
+    b = 0.5 / N
+    d[0] = d[1] = 1e-12 * b
+    A[np.arange(N), np.arange(N)] = d
+    A[0, 1] = A[1, 0] = 1.0
+    return A
+
+
 @pytest.mark.parametrize("N", [64, 300])
-def test_symm_h_growth_reported(N):
-    """symm 'h' factors without pivoting: on a matrix whose unpivoted L D L'
-    grows (one box, so K(rd,rd) is the whole matrix), the register (N = 64)
-    and global-scratch (N = 300) eliminations report FLAM:rskx:ldl instead of
-    returning an inaccurate factor; 'n' (LU with partial pivoting) solves
-    the same matrix to the dense answer."""
+def test_symm_h_bunch_kaufman(N):
+    """symm 'h' with rskelf.m:229's Bunch-Kaufman pivots, on indefinite
+    matrices whose unpivoted L D L' would break down (one box, so K(rd,rd)
+    is the whole matrix; N = 64 eliminates in shared memory, 300 in global
+    scratch).  The tiny-pivot circle (a 1 x 1 pivot after an interchange)
+    and a matrix needing a 2 x 2 pivot (given to the library as a host
+    callback): solve and apply to the dense answers, L unit lower and
+    bounded by 1 / (1 - alpha), L D L' = A(rd, rd) in the exported order,
+    D block diagonal with the 2 x 2 block, logdet equal to the dense
+    slogdet, and the same solve as the oracle's 'h' (SciPy's
+    Bunch-Kaufman)."""
     flam_amd, geometry = _dev()
-    print(f"symm h growth: indefinite circle N={N} (one box)", flush=True)
-    g, A = _indefinite_circle(geometry, N)
-    with pytest.raises(flam_amd.FlamError) as e:
-        flam_amd.rskelf(g.kernel, g.x, N, 1e-12, g.proxy, dict(symm="h"))
-    assert e.value.ident == "FLAM:rskx:ldl"
-    F = flam_amd.rskelf(g.kernel, g.x, N, 1e-12, g.proxy, dict(symm="n"))
+    print(f"symm h bunch-kaufman: N={N} (one box)", flush=True)
+    g, A1 = _indefinite_circle(geometry, N)
+    A2 = _two_by_two_matrix(N)
+    pr = opr.CurveDL(N, "circle")
+    afun2 = lambda I, J: A2[np.ix_(np.asarray(I, dtype=np.int64), np.asarray(J, dtype=np.int64))]
+    alpha = (1 + np.sqrt(17)) / 8
     b = np.random.default_rng(5).standard_normal(N)
-    x = flam_amd.rskelf_sv(F, torch.from_numpy(b).cuda()).cpu().numpy()
-    xe = np.linalg.solve(A, b)
-    assert np.linalg.norm(x - xe) <= 1e-10 * np.linalg.norm(xe)
+    for A, F, blocks in (
+            (A1, flam_amd.rskelf(g.kernel, g.x, N, 1e-12, g.proxy, dict(symm="h")), False),
+            (A2, flam_amd.rskelf(afun2, pr.x, N, 1e-12, pr.pxyfun, dict(symm="h")), True)):
+        x = flam_amd.rskelf_sv(F, torch.from_numpy(b).cuda()).cpu().numpy()
+        xe = np.linalg.solve(A, b)
+        assert np.linalg.norm(x - xe) <= 1e-10 * np.linalg.norm(xe)
+        y = flam_amd.rskelf_mv(F, torch.from_numpy(xe).cuda()).cpu().numpy()
+        assert np.linalg.norm(y - b) <= 1e-10 * np.linalg.norm(b)
+        f = F.export(0)
+        L, U, rd = f["L"], f["U"], np.asarray(f["rd"])
+        assert np.array_equal(np.sort(rd), np.arange(N))
+        assert np.allclose(np.diag(L), 1.0) and np.allclose(np.triu(L, 1), 0.0)
+        assert np.abs(L).max() <= 1 / (1 - alpha) + 1e-12
+        assert np.abs(np.triu(U, 2)).max() == 0 and np.abs(np.tril(U, -2)).max() == 0
+        assert (np.abs(np.diag(U, -1)).max() > 0) == blocks
+        R = L @ U @ L.T - A[np.ix_(rd, rd)]
+        assert np.abs(R).max() <= 1e-12 * np.abs(A).max()
+        sgn, ld = np.linalg.slogdet(A)
+        ldF = flam_amd.rskelf_logdet(F)
+        assert abs(ldF.real - ld) <= 1e-10 * max(1.0, abs(ld))
+        assert abs(np.exp(1j * ldF.imag) - sgn) < 1e-10
+        afun = (lambda I, J: A[np.ix_(np.asarray(I, dtype=np.int64), np.asarray(J, dtype=np.int64))])
+        Fo = ofl.rskelf(afun, pr.x, N, 1e-12, pr.pxyfun, dict(symm="h", order="level"))
+        xo = ofl.rskelf_sv(Fo, b)
+        assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
 
 
 def test_large_root_global_paths():

@@ -500,9 +500,8 @@ def test_sphere_single_layer_identities():
 
 def test_symm_h_bunch_kaufman_on_indefinite():
     """rskelf.m:229's ldl pivots (Bunch-Kaufman): on an indefinite matrix
-    whose first pivot is tiny against its column the oracle's 'h' still
-    solves to the dense answer (the device, which factors without
-    pivoting, reports FLAM:rskx:ldl on the same matrix instead)."""
+    whose first pivot is tiny against its column the oracle's 'h' solves
+    to the dense answer, where an unpivoted L D L' would grow past 1e8."""
     N = 64
     pr = problems.CurveDL(N, "circle")
     d = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
