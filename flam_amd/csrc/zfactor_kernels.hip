@@ -26,14 +26,15 @@ constexpr double kTwoPiZ = 6.283185307179586476925286766559;
 //   G = (-Y0(kr) + i J0(kr)) / 4,
 //   dG/dn_y = (i k / 4) H1^(1)(kr) ((t - s) . n) / r
 //           = (k c / 4) (-Y1(kr) + i J1(kr)),  c = ((t - s) . n) / r,
-// so the entry is w ((eta J0 - k c Y1) + i (eta Y0 + k c J1)) / 4.  The
-// Bessel functions are HIP's device j0, y0, j1, y1.
+// so the entry is w ((eta J0 - k c Y1) + i (eta Y0 + k c J1)) / 4 (the
+// Bessel functions by bessel01).
 __device__ inline zd cfie_entry(double k, double eta, double tx, double ty, double sx, double sy,
                                 double nx, double ny, double w) {
   const double dx = tx - sx, dy = ty - sy;
   const double r = sqrt(dx * dx + dy * dy);
   const double z = k * r, c = (dx * nx + dy * ny) / r;
-  const double J0 = j0(z), Y0 = y0(z), J1 = j1(z), Y1 = y1(z);
+  double J0, Y0, J1, Y1;
+  bessel01(z, J0, Y0, J1, Y1);
   const double q = 0.25 * w;
   return zd{q * (eta * J0 - k * c * Y1), q * (eta * Y0 + k * c * J1)};
 }

@@ -225,12 +225,14 @@ constexpr int kZMvTile = 256;
 constexpr int kZMvCols = 8;
 
 namespace {
-__device__ inline zd cfie_entry_mv(double k, double eta, double tx, double ty, double sx, double sy,
-                                   double nx, double ny, double w) {
+// ik = 1 / k: one division per pair (1 / r) serves c and 1 / z
+__device__ inline zd cfie_entry_mv(double k, double ik, double eta, double tx, double ty,
+                                   double sx, double sy, double nx, double ny, double w) {
   const double dx = tx - sx, dy = ty - sy;
-  const double r = sqrt(dx * dx + dy * dy);
-  const double z = k * r, c = (dx * nx + dy * ny) / r;
-  const double J0 = j0(z), Y0 = y0(z), J1 = j1(z), Y1 = y1(z);
+  const double r = sqrt(dx * dx + dy * dy), ir = 1.0 / r;
+  const double z = k * r, c = (dx * nx + dy * ny) * ir;
+  double J0, Y0, J1, Y1;
+  bessel01(z, ir * ik, J0, Y0, J1, Y1);
   const double q = 0.25 * w;
   return zd{q * (eta * J0 - k * c * Y1), q * (eta * Y0 + k * c * J1)};
 }
@@ -248,6 +250,7 @@ __global__ void __launch_bounds__(kZMvTile) k_zmatvec(const double* __restrict__
   const int64_t i = blockIdx.x * (int64_t)kZMvTile + threadIdx.x;
   const bool act = i < N;
   const double tx = act ? x[2 * i] : 0.0, ty = act ? x[2 * i + 1] : 0.0;
+  const double ik = 1.0 / k;
   zd acc[kZMvCols];
 #pragma unroll
   for (int c = 0; c < kZMvCols; ++c) acc[c] = zd{0, 0};
@@ -270,7 +273,7 @@ __global__ void __launch_bounds__(kZMvTile) k_zmatvec(const double* __restrict__
     for (int t = 0; t < cnt; ++t) {
       const int64_t jj = j0s + t;
       const zd a = jj == i ? zd{diag[2 * i], diag[2 * i + 1]}
-                           : cfie_entry_mv(k, eta, tx, ty, sx[t], sy[t], snx[t], sny[t], sw[t]);
+                           : cfie_entry_mv(k, ik, eta, tx, ty, sx[t], sy[t], snx[t], sny[t], sw[t]);
 #pragma unroll
       for (int c = 0; c < kZMvCols; ++c) acc[c] += a * sX[c][t];
     }

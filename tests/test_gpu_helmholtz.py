@@ -170,3 +170,19 @@ def test_cfie_fp32_refined(kind, symm):
           torch.linalg.vector_norm(B, dim=0)).max().item()
     print(f"one FP32 solve: relres {r1:.2e}", flush=True)
     assert r1 < 1e-3
+
+
+def test_cfie_matvec_far_field_against_oracle():
+    """At k = 200 most pairs have k r >= 30, where the device evaluates
+    H0 and H1 by the Hankel asymptotic expansion (DLMF 10.17.3-4) instead
+    of j0 / y0 / j1 / y1: the product against the dense oracle matrix
+    (SciPy's hankel1) on the star to 1e-12."""
+    flam_amd, geometry = _dev()
+    N = 2048
+    print(f"cfie matvec far field: star N={N} k=200", flush=True)
+    g = geometry.helmholtz_curve(N, "star", k=200.0)
+    pr = _oracle(g)
+    X = _crand((N, 2), 7)
+    Y = _np(flam_amd.kernel_matvec(g.kernel, g.x, torch.from_numpy(X).cuda()))
+    Ye = pr.dense() @ X
+    assert np.abs(Y - Ye).max() <= 1e-12 * np.abs(Ye).max()
