@@ -12,7 +12,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = ["api.hip", "factor_kernels.hip", "sweep_kernels.hip", "tree_kernels.hip", "tree.cpp",
-           "refine_kernels.hip", "zfactor_kernels.hip", "zsweep_kernels.hip", "mlp_kernels.hip"]
+           "refine_kernels.hip", "zfactor_kernels.hip", "zsweep_kernels.hip", "mlp_kernels.hip",
+           "elim_mfma.hip"]
 LIB = os.path.join(HERE, "librskx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -58,6 +58,12 @@ template <typename S>
 void launch_elim(bool, bool, int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*,
                  int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
                  int*, hipStream_t);
+int elim_mfma_shape(int n, int k, int r, size_t es, size_t* shmem);
+constexpr int kMfmaShapes = 3;
+template <typename S>
+void launch_elim_mfma(int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
+                      const int*, const double*, const S*, S*, S*, int*, const PointsDev&, int*,
+                      hipStream_t);
 template <typename S>
 void launch_sweep(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int, int,
                   int, size_t, S*, int64_t, hipStream_t);
@@ -143,6 +149,7 @@ struct Tuning {
   int id_lds = 160 * 1024;         // shared bytes above which the QR ID keeps its Householder
                                    // vectors in global memory
   int id_reg = 1;                  // register QRCP classes for the boxes that fit them
+  int elim_mfma = 1;               // matrix-core elimination for the Cholesky ('p') boxes
 };
 static Tuning g_tune;
 
@@ -1535,13 +1542,25 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // Boxes whose augmented matrix fits registers go to k_elim_reg: 8-wave
     // workgroups (two per CU, n <= 192 in FP32, 128 in FP64) when the box
     // and its shared memory allow, else 16-wave workgroups
-    std::vector<int> cls[3], creg[2];
-    size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0};
+    std::vector<int> cls[3], creg[2], cmf[kMfmaShapes];
+    size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0}, shm_mf[kMfmaShapes] = {0, 0, 0};
     const int nreg8 = std::min(elim_reg_nmax(sizeof(S), 8), g_tune.elim8_nmax),
               nreg16 = elim_reg_nmax(sizeof(S), 16);
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
       const int n = hb[ib].n;
+      // Cholesky boxes go to the matrix-core elimination when its tiles and
+      // shared memory hold them
+      if (O.r > 0 && !lu && !ldl && g_tune.elim_mfma) {
+        size_t by = 0;
+        const int c = elim_mfma_shape(n, O.k, O.r, sizeof(S), &by);
+        if (c >= 0) {
+          cmf[c].push_back((int)ib);
+          regbox[ib] = 1;
+          shm_mf[c] = std::max(shm_mf[c], by);
+          continue;
+        }
+      }
       // 'h' (Bunch-Kaufman interchanges and 2 x 2 pivots) always takes k_elim
       if (O.r > 0 && n <= nreg8 && !ldl) {
         const size_t by = elim_reg_shmem(n, O.k, O.r, sizeof(S), 8, lu);
@@ -1567,20 +1586,35 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       cls[c].push_back((int)ib);
       if (c < 2) shm_cls[c] = std::max(shm_cls[c], by);   // class 2: all in global scratch
     }
-    std::vector<int> allids(creg[1]);
+    std::vector<int> allids;
+    for (int c = kMfmaShapes - 1; c >= 0; --c) allids.insert(allids.end(), cmf[c].begin(), cmf[c].end());
+    allids.insert(allids.end(), creg[1].begin(), creg[1].end());
     allids.insert(allids.end(), creg[0].begin(), creg[0].end());
     for (int c = 2; c >= 0; --c) allids.insert(allids.end(), cls[c].begin(), cls[c].end());
     RSKX_TRY(upload(ws.ids, allids));
     prof.begin("factor", ev);
     {
-      // the classes run concurrently: 16-wave boxes on one side stream,
-      // 8-wave boxes on g_stream, the shared-memory / global classes of
+      // the classes run concurrently: the largest matrix-core shape and the
+      // 16-wave boxes on one side stream, the other matrix-core shapes and
+      // the 8-wave boxes on g_stream, the shared-memory / global classes of
       // k_elim on the other side stream; joined before anything reads them
-      const int nclass = (creg[1].empty() ? 0 : 1) + (creg[0].empty() ? 0 : 1) +
+      int nmf = 0;
+      for (int c = 0; c < kMfmaShapes; ++c) nmf += cmf[c].empty() ? 0 : 1;
+      const int nclass = nmf + (creg[1].empty() ? 0 : 1) + (creg[0].empty() ? 0 : 1) +
                          ((cls[0].empty() && cls[1].empty() && cls[2].empty()) ? 0 : 1);
       const bool conc = nclass > 1 && g_tune.concurrent;
       if (conc) RSKX_TRY(side_fork());
       int off = 0;
+      for (int c = kMfmaShapes - 1; c >= 0; --c) {
+        if (cmf[c].empty()) continue;
+        launch_elim_mfma<S>(c, shm_mf[c], ws.ids.as<int>() + off, (int)cmf[c].size(),
+                            ws.boxes.as<LvlBox>(), ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(),
+                            ws.pool.as<int>(), ws.perm.as<int>(), ws.T.as<double>(),
+                            ws.M[mpar].as<S>(), ws.M[mcur].as<S>(), fval, fidx, P,
+                            ws.flag.as<int>(),
+                            conc && c == kMfmaShapes - 1 ? g_side.s[0] : g_stream);
+        off += (int)cmf[c].size();
+      }
       for (int c = 1; c >= 0; --c) {           // 16-wave boxes first, then 8-wave
         if (creg[c].empty()) continue;
         launch_elim_reg<S>(lu, c == 0 ? 8 : 16, shm_reg[c], ws.ids.as<int>() + off,
@@ -2013,6 +2047,7 @@ static int* tuning_slot(const char* name) {
   if (s == "sweep_stage") return &g_tune.sweep_stage;
   if (s == "id_lds") return &g_tune.id_lds;
   if (s == "id_reg") return &g_tune.id_reg;
+  if (s == "elim_mfma") return &g_tune.elim_mfma;
   return nullptr;
 }
 

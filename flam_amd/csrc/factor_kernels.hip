@@ -9,38 +9,10 @@
 //               the skeleton Schur complement (rskelf.m:218-261)
 #include "common.hpp"
 #include "devutil.hpp"
+#include "entries.hpp"
 #include "rskx.h"
 
 namespace rskx {
-
-constexpr double kTwoPi = 6.283185307179586476925286766559;
-constexpr double kFourPi = 12.566370614359172953850573533118;
-
-// ------------------------------------------------- point fields, kernels --
-// Every kernel stages five fields per point: x, y, f2, f3, w, where
-// (f2, f3) = (nu_x, nu_y) for the 2D double layer and (z, 0) for the 3D
-// single layer, which needs no normals.
-__device__ inline void point_fields(const PointsDev& P, int pt, double& f0, double& f1, double& f2,
-                                    double& f3, double& fw) {
-  f0 = P.x[0][pt];
-  f1 = P.x[1][pt];
-  if (P.kind == RSKX_KERNEL_LAPLACE3D_SL) {
-    f2 = P.x[2][pt];
-    f3 = 0;
-  } else {
-    f2 = P.nu[0][pt];
-    f3 = P.nu[1][pt];
-  }
-  fw = P.w[pt];
-}
-
-// A_ij = scale w_j / (4 pi |x_i - x_j|) (RSKX_KERNEL_LAPLACE3D_SL, i != j)
-// from target (t0, t1, t2) and source (s0, s1, s2) coordinates
-__device__ inline double sl3_entry(const PointsDev& P, double t0, double t1, double t2, double s0,
-                                   double s1, double s2, double sw) {
-  const double dx = t0 - s0, dy = t1 - s1, dz = t2 - s2;
-  return P.scale * sw / (kFourPi * sqrt(dx * dx + dy * dy + dz * dz));
-}
 
 // proxy point q of box B: the ring of ie_circle.m:35,107 / ie_ellipse.m:127,
 // or the caller's points scaled by the box size (ie_sphere.m:175)
@@ -545,26 +517,6 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
 }
 
 // ------------------------------------------------------------- k_factor --
-// child block lookup for the modified diagonal M (rskelf.m:190-199)
-struct ChildMap {
-  int nc;
-  int start[9];
-  int64_t moff[8];
-};
-
-template <typename S>
-__device__ inline S mval(const ChildMap& cm, const S* Mprev, int a, int b) {
-  for (int c = 0; c < cm.nc; ++c) {
-    int s0 = cm.start[c], s1 = cm.start[c + 1];
-    if (a >= s0 && a < s1) {
-      if (b < s0 || b >= s1 || cm.moff[c] < 0) return (S)0;
-      int len = s1 - s0;
-      return Mprev[cm.moff[c] + (a - s0) + (int64_t)(b - s0) * len];
-    }
-  }
-  return (S)0;
-}
-
 // Elimination of one box's redundant points (rskelf.m:218-261).
 //
 // With the skeleton-first order [sk rd] and K = A(slf,slf) + M, the two
