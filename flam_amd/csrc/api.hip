@@ -59,7 +59,7 @@ void launch_elim(bool, bool, int, size_t, const int*, int, const LvlBox*, const 
                  int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
                  int*, hipStream_t);
 int elim_mfma_shape(int n, int k, int r, size_t es, size_t* shmem);
-constexpr int kMfmaShapes = 3;
+constexpr int kMfmaShapes = 4;
 template <typename S>
 void launch_elim_mfma(int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
                       const int*, const double*, const S*, S*, S*, int*, const PointsDev&, int*,
@@ -1543,7 +1543,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // workgroups (two per CU, n <= 192 in FP32, 128 in FP64) when the box
     // and its shared memory allow, else 16-wave workgroups
     std::vector<int> cls[3], creg[2], cmf[kMfmaShapes];
-    size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0}, shm_mf[kMfmaShapes] = {0, 0, 0};
+    size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0}, shm_mf[kMfmaShapes] = {0, 0, 0, 0};
     const int nreg8 = std::min(elim_reg_nmax(sizeof(S), 8), g_tune.elim8_nmax),
               nreg16 = elim_reg_nmax(sizeof(S), 16);
     for (int64_t ib = 0; ib < nb; ++ib) {
