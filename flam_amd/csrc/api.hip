@@ -41,6 +41,7 @@ void launch_gram_id(int, const int*, const LvlBox*, const int*, const int*, cons
 int gram_id_nmax(int cls);
 int gram_id_class(int n);
 size_t gram_id_shmem(int cls, int n, int mn, int m, int p);
+void set_gram_mfma(bool on);
 constexpr int kGramClasses = 3;
 template <typename S>
 void launch_id_reg(int, const int*, const LvlBox*, const int*, const int*, const int*, int*, S*,
@@ -150,6 +151,8 @@ struct Tuning {
                                    // vectors in global memory
   int id_reg = 1;                  // register QRCP classes for the boxes that fit them
   int elim_mfma = 1;               // matrix-core elimination for the Cholesky ('p') boxes
+  int gram_mfma = 0;               // matrix-core accumulation of G in the Gram ID (measured
+                                   // slower than the vector FP64 one: 5.3 vs 4.9 ms)
 };
 static Tuning g_tune;
 
@@ -2048,6 +2051,7 @@ static int* tuning_slot(const char* name) {
   if (s == "id_lds") return &g_tune.id_lds;
   if (s == "id_reg") return &g_tune.id_reg;
   if (s == "elim_mfma") return &g_tune.elim_mfma;
+  if (s == "gram_mfma") return &g_tune.gram_mfma;
   return nullptr;
 }
 
@@ -2060,6 +2064,7 @@ int rskx_set_tuning(const char* name, double value) {
     return fail(RSKX_ERR_ARG, "FLAM:rskx:tuning: shared-memory budgets lie in [4096, 163840] bytes");
   RSKX_TRY(sync_stream());                 // queued work keeps the values it was launched with
   *slot = (int)value;
+  set_gram_mfma(g_tune.gram_mfma != 0);
   return RSKX_OK;
 }
 

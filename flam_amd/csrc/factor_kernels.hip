@@ -10,6 +10,7 @@
 #include "common.hpp"
 #include "devutil.hpp"
 #include "entries.hpp"
+#include "mfma.hpp"
 #include "rskx.h"
 
 namespace rskx {
@@ -1422,6 +1423,251 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   if (tid == 0) kout[b] = k;
 }
 
+// k_gram_id on the matrix cores: the same column ID, with G = K'K
+// accumulated by v_mfma_f64_16x16x4f64 into 16 x 16 tiles of its lower
+// triangle (spread round-robin over the waves, MS per wave) from each
+// 16-row chunk of K staged transposed in LDS (row j of the chunk = column j
+// of K over the chunk's rows), so a tile's four products read their
+// operands as two 32-byte rows per lane.  The pivoted Cholesky then works
+// on the accumulator registers: lane l of a tile holds G(16 I + (l >> 4) +
+// 4 v, 16 J + (l & 15)), v < 4.
+template <int NW, int MS>
+__global__ void __launch_bounds__(64 * NW)
+k_gram_id_mf(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
+             const int* __restrict__ pool, const int* __restrict__ nbrar,
+             const int* __restrict__ mnbr, int* __restrict__ mrows, double* __restrict__ Tar,
+             int* __restrict__ permar, PointsDev P, LevelParams L, IdParams prm,
+             int* __restrict__ kout) {
+  typedef Mf<double>::V V;
+  constexpr int NT = 64 * NW;
+  constexpr int RC = 16;                          // rows of K per chunk
+  constexpr int LD = RC + kMfPad;                 // chunk row length (doubles)
+  extern __shared__ double sh[];
+  const int b = ids[blockIdx.x];
+  const LvlBox B = boxes[b];
+  const int n = B.n, mn = mnbr[b];
+  const int mt = L.two_sided ? 2 * mn : mn;
+  const int m = mt + L.p;
+  const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nt16 = (n + 15) >> 4, np = 16 * nt16;
+  int* perm = permar + B.perm_off;
+  const int* slf = pool + B.slf_off;
+  const int* nbr = nbrar + B.nbr_off;
+  double* chunk = sh;                // 2 x np x LD: chunk[j LD + rr] = K(r0 + rr, j)
+  double* dg = chunk + 2 * np * LD;  // remaining diagonal of G (np)
+  double* prow = dg + np;            // pivot row of the updated G (np)
+  double* red = prow + np;           // 2 NW values, 2 NW indices
+  double* ppx = red + 4 * NW;        // proxy points
+  double* ppy = ppx + L.p;
+  double* ppz = ppy + L.p;
+  double* sx = ppz + L.p;            // column points (fields as point_fields)
+  double* sy = sx + n;
+  double* snx = sy + n;
+  double* sny = snx + n;
+  double* sw = sny + n;
+  double* rx = sw + n;               // neighbour (row) points
+  double* ry = rx + mn;
+  double* rnx = ry + mn;
+  double* rny = rnx + mn;
+  double* rw = rny + mn;
+  int* p2c = reinterpret_cast<int*>(rw + mn);        // position -> column (n)
+  const int ldt = B.ldt;                             // R rows kept (the rank cap)
+  double* Rc = Tar + B.T_off + (int64_t)ldt * n;     // R rows by column
+  double* Rp = Rc + (int64_t)ldt * n;                // R rows by position
+  for (int j = tid; j < n; j += NT) {
+    point_fields(P, slf[j], sx[j], sy[j], snx[j], sny[j], sw[j]);
+    p2c[j] = j;
+  }
+  for (int i = tid; i < mn; i += NT)
+    point_fields(P, nbr[i], rx[i], ry[i], rnx[i], rny[i], rw[i]);
+  for (int q = tid; q < L.p; q += NT) proxy_point(L, B, q, ppx[q], ppy[q], ppz[q]);
+  for (int j = tid; j < np; j += NT) { prow[j] = 0; dg[j] = 0; }
+  if (tid == 0) mrows[b] = m;
+  __syncthreads();
+  if (m == 0 || n == 0) {
+    for (int j = tid; j < n; j += NT) perm[j] = j;
+    if (tid == 0) kout[b] = 0;
+    return;
+  }
+  // this wave's tiles (column-major order of the lower triangle); lane s of
+  // tIv / tJv holds slot s, read back with readlane where used
+  int tIv = -1, tJv = -1;
+  {
+    const int ntile = nt16 * (nt16 + 1) / 2;
+    int t = wid + NW * lane;
+    if (lane < MS && t < ntile) {
+      int J = 0, cnt = nt16;
+      while (t >= cnt) { t -= cnt; ++J; --cnt; }
+      tIv = J + t;
+      tJv = J;
+    }
+  }
+  auto tI = [&](int s) -> int { return __builtin_amdgcn_readlane(tIv, s); };
+  auto tJ = [&](int s) -> int { return __builtin_amdgcn_readlane(tJv, s); };
+  V acc[MS];
+#pragma unroll
+  for (int s = 0; s < MS; ++s)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[s][v] = 0;
+  // G = K'K over chunks of RC rows of K = [A(nbr,slf); A(slf,nbr)'; Kpxy]
+  // (rskelf.m:201-212), entries as in k_assemble
+  const double c0 = P.scale / kTwoPi;
+  const bool sl3 = P.kind == RSKX_KERNEL_LAPLACE3D_SL;
+  // entries of rows r0.. of K into a chunk buffer (transposed)
+  auto eval_chunk = [&](int r0, double* ch) {
+    for (int e = tid; e < RC * np; e += NT) {
+      const int rr = e & (RC - 1), j = e >> 4, i = r0 + rr;
+      double val = 0;
+      if (i < m && j < n) {
+        const double xj = sx[j], yj = sy[j];
+        if (i < mn) {
+          if (sl3) {
+            val = sl3_entry(P, rx[i], ry[i], rnx[i], xj, yj, snx[j], sw[j]);
+          } else {
+            const double dx = rx[i] - xj, dy = ry[i] - yj;
+            val = c0 * sw[j] * (dx * snx[j] + dy * sny[j]) / (dx * dx + dy * dy);
+          }
+        } else if (i < mt) {
+          const int q = i - mn;
+          if (sl3) {
+            val = sl3_entry(P, xj, yj, snx[j], rx[q], ry[q], rnx[q], rw[q]);
+          } else {
+            const double dx = xj - rx[q], dy = yj - ry[q];
+            val = c0 * rw[q] * (dx * rnx[q] + dy * rny[q]) / (dx * dx + dy * dy);
+          }
+        } else {
+          const int q = i - mt;
+          if (L.pkind == RSKX_PROXY_POINTS_SL) {
+            val = proxy_sl3(L, B, q, ppx[q], ppy[q], ppz[q], xj, yj, snx[j], sw[j]);
+          } else {
+            const double dx = ppx[q] - xj, dy = ppy[q] - yj;
+            const double r2 = dx * dx + dy * dy;
+            val = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
+                                                : sw[j] * (dx * snx[j] + dy * sny[j]) / (kTwoPi * r2);
+          }
+        }
+      }
+      ch[j * LD + rr] = val;
+    }
+  };
+  // two chunk buffers: the products of one chunk (matrix cores) are issued
+  // before the next chunk's entries are evaluated (vector FP64), so the two
+  // pipes overlap
+  eval_chunk(0, chunk);
+  for (int r0 = 0; r0 < m; r0 += RC) {
+    asm volatile("" : "+v"(tIv), "+v"(tJv));
+    __syncthreads();
+    const double* cur = chunk + ((r0 / RC) & 1) * np * LD;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      if (s && !(s & 3)) asm volatile("" ::: "memory");
+      const int I = tI(s);
+      if (I < 0) continue;
+      acc[s] = mac_rows<double, false>(acc[s], cur + 16 * I * LD, LD, cur + 16 * tJ(s) * LD,
+                                       LD, RC, lane);
+    }
+    if (r0 + RC < m) eval_chunk(r0 + RC, chunk + (((r0 / RC) + 1) & 1) * np * LD);
+  }
+  const int lrow = lane & 15;
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    const int I = tI(s);
+    if (I < 0 || I != tJ(s)) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (Mf<double>::row(lane, v) == lrow) dg[16 * I + lrow] = acc[s][v];
+  }
+  __syncthreads();
+  // diagonally pivoted Cholesky, right-looking on the register tiles,
+  // stopped as id.m:105-111 stops the QR
+  int kmax = prm.kmax;
+  if (kmax <= 0 || kmax > n) kmax = n;
+  const int kuser = kmax;
+  if (kmax > ldt) kmax = ldt;            // the box's buffers hold ldt rows of R
+  const int smax = m < n ? m : n;
+  double tol_abs = 0;
+  int k = 0;
+  for (int s = 0; s < smax; ++s) {
+    asm volatile("" : "+v"(tIv), "+v"(tJv));
+    double bv = -1.0;
+    int bi = n;
+    for (int pos = s + tid; pos < n; pos += NT) argmax_merge(bv, bi, dg[p2c[pos]], pos);
+    // block argmax through LDS barriers only (R's stores stay in flight);
+    // the reduction slots alternate between steps
+    wave_argmax(bv, bi);
+    double* rv = red + (s & 1) * NW;
+    int* ri = reinterpret_cast<int*>(red + 2 * NW) + (s & 1) * NW;
+    if (lane == 0) { rv[wid] = bv; ri[wid] = bi; }
+    lds_barrier();
+    bv = rv[0];
+    bi = ri[0];
+    for (int w = 1; w < NW; ++w) argmax_merge(bv, bi, rv[w], ri[w]);
+    if (bi >= n) break;                // no finite candidate
+    const int pc = p2c[bi];
+    const double nrm = sqrt(fmax(bv, 0.0));
+    if (s == 0) tol_abs = nrm * prm.tol;
+    if (!(nrm > tol_abs)) break;
+    k = s + 1;
+    // row pc of G: from the tiles of block row pc / 16 and block column pc / 16
+    const int pt = pc >> 4, pl = pc & 15;
+#pragma unroll
+    for (int q = 0; q < MS; ++q) {
+      const int I = tI(q), J = tJ(q);
+      if (I < 0) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rv = Mf<double>::row(lane, v);
+        if (I == pt && rv == pl) prow[16 * J + lrow] = acc[q][v];
+        if (J == pt && lrow == pl) prow[16 * I + rv] = acc[q][v];
+      }
+    }
+    lds_barrier();
+    if (tid == 0 && bi != s) { p2c[bi] = p2c[s]; p2c[s] = pc; }
+    const double rinv = 1.0 / nrm;
+    for (int j = tid; j < n; j += NT) {
+      const double r = prow[j] * rinv;
+      Rc[s + (int64_t)j * ldt] = r;
+      dg[j] -= r * r;
+    }
+    const double r2 = rinv * rinv;
+#pragma unroll
+    for (int q = 0; q < MS; ++q) {
+      const int I = tI(q), J = tJ(q);
+      if (I < 0) continue;
+      const double cj = prow[16 * J + lrow] * r2;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[q][v] -= prow[16 * I + Mf<double>::row(lane, v)] * cj;
+    }
+    lds_barrier();
+    if (k == kmax) break;
+  }
+  __syncthreads();                       // R's stores, before they are read back
+  if (k == ldt && ldt < kuser && k < smax) {
+    // stopped by the rank cap: if a remaining column is still above the
+    // tolerance the true rank exceeds the cap (an under-predicted rank);
+    // kout = ldt + 1 tells the host to redo this box with full buffers
+    double bv = -1.0;
+    int bi = n;
+    for (int pos = k + tid; pos < n; pos += NT) argmax_merge(bv, bi, dg[p2c[pos]], pos);
+    block_argmax(bv, bi, red);
+    if (bi < n && sqrt(fmax(bv, 0.0)) > tol_abs) {
+      if (tid == 0) kout[b] = ldt + 1;
+      return;
+    }
+  }
+  // R's leading k rows by pivot position
+  for (int e = tid; e < k * n; e += NT) {
+    const int i = e % k, pos = e / k;
+    Rp[i + (int64_t)pos * ldt] = Rc[i + (int64_t)p2c[pos] * ldt];
+  }
+  for (int pos = tid; pos < n; pos += NT) perm[pos] = p2c[pos];
+  if (tid == 0) kout[b] = k;
+}
+
+// the matrix-core Gram ID for every class (set from the host's tuning)
+static bool g_gram_mfma = false;
+void set_gram_mfma(bool on) { g_gram_mfma = on; }
+
 void launch_gram_id(int cls, const int* ids, const LvlBox* boxes, const int* pool,
                     const int* nbrar, const int* mnbr, int* mrows, double* Tar, int* permar,
                     const PointsDev& P, const LevelParams& L, const IdParams& prm, int* kout,
@@ -1433,7 +1679,19 @@ void launch_gram_id(int cls, const int* ids, const LvlBox* boxes, const int* poo
     hipLaunchKernelGGL(kern, dim3(nb), dim3(NTV), shmem, st, ids, boxes, pool, nbrar, mnbr,      \
                        mrows, Tar, permar, P, L, prm, kout);                                     \
   }
-  if (cls == 0) RSKX_GID(256, 2) else if (cls == 1) RSKX_GID(512, 2) else RSKX_GID(512, 5)
+#define RSKX_GIDM(NWV, MSV)                                                                     \
+  {                                                                                              \
+    auto kern = k_gram_id_mf<NWV, MSV>;                                                          \
+    ensure_dyn_lds((const void*)kern, shmem);                                                     \
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(64 * NWV), shmem, st, ids, boxes, pool, nbrar, mnbr, \
+                       mrows, Tar, permar, P, L, prm, kout);                                     \
+  }
+  if (g_gram_mfma) {
+    if (cls == 0) RSKX_GIDM(4, 9) else if (cls == 1) RSKX_GIDM(8, 9) else RSKX_GIDM(8, 22)
+  } else {
+    if (cls == 0) RSKX_GID(256, 2) else if (cls == 1) RSKX_GID(512, 2) else RSKX_GID(512, 5)
+  }
+#undef RSKX_GIDM
 #undef RSKX_GID
 }
 
@@ -1446,6 +1704,12 @@ int gram_id_class(int n) {
   return -1;
 }
 size_t gram_id_shmem(int cls, int n, int mn, int m, int p) {
+  if (g_gram_mfma) {
+    // two chunks np16 x (16 + pad), dg, prow, red, proxies, point data, p2c
+    const int np = (n + 15) / 16 * 16, nw = cls == 0 ? 4 : 8;
+    return (size_t)(2 * np * kMfLdp + 3 * p + 5 * n + 5 * mn + 2 * np + 4 * nw) * sizeof(double) +
+           (size_t)n * sizeof(int) + 64;
+  }
   const int np = (n + 3) / 4 * 4, nw = (cls == 0 ? 256 : 512) / 64;
   return (size_t)(kGramRows * np + 3 * p + 5 * n + 5 * mn + 2 * np + 2 * nw) * sizeof(double) +
          (size_t)n * sizeof(int) + 64;

@@ -33,6 +33,8 @@ SPACE = {
     "gram_n0": [0, 96, 124],
     "gram_n1": [0, 148, 176],
     "sweep_stage": [48 * 1024, 64 * 1024, 76 * 1024],
+    "elim_mfma": [0, 1],
+    "gram_mfma": [0, 1],
 }
 
 
