@@ -112,28 +112,49 @@ def measured_traffic(kernel, workload="circle_N1048576_fp32"):
     return None, None
 
 
-def roofline_of(F, F_full, precision, workload=None):
-    """Dominant kernel class by time in the fully profiled step F_full, with
-    algorithmic flops / bytes; its launch times come from the last timed
-    step F when that step timed it (the elimination class), else from
-    F_full."""
+def roofline_of(F, F_full, precision, workload=None, name=None):
+    """Roofline of one kernel class (default: the dominant class by time in
+    the fully profiled step F_full), with algorithmic flops / bytes; its
+    launch times come from the last timed step F when that step timed it
+    (the elimination class), else from F_full.  `pipe` names the unit the
+    peak belongs to: the elimination runs on the matrix cores (MFMA, FP32
+    or FP64), the Gram ID on the vector FP64 units (whose peak on MI355X
+    equals the FP64 matrix-core peak), the sweeps are bound by HBM."""
     kt = F_full.kernel_times()
-    name, st = max(((k, v) for k, v in kt.items() if k != "tree" and not k.startswith("h:")),
-                   key=lambda kv: kv[1]["ms"])
+    if name is None:
+        name, st = max(((k, v) for k, v in kt.items() if k != "tree" and not k.startswith("h:")),
+                       key=lambda kv: kv[1]["ms"])
+    elif name not in kt:
+        return None, kt
+    else:
+        st = kt[name]
     st = F.kernel_times().get(name, st)
     avg_ms = st["ms"] / max(st["launches"], 1)
     if st["flops"] > 0:
         ach = st["flops"] / st["launches"] / (avg_ms * 1e-3)
         peak = PEAK["fp64"] if name == "id" or precision == "fp64" else PEAK["fp32"]
+        pipe = ("valu-fp64" if name == "id" else
+                "mfma-fp64" if precision == "fp64" else "mfma-fp32")
         traffic, src = measured_traffic(name, workload) if workload else (None, None)
-        return dict(kernel=name, bound="mfma", achieved=ach / 1e12, peak=peak / 1e12,
+        return dict(kernel=name, bound="mfma", pipe=pipe, achieved=ach / 1e12, peak=peak / 1e12,
                     unit="TFLOP/s", frac=ach / peak, traffic=traffic, traffic_source=src,
                     avg_launch_ms=avg_ms, launches=st["launches"]), kt
     ach = st["bytes"] / st["launches"] / (avg_ms * 1e-3)
     traffic, src = measured_traffic(name, workload) if workload else (None, None)
-    return dict(kernel=name, bound="hbm", achieved=ach / 1e9, peak=PEAK["hbm"] / 1e9,
+    return dict(kernel=name, bound="hbm", pipe="hbm", achieved=ach / 1e9, peak=PEAK["hbm"] / 1e9,
                 unit="GB/s", frac=ach / PEAK["hbm"], traffic=traffic, traffic_source=src,
                 avg_launch_ms=avg_ms, launches=st["launches"]), kt
+
+
+def class_rooflines(F, F_full, precision, workload):
+    """Rooflines of the three main kernel classes: elimination (factor),
+    Gram ID (id) and sweeps (sweep)."""
+    out = {}
+    for name in ("factor", "id", "sweep"):
+        roof, _ = roofline_of(F, F_full, precision, workload, name)
+        if roof is not None:
+            out[name] = roof
+    return out
 
 
 def cpu_baseline(N, occ, tol, nrhs, target, log, runs=3):
@@ -349,6 +370,7 @@ def main():
     t_mix, i_mix, F_mix, Ff_mix = run_path(flam_amd, g, B, args, "fp32", args.steps, args.warmup,
                                            log)
     roof, kt = roofline_of(F_mix, Ff_mix, "fp32", f"circle_N{N}_fp32")
+    roofs = class_rooflines(F_mix, Ff_mix, "fp32", f"circle_N{N}_fp32")
     del Ff_mix
     info_mix = F_mix.info()
     table = F_mix.level_table()
@@ -357,9 +379,10 @@ def main():
         t_pl, i_pl, F_pl, Ff_pl = run_path(flam_amd, g, B, args, "fp64", args.steps, args.warmup,
                                            log)
         roof64, kt64 = roofline_of(F_pl, Ff_pl, "fp64", f"circle_N{N}_fp64")
+        roofs64 = class_rooflines(F_pl, Ff_pl, "fp64", f"circle_N{N}_fp64")
         del Ff_pl
         plain = dict(value=N / float(np.mean(t_pl)), ms_per_step=1e3 * float(np.mean(t_pl)),
-                     steps=i_pl, roofline=roof64,
+                     steps=i_pl, roofline=roof64, rooflines=roofs64,
                      kernels={k: dict(ms=v["ms"], launches=v["launches"]) for k, v in kt64.items()})
         del F_pl
     cpu = None
@@ -382,6 +405,7 @@ def main():
         "factor": dict(info_mix, levels=table),
         "kernels": {k: dict(ms=v["ms"], launches=v["launches"]) for k, v in kt.items()},
         "roofline": roof,
+        "rooflines": roofs,
         "plain_fp64": plain,
         "mixed_over_plain": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,

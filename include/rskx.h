@@ -268,9 +268,11 @@ void rskx_graph_free(rskx_graph* G);
  * and 1 ("gram_n0", "gram_n1"), concurrent size classes ("concurrent"),
  * the sweep's row / stage buffer budget ("sweep_stage"), the shared
  * bytes above which the QR ID keeps Householder vectors in global memory
- * ("id_lds") and the register QRCP classes on or off ("id_reg").  Values
- * apply to later calls; results do not depend on them (only which kernel
- * shape runs). */
+ * ("id_lds"), the register QRCP classes on or off ("id_reg"), the
+ * matrix-core elimination of symm 'p' boxes ("elim_mfma", on) and the
+ * matrix-core accumulation of G in the Gram ID ("gram_mfma", off).  Values
+ * apply to later calls; results do not depend on them beyond rounding (only
+ * which kernel shape runs). */
 int rskx_set_tuning(const char* name, double value);
 int rskx_get_tuning(const char* name, double* value);
 

@@ -25,10 +25,14 @@ def test_traffic_summary_is_read():
     finally:
         sys.path.pop(0)
     traffic, src = bench.measured_traffic("factor")
-    assert src == "rskelf_circle_N1048576_fp32_elimination_traffic.md"
-    # the elimination writes the factor (~1.7 GB per factorization over 12
+    assert src == "rskelf_circle_N1048576_fp32_traffic.md"
+    # the elimination writes the factor (over a GB per factorization over 12
     # launches): the measured bytes per launch are of that order
     assert 5e7 < traffic < 1e9
+    # the Gram ID and the sweeps have their own entries
+    for cls in ("id", "sweep"):
+        t, s = bench.measured_traffic(cls)
+        assert s == "rskelf_circle_N1048576_fp32_traffic.md" and t > 1e6
     assert bench.measured_traffic("no-such-kernel") == (None, None)
     # keyed by workload: another workload's class of the same name gets none
     assert bench.measured_traffic("factor", "ellipses_fp32") == (None, None)
