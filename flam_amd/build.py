@@ -13,9 +13,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = ["api.hip", "factor_kernels.hip", "sweep_kernels.hip", "tree_kernels.hip", "tree.cpp",
            "refine_kernels.hip", "zfactor_kernels.hip", "zsweep_kernels.hip", "mlp_kernels.hip",
-           "elim_mfma.hip"]
+           "elim_mfma.hip", "big_kernels.hip"]
 LIB = os.path.join(HERE, "librskx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# the largest boxes' dense factorizations and products run through rocSOLVER /
+# rocBLAS (the same sonames torch loads, so one copy of each per process)
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -56,7 +59,7 @@ def build(verbose=False, src_dir=None, out=None):
         with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
             list(ex.map(_run, [c for c, _ in jobs]))
         tmp = out + ".tmp"
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *[o for _, o in jobs], "-o", tmp])
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *[o for _, o in jobs], *LIBS, "-o", tmp])
         os.replace(tmp, out)
     return out
 

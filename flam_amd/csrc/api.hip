@@ -60,6 +60,17 @@ void launch_elim(bool, bool, int, size_t, const int*, int, const LvlBox*, const 
                  int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
                  int*, hipStream_t);
 int elim_mfma_shape(int n, int k, int r, size_t es, size_t* shmem);
+size_t big_id_scratch(int n, int mb);
+int big_id(int b, int n, int mb, int kmax, int smax, double tol, const LvlBox* boxes,
+           const int* pool, const int* nbrar, const int* mnbr, int* mrows, double* Tar,
+           int* permar, int* kout, const PointsDev& P, const LevelParams& L, double* scratch,
+           hipStream_t st);
+size_t big_elim_scratch(int n, int k, int r, size_t es);
+template <typename S>
+int big_elim(bool lu, int b, int n, int k, int r, const LvlBox* boxes, const BoxOut* outs,
+             const Seg* csegs, int* pool, const int* permar, const double* Tar, const S* Mprev,
+             S* Mcur, S* fval, int* fidx, const PointsDev& P, int* fail, void* scratch,
+             hipStream_t st);
 constexpr int kMfmaShapes = 4;
 template <typename S>
 void launch_elim_mfma(int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
@@ -151,6 +162,8 @@ struct Tuning {
                                    // vectors in global memory
   int id_reg = 1;                  // register QRCP classes for the boxes that fit them
   int elim_mfma = 1;               // matrix-core elimination for the Cholesky ('p') boxes
+  int big_n = 1024;                // boxes with more points take the whole-GPU path (library
+                                   // GEMM / LU / Cholesky, grid-wide pivoted Cholesky of G)
   int gram_mfma = 0;               // matrix-core accumulation of G in the Gram ID (measured
                                    // slower than the vector FP64 one: 5.3 vs 4.9 ms)
 };
@@ -284,6 +297,7 @@ struct Workspace {
   DevBuf cgs;                   // Gram-Schmidt partial sums
   DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs, poff;
   DevBuf Ablk;                  // host-callback matrices: A(slf,slf) of the level's boxes
+  DevBuf big;                   // scratch of the whole-GPU path for the largest boxes
   int* hflag = nullptr;
 };
 static Workspace g_ws;
@@ -1235,13 +1249,18 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       RSKX_TRY(download(hmn.data(), ws.mnbr.p, nb * sizeof(int)));
       RSKX_TRY(sync_stream());
     }
-    std::vector<int> idg, idr[2], idq[kGramClasses], idall;
+    std::vector<int> idg, idr[2], idq[kGramClasses], idall, idb;
     size_t shm_r[2] = {0, 0}, shm_q[kGramClasses] = {0, 0, 0};
     int nmax_g = 0, mcapmax_g = 0;
     // a box whose point data would not fit a class's shared memory goes to
-    // the next route: Gram -> register QRCP -> QRCP on K in global memory
+    // the next route: Gram -> register QRCP -> QRCP on K in global memory;
+    // on the Gram route the largest boxes take the whole-GPU path instead
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
+      if (gram && n > g_tune.big_n && mb > 0 && hb[ib].ldt == n) {
+        idb.push_back((int)ib);
+        continue;
+      }
       if (gram && n > 0 && mb > 0 && gram_id_class(n) >= 0) {
         const int c = gram_class_tuned(n);
         const size_t by = gram_id_shmem(c, n, hmn[ib], mb, L.p);
@@ -1291,6 +1310,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     idall.insert(idall.end(), idr[0].begin(), idr[0].end());
     idall.insert(idall.end(), idr[1].begin(), idr[1].end());
     for (int c = kGramClasses - 1; c >= 0; --c) idall.insert(idall.end(), idq[c].begin(), idq[c].end());
+    idall.insert(idall.end(), idb.begin(), idb.end());    // after the Gram boxes: the same tail
     RSKX_TRY(upload(ws.ids, idall));
     const int* dids = ws.ids.as<int>();
     const int ng = (int)idg.size(), nr0 = (int)idr[0].size(), nr1 = (int)idr[1].size();
@@ -1355,12 +1375,37 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       }
       if (conc) RSKX_TRY(side_join());
     }
-    if (nq) {
+    // the largest boxes, one at a time with the whole GPU: K, G = K'K, the
+    // pivoted Cholesky of G; their T and refinement run in the Gram tail
+    if (!idb.empty()) {
+      size_t need = 0;
+      for (int ib : idb) {
+        const int mb = hmn[ib] * (two ? 2 : 1) + L.p;
+        need = std::max(need, big_id_scratch(hb[ib].n, mb));
+      }
+      RSKX_TRY(ws.big.ensure(need * sizeof(double)));
+      for (int ib : idb) {
+        const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
+        int kmx = prm.kmax;
+        if (kmx <= 0 || kmx > n) kmx = n;
+        RSKX_TRY(big_id(ib, n, mb, kmx, std::min(mb, n), prm.tol, ws.boxes.as<LvlBox>(),
+                        ws.pool.as<int>(), ws.nbr.as<int>(), ws.mnbr.as<int>(), ws.mrows.as<int>(),
+                        ws.T.as<double>(), ws.perm.as<int>(), ws.kout.as<int>(), P, L,
+                        ws.big.as<double>(), g_stream));
+      }
+    }
+    const int nt2 = nq + (int)idb.size();
+    if (nt2) {
       int nmx = 0;
       for (int c = 0; c < kGramClasses; ++c) for (int ib : idq[c]) nmx = std::max(nmx, hb[ib].n);
+      for (int ib : idb) nmx = std::max(nmx, hb[ib].n);
+      const size_t sht = (size_t)(3 * nmx + 32) * sizeof(double);
+      if (sht > 160 * 1024)
+        return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box too large for the ID's refinement kernel (n=" +
+                                         std::to_string(nmx) + ")");
       launch_id<S>(2, qids, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
                    ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
-                   nq, (size_t)(3 * nmx + 32) * sizeof(double), nullptr, 0, g_stream);
+                   nt2, sht, nullptr, 0, g_stream);
     }
     prof.end("id", ev);
     }
@@ -1443,7 +1488,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       O.k = hk[ib];
       O.r = B.n - O.k;
       const int64_t k = O.k, r = O.r, m = hm[ib], n = B.n;
-      if (gram && n > 0 && m > 0 && gram_id_class((int)n) >= 0)   // G = K'K, pivoted Cholesky, T
+      if (gram && n > 0 && m > 0 && (gram_id_class((int)n) >= 0 || n > g_tune.big_n))   // G = K'K, pivoted Cholesky, T
         id_flops += (double)m * n * (n + 1) + 2.0 * (double)n * n * k + (double)k * k * r;
       else                                                  // truncated QRCP, T (x4 complex)
         id_flops += (CPLX ? 4.0 : 1.0) * (4.0 * m * n * k - 2.0 * k * k * (m + n) +
@@ -1545,13 +1590,20 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     // Boxes whose augmented matrix fits registers go to k_elim_reg: 8-wave
     // workgroups (two per CU, n <= 192 in FP32, 128 in FP64) when the box
     // and its shared memory allow, else 16-wave workgroups
-    std::vector<int> cls[3], creg[2], cmf[kMfmaShapes];
+    std::vector<int> cls[3], creg[2], cmf[kMfmaShapes], cbig;
     size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0}, shm_mf[kMfmaShapes] = {0, 0, 0, 0};
     const int nreg8 = std::min(elim_reg_nmax(sizeof(S), 8), g_tune.elim8_nmax),
               nreg16 = elim_reg_nmax(sizeof(S), 16);
+    size_t big_need = 0;
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
       const int n = hb[ib].n;
+      // the largest boxes: dense LU / Cholesky with the whole GPU
+      if (O.r > 0 && !ldl && n > g_tune.big_n) {
+        cbig.push_back((int)ib);
+        big_need = std::max(big_need, big_elim_scratch(n, O.k, O.r, sizeof(S)));
+        continue;
+      }
       // Cholesky boxes go to the matrix-core elimination when its tiles and
       // shared memory hold them
       if (O.r > 0 && !lu && !ldl && g_tune.elim_mfma) {
@@ -1595,6 +1647,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     allids.insert(allids.end(), creg[0].begin(), creg[0].end());
     for (int c = 2; c >= 0; --c) allids.insert(allids.end(), cls[c].begin(), cls[c].end());
     RSKX_TRY(upload(ws.ids, allids));
+    if (!cbig.empty()) RSKX_TRY(ws.big.ensure(big_need));
     prof.begin("factor", ev);
     {
       // the classes run concurrently: the largest matrix-core shape and the
@@ -1637,6 +1690,13 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         off += (int)cls[c].size();
       }
       if (conc) RSKX_TRY(side_join());
+      // the largest boxes one after another, each with the whole GPU
+      for (int ib : cbig)
+        RSKX_TRY(big_elim<S>(lu, ib, hb[ib].n, ho[ib].k, ho[ib].r, ws.boxes.as<LvlBox>(),
+                             ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
+                             ws.perm.as<int>(), ws.T.as<double>(), ws.M[mpar].as<S>(),
+                             ws.M[mcur].as<S>(), fval, fidx, P, ws.flag.as<int>(), ws.big.p,
+                             g_stream));
     }
     prof.end("factor", ev, 0, el_flops);
     }
@@ -2052,6 +2112,7 @@ static int* tuning_slot(const char* name) {
   if (s == "id_reg") return &g_tune.id_reg;
   if (s == "elim_mfma") return &g_tune.elim_mfma;
   if (s == "gram_mfma") return &g_tune.gram_mfma;
+  if (s == "big_n") return &g_tune.big_n;
   return nullptr;
 }
 

@@ -35,6 +35,34 @@ __device__ inline double sl3_entry(const PointsDev& P, double t0, double t1, dou
   return P.scale * sw / (kFourPi * sqrt(dx * dx + dy * dy + dz * dz));
 }
 
+// proxy point q of box B: the ring of ie_circle.m:35,107 / ie_ellipse.m:127,
+// or the caller's points scaled by the box size (ie_sphere.m:175)
+__device__ inline void proxy_point(const LevelParams& L, const LvlBox& B, int q, double& px,
+                                   double& py, double& pz) {
+  if (L.pkind == RSKX_PROXY_POINTS_SL) {
+    const int u = q % L.np;
+    px = L.ppts[3 * u] * B.l[0] + B.ctr[0];
+    py = L.ppts[3 * u + 1] * B.l[1] + B.ctr[1];
+    pz = L.ppts[3 * u + 2] * B.l[2] + B.ctr[2];
+  } else {
+    const double th = (double)(q + 1) * kTwoPi / (double)L.p;
+    px = L.radius * cos(th) * B.l[0] + B.ctr[0];
+    py = L.radius * sin(th) * B.l[1] + B.ctr[1];
+    pz = 0;
+  }
+}
+
+// RSKX_PROXY_POINTS_SL row q against source point (s0, s1, s2) of weight sw:
+// rows q < np are the outgoing interactions w_j G(pxy, x_j), rows q >= np
+// (symm 'n') the incoming ones, weighted by the problem's proxy weight
+// (ie_sphere.m:178-183)
+__device__ inline double proxy_sl3(const LevelParams& L, const LvlBox& B, int q, double px,
+                                   double py, double pz, double s0, double s1, double s2,
+                                   double sw) {
+  const double dx = px - s0, dy = py - s1, dz = pz - s2;
+  return (q < L.np ? sw : B.pw) / (kFourPi * sqrt(dx * dx + dy * dy + dz * dz));
+}
+
 // child block lookup for the modified diagonal M (rskelf.m:190-199)
 struct ChildMap {
   int nc;

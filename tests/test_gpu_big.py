@@ -1,0 +1,86 @@
+"""The whole-GPU path for the largest boxes against the per-workgroup kernels.
+
+Boxes with more than big_n points take the whole-GPU route: K assembled in
+FP64, G = K'K by a library product and a grid-wide pivoted Cholesky for the
+ID, then a dense LU / Cholesky with library triangular solves and products
+for the elimination (rskelf.m:218-246).  Lowering big_n sends most non-leaf
+boxes there.  The route changes only the order of the arithmetic: the
+per-level skeleton totals must agree with the default route within 2% (the
+pivoted Cholesky may break near-ties the other way), and the solves must
+meet the same accuracy against the FP64 operator as the default route's.
+The sphere case also takes its root box (no neighbours, everything
+redundant: a plain dense LU) through the library LU.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(flam_amd, tune, big_n, g, occ, tol, opts):
+    tune.set_tuning("big_n", big_n)
+    try:
+        Fa = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, opts)
+    finally:
+        tune.set_tuning("big_n", 1024)
+    Fb = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, opts)
+    return Fa, Fb
+
+
+def _tables_close(ta, tb):
+    assert len(ta) == len(tb)
+    for a, b in zip(ta, tb):
+        assert (a[0], a[1]) == (b[0], b[1]), (a, b)
+        assert abs(a[3] - b[3]) <= max(2, 0.02 * b[3]), (a, b)
+
+
+def _relres(flam_amd, g, X, B):
+    R = B - flam_amd.kernel_matvec(g.kernel, g.x, X)
+    return (torch.linalg.vector_norm(R, dim=0) / torch.linalg.vector_norm(B, dim=0)).max().item()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_big_path_sphere_lu(prec):
+    import flam_amd
+    from flam_amd import geometry, tune
+    g = geometry.sphere(5120)
+    print(f"big path: sphere nface=5120 occ=128 tol=1e-6 symm=n {prec} big_n=200", flush=True)
+    opts = dict(symm="n", precision=prec, id_route="gram")
+    Fa, Fb = _pair(flam_amd, tune, 200, g, 128, 1e-6, opts)
+    ta, tb = Fa.level_table(), Fb.level_table()
+    print("big:", ta, "\ndefault:", tb, flush=True)
+    _tables_close(ta, tb)
+    B = torch.from_numpy(np.random.default_rng(11).standard_normal((g.N, 3))).cuda()
+    ea = _relres(flam_amd, g, flam_amd.rskelf_sv(Fa, B), B)
+    eb = _relres(flam_amd, g, flam_amd.rskelf_sv(Fb, B), B)
+    print(f"relative residual: big {ea:.2e}, default {eb:.2e}", flush=True)
+    assert ea <= max(3 * eb, 1e-5 if prec == "fp32" else 1e-9)
+    # the transposed solve reads the same factor through the other sweeps
+    Xt = flam_amd.rskelf_sv(Fa, B, "t")
+    Xtb = flam_amd.rskelf_sv(Fb, B, "t")
+    dt = ((Xt - Xtb).abs().max() / Xtb.abs().max()).item()
+    print(f"transposed solves differ by {dt:.2e}", flush=True)
+    assert dt <= (1e-2 if prec == "fp32" else 1e-3)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_big_path_circle_cholesky(prec):
+    import flam_amd
+    from flam_amd import geometry, tune
+    N = 1 << 14
+    g = geometry.circle(N)
+    print(f"big path: circle N={N} occ=64 tol=1e-6 symm=p {prec} big_n=64", flush=True)
+    opts = dict(symm="p", precision=prec, id_route="gram")
+    Fa, Fb = _pair(flam_amd, tune, 64, g, 64, 1e-6, opts)
+    ta, tb = Fa.level_table(), Fb.level_table()
+    _tables_close(ta, tb)
+    B = torch.from_numpy(np.random.default_rng(12).standard_normal((N, 2))).cuda()
+    Xa, Xb = flam_amd.rskelf_sv(Fa, B), flam_amd.rskelf_sv(Fb, B)
+    ea, eb = _relres(flam_amd, g, Xa, B), _relres(flam_amd, g, Xb, B)
+    print(f"relative residual: big {ea:.2e}, default {eb:.2e}", flush=True)
+    assert ea <= max(3 * eb, 1e-5 if prec == "fp32" else 1e-9)
+    # the Cholesky factor's log-determinant agrees with the default route's
+    la, lb = flam_amd.rskelf_logdet(Fa), flam_amd.rskelf_logdet(Fb)
+    print(f"logdet: big {la}, default {lb}", flush=True)
+    assert abs(complex(la) - complex(lb)) <= (1e-3 if prec == "fp32" else 1e-6) * max(1.0, abs(complex(lb)))
