@@ -60,11 +60,13 @@ void launch_elim(bool, bool, int, size_t, const int*, int, const LvlBox*, const 
                  int*, const int*, const double*, const S*, S*, S*, S*, int*, const PointsDev&,
                  int*, hipStream_t);
 int elim_mfma_shape(int n, int k, int r, size_t es, size_t* shmem);
-size_t big_id_scratch(int n, int mb);
-int big_id(int b, int n, int mb, int kmax, int smax, double tol, const LvlBox* boxes,
-           const int* pool, const int* nbrar, const int* mnbr, int* mrows, double* Tar,
-           int* permar, int* kout, const PointsDev& P, const LevelParams& L, double* scratch,
-           hipStream_t st);
+int64_t gram_wide_stride(int nmax);
+template <typename S>
+int gram_wide_id(const int* ids, int nw, int nmax, const LvlBox* boxes, const int* mrows,
+                 const S* Kar, double* Tar, int* permar, int* kout, const IdParams& prm, double* wsp,
+                 hipStream_t st);
+int gram_wide_tsolve(const LvlBox* boxes, int b, const LvlBox& hbox, int k, double* Tar,
+                     hipStream_t st);
 size_t big_elim_scratch(int n, int k, int r, size_t es);
 template <typename S>
 int big_elim(bool lu, int b, int n, int k, int r, const LvlBox* boxes, const BoxOut* outs,
@@ -297,7 +299,8 @@ struct Workspace {
   DevBuf cgs;                   // Gram-Schmidt partial sums
   DevBuf pts, xw, txs, tord, tpart, tchunks, tcounts, xs, poff;
   DevBuf Ablk;                  // host-callback matrices: A(slf,slf) of the level's boxes
-  DevBuf big;                   // scratch of the whole-GPU path for the largest boxes
+  DevBuf big;                   // scratch of the wide-box ID and of the whole-GPU elimination
+  DevBuf wids;                  // the wide-box ID's tail lists
   int* hflag = nullptr;
 };
 static Workspace g_ws;
@@ -1249,18 +1252,17 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       RSKX_TRY(download(hmn.data(), ws.mnbr.p, nb * sizeof(int)));
       RSKX_TRY(sync_stream());
     }
-    std::vector<int> idg, idr[2], idq[kGramClasses], idall, idb;
+    std::vector<int> idg, idr[2], idq[kGramClasses], idall, idw;
     size_t shm_r[2] = {0, 0}, shm_q[kGramClasses] = {0, 0, 0};
-    int nmax_g = 0, mcapmax_g = 0;
+    int nmax_g = 0, mcapmax_g = 0, nmax_w = 0;
     // a box whose point data would not fit a class's shared memory goes to
     // the next route: Gram -> register QRCP -> QRCP on K in global memory;
-    // on the Gram route the largest boxes take the whole-GPU path instead
+    // on the Gram route, boxes beyond the register-resident Gram classes
+    // take the batched wide-box Gram ID (K in global memory, G by MFMA
+    // tiles, blocked pivoted Cholesky) while the tail's shared memory
+    // (3 n + 32 doubles) holds them
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
-      if (gram && n > g_tune.big_n && mb > 0 && hb[ib].ldt == n) {
-        idb.push_back((int)ib);
-        continue;
-      }
       if (gram && n > 0 && mb > 0 && gram_id_class(n) >= 0) {
         const int c = gram_class_tuned(n);
         const size_t by = gram_id_shmem(c, n, hmn[ib], mb, L.p);
@@ -1269,6 +1271,11 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
           shm_q[c] = std::max(shm_q[c], by);
           continue;
         }
+      }
+      if (gram && n > 0 && mb > 0 && (size_t)(3 * n + 32) * sizeof(double) <= 160 * 1024) {
+        idw.push_back((int)ib);
+        nmax_w = std::max(nmax_w, n);
+        continue;
       }
       int cls = -1;
       for (int c = 1; c >= 0 && cls < 0; --c) {
@@ -1306,22 +1313,28 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       RSKX_TRY(upload(ws.boxes, hb));
     }
     F->id_scratch = std::max<int64_t>(F->id_scratch, off_T * (int64_t)sizeof(double));
+    // QR boxes and wide Gram boxes first (both assemble K in global memory)
     idall = idg;
+    idall.insert(idall.end(), idw.begin(), idw.end());
     idall.insert(idall.end(), idr[0].begin(), idr[0].end());
     idall.insert(idall.end(), idr[1].begin(), idr[1].end());
     for (int c = kGramClasses - 1; c >= 0; --c) idall.insert(idall.end(), idq[c].begin(), idq[c].end());
-    idall.insert(idall.end(), idb.begin(), idb.end());    // after the Gram boxes: the same tail
     RSKX_TRY(upload(ws.ids, idall));
+    const int ng = (int)idg.size(), nwd = (int)idw.size(), nr0 = (int)idr[0].size(),
+              nr1 = (int)idr[1].size();
     const int* dids = ws.ids.as<int>();
-    const int ng = (int)idg.size(), nr0 = (int)idr[0].size(), nr1 = (int)idr[1].size();
+    const int* wids = dids + ng;
+    const int* rids = wids + nwd;          // register QR boxes
     int nq = 0;
     for (int c = 0; c < kGramClasses; ++c) nq += (int)idq[c].size();
     int64_t Kbytes = 0;
     for (int ib : idg) Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
+    for (int ib : idw) Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
     prof.begin("assemble", ev);
-    if (ng && !cb)                         // callbacks uploaded K already
+    if (ng + nwd && !cb)                   // callbacks uploaded K already
       launch_assemble<S>(dids, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
-                         ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng, g_stream);
+                         ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng + nwd,
+                         g_stream);
     prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
     prof.begin("id", ev);
     if (ng) {
@@ -1343,18 +1356,18 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     for (int c = 0; c < 2; ++c) {
       const int cnt = c == 0 ? nr0 : nr1;
       if (!cnt) continue;
-      launch_id_reg<S>(c, dids + ng + (c == 0 ? 0 : nr0), ws.boxes.as<LvlBox>(), ws.pool.as<int>(),
+      launch_id_reg<S>(c, rids + (c == 0 ? 0 : nr0), ws.boxes.as<LvlBox>(), ws.pool.as<int>(),
                        ws.nbr.as<int>(), ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(),
                        ws.perm.as<int>(), P, L, prm, ws.kout.as<int>(), cnt, shm_r[c], g_stream);
     }
     if (nr0 + nr1) {
       int nmx = 0;
       for (int c = 0; c < 2; ++c) for (int ib : idr[c]) nmx = std::max(nmx, hb[ib].n);
-      launch_id<S>(1, dids + ng, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
+      launch_id<S>(1, rids, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
                    ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
                    nr0 + nr1, (size_t)(3 * nmx + 32) * sizeof(double), nullptr, 0, g_stream);
     }
-    const int* qids = dids + ng + nr0 + nr1;
+    const int* qids = rids + nr0 + nr1;
     {
       // the Gram classes run concurrently (largest boxes on a side stream,
       // the smallest on the other, the middle class on g_stream); the
@@ -1375,37 +1388,58 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       }
       if (conc) RSKX_TRY(side_join());
     }
-    // the largest boxes, one at a time with the whole GPU: K, G = K'K, the
-    // pivoted Cholesky of G; their T and refinement run in the Gram tail
-    if (!idb.empty()) {
-      size_t need = 0;
-      for (int ib : idb) {
-        const int mb = hmn[ib] * (two ? 2 : 1) + L.p;
-        need = std::max(need, big_id_scratch(hb[ib].n, mb));
-      }
-      RSKX_TRY(ws.big.ensure(need * sizeof(double)));
-      for (int ib : idb) {
-        const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
-        int kmx = prm.kmax;
-        if (kmx <= 0 || kmx > n) kmx = n;
-        RSKX_TRY(big_id(ib, n, mb, kmx, std::min(mb, n), prm.tol, ws.boxes.as<LvlBox>(),
-                        ws.pool.as<int>(), ws.nbr.as<int>(), ws.mnbr.as<int>(), ws.mrows.as<int>(),
-                        ws.T.as<double>(), ws.perm.as<int>(), ws.kout.as<int>(), P, L,
-                        ws.big.as<double>(), g_stream));
+    // wide Gram boxes: G, the blocked pivoted Cholesky, R by position
+    if (nwd) {
+      RSKX_TRY(ws.big.ensure((size_t)nwd * gram_wide_stride(nmax_w) * sizeof(double)));
+      RSKX_TRY(gram_wide_id<S>(wids, nwd, nmax_w, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(),
+                               ws.K.as<S>(), ws.T.as<double>(), ws.perm.as<int>(),
+                               ws.kout.as<int>(), prm, ws.big.as<double>(), g_stream));
+    }
+    // T = R11 \ R12: the Gram tail for the register-class boxes and the
+    // wide boxes of modest rank; rocBLAS dtrsm, then the tail's refinement
+    // (MODE 3), for wide boxes of rank >= kGwTrsm (the host reads their
+    // ranks first)
+    constexpr int kGwTrsm = 256;
+    std::vector<int> wt2, wt3;
+    if (nwd) {
+      bool any = false;
+      for (int ib : idw) any = any || hb[ib].n > kGwTrsm;
+      if (any) {
+        std::vector<int> kw(nb);
+        RSKX_TRY(download(kw.data(), ws.kout.p, nb * sizeof(int)));
+        RSKX_TRY(sync_stream());
+        for (int ib : idw) (kw[ib] >= kGwTrsm && kw[ib] < hb[ib].n ? wt3 : wt2).push_back(ib);
+        for (int ib : wt3)
+          RSKX_TRY(gram_wide_tsolve(ws.boxes.as<LvlBox>(), ib, hb[ib], kw[ib], ws.T.as<double>(),
+                                    g_stream));
+      } else {
+        wt2 = idw;
       }
     }
-    const int nt2 = nq + (int)idb.size();
-    if (nt2) {
+    if (nq) {
       int nmx = 0;
       for (int c = 0; c < kGramClasses; ++c) for (int ib : idq[c]) nmx = std::max(nmx, hb[ib].n);
-      for (int ib : idb) nmx = std::max(nmx, hb[ib].n);
-      const size_t sht = (size_t)(3 * nmx + 32) * sizeof(double);
-      if (sht > 160 * 1024)
-        return fail(RSKX_ERR_UNSUP, "FLAM:rskx:size: box too large for the ID's refinement kernel (n=" +
-                                         std::to_string(nmx) + ")");
       launch_id<S>(2, qids, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
                    ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(), ws.iter.as<int>(),
-                   nt2, sht, nullptr, 0, g_stream);
+                   nq, (size_t)(3 * nmx + 32) * sizeof(double), nullptr, 0, g_stream);
+    }
+    if (!wt2.empty() || !wt3.empty()) {
+      std::vector<int> wl(wt2);
+      wl.insert(wl.end(), wt3.begin(), wt3.end());
+      RSKX_TRY(upload(ws.wids, wl));
+      int nm2 = 0, nm3 = 0;
+      for (int ib : wt2) nm2 = std::max(nm2, hb[ib].n);
+      for (int ib : wt3) nm3 = std::max(nm3, hb[ib].n);
+      if (!wt2.empty())
+        launch_id<S>(2, ws.wids.as<int>(), ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
+                     ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(),
+                     ws.iter.as<int>(), (int)wt2.size(), (size_t)(3 * nm2 + 32) * sizeof(double),
+                     nullptr, 0, g_stream);
+      if (!wt3.empty())
+        launch_id<S>(3, ws.wids.as<int>() + wt2.size(), ws.boxes.as<LvlBox>(), ws.mrows.as<int>(),
+                     ws.K.as<S>(), ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(),
+                     ws.iter.as<int>(), (int)wt3.size(), (size_t)(3 * nm3 + 32) * sizeof(double),
+                     nullptr, 0, g_stream);
     }
     prof.end("id", ev);
     }
@@ -1488,7 +1522,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       O.k = hk[ib];
       O.r = B.n - O.k;
       const int64_t k = O.k, r = O.r, m = hm[ib], n = B.n;
-      if (gram && n > 0 && m > 0 && (gram_id_class((int)n) >= 0 || n > g_tune.big_n))   // G = K'K, pivoted Cholesky, T
+      if (gram && n > 0 && m > 0)   // G = K'K, pivoted Cholesky, T
         id_flops += (double)m * n * (n + 1) + 2.0 * (double)n * n * k + (double)k * k * r;
       else                                                  // truncated QRCP, T (x4 complex)
         id_flops += (CPLX ? 4.0 : 1.0) * (4.0 * m * n * k - 2.0 * k * k * (m + n) +

@@ -1,24 +1,27 @@
-// Boxes too large for one workgroup: the few top-level boxes of 3D problems
-// (n in the thousands), each processed with the whole GPU.
+// Boxes too large for one workgroup's registers or shared memory.
 //
-// ID (tol >= 1e-7, where the Gram route is accurate): K = [A(nbr,slf);
-// A(slf,nbr)' (two-sided); Kpxy] (rskelf.m:201-212) assembled in FP64, one
-// workgroup per column; G = K'K by rocBLAS dsyrk; then the diagonally
-// pivoted Cholesky of G, stopped as id.m:105-111 stops the QR, computed
-// left-looking so G is never rewritten: step s picks the largest remaining
-// diagonal (one workgroup), then every column's R entry is formed as
-// (G(pc, j) - R(0:s, pc)' R(0:s, j)) / R(s, pc) by a grid-wide pass.  R's
-// leading rows by pivot position go where k_gram_id leaves them, and the
-// same tail (k_id<S, 2>: T = R11 \ R12 and the Gu-Eisenstat refinement)
-// finishes the ID.
+// ID (Gram route, tol >= 1e-7): boxes beyond the register-resident Gram
+// kernels, batched over a level.  K (rskelf.m:201-212: [A(nbr,slf);
+// A(slf,nbr)' (two-sided); Kpxy], stored as S) comes from k_assemble; then
+//   G = K'K in FP64 on 16 x 16 x 4 FP64 MFMA tiles (k_gw_gram): products of
+//     S values are exact in FP64, so G is the Gram matrix of the stored K;
+//   the diagonally pivoted Cholesky of G, blocked by 32 pivots: a panel
+//     kernel (one workgroup per box) picks each pivot and forms its row of
+//     R from G as updated through the previous panels minus the panel's
+//     earlier rows, then a trailing update G -= R_p' R_p (k_gw_update);
+//     stopped as id.m:105-111 stops the QR;
+//   k_gw_finish leaves R's leading rows by pivot position in the box's
+//     third T slice, the pivot order and the rank, as k_gram_id does.
+// T = R11 \ R12 (id.m:114) then comes from the Gram tail (k_id<S, 2>), or,
+// for the largest ranks, from rocBLAS dtrsm with k_id<S, 3> refining.
 //
-// Elimination (symm 'n', 's' (real) and 'p'): K = A(slf,slf) + M in the
-// order [rd; sk] assembled densely, the two sparsification products, LU
-// with partial pivoting (rocSOLVER getrf) or Cholesky (potrf), E and G by
-// triangular solves and the skeleton Schur update by a GEMM (rocBLAS),
-// exactly rskelf.m:218-246; the factor is then written in the layout of the
-// other elimination kernels (k_pack adds the transposed copies and the
-// inverted diagonal blocks).
+// Elimination (symm 'n', 's' (real) and 'p') of the largest boxes, one at a
+// time with the whole GPU: K = A(slf,slf) + M in the order [rd; sk]
+// assembled densely, the two sparsification products, LU with partial
+// pivoting (rocSOLVER getrf) or Cholesky (potrf), E and G by triangular
+// solves and the skeleton Schur update by a GEMM (rocBLAS), exactly
+// rskelf.m:218-246; the factor is then written in the layout of the other
+// elimination kernels (k_pack adds the transposed copies).
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -28,153 +31,268 @@
 #include "common.hpp"
 #include "devutil.hpp"
 #include "entries.hpp"
+#include "mfma.hpp"
 #include "rskx.h"
 
 namespace rskx {
 
-// ------------------------------------------------------------- the ID --
-// column j of K (mb rows, leading dimension mb; rows past the box's m are 0)
-__global__ void __launch_bounds__(256) k_big_assemble(int b, const LvlBox* __restrict__ boxes,
-                                                      const int* __restrict__ pool,
-                                                      const int* __restrict__ nbrar,
-                                                      const int* __restrict__ mnbr,
-                                                      int* __restrict__ mrows, double* __restrict__ K,
-                                                      int mb, PointsDev P, LevelParams L) {
+// ------------------------------------------------- the ID of wide boxes --
+// per-box state of the blocked pivoted Cholesky, followed in the box's
+// scratch by dg (n doubles: remaining diagonal of G) and p2c (n ints:
+// position -> column)
+struct GwState {
+  int k;           // rows of R formed
+  int stop;        // set once the rank is decided
+  int pad0, pad1;
+  double tol_abs;  // tol times the first pivot (id.m:105-111)
+  double pad2;
+};
+constexpr int kGwPanel = 32;   // pivots per panel
+
+// lower 64 x 64 tile t of an nt x nt tile triangle: t = ti (ti + 1) / 2 + tj
+__device__ inline void gw_tile(int t, int& ti, int& tj) {
+  ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (ti * (ti + 1) / 2 > t) --ti;
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  tj = t - ti * (ti + 1) / 2;
+}
+
+// G = K'K: workgroup (tile, box), four waves of 32 x 32.  MFMA D(j, i):
+// lane l holds i = i0 + (l & 15) and rows j = j0 + (l >> 4) + 4 v, so the
+// stores of G(i, j) (column-major, leading dimension n) are coalesced.  Per
+// 16 rows of K, lane l reads rows 4 (l >> 4) + [0, 4) of its columns and
+// MFMA c multiplies their c-th entries (both operands pair the same row).
+template <typename S>
+__global__ void __launch_bounds__(256) k_gw_gram(const int* __restrict__ ids,
+                                                 const LvlBox* __restrict__ boxes,
+                                                 const int* __restrict__ mrows,
+                                                 const S* __restrict__ Kar,
+                                                 double* __restrict__ Tar) {
+  const int b = ids[blockIdx.y];
   const LvlBox B = boxes[b];
-  const int j = blockIdx.x;
-  const int n = B.n, mn = mnbr[b];
-  const int mt = L.two_sided ? 2 * mn : mn;
-  const int m = mt + L.p;
-  if (j == 0 && threadIdx.x == 0) mrows[b] = m;
-  const int* slf = pool + B.slf_off;
-  const int* nbr = nbrar + B.nbr_off;
-  double xj, yj, nxj, nyj, wj;
-  point_fields(P, slf[j], xj, yj, nxj, nyj, wj);
-  const bool sl3 = P.kind == RSKX_KERNEL_LAPLACE3D_SL;
-  const double c0 = P.scale / kTwoPi;
-  double* Kj = K + (int64_t)j * mb;
-  for (int i = threadIdx.x; i < mb; i += blockDim.x) {
-    double v = 0;
-    if (i < mn) {                                           // A(nbr, slf)
-      double x, y, f2, f3, w;
-      point_fields(P, nbr[i], x, y, f2, f3, w);
-      if (sl3) {
-        v = sl3_entry(P, x, y, f2, xj, yj, nxj, wj);
-      } else {
-        const double dx = x - xj, dy = y - yj;
-        v = c0 * wj * (dx * nxj + dy * nyj) / (dx * dx + dy * dy);
+  const int n = B.n, nt = (n + 63) >> 6;
+  if ((int)blockIdx.x >= nt * (nt + 1) / 2) return;
+  int ti, tj;
+  gw_tile(blockIdx.x, ti, tj);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int i0 = 64 * ti + 32 * (wid >> 1), j0 = 64 * tj + 32 * (wid & 1);
+  const int m = mrows[b], ld = B.mcap, rl = 4 * (lane >> 4);
+  const S* K = Kar + B.K_off;
+  // columns past n read column n - 1; their results are never stored
+  const S* pi[2];
+  const S* pj[2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    pi[x] = K + (int64_t)min(i0 + 16 * x + (lane & 15), n - 1) * ld + rl;
+    pj[x] = K + (int64_t)min(j0 + 16 * x + (lane & 15), n - 1) * ld + rl;
+  }
+  mf_d4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = mf_d4{0, 0, 0, 0};
+  auto step = [&](int r0, bool tail) {
+    double ai[2][4], aj[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bool ok = !tail || r0 + rl + c < m;
+        ai[x][c] = ok ? (double)pi[x][r0 + c] : 0.0;
+        aj[x][c] = ok ? (double)pj[x][r0 + c] : 0.0;
       }
-    } else if (i < mt) {                                    // A(slf, nbr)'
-      double x, y, f2, f3, w;
-      point_fields(P, nbr[i - mn], x, y, f2, f3, w);
-      if (sl3) {
-        v = sl3_entry(P, xj, yj, nxj, x, y, f2, w);
-      } else {
-        const double dx = xj - x, dy = yj - y;
-        v = c0 * w * (dx * f2 + dy * f3) / (dx * dx + dy * dy);
-      }
-    } else if (i < m) {                                     // proxy rows
-      const int q = i - mt;
-      double px, py, pz;
-      proxy_point(L, B, q, px, py, pz);
-      if (L.pkind == RSKX_PROXY_POINTS_SL) {
-        v = proxy_sl3(L, B, q, px, py, pz, xj, yj, nxj, wj);
-      } else {
-        const double dx = px - xj, dy = py - yj;
-        const double r2 = dx * dx + dy * dy;
-        v = L.pkind == RSKX_PROXY_RING_SL ? -0.5 * log(r2) / kTwoPi * B.pw
-                                          : wj * (dx * nxj + dy * nyj) / (kTwoPi * r2);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) acc[y][x] = Mf<double>::mac(aj[y][c], ai[x][c], acc[y][x]);
+  };
+  int r0 = 0;
+  for (; r0 + 16 <= m; r0 += 16) step(r0, false);
+  if (r0 < m) step(r0, true);
+  double* G = Tar + B.T_off;
+#pragma unroll
+  for (int y = 0; y < 2; ++y)
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const int i = i0 + 16 * x + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = j0 + 16 * y + Mf<double>::row(lane, v);
+        if (i < n && j < n) G[i + (int64_t)j * n] = acc[y][x][v];
       }
     }
-    Kj[i] = v;
-  }
 }
 
-// state of the pivoted Cholesky on the device
-struct BigPiv {
-  int k;           // rows of R formed so far
-  int stop;        // set when the ID's rank is reached
-  int pc;          // pivot column of the row being formed
-  int pad;
-  double nrm;      // its pivot (the remaining column norm)
-  double tol_abs;  // tol times the first pivot (id.m:105-111)
-};
-
-__global__ void k_big_init(BigPiv* st, const double* __restrict__ G, double* __restrict__ dg,
-                           int* __restrict__ p2c, int n) {
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    dg[j] = G[j + (int64_t)j * n];
-    p2c[j] = j;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->k = 0;
-    st->stop = 0;
-  }
-}
-
-// pivot of step s = st->k: the largest remaining diagonal over positions
-// s.. (ties to the smaller position), with the stopping tests
-__global__ void __launch_bounds__(1024) k_big_pivot(BigPiv* st, const double* __restrict__ dg,
-                                                     int* __restrict__ p2c, int n, int smax,
-                                                     int kmax, double tol) {
+// Panel p of the pivoted Cholesky: pivots 32 p .. 32 p + 31 of one box per
+// workgroup.  G holds the lower triangle updated through panel p - 1; R is
+// row-major (row t at Rrow + t n) in the second T slice.
+__global__ void __launch_bounds__(1024) k_gw_panel(const int* __restrict__ ids,
+                                                   const LvlBox* __restrict__ boxes,
+                                                   const int* __restrict__ mrows,
+                                                   double* __restrict__ Tar, double* __restrict__ wsp,
+                                                   int64_t wstride, IdParams prm, int p) {
+  extern __shared__ double sh[];
   __shared__ double red[2 * 16];
-  if (st->stop) return;
-  const int s = st->k;
-  if (s >= smax || s >= kmax) {
-    if (threadIdx.x == 0) st->stop = 1;
-    return;
-  }
-  double bv = -1.0;
-  int bi = n;
-  for (int pos = s + threadIdx.x; pos < n; pos += blockDim.x) argmax_merge(bv, bi, dg[p2c[pos]], pos);
-  block_argmax(bv, bi, red);
-  if (threadIdx.x == 0) {
-    if (bi >= n) { st->stop = 1; return; }
-    const double nrm = sqrt(fmax(bv, 0.0));
-    if (s == 0) st->tol_abs = nrm * tol;
-    if (!(nrm > st->tol_abs)) { st->stop = 1; return; }
-    const int pc = p2c[bi];
-    p2c[bi] = p2c[s];
-    p2c[s] = pc;
-    st->pc = pc;
-    st->nrm = nrm;
-    st->k = s + 1;
-  }
-}
-
-// row s = k - 1 of R for every column: (G(pc, j) - sum_{t<s} R(t, pc) R(t, j)) / nrm,
-// R row-major (row t at Rrow + t n); G holds its lower triangle
-__global__ void __launch_bounds__(256) k_big_row(const BigPiv* st, const double* __restrict__ G,
-                                                 int n, double* __restrict__ Rrow,
-                                                 double* __restrict__ dg, int last_k) {
-  if (st->k != last_k + 1) return;           // this step formed no row
-  const int s = st->k - 1, pc = st->pc;
-  const double inv = 1.0 / st->nrm;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  double g = j <= pc ? G[pc + (int64_t)j * n] : G[j + (int64_t)pc * n];
-  for (int t = 0; t < s; ++t) g -= Rrow[(int64_t)t * n + pc] * Rrow[(int64_t)t * n + j];
-  const double r = g * inv;
-  Rrow[(int64_t)s * n + j] = r;
-  dg[j] -= r * r;
-}
-
-// R's leading k rows by pivot position into the box's third T slice (as
-// k_gram_id leaves them), the pivot order and the rank
-__global__ void k_big_finish(int b, const LvlBox* __restrict__ boxes, const BigPiv* st,
-                             const double* __restrict__ Rrow, const int* __restrict__ p2c,
-                             double* __restrict__ Tar, int* __restrict__ permar,
-                             int* __restrict__ kout) {
+  const int w = blockIdx.x, b = ids[w];
   const LvlBox B = boxes[b];
-  const int n = B.n, ldt = B.ldt, k = st->k;
-  double* Rp = Tar + B.T_off + 2 * (int64_t)ldt * n;
-  int* perm = permar + B.perm_off;
-  for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += gridDim.x * blockDim.x) {
-    const int c = p2c[pos];
-    perm[pos] = c;
-    for (int i = 0; i < k; ++i) Rp[i + (int64_t)pos * ldt] = Rrow[(int64_t)i * n + c];
+  const int n = B.n, m = mrows[b], tid = threadIdx.x, nt = blockDim.x;
+  GwState* st = reinterpret_cast<GwState*>(wsp + w * wstride);
+  double* dgg = wsp + w * wstride + sizeof(GwState) / sizeof(double);
+  int* p2cg = reinterpret_cast<int*>(dgg + n);
+  if (p > 0 && st->stop) return;
+  const double* G = Tar + B.T_off;
+  double* Rrow = Tar + B.T_off + (int64_t)n * n;
+  double* dg = sh;
+  int* p2c = reinterpret_cast<int*>(sh + n);
+  int kmax = prm.kmax;
+  if (kmax <= 0 || kmax > n) kmax = n;
+  const int lim = min(min(m, n), kmax);
+  double tol_abs = 0;
+  if (p == 0) {
+    for (int j = tid; j < n; j += nt) { dg[j] = G[j + (int64_t)j * n]; p2c[j] = j; }
+  } else {
+    for (int j = tid; j < n; j += nt) { dg[j] = dgg[j]; p2c[j] = p2cg[j]; }
+    tol_abs = st->tol_abs;
   }
+  __syncthreads();
+  const int s0 = kGwPanel * p, s1 = min(s0 + kGwPanel, lim);
+  int k = s0 < lim ? s0 : lim, stop = s1 >= lim ? 1 : 0;
+  for (int s = s0; s < s1; ++s) {
+    double bv = -1.0;
+    int bi = n;
+    for (int pos = s + tid; pos < n; pos += nt) argmax_merge(bv, bi, dg[p2c[pos]], pos);
+    block_argmax(bv, bi, red);
+    if (bi >= n) { stop = 1; break; }                 // no finite candidate
+    const int pc = p2c[bi];
+    const double nrm = sqrt(fmax(bv, 0.0));
+    if (s == 0) tol_abs = nrm * prm.tol;
+    if (!(nrm > tol_abs)) { stop = 1; break; }
+    k = s + 1;
+    __syncthreads();                                   // p2c[bi] read by every thread
+    if (tid == 0) { p2c[bi] = p2c[s]; p2c[s] = pc; }
+    const double rinv = 1.0 / nrm;
+    const double* Rpc = Rrow + pc;
+    for (int j = tid; j < n; j += nt) {
+      double g = j <= pc ? G[pc + (int64_t)j * n] : G[j + (int64_t)pc * n];
+      for (int t = s0; t < s; ++t) g -= Rpc[(int64_t)t * n] * Rrow[(int64_t)t * n + j];
+      const double r = g * rinv;
+      Rrow[(int64_t)s * n + j] = r;
+      dg[j] -= r * r;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += nt) { dgg[j] = dg[j]; p2cg[j] = p2c[j]; }
+  if (tid == 0) {
+    st->k = k;
+    st->stop = stop;
+    st->tol_abs = tol_abs;
+  }
+}
+
+// G -= R_p' R_p over the lower 64 x 64 tiles after a complete panel p
+// (nothing once the rank is decided: no later panel reads G)
+__global__ void __launch_bounds__(256) k_gw_update(const int* __restrict__ ids,
+                                                   const LvlBox* __restrict__ boxes,
+                                                   double* __restrict__ Tar,
+                                                   const double* __restrict__ wsp, int64_t wstride,
+                                                   int p) {
+  const int w = blockIdx.y;
+  const GwState* st = reinterpret_cast<const GwState*>(wsp + w * wstride);
+  if (st->stop) return;
+  const LvlBox B = boxes[ids[w]];
+  const int n = B.n, nt = (n + 63) >> 6;
+  if ((int)blockIdx.x >= nt * (nt + 1) / 2) return;
+  int ti, tj;
+  gw_tile(blockIdx.x, ti, tj);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int i0 = 64 * ti + 32 * (wid >> 1), j0 = 64 * tj + 32 * (wid & 1);
+  double* G = Tar + B.T_off;
+  const double* R = Tar + B.T_off + (int64_t)n * n + (int64_t)kGwPanel * p * n;
+  const int rl = 4 * (lane >> 4);
+  mf_d4 acc[2][2];
+#pragma unroll
+  for (int y = 0; y < 2; ++y)
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const int i = i0 + 16 * x + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = j0 + 16 * y + Mf<double>::row(lane, v);
+        acc[y][x][v] = (i < n && j < n) ? G[i + (int64_t)j * n] : 0.0;
+      }
+    }
+  int ci[2], cj[2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    ci[x] = min(i0 + 16 * x + (lane & 15), n - 1);
+    cj[x] = min(j0 + 16 * x + (lane & 15), n - 1);
+  }
+#pragma unroll
+  for (int r0 = 0; r0 < kGwPanel; r0 += 16) {
+    double ai[2][4], aj[2][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double* Rt = R + (int64_t)(r0 + rl + c) * n;
+#pragma unroll
+      for (int x = 0; x < 2; ++x) { ai[x][c] = Rt[ci[x]]; aj[x][c] = -Rt[cj[x]]; }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) acc[y][x] = Mf<double>::mac(aj[y][c], ai[x][c], acc[y][x]);
+  }
+#pragma unroll
+  for (int y = 0; y < 2; ++y)
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const int i = i0 + 16 * x + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = j0 + 16 * y + Mf<double>::row(lane, v);
+        if (i < n && j < n) G[i + (int64_t)j * n] = acc[y][x][v];
+      }
+    }
+}
+
+// R's leading k rows by pivot position into the third T slice (leading
+// dimension n), the pivot order and the rank; one wave per position
+__global__ void __launch_bounds__(256) k_gw_finish(const int* __restrict__ ids,
+                                                   const LvlBox* __restrict__ boxes,
+                                                   double* __restrict__ Tar,
+                                                   const double* __restrict__ wsp, int64_t wstride,
+                                                   int* __restrict__ permar, int* __restrict__ kout) {
+  const int w = blockIdx.y, b = ids[w];
+  const LvlBox B = boxes[b];
+  const int n = B.n;
+  const int pos = 4 * blockIdx.x + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const GwState* st = reinterpret_cast<const GwState*>(wsp + w * wstride);
+  const int* p2c = reinterpret_cast<const int*>(wsp + w * wstride + sizeof(GwState) / sizeof(double) + n);
+  const int k = st->k;
   if (blockIdx.x == 0 && threadIdx.x == 0) kout[b] = k;
+  if (pos >= n) return;
+  const int c = p2c[pos];
+  if (lane == 0) permar[B.perm_off + pos] = c;
+  const double* Rrow = Tar + B.T_off + (int64_t)n * n;
+  double* Rp = Tar + B.T_off + 2 * (int64_t)n * n + (int64_t)pos * n;
+  for (int i = lane; i < k; i += 64) Rp[i] = Rrow[(int64_t)i * n + c];
+}
+
+// T = R11 \ R12 (k x r, leading dimension n) starts as a copy of R12
+__global__ void k_gw_tcopy(const LvlBox* __restrict__ boxes, int b, int k, double* __restrict__ Tar) {
+  const LvlBox B = boxes[b];
+  const int n = B.n, r = n - k;
+  double* T = Tar + B.T_off;
+  const double* X = Tar + B.T_off + 2 * (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)k * r;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % k), c = (int)(e / k);
+    T[i + (int64_t)c * n] = X[i + (int64_t)(k + c) * n];
+  }
 }
 
 // ------------------------------------------------------ the elimination --
@@ -398,50 +516,60 @@ static rocblas_handle big_handle(hipStream_t st) {
     }                                                                                     \
   } while (0)
 
-// device scratch of the big-box path: sizes in elements
-size_t big_id_scratch(int n, int mb) {
-  return (size_t)mb * n + (size_t)n * n + (size_t)n * n + 2 * (size_t)n + 64;   // K, G, R rows, dg, p2c, state
-}
+// scratch of the wide-box ID per box (doubles): state, dg, p2c
+int64_t gram_wide_stride(int nmax) { return (int64_t)(sizeof(GwState) / sizeof(double)) + nmax + (nmax + 1) / 2 + 2; }
 
-int big_id(int b, int n, int mb, int kmax, int smax, double tol, const LvlBox* boxes,
-           const int* pool, const int* nbrar, const int* mnbr, int* mrows, double* Tar,
-           int* permar, int* kout, const PointsDev& P, const LevelParams& L, double* scratch,
-           hipStream_t st) {
-  double* K = scratch;
-  double* G = K + (size_t)mb * n;
-  double* Rrow = G + (size_t)n * n;
-  double* dg = Rrow + (size_t)n * n;
-  int* p2c = reinterpret_cast<int*>(dg + n);
-  BigPiv* state = reinterpret_cast<BigPiv*>(dg + 2 * (size_t)n);
-  rocblas_handle h = big_handle(st);
-  if (!h) return (set_error("FLAM:rskx:blas: rocblas_create_handle failed"), RSKX_ERR_HIP);
-  hipLaunchKernelGGL(k_big_assemble, dim3(n), dim3(256), 0, st, b, boxes, pool, nbrar, mnbr, mrows,
-                     K, mb, P, L);
-  const double one = 1.0, zero = 0.0;
-  RSKX_BLAS(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, n, mb, &one, K, mb,
-                          &zero, G, n));
-  hipLaunchKernelGGL(k_big_init, dim3((n + 255) / 256), dim3(256), 0, st, state, G, dg, p2c, n);
-  int steps = smax < kmax ? smax : kmax;
-  if (steps > n) steps = n;
-  int done = 0, hstop = 0, hk = 0;
-  // steps in chunks, the host reading the stop flag after each chunk
-  while (done < steps && !hstop) {
-    const int chunk = steps - done < 32 ? steps - done : 32;
-    for (int c = 0; c < chunk; ++c) {
-      hipLaunchKernelGGL(k_big_pivot, dim3(1), dim3(1024), 0, st, state, dg, p2c, n, smax, kmax, tol);
-      hipLaunchKernelGGL(k_big_row, dim3((n + 255) / 256), dim3(256), 0, st, state, G, n, Rrow, dg,
-                         done + c);
-    }
-    done += chunk;
-    RSKX_HIP(hipMemcpyAsync(&hstop, &state->stop, sizeof(int), hipMemcpyDeviceToHost, st));
-    RSKX_HIP(hipMemcpyAsync(&hk, &state->k, sizeof(int), hipMemcpyDeviceToHost, st));
-    RSKX_HIP(hipStreamSynchronize(st));
+int gram_wide_rounds(int nmax) { return (nmax + kGwPanel - 1) / kGwPanel; }
+
+// G, the blocked pivoted Cholesky and R by position for the nw boxes of
+// ids (device list), K assembled; no host synchronization
+template <typename S>
+int gram_wide_id(const int* ids, int nw, int nmax, const LvlBox* boxes, const int* mrows,
+                 const S* Kar, double* Tar, int* permar, int* kout, const IdParams& prm, double* wsp,
+                 hipStream_t st) {
+  const int nt = (nmax + 63) / 64, ntile = nt * (nt + 1) / 2;
+  const int64_t wstride = gram_wide_stride(nmax);
+  hipLaunchKernelGGL(k_gw_gram<S>, dim3(ntile, nw), dim3(256), 0, st, ids, boxes, mrows, Kar, Tar);
+  const size_t shm = (size_t)nmax * sizeof(double) + (size_t)nmax * sizeof(int) + 16;
+  ensure_dyn_lds((const void*)k_gw_panel, shm);
+  const int rounds = gram_wide_rounds(prm.kmax > 0 ? std::min(nmax, prm.kmax) : nmax);
+  for (int p = 0; p < rounds; ++p) {
+    hipLaunchKernelGGL(k_gw_panel, dim3(nw), dim3(1024), shm, st, ids, boxes, mrows, Tar, wsp,
+                       wstride, prm, p);
+    if (p + 1 < rounds)
+      hipLaunchKernelGGL(k_gw_update, dim3(ntile, nw), dim3(256), 0, st, ids, boxes, Tar, wsp,
+                         wstride, p);
   }
-  hipLaunchKernelGGL(k_big_finish, dim3((n + 255) / 256), dim3(256), 0, st, b, boxes, state, Rrow,
-                     p2c, Tar, permar, kout);
+  hipLaunchKernelGGL(k_gw_finish, dim3((nmax + 3) / 4, nw), dim3(256), 0, st, ids, boxes, Tar, wsp,
+                     wstride, permar, kout);
   RSKX_HIP(hipGetLastError());
   return RSKX_OK;
 }
+
+// T = R11 \ R12 for one wide box of rank k with the whole GPU (FP64)
+int gram_wide_tsolve(const LvlBox* boxes, int b, const LvlBox& hbox, int k, double* Tar,
+                     hipStream_t st) {
+  const int n = hbox.n, r = n - k;
+  if (k <= 0 || r <= 0) return RSKX_OK;
+  rocblas_handle h = big_handle(st);
+  if (!h) return (set_error("FLAM:rskx:blas: rocblas_create_handle failed"), RSKX_ERR_HIP);
+  const int64_t e = (int64_t)k * r;
+  const int grid = (int)std::min<int64_t>((e + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_gw_tcopy, dim3(grid), dim3(256), 0, st, boxes, b, k, Tar);
+  double* T = Tar + hbox.T_off;
+  const double* X = Tar + hbox.T_off + 2 * (int64_t)n * n;
+  const double one = 1.0;
+  RSKX_BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, k, r, &one, X, n, T, n));
+  RSKX_HIP(hipGetLastError());
+  return RSKX_OK;
+}
+
+template int gram_wide_id<float>(const int*, int, int, const LvlBox*, const int*, const float*,
+                                 double*, int*, int*, const IdParams&, double*, hipStream_t);
+template int gram_wide_id<double>(const int*, int, int, const LvlBox*, const int*, const double*,
+                                  double*, int*, int*, const IdParams&, double*, hipStream_t);
+
 
 size_t big_elim_scratch(int n, int k, int r, size_t es) {
   return ((size_t)n * n + (size_t)k * r + 64) * es + (size_t)(r + 16) * sizeof(int);

@@ -233,7 +233,9 @@ __device__ void swap_skel(int* perm, double* T, int ldt, int r, double* R, int l
 // rows of R are already there by pivot position, perm holds the pivot order
 // and kout the rank, and only T and the refinement remain; R is in K
 // (MODE 1, from k_id_reg) or, in FP64 with leading dimension n, in the
-// third n x n slice of the box's T scratch (MODE 2, from k_gram_id).
+// third n x n slice of the box's T scratch (MODE 2, from k_gram_id; MODE 3
+// as MODE 2 with T = R11 \ R12 already in the first slice, formed by a
+// library triangular solve for the largest boxes).
 template <typename S, int MODE>
 __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
                                             const LvlBox* __restrict__ boxes,
@@ -363,7 +365,7 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
   double* X = R + (int64_t)ldt * n;
   // leading rows of R by position
   auto rv = [&](int i, int j) -> double {
-    return MODE == 2 ? X[i + (int64_t)j * ldt] : (double)K[i + (int64_t)j * ld];
+    return MODE >= 2 ? X[i + (int64_t)j * ldt] : (double)K[i + (int64_t)j * ld];
   };
   if (!QR) {
     k = kout[b];
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
     tol_abs = k > 0 ? fabs(rv(0, 0)) * prm.tol : 0.0;
   }
   const int r0 = n - k;
-  for (int c = tid; c < r0; c += nt) {
+  for (int c = tid; c < (MODE == 3 ? 0 : r0); c += nt) {   // MODE 3: T is already formed
     for (int l = k - 1; l >= 0; --l) {
       double s = rv(l, k + c);
       for (int q = l + 1; q < k; ++q) s -= rv(l, q) * T[q + (int64_t)c * ldt];
@@ -1732,7 +1734,7 @@ template <typename S>
 void launch_id(int mode, const int* ids, const LvlBox* boxes, const int* mrows, S* Kar,
                int* permar, double* Tar, const IdParams& prm, int* kout, int* iterout, int nb,
                size_t shmem, double* vglob, int64_t vstride, hipStream_t st) {
-  auto kern = mode == 0 ? k_id<S, 0> : mode == 1 ? k_id<S, 1> : k_id<S, 2>;
+  auto kern = mode == 0 ? k_id<S, 0> : mode == 1 ? k_id<S, 1> : mode == 2 ? k_id<S, 2> : k_id<S, 3>;
   ensure_dyn_lds((const void*)kern, shmem);
   hipLaunchKernelGGL(kern, dim3(nb), dim3(256), shmem, st, ids, boxes, mrows, Kar, permar, Tar, prm,
                      kout, iterout, vglob, vstride);

@@ -1,21 +1,55 @@
-"""The whole-GPU path for the largest boxes against the per-workgroup kernels.
+r"""Large boxes: the wide-box Gram ID and the whole-GPU elimination.
 
-Boxes with more than big_n points take the whole-GPU route: K assembled in
-FP64, G = K'K by a library product and a grid-wide pivoted Cholesky for the
-ID, then a dense LU / Cholesky with library triangular solves and products
-for the elimination (rskelf.m:218-246).  Lowering big_n sends most non-leaf
-boxes there.  The route changes only the order of the arithmetic: the
-per-level skeleton totals must agree with the default route within 2% (the
-pivoted Cholesky may break near-ties the other way), and the solves must
-meet the same accuracy against the FP64 operator as the default route's.
-The sphere case also takes its root box (no neighbours, everything
-redundant: a plain dense LU) through the library LU.
+Gram-route boxes beyond the register-resident Gram classes (n > 284) take
+the batched wide-box ID: K in global memory, G = K'K on FP64 matrix-core
+tiles, a pivoted Cholesky blocked by 32 pivots, T = R11 \ R12 by the Gram
+tail or (rank >= 256) by a library triangular solve.  The first test pins
+it against the oracle's QR ID on the sphere, whose upper levels hold such
+boxes.
+
+Boxes with more than big_n points are eliminated with the whole GPU: a
+dense LU / Cholesky (rocSOLVER) with library triangular solves and products
+(rskelf.m:218-246).  Lowering big_n sends most non-leaf boxes there.  The
+route changes only the order of the arithmetic: the per-level skeleton
+totals must agree with the default route within 2% (the ranks of the next
+level depend on the rounding of M), and the solves must meet the same
+accuracy against the FP64 operator as the default route's.  The sphere case
+also takes its root box (no neighbours, everything redundant: a plain dense
+LU) through the library LU.
 """
 import numpy as np
 import pytest
 import torch
 
 pytestmark = pytest.mark.gpu
+
+
+def test_wide_gram_id_sphere_vs_oracle():
+    import flam_amd
+    from flam_amd import geometry
+    from oracle import flam as ofl
+    from oracle import problems as opr
+    g = geometry.sphere(5120)
+    print("wide Gram ID: sphere nface=5120 occ=128 tol=1e-6 symm=n fp64 id_route=gram", flush=True)
+    F = flam_amd.rskelf(g.kernel, g.x, 128, 1e-6, g.proxy, dict(symm="n", id_route="gram"))
+    got = F.level_table()
+    print("levels:", got, flush=True)
+    # the top levels' boxes are beyond the register Gram classes
+    assert max(r[2] / r[1] for r in got if r[3] > 0) > 284
+    pr = opr.SphereSL.from_arrays(g.x.cpu().numpy(), g.w.cpu().numpy(), g.diag.cpu().numpy(),
+                                  g.proxy.pts.cpu().numpy(), symm="n")
+    Fo = ofl.rskelf(pr.afun, pr.x, 128, 1e-6, pr.pxyfun, dict(symm="n", order="level"))
+    ref = ofl.rank_table(Fo)
+    _tables_close(got, ref)
+    A = pr.dense()
+    b = np.random.default_rng(10).standard_normal(g.N)
+    xe = np.linalg.solve(A, b)
+    x = flam_amd.rskelf_sv(F, torch.from_numpy(b).cuda()).cpu().numpy()
+    xo = ofl.rskelf_sv(Fo, b)
+    err = np.linalg.norm(x - xe) / np.linalg.norm(xe)
+    err_o = np.linalg.norm(xo - xe) / np.linalg.norm(xe)
+    print(f"solve error: device {err:.2e}, oracle {err_o:.2e}", flush=True)
+    assert err <= max(10 * err_o, 1e-12)
 
 
 def _pair(flam_amd, tune, big_n, g, occ, tol, opts):
