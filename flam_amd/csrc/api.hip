@@ -31,7 +31,7 @@ void launch_lists(const LvlBox*, const Seg*, const Seg*, int*, int*, int*, const
                   const LevelParams&, int, hipStream_t);
 template <typename S>
 void launch_assemble(const int*, const LvlBox*, const int*, const int*, const int*, int*, S*,
-                     const PointsDev&, const LevelParams&, int, hipStream_t);
+                     const PointsDev&, const LevelParams&, int, int, hipStream_t);
 template <typename S>
 void launch_id(int, const int*, const LvlBox*, const int*, S*, int*, double*, const IdParams&, int*,
                int*, int, size_t, double*, int64_t, hipStream_t);
@@ -62,9 +62,11 @@ void launch_elim(bool, bool, int, size_t, const int*, int, const LvlBox*, const 
 int elim_mfma_shape(int n, int k, int r, size_t es, size_t* shmem);
 int64_t gram_wide_stride(int nmax);
 template <typename S>
-int gram_wide_id(const int* ids, int nw, int nmax, const LvlBox* boxes, const int* mrows,
-                 const S* Kar, double* Tar, int* permar, int* kout, const IdParams& prm, double* wsp,
-                 hipStream_t st);
+void gram_wide_gram(const int* ids, int nw, int nmax, const LvlBox* boxes, const int* mrows,
+                    const S* Kar, double* Tar, hipStream_t st);
+int gram_wide_chol(const int* ids, int nw, int nmax, const LvlBox* boxes, const int* mrows,
+                   double* Tar, int* permar, int* kout, const IdParams& prm, double* wsp,
+                   hipStream_t st);
 int gram_wide_tsolve(const LvlBox* boxes, int b, const LvlBox& hbox, int k, double* Tar,
                      hipStream_t st);
 size_t big_elim_scratch(int n, int k, int r, size_t es);
@@ -84,7 +86,7 @@ void launch_sweep(int, bool, const FacRec*, int, const S*, const int*, S*, int64
 template <typename U, typename S>
 void launch_perm(bool, void*, int64_t, S*, int64_t, int, int, const int*, hipStream_t);
 template <typename S>
-void launch_pack(bool, const FacRec*, int, S*, hipStream_t);
+void launch_pack(bool, const FacRec*, int, int, S*, hipStream_t);
 template <typename S>
 void launch_solve(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int, int,
                   size_t, S*, int64_t, hipStream_t);
@@ -301,9 +303,16 @@ struct Workspace {
   DevBuf Ablk;                  // host-callback matrices: A(slf,slf) of the level's boxes
   DevBuf big;                   // scratch of the wide-box ID and of the whole-GPU elimination
   DevBuf wids;                  // the wide-box ID's tail lists
+  DevBuf wv;                    // the wide-box tails' vectors when shared memory is too small
   int* hflag = nullptr;
 };
 static Workspace g_ws;
+
+// the wide-box Gram ID takes boxes up to kWideNmax points (G and the three
+// T slices are n^2 doubles each); their K is assembled in groups of at most
+// kWideKBytes
+constexpr int kWideNmax = 32768;
+constexpr int64_t kWideKBytes = int64_t(32) << 30;
 
 #define RSKX_TRY(expr) do { int rc_ = (expr); if (rc_ != RSKX_OK) return rc_; } while (0)
 
@@ -1112,7 +1121,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     hb.assign(nb, LvlBox{});
     hn.clear();
     hc.clear();
-    int64_t off_slf = 0, off_nbr = 0, off_K = 0, off_perm = 0, off_T = 0, off_W = 0;
+    int64_t off_slf = 0, off_nbr = 0, off_K = 0, off_perm = 0, off_T = 0;
     int nmax = 0, mcapmax = 0, mcmax = 0;
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int b = (int)(b0 + ib);
@@ -1153,9 +1162,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       B.perm_off = off_perm;           off_perm += n;
       B.ldt = n;
       B.T_off = off_T;                 off_T += (CPLX ? 6 : 3) * (int64_t)n * n;
-      // elimination scratch of the global class: 2 n^2 elements, then a
-      // 6 n double point cache (12 n elements keep the next box 8-byte aligned)
-      B.W_off = off_W;                 off_W += 2 * (int64_t)n * n + 12 * (int64_t)n;
+      B.W_off = 0;                     // laid out by the elimination routing
       for (int i = 0; i < d; ++i) {
         B.ctr[i] = t.ctr[(size_t)d * b + i];
         B.l[i] = t.bsize[(size_t)d * b + i];
@@ -1174,12 +1181,13 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     RSKX_TRY(ws.mrows.ensure(nb * sizeof(int)));
     RSKX_TRY(ws.kout.ensure(nb * sizeof(int)));
     RSKX_TRY(ws.iter.ensure(nb * sizeof(int)));
-    RSKX_TRY(ws.K.ensure(std::max<int64_t>(off_K, 1) * sizeof(S)));
+    // real factorizations without callbacks lay K out after the ID routing,
+    // for the boxes that use it
+    if (CPLX || cb) RSKX_TRY(ws.K.ensure(std::max<int64_t>(off_K, 1) * sizeof(S)));
     RSKX_TRY(ws.perm.ensure(std::max<int64_t>(off_perm, 1) * sizeof(int)));
     // with a rank model the real ID's buffers are laid out after routing
     const bool sized_by_model = !kcap.empty() && !CPLX;
     if (!sized_by_model) RSKX_TRY(ws.T.ensure(std::max<int64_t>(off_T, 1) * sizeof(double)));
-    RSKX_TRY(ws.W.ensure(std::max<int64_t>(off_W, 1) * sizeof(S)));
     hlap("h:segments");
 
     hipEvent_t ev;
@@ -1272,7 +1280,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
           continue;
         }
       }
-      if (gram && n > 0 && mb > 0 && (size_t)(3 * n + 32) * sizeof(double) <= 160 * 1024) {
+      if (gram && n > 0 && mb > 0 && n <= kWideNmax) {
         idw.push_back((int)ib);
         nmax_w = std::max(nmax_w, n);
         continue;
@@ -1292,10 +1300,35 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         shm_r[cls] = std::max(shm_r[cls], by);
       } else {
         idg.push_back((int)ib);
-        nmax_g = std::max(nmax_g, n);
-        mcapmax_g = std::max(mcapmax_g, hb[ib].mcap);
+        if (mb > 0 || cb) {              // without rows (the root) k_id returns rank 0 at once
+          nmax_g = std::max(nmax_g, n);
+          mcapmax_g = std::max(mcapmax_g, hb[ib].mcap);
+        }
       }
     }
+    // K for the boxes that use it (QR, register QR, wide Gram); the wide
+    // boxes' K in groups of at most kWideKBytes, each assembled once the
+    // previous group's G is formed
+    std::vector<int> wgrp{0};
+    if (!cb) {
+      int64_t oK = 0;
+      auto place = [&](int ib) { hb[ib].K_off = oK; oK += (int64_t)hb[ib].mcap * hb[ib].n; };
+      for (int ib : idg) place(ib);
+      for (int c = 0; c < 2; ++c) for (int ib : idr[c]) place(ib);
+      const int64_t kbase = oK, cap = kWideKBytes / (int64_t)sizeof(S);
+      int64_t acc = 0, gmax = 0;
+      for (size_t i = 0; i < idw.size(); ++i) {
+        const int ib = idw[i];
+        const int64_t sz = (int64_t)hb[ib].mcap * hb[ib].n;
+        if (acc > 0 && acc + sz > cap) { wgrp.push_back((int)i); acc = 0; }
+        hb[ib].K_off = kbase + acc;
+        acc += sz;
+        gmax = std::max(gmax, acc);
+      }
+      RSKX_TRY(ws.K.ensure(std::max<int64_t>(kbase + gmax, 1) * sizeof(S)));
+      RSKX_TRY(upload(ws.boxes, hb));
+    }
+    wgrp.push_back((int)idw.size());
     if (sized_by_model) {
       // the Gram ID keeps only its rank cap of R rows: size every Gram
       // box's R / T buffers (3 ldt n doubles) by the predicted cap
@@ -1328,13 +1361,30 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     int nq = 0;
     for (int c = 0; c < kGramClasses; ++c) nq += (int)idq[c].size();
     int64_t Kbytes = 0;
-    for (int ib : idg) Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
+    int nmax_a = 0;
+    for (int ib : idg) {
+      Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
+      nmax_a = std::max(nmax_a, hb[ib].n);
+    }
     for (int ib : idw) Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
     prof.begin("assemble", ev);
-    if (ng + nwd && !cb)                   // callbacks uploaded K already
+    if (ng && !cb)                         // callbacks uploaded K already
       launch_assemble<S>(dids, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
-                         ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng + nwd,
+                         ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng, nmax_a,
                          g_stream);
+    // wide boxes: K by groups, each group's G = K'K before the next group's
+    // K overwrites it
+    for (size_t g = 0; g + 1 < wgrp.size(); ++g) {
+      const int a = wgrp[g], e = wgrp[g + 1];
+      if (e == a) continue;
+      int nm = 0;
+      for (int i = a; i < e; ++i) nm = std::max(nm, hb[idw[i]].n);
+      launch_assemble<S>(wids + a, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
+                         ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, e - a, nm,
+                         g_stream);
+      gram_wide_gram<S>(wids + a, e - a, nm, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(),
+                        ws.K.as<S>(), ws.T.as<double>(), g_stream);
+    }
     prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
     prof.begin("id", ev);
     if (ng) {
@@ -1388,12 +1438,12 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       }
       if (conc) RSKX_TRY(side_join());
     }
-    // wide Gram boxes: G, the blocked pivoted Cholesky, R by position
+    // wide Gram boxes: the blocked pivoted Cholesky of G, R by position
     if (nwd) {
       RSKX_TRY(ws.big.ensure((size_t)nwd * gram_wide_stride(nmax_w) * sizeof(double)));
-      RSKX_TRY(gram_wide_id<S>(wids, nwd, nmax_w, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(),
-                               ws.K.as<S>(), ws.T.as<double>(), ws.perm.as<int>(),
-                               ws.kout.as<int>(), prm, ws.big.as<double>(), g_stream));
+      RSKX_TRY(gram_wide_chol(wids, nwd, nmax_w, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(),
+                              ws.T.as<double>(), ws.perm.as<int>(), ws.kout.as<int>(), prm,
+                              ws.big.as<double>(), g_stream));
     }
     // T = R11 \ R12: the Gram tail for the register-class boxes and the
     // wide boxes of modest rank; rocBLAS dtrsm, then the tail's refinement
@@ -1427,19 +1477,27 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       std::vector<int> wl(wt2);
       wl.insert(wl.end(), wt3.begin(), wt3.end());
       RSKX_TRY(upload(ws.wids, wl));
-      int nm2 = 0, nm3 = 0;
-      for (int ib : wt2) nm2 = std::max(nm2, hb[ib].n);
-      for (int ib : wt3) nm3 = std::max(nm3, hb[ib].n);
-      if (!wt2.empty())
-        launch_id<S>(2, ws.wids.as<int>(), ws.boxes.as<LvlBox>(), ws.mrows.as<int>(), ws.K.as<S>(),
-                     ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(),
-                     ws.iter.as<int>(), (int)wt2.size(), (size_t)(3 * nm2 + 32) * sizeof(double),
-                     nullptr, 0, g_stream);
-      if (!wt3.empty())
-        launch_id<S>(3, ws.wids.as<int>() + wt2.size(), ws.boxes.as<LvlBox>(), ws.mrows.as<int>(),
-                     ws.K.as<S>(), ws.perm.as<int>(), ws.T.as<double>(), prm, ws.kout.as<int>(),
-                     ws.iter.as<int>(), (int)wt3.size(), (size_t)(3 * nm3 + 32) * sizeof(double),
-                     nullptr, 0, g_stream);
+      // the tails' three n-vectors in shared memory, or, for boxes too wide
+      // for it, in global memory (32 doubles of shared reduction scratch)
+      for (int mode = 2; mode <= 3; ++mode) {
+        const std::vector<int>& wt = mode == 2 ? wt2 : wt3;
+        if (wt.empty()) continue;
+        int nm = 0;
+        for (int ib : wt) nm = std::max(nm, hb[ib].n);
+        // MODE 3's refinement keeps six n-vectors in global memory always
+        const int nv = mode == 3 ? 6 : 3;
+        size_t shm = (size_t)(3 * nm + 32) * sizeof(double);
+        double* vg = nullptr;
+        if (mode == 3 || shm > 160 * 1024) {
+          RSKX_TRY(ws.wv.ensure((size_t)wt.size() * nv * nm * sizeof(double)));
+          vg = ws.wv.as<double>();
+          shm = 32 * sizeof(double);
+        }
+        launch_id<S>(mode, ws.wids.as<int>() + (mode == 2 ? 0 : wt2.size()), ws.boxes.as<LvlBox>(),
+                     ws.mrows.as<int>(), ws.K.as<S>(), ws.perm.as<int>(), ws.T.as<double>(), prm,
+                     ws.kout.as<int>(), ws.iter.as<int>(), (int)wt.size(), shm, vg,
+                     (int64_t)nv * nm, g_stream);
+      }
     }
     prof.end("id", ev);
     }
@@ -1601,6 +1659,15 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         if (O.r > 0 && by <= 158 * 1024) { zc[0].push_back((int)ib); shz = std::max(shz, by); }
         else zc[1].push_back((int)ib);
       }
+      if (!zc[1].empty()) {                // global scratch for these boxes only
+        int64_t oW = 0;
+        for (int ib : zc[1]) {
+          hb[ib].W_off = oW;
+          oW += 2 * (int64_t)hb[ib].n * hb[ib].n + 12 * (int64_t)hb[ib].n;
+        }
+        RSKX_TRY(ws.W.ensure(oW * sizeof(S)));
+        RSKX_TRY(upload(ws.boxes, hb));
+      }
       std::vector<int> zall(zc[0]);
       zall.insert(zall.end(), zc[1].begin(), zc[1].end());
       RSKX_TRY(upload(ws.ids, zall));
@@ -1674,6 +1741,17 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       int c = (O.r > 0 && by <= kHalf) ? 0 : (O.r > 0 && by <= kFull) ? 1 : 2;
       cls[c].push_back((int)ib);
       if (c < 2) shm_cls[c] = std::max(shm_cls[c], by);   // class 2: all in global scratch
+    }
+    // global-scratch class: 2 n^2 elements of W, then a 6 n double point
+    // cache (12 n elements keep the next box 8-byte aligned)
+    if (!cls[2].empty()) {
+      int64_t oW = 0;
+      for (int ib : cls[2]) {
+        hb[ib].W_off = oW;
+        oW += 2 * (int64_t)hb[ib].n * hb[ib].n + 12 * (int64_t)hb[ib].n;
+      }
+      RSKX_TRY(ws.W.ensure(oW * sizeof(S)));
+      RSKX_TRY(upload(ws.boxes, hb));
     }
     std::vector<int> allids;
     for (int c = kMfmaShapes - 1; c >= 0; --c) allids.insert(allids.end(), cmf[c].begin(), cmf[c].end());
@@ -1782,7 +1860,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
                                              (lu ? 2 : 1) * nbk * kDiagBlock * kDiagBlock);
         }
         prof.begin("pack", ev);
-        launch_pack<S>(lu, ws.lrecs.as<FacRec>(), (int)nl, fval, g_stream);
+        launch_pack<S>(lu, ws.lrecs.as<FacRec>(), (int)nl, rmx, fval, g_stream);
         prof.end("pack", ev, bytes);
       }
     }
@@ -2772,7 +2850,9 @@ static int import_impl(int64_t N, char symm, int nlvl, const int64_t* lvp,
       DevBuf recs;
       RSKX_TRY(recs.ensure((f1 - f0) * sizeof(FacRec)));
       RSKX_HIP(hipMemcpy(recs.p, F->recs.data() + f0, (f1 - f0) * sizeof(FacRec), hipMemcpyHostToDevice));
-      launch_pack<S>(lu, recs.as<FacRec>(), (int)(f1 - f0), F->lval.back().as<S>(), g_stream);
+      int rmx = 0;
+      for (int64_t f = f0; f < f1; ++f) rmx = std::max(rmx, F->recs[f].r);
+      launch_pack<S>(lu, recs.as<FacRec>(), (int)(f1 - f0), rmx, F->lval.back().as<S>(), g_stream);
       RSKX_HIP(hipGetLastError());
       RSKX_TRY(sync_stream());
       recs.release();

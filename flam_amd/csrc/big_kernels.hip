@@ -128,6 +128,7 @@ __global__ void __launch_bounds__(256) k_gw_gram(const int* __restrict__ ids,
 // Panel p of the pivoted Cholesky: pivots 32 p .. 32 p + 31 of one box per
 // workgroup.  G holds the lower triangle updated through panel p - 1; R is
 // row-major (row t at Rrow + t n) in the second T slice.
+template <bool SH>
 __global__ void __launch_bounds__(1024) k_gw_panel(const int* __restrict__ ids,
                                                    const LvlBox* __restrict__ boxes,
                                                    const int* __restrict__ mrows,
@@ -144,8 +145,9 @@ __global__ void __launch_bounds__(1024) k_gw_panel(const int* __restrict__ ids,
   if (p > 0 && st->stop) return;
   const double* G = Tar + B.T_off;
   double* Rrow = Tar + B.T_off + (int64_t)n * n;
-  double* dg = sh;
-  int* p2c = reinterpret_cast<int*>(sh + n);
+  // dg and p2c in shared memory, or (boxes too wide for it) in place
+  double* dg = SH ? sh : dgg;
+  int* p2c = SH ? reinterpret_cast<int*>(sh + n) : p2cg;
   int kmax = prm.kmax;
   if (kmax <= 0 || kmax > n) kmax = n;
   const int lim = min(min(m, n), kmax);
@@ -153,7 +155,8 @@ __global__ void __launch_bounds__(1024) k_gw_panel(const int* __restrict__ ids,
   if (p == 0) {
     for (int j = tid; j < n; j += nt) { dg[j] = G[j + (int64_t)j * n]; p2c[j] = j; }
   } else {
-    for (int j = tid; j < n; j += nt) { dg[j] = dgg[j]; p2c[j] = p2cg[j]; }
+    if (SH)
+      for (int j = tid; j < n; j += nt) { dg[j] = dgg[j]; p2c[j] = p2cg[j]; }
     tol_abs = st->tol_abs;
   }
   __syncthreads();
@@ -184,7 +187,8 @@ __global__ void __launch_bounds__(1024) k_gw_panel(const int* __restrict__ ids,
     __syncthreads();
   }
   __syncthreads();
-  for (int j = tid; j < n; j += nt) { dgg[j] = dg[j]; p2cg[j] = p2c[j]; }
+  if (SH)
+    for (int j = tid; j < n; j += nt) { dgg[j] = dg[j]; p2cg[j] = p2c[j]; }
   if (tid == 0) {
     st->k = k;
     st->stop = stop;
@@ -521,21 +525,32 @@ int64_t gram_wide_stride(int nmax) { return (int64_t)(sizeof(GwState) / sizeof(d
 
 int gram_wide_rounds(int nmax) { return (nmax + kGwPanel - 1) / kGwPanel; }
 
-// G, the blocked pivoted Cholesky and R by position for the nw boxes of
-// ids (device list), K assembled; no host synchronization
+// G = K'K for the nw boxes of ids (device list), K assembled
 template <typename S>
-int gram_wide_id(const int* ids, int nw, int nmax, const LvlBox* boxes, const int* mrows,
-                 const S* Kar, double* Tar, int* permar, int* kout, const IdParams& prm, double* wsp,
-                 hipStream_t st) {
+void gram_wide_gram(const int* ids, int nw, int nmax, const LvlBox* boxes, const int* mrows,
+                    const S* Kar, double* Tar, hipStream_t st) {
+  const int nt = (nmax + 63) / 64, ntile = nt * (nt + 1) / 2;
+  hipLaunchKernelGGL(k_gw_gram<S>, dim3(ntile, nw), dim3(256), 0, st, ids, boxes, mrows, Kar, Tar);
+}
+
+// the blocked pivoted Cholesky of every box's G and R by position; no host
+// synchronization
+int gram_wide_chol(const int* ids, int nw, int nmax, const LvlBox* boxes, const int* mrows,
+                   double* Tar, int* permar, int* kout, const IdParams& prm, double* wsp,
+                   hipStream_t st) {
   const int nt = (nmax + 63) / 64, ntile = nt * (nt + 1) / 2;
   const int64_t wstride = gram_wide_stride(nmax);
-  hipLaunchKernelGGL(k_gw_gram<S>, dim3(ntile, nw), dim3(256), 0, st, ids, boxes, mrows, Kar, Tar);
   const size_t shm = (size_t)nmax * sizeof(double) + (size_t)nmax * sizeof(int) + 16;
-  ensure_dyn_lds((const void*)k_gw_panel, shm);
+  const bool sh = shm <= 160 * 1024;
+  if (sh) ensure_dyn_lds((const void*)k_gw_panel<true>, shm);
   const int rounds = gram_wide_rounds(prm.kmax > 0 ? std::min(nmax, prm.kmax) : nmax);
   for (int p = 0; p < rounds; ++p) {
-    hipLaunchKernelGGL(k_gw_panel, dim3(nw), dim3(1024), shm, st, ids, boxes, mrows, Tar, wsp,
-                       wstride, prm, p);
+    if (sh)
+      hipLaunchKernelGGL(k_gw_panel<true>, dim3(nw), dim3(1024), shm, st, ids, boxes, mrows, Tar,
+                         wsp, wstride, prm, p);
+    else
+      hipLaunchKernelGGL(k_gw_panel<false>, dim3(nw), dim3(1024), 0, st, ids, boxes, mrows, Tar,
+                         wsp, wstride, prm, p);
     if (p + 1 < rounds)
       hipLaunchKernelGGL(k_gw_update, dim3(ntile, nw), dim3(256), 0, st, ids, boxes, Tar, wsp,
                          wstride, p);
@@ -565,10 +580,10 @@ int gram_wide_tsolve(const LvlBox* boxes, int b, const LvlBox& hbox, int k, doub
   return RSKX_OK;
 }
 
-template int gram_wide_id<float>(const int*, int, int, const LvlBox*, const int*, const float*,
-                                 double*, int*, int*, const IdParams&, double*, hipStream_t);
-template int gram_wide_id<double>(const int*, int, int, const LvlBox*, const int*, const double*,
-                                  double*, int*, int*, const IdParams&, double*, hipStream_t);
+template void gram_wide_gram<float>(const int*, int, int, const LvlBox*, const int*, const float*,
+                                    double*, hipStream_t);
+template void gram_wide_gram<double>(const int*, int, int, const LvlBox*, const int*,
+                                     const double*, double*, hipStream_t);
 
 
 size_t big_elim_scratch(int n, int k, int r, size_t es) {

@@ -84,10 +84,11 @@ __global__ void __launch_bounds__(256) k_lists(const LvlBox* __restrict__ boxes,
 }
 
 // ----------------------------------------------------------- k_assemble --
-// K of one box in global memory, for the boxes no register ID class takes.
-// The neighbour (row) fields are staged in LDS a chunk of kAsmRows at a
-// time; a column's own fields are the same for every thread and are read
-// directly.  No limit on the box or neighbourhood size.
+// K of the boxes no register ID class takes, in global memory; gridDim.y
+// workgroups share a box's columns.  The neighbour (row) fields are staged
+// in LDS a chunk of kAsmRows at a time; a column's own fields are the same
+// for every thread and are read directly.  No limit on the box or
+// neighbourhood size.
 constexpr int kAsmRows = 1024;
 
 template <typename S>
@@ -104,7 +105,10 @@ __global__ void __launch_bounds__(256) k_assemble(const int* __restrict__ ids,
   const int n = B.n, mn = mnbr[bidx];
   const int mt = L.two_sided ? 2 * mn : mn;
   const int m = mt + L.p;
-  if (threadIdx.x == 0) mrows[bidx] = m;
+  // workgroup (box, y) assembles columns y, y + gridDim.y, ...
+  const int j0 = blockIdx.y, js = gridDim.y;
+  if (j0 >= n) return;
+  if (j0 == 0 && threadIdx.x == 0) mrows[bidx] = m;
   const int* slf = pool + B.slf_off;
   const int* nbr = nbrar + B.nbr_off;
   S* K = Kar + B.K_off;
@@ -117,7 +121,7 @@ __global__ void __launch_bounds__(256) k_assemble(const int* __restrict__ ids,
     for (int i = threadIdx.x; i < rc; i += blockDim.x)
       point_fields(P, nbr[r0 + i], rf[0][i], rf[1][i], rf[2][i], rf[3][i], rf[4][i]);
     __syncthreads();
-    for (int j = 0; j < n; ++j) {
+    for (int j = j0; j < n; j += js) {
       double xj, yj, nxj, nyj, wj;
       point_fields(P, slf[j], xj, yj, nxj, nyj, wj);
       S* Kj = K + (int64_t)j * ld + r0;
@@ -144,7 +148,7 @@ __global__ void __launch_bounds__(256) k_assemble(const int* __restrict__ ids,
         }
     }
   }
-  for (int j = 0; j < n; ++j) {                              // proxy rows
+  for (int j = j0; j < n; j += js) {                         // proxy rows
     double xj, yj, nxj, nyj, wj;
     point_fields(P, slf[j], xj, yj, nxj, nyj, wj);
     S* Kj = K + (int64_t)j * ld;
@@ -228,14 +232,231 @@ __device__ void swap_skel(int* perm, double* T, int ldt, int r, double* R, int l
   __syncthreads();
 }
 
-// Shared memory: vn1[n], vn2[n], v[max(m, n)], red[32].
+// ---- refinement for large ranks (the tails of the widest boxes) --------
+// Exchange columns i and k-1 of the upper-triangular k x k R and make it
+// triangular again: what qrupdate returns in id.m:139-141, up to row signs
+// (which change neither T nor the inverse row norms).  Rotations on rows
+// (rr-1, rr), rr = k-1 .. i+1, clear column i below its diagonal (their
+// parameters depend on column i alone, so one thread forms them and every
+// other column applies the whole chain on its own); the subdiagonal they
+// leave in columns i+1 .. k-2 is cleared top-down, one rotation per step.
+__device__ void col_exchange(double* R, int k, int ldr, int i, double* cs, double* sn) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ double rot[2];
+  for (int l = tid; l < k; l += nt) {
+    const double a = R[l + (int64_t)i * ldr];
+    R[l + (int64_t)i * ldr] = R[l + (int64_t)(k - 1) * ldr];
+    R[l + (int64_t)(k - 1) * ldr] = a;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double b = R[(k - 1) + (int64_t)i * ldr];
+    for (int rr = k - 1; rr > i; --rr) {
+      const double a = R[rr - 1 + (int64_t)i * ldr];
+      const double rho = hypot(a, b);
+      cs[rr] = rho > 0 ? a / rho : 1.0;
+      sn[rr] = rho > 0 ? b / rho : 0.0;
+      b = rho;
+    }
+    R[i + (int64_t)i * ldr] = b;
+  }
+  __syncthreads();
+  for (int rr = i + 1 + tid; rr < k; rr += nt) R[rr + (int64_t)i * ldr] = 0.0;
+  for (int col = i + 1 + tid; col < k; col += nt) {
+    double* Rc = R + (int64_t)col * ldr;
+    const int start = col + 1 < k - 1 ? col + 1 : k - 1;
+    double y = Rc[start];
+    for (int rr = start; rr > i; --rr) {
+      const double x = Rc[rr - 1], c = cs[rr], s = sn[rr];
+      Rc[rr] = -s * x + c * y;
+      y = c * x + s * y;
+    }
+    Rc[i] = y;
+  }
+  __syncthreads();
+  // top-down: clear R(c+1, c), c = i+1 .. k-2; the owner of column c+1
+  // forms the next rotation once it has applied the current one
+  if (tid == 0 && i + 1 < k - 1) {
+    const double a = R[(i + 1) + (int64_t)(i + 1) * ldr], b = R[(i + 2) + (int64_t)(i + 1) * ldr];
+    const double rho = hypot(a, b);
+    rot[0] = rho > 0 ? a / rho : 1.0;
+    rot[1] = rho > 0 ? b / rho : 0.0;
+  }
+  __syncthreads();
+  for (int c0 = i + 1; c0 < k - 1; ++c0) {
+    const double c = rot[0], s = rot[1];
+    __syncthreads();                     // every thread holds this step's rotation
+    for (int col = c0 + tid; col < k; col += nt) {
+      double* Rc = R + (int64_t)col * ldr;
+      const double x = Rc[c0], y = Rc[c0 + 1];
+      Rc[c0] = c * x + s * y;
+      Rc[c0 + 1] = col == c0 ? 0.0 : -s * x + c * y;
+      if (col == c0 + 1 && c0 + 1 < k - 1) {
+        const double a = Rc[c0 + 1], b = Rc[c0 + 2];
+        const double rho = hypot(a, b);
+        rot[0] = rho > 0 ? a / rho : 1.0;
+        rot[1] = rho > 0 ? b / rho : 0.0;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// x = R(0:kk, 0:kk) \ b (upper triangular, column-oriented: one barrier per
+// column); w is work space, b may alias neither
+__device__ void trsv_upper(const double* R, int kk, int ldr, const double* b, double* w, double* x) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int l = tid; l < kk; l += nt) w[l] = b[l];
+  __syncthreads();
+  for (int q = kk - 1; q >= 0; --q) {
+    const double xq = w[q] / R[q + (int64_t)q * ldr];
+    if (tid == 0) x[q] = xq;
+    for (int l = tid; l < q; l += nt) w[l] -= R[l + (int64_t)q * ldr] * xq;
+    __syncthreads();
+  }
+}
+
+// Gu-Eisenstat refinement with R11 in R (upper, leading dimension ldt) and
+// T = R11 \ R12 (k x r), in O(k^2 + k r) per exchange like id.m:116-236
+// with c2 = 0: the inverse row norms r2 are formed once (when the first
+// test fails) and then updated by triangular solves (id.m:151-152,
+// 191-192, 218); vb holds six n-vectors
+__device__ void refine_large(int& k, int n, double* T, double* R, double* X, int ldt, int* perm,
+                             const IdParams& prm, double tol_abs, double* vb, double* red,
+                             int& iter, bool& conv) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double* r2 = vb;
+  double* u = vb + n;
+  double* w = vb + 2 * n;
+  double* x = vb + 3 * n;
+  double* cs = vb + 4 * n;
+  double* sn = vb + 5 * n;
+  const double f2 = prm.Tmax * prm.Tmax;
+  const double cap = fmin(prm.rrqr_iter, 4.0 * n + 64.0);
+  bool have_r2 = false;
+  conv = false;
+  while ((double)iter < cap) {
+    const int r = n - k;
+    double bv = -1.0;
+    int bi = k * r;
+    for (int e = tid; e < k * r; e += nt) {
+      const int i = e % k, c = e / k;
+      const double a = T[i + (int64_t)c * ldt];
+      argmax_merge(bv, bi, a * a, e);
+    }
+    block_argmax(bv, bi, red);
+    if (bv <= f2) { conv = true; break; }
+    if (!have_r2) {                       // id.m:124: r2 = sum(inv(R11).^2, 2)
+      inv_row_norms(R, k, ldt, X, ldt, r2);
+      have_r2 = true;
+    }
+    ++iter;
+    const int i = bi % k, j = bi / k;
+    if (i < k - 1) {                      // id.m:136-143
+      col_exchange(R, k, ldt, i, cs, sn);
+      for (int c = tid; c < r; c += nt) {
+        const double a = T[i + (int64_t)c * ldt];
+        T[i + (int64_t)c * ldt] = T[k - 1 + (int64_t)c * ldt];
+        T[k - 1 + (int64_t)c * ldt] = a;
+      }
+      if (tid == 0) {
+        double a = r2[i]; r2[i] = r2[k - 1]; r2[k - 1] = a;
+        int q = perm[i]; perm[i] = perm[k - 1]; perm[k - 1] = q;
+      }
+      __syncthreads();                    // rows i, k-1 of T before its columns move
+    }
+    if (j > 0) {                          // id.m:146-150
+      for (int l = tid; l < k; l += nt) {
+        const double a = T[l]; T[l] = T[l + (int64_t)j * ldt]; T[l + (int64_t)j * ldt] = a;
+      }
+      if (tid == 0) { int q = perm[k]; perm[k] = perm[k + j]; perm[k + j] = q; }
+    }
+    __syncthreads();
+    // downdate r2 (id.m:152-153)
+    const double* Rl = R + (int64_t)(k - 1) * ldt;
+    trsv_upper(R, k - 1, ldt, Rl, w, x);
+    const double dk = R[k - 1 + (int64_t)(k - 1) * ldt];
+    for (int l = tid; l < k - 1; l += nt) r2[l] -= (x[l] / dk) * (x[l] / dk);
+    for (int l = tid; l < k; l += nt) u[l] = T[l];
+    __syncthreads();
+    // the new last skeleton column of R11 is R11 u (id.m:168-170, 207-208)
+    for (int l = tid; l < k; l += nt) {
+      double acc = 0;
+      for (int q = l; q < k; ++q) acc += R[l + (int64_t)q * ldt] * u[q];
+      w[l] = acc;
+    }
+    __syncthreads();
+    for (int l = tid; l < k; l += nt) R[l + (int64_t)(k - 1) * ldt] = w[l];
+    if (tid == 0) { int q = perm[k - 1]; perm[k - 1] = perm[k]; perm[k] = q; }
+    // Sherman-Morrison (id.m:212-213): column 0 of the half-updated T is
+    // e_{k-1}; rho_c = T(k-1, c) / u(k-1)
+    const double g = 1.0 / u[k - 1];
+    for (int c = tid; c < r; c += nt) cs[c] = c == 0 ? g : T[k - 1 + (int64_t)c * ldt] * g;
+    __syncthreads();
+    for (int e = tid; e < k * r; e += nt) {
+      const int l = e % k, c = e / k;
+      double* Tc = T + (int64_t)c * ldt;
+      if (l == k - 1) Tc[l] = cs[c];
+      else Tc[l] = (c == 0 ? 0.0 : Tc[l]) - cs[c] * u[l];
+    }
+    __syncthreads();
+    // update r2 with the new R11 (id.m:208-209)
+    trsv_upper(R, k - 1, ldt, Rl, w, x);
+    const double dn = R[k - 1 + (int64_t)(k - 1) * ldt];
+    for (int l = tid; l < k - 1; l += nt) r2[l] += (x[l] / dn) * (x[l] / dn);
+    if (tid == 0) r2[k - 1] = 1.0 / (dn * dn);
+    __syncthreads();
+    // drop a skeleton within tol of the others' span (id.m:215-231)
+    double best = -1.0;
+    int ib = k;
+    for (int l = tid; l < k; l += nt) argmax_merge(best, ib, r2[l], l);
+    block_argmax(best, ib, red);
+    if (1.0 / sqrt(best) > tol_abs) continue;
+    if (ib < k - 1) {
+      col_exchange(R, k, ldt, ib, cs, sn);
+      for (int c = tid; c < r; c += nt) {
+        const double a = T[ib + (int64_t)c * ldt];
+        T[ib + (int64_t)c * ldt] = T[k - 1 + (int64_t)c * ldt];
+        T[k - 1 + (int64_t)c * ldt] = a;
+      }
+      if (tid == 0) {
+        double a = r2[ib]; r2[ib] = r2[k - 1]; r2[k - 1] = a;
+        int q = perm[ib]; perm[ib] = perm[k - 1]; perm[k - 1] = q;
+      }
+      __syncthreads();
+    }
+    // skeleton k-1 becomes the first redundant column (id.m:226-230)
+    const int k1 = k - 1;
+    trsv_upper(R, k1, ldt, R + (int64_t)k1 * ldt, w, x);
+    const double d1 = R[k1 + (int64_t)k1 * ldt];
+    for (int l = tid; l < k1; l += nt) r2[l] -= (x[l] / d1) * (x[l] / d1);
+    for (int e = tid; e < k1 * r; e += nt) {          // shift columns right by one
+      const int l = e % k1, c = e / k1;
+      X[l + (int64_t)(c + 1) * ldt] = T[l + (int64_t)c * ldt] + x[l] * T[k1 + (int64_t)c * ldt];
+    }
+    for (int l = tid; l < k1; l += nt) X[l] = x[l];
+    __syncthreads();
+    for (int e = tid; e < k1 * (r + 1); e += nt) {
+      const int l = e % k1, c = e / k1;
+      T[l + (int64_t)c * ldt] = X[l + (int64_t)c * ldt];
+    }
+    __syncthreads();
+    k = k1;
+    if (k == 0) break;
+  }
+}
+
+// Shared memory: vn1[n], vn2[n], v[max(m, n)], red[32]; with vglob, v (the
+// QR) or vn1, vn2 and v (the tails) in global memory.
 // MODE 0: run the column-pivoted QR on K first.  Otherwise the leading k
 // rows of R are already there by pivot position, perm holds the pivot order
 // and kout the rank, and only T and the refinement remain; R is in K
 // (MODE 1, from k_id_reg) or, in FP64 with leading dimension n, in the
 // third n x n slice of the box's T scratch (MODE 2, from k_gram_id; MODE 3
 // as MODE 2 with T = R11 \ R12 already in the first slice, formed by a
-// library triangular solve for the largest boxes).
+// library triangular solve for the largest ranks, and the refinement done
+// by refine_large in O(k^2 + k r) per exchange, six n-vectors per box in
+// vglob).
 template <typename S, int MODE>
 __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
                                             const LvlBox* __restrict__ boxes,
@@ -256,10 +477,14 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
   // shared: norms (2n), then the QR's Householder vector (m, or in this
   // workgroup's slice of vglob when m rows do not fit) or the tails' k-vector
   // (at most n), then the reduction scratch
-  double* vn1 = sh;
-  double* vn2 = sh + n;
-  double* v = QR && vglob ? vglob + blockIdx.x * vstride : sh + 2 * n;
-  double* red = QR && !vglob ? sh + 2 * n + (m > n ? m : n) : sh + 3 * n;
+  // the tails (MODE >= 1) of boxes too wide for shared memory keep their
+  // three n-vectors in this workgroup's slice of vglob instead
+  const bool gv = !QR && vglob != nullptr;
+  double* vb = gv ? vglob + blockIdx.x * vstride : sh;
+  double* vn1 = vb;
+  double* vn2 = vb + n;
+  double* v = QR && vglob ? vglob + blockIdx.x * vstride : vb + 2 * n;
+  double* red = gv ? sh : QR && !vglob ? sh + 2 * n + (m > n ? m : n) : sh + 3 * n;
   __shared__ int s_piv;
 
   if (QR)
@@ -385,7 +610,16 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
   }
   int iter = 0;
   bool conv = true;
-  if (isfinite(prm.Tmax) && prm.rrqr_iter > 0 && k > 0 && k < n) {
+  if (MODE == 3 && isfinite(prm.Tmax) && prm.rrqr_iter > 0 && k > 0 && k < n) {
+    __syncthreads();
+    for (int e = tid; e < k * k; e += nt) {
+      int i = e % k, j = e / k;
+      R[i + (int64_t)j * ldt] = i <= j ? rv(i, j) : 0.0;
+    }
+    __syncthreads();
+    refine_large(k, n, T, R, X, ldt, perm, prm, tol_abs, vglob + blockIdx.x * vstride, red, iter,
+                 conv);
+  } else if (isfinite(prm.Tmax) && prm.rrqr_iter > 0 && k > 0 && k < n) {
     // Gu-Eisenstat refinement in the fixed-precision form the reference
     // takes when R has no trailing block (id.m:116-236, c2 = 0).
     __syncthreads();
@@ -1725,8 +1959,11 @@ void launch_lists(const LvlBox* boxes, const Seg* nsegs, const Seg* csegs, int* 
 template <typename S>
 void launch_assemble(const int* ids, const LvlBox* boxes, const int* pool, const int* nbrar,
                      const int* mnbr, int* mrows, S* Kar, const PointsDev& P, const LevelParams& L,
-                     int nb, hipStream_t st) {
-  hipLaunchKernelGGL(k_assemble<S>, dim3(nb), dim3(256), 0, st, ids, boxes, pool, nbrar, mnbr,
+                     int nb, int nmax, hipStream_t st) {
+  // enough workgroups per box to fill the chip, at least 8 columns each
+  int ys = (nmax + 7) / 8;
+  ys = std::max(1, std::min(ys, std::max(1, 2048 / std::max(nb, 1))));
+  hipLaunchKernelGGL(k_assemble<S>, dim3(nb, ys), dim3(256), 0, st, ids, boxes, pool, nbrar, mnbr,
                      mrows, Kar, P, L);
 }
 
@@ -2353,7 +2590,7 @@ void launch_elim(bool lu, bool ldl, int mode, size_t shmem, const int* ids, int 
 #define RSKX_INST(S)                                                                          \
   template void launch_assemble<S>(const int*, const LvlBox*, const int*, const int*,          \
                                    const int*, int*, S*, const PointsDev&, const LevelParams&,  \
-                                   int, hipStream_t);                                           \
+                                   int, int, hipStream_t);                                      \
   template void launch_id<S>(int, const int*, const LvlBox*, const int*, S*, int*, double*,    \
                              const IdParams&, int*, int*, int, size_t, double*, int64_t,       \
                              hipStream_t);                                                     \

@@ -369,21 +369,23 @@ void launch_sweep(int prog, bool up, const FacRec* recs, int nrec, const S* fval
 template <typename S>
 __device__ void transpose_into(S* __restrict__ dst, const S* __restrict__ src, int R, int C,
                                S (*tile)[33]) {
-  // src R x C (leading dimension R) -> dst C x R (leading dimension C)
+  // src R x C (leading dimension R) -> dst C x R (leading dimension C); the
+  // gridDim.y workgroups of a record share its 32 x 32 tiles
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5, ny = blockDim.x >> 5;
-  for (int i0 = 0; i0 < R; i0 += 32)
-    for (int j0 = 0; j0 < C; j0 += 32) {
-      for (int q = ty; q < 32; q += ny) {
-        const int i = i0 + tx, j = j0 + q;
-        tile[q][tx] = (i < R && j < C) ? src[i + (int64_t)j * R] : (S)0;
-      }
-      __syncthreads();
-      for (int q = ty; q < 32; q += ny) {
-        const int j = j0 + tx, i = i0 + q;
-        if (i < R && j < C) dst[j + (int64_t)i * C] = tile[tx][q];
-      }
-      __syncthreads();
+  const int nti = (R + 31) / 32, ntj = (C + 31) / 32;
+  for (int64_t t = blockIdx.y; t < (int64_t)nti * ntj; t += gridDim.y) {
+    const int i0 = (int)(t % nti) * 32, j0 = (int)(t / nti) * 32;
+    for (int q = ty; q < 32; q += ny) {
+      const int i = i0 + tx, j = j0 + q;
+      tile[q][tx] = (i < R && j < C) ? src[i + (int64_t)j * R] : (S)0;
     }
+    __syncthreads();
+    for (int q = ty; q < 32; q += ny) {
+      const int j = j0 + tx, i = i0 + q;
+      if (i < R && j < C) dst[j + (int64_t)i * C] = tile[tx][q];
+    }
+    __syncthreads();
+  }
 }
 
 template <typename S, bool LU>
@@ -404,10 +406,14 @@ __global__ void __launch_bounds__(256) k_pack(const FacRec* __restrict__ recs, S
 }
 
 template <typename S>
-void launch_pack(bool lu, const FacRec* recs, int nrec, S* fval, hipStream_t st) {
+void launch_pack(bool lu, const FacRec* recs, int nrec, int rmax, S* fval, hipStream_t st) {
   if (nrec <= 0) return;
-  if (lu) hipLaunchKernelGGL((k_pack<S, true>), dim3(nrec), dim3(256), 0, st, recs, fval);
-  else hipLaunchKernelGGL((k_pack<S, false>), dim3(nrec), dim3(256), 0, st, recs, fval);
+  // workgroups per record: enough for the largest factor's tiles to fill
+  // the chip when the level has few records
+  const int64_t tiles = (int64_t)((rmax + 31) / 32) * ((rmax + 31) / 32);
+  const int ys = (int)std::max<int64_t>(1, std::min<int64_t>(tiles / 16, 2048 / nrec));
+  if (lu) hipLaunchKernelGGL((k_pack<S, true>), dim3(nrec, ys), dim3(256), 0, st, recs, fval);
+  else hipLaunchKernelGGL((k_pack<S, false>), dim3(nrec, ys), dim3(256), 0, st, recs, fval);
 }
 
 // ------------------------------------------------------- k_solve -------
@@ -947,7 +953,7 @@ void launch_gather_points(const double* x, const double* nu, const double* w, co
                                        const int*, hipStream_t);                              \
   template void launch_perm<float, S>(bool, void*, int64_t, S*, int64_t, int, int,          \
                                       const int*, hipStream_t);                               \
-  template void launch_pack<S>(bool, const FacRec*, int, S*, hipStream_t);                    \
+  template void launch_pack<S>(bool, const FacRec*, int, int, S*, hipStream_t);               \
   template void launch_logdet<S>(const FacRec*, int, const S*, const int*, int, double*,      \
                                  hipStream_t);                                                \
   template void launch_solve<S>(int, bool, const FacRec*, int, const S*, const int*, S*,      \
