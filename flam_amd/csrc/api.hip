@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -89,7 +90,11 @@ template <typename S>
 void launch_pack(bool, const FacRec*, int, int, S*, hipStream_t);
 template <typename S>
 void launch_solve(int, bool, const FacRec*, int, const S*, const int*, S*, int64_t, int, int, int,
-                  size_t, S*, int64_t, hipStream_t);
+                  size_t, S*, int64_t, int, hipStream_t);
+size_t big_solve_scratch(int k, int r, int ncol);
+template <typename S>
+int big_solve(int prog, bool up, const FacRec& f, const S* fval, const int* fidx, S* X,
+              int64_t ldx, int ncol, S* scratch, hipStream_t st);
 size_t solve_shmem(int prog, int kmax, int rmax, size_t es);
 template <typename S>
 void launch_logdet(const FacRec*, int, const S*, const int*, int, double*, hipStream_t);
@@ -166,6 +171,8 @@ struct Tuning {
                                    // vectors in global memory
   int id_reg = 1;                  // register QRCP classes for the boxes that fit them
   int elim_mfma = 1;               // matrix-core elimination for the Cholesky ('p') boxes
+  int lib_solve_r = 4096;          // solve sweeps: factors with this many redundant points or
+                                   // more take library triangular solves and products
   int big_n = 1024;                // boxes with more points take the whole-GPU path (library
                                    // GEMM / LU / Cholesky, grid-wide pivoted Cholesky of G)
   int gram_mfma = 0;               // matrix-core accumulation of G in the Gram ID (measured
@@ -304,6 +311,7 @@ struct Workspace {
   DevBuf big;                   // scratch of the wide-box ID and of the whole-GPU elimination
   DevBuf wids;                  // the wide-box ID's tail lists
   DevBuf wv;                    // the wide-box tails' vectors when shared memory is too small
+  DevBuf bsv;                   // the library solve's gathered rows
   int* hflag = nullptr;
 };
 static Workspace g_ws;
@@ -1963,6 +1971,14 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
   };
   hipEvent_t ev;
   Prof& prof = ext ? g_noprof : F->prof;
+  // factors with r >= rlib are solved by library calls with the whole GPU
+  // (not in the allocation-free entry point, nor under graph capture)
+  int rlib = INT_MAX;
+  if (!ext && solve) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(g_stream, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone)
+      rlib = g_tune.lib_solve_r > 0 ? g_tune.lib_solve_r : INT_MAX;
+  }
   prof.begin("permute", ev);
   launch_perm<U, S>(true, X, ldx, Y, N, nrhs, ncol, F->order.as<int>(), g_stream);
   prof.end("permute", ev, (int64_t)N * nrhs * sizeof(U) + N * ncol * sizeof(S) + N * 4);
@@ -2004,7 +2020,16 @@ int sweep_impl(const rskx_factor* Fc, void* X, int64_t ldx, int nrhs, char trans
         }
         prof.begin("sweep", ev);
         launch_solve<S>(prog, up, F->d_recs.as<FacRec>() + f0, (int)(f1 - f0), F->lval[lv].as<S>(),
-                        F->lidx[lv].as<int>(), Y, N, ncol, kmx, rmx, shs, gws, gstride, g_stream);
+                        F->lidx[lv].as<int>(), Y, N, ncol, kmx, rmx, shs, gws, gstride, rlib,
+                        g_stream);
+        if (rmx >= rlib)
+          for (int64_t f = f0; f < f1; ++f) {
+            const FacRec& fr = F->recs[f];
+            if (fr.r < rlib || fr.D >= 0) continue;
+            RSKX_TRY(g_ws.bsv.ensure(big_solve_scratch(fr.k, fr.r, ncol) * sizeof(S)));
+            RSKX_TRY(big_solve<S>(prog, up, fr, F->lval[lv].as<S>(), F->lidx[lv].as<int>(), Y, N,
+                                  ncol, g_ws.bsv.as<S>(), g_stream));
+          }
         prof.end("sweep", ev, bytes);
         continue;
       }
@@ -2225,6 +2250,7 @@ static int* tuning_slot(const char* name) {
   if (s == "elim_mfma") return &g_tune.elim_mfma;
   if (s == "gram_mfma") return &g_tune.gram_mfma;
   if (s == "big_n") return &g_tune.big_n;
+  if (s == "lib_solve_r") return &g_tune.lib_solve_r;
   return nullptr;
 }
 
@@ -2255,7 +2281,7 @@ int rskx_release_cache(void) {
                     &ws.mnbr, &ws.mrows, &ws.kout, &ws.iter, &ws.K, &ws.perm, &ws.T, &ws.W,
                     &ws.M[0], &ws.M[1], &ws.flag, &ws.V, &ws.lrecs, &ws.sweep, &ws.cgs, &ws.pts,
                     &ws.xw, &ws.txs, &ws.tord, &ws.tpart, &ws.tchunks, &ws.tcounts, &ws.xs,
-                    &ws.poff})
+                    &ws.poff, &ws.Ablk, &ws.big, &ws.wids, &ws.wv, &ws.bsv})
     b->release();
   block_trim_all();
   RSKX_HIP(hipGetLastError());

@@ -561,7 +561,8 @@ int gram_wide_chol(const int* ids, int nw, int nmax, const LvlBox* boxes, const 
   return RSKX_OK;
 }
 
-// T = R11 \ R12 for one wide box of rank k with the whole GPU (FP64)
+// T = R11 \ R12 and inv(R11) for one wide box of rank k with the whole GPU
+// (FP64)
 int gram_wide_tsolve(const LvlBox* boxes, int b, const LvlBox& hbox, int k, double* Tar,
                      hipStream_t st) {
   const int n = hbox.n, r = n - k;
@@ -576,6 +577,9 @@ int gram_wide_tsolve(const LvlBox* boxes, int b, const LvlBox& hbox, int k, doub
   const double one = 1.0;
   RSKX_BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                           rocblas_diagonal_non_unit, k, r, &one, X, n, T, n));
+  // inv(R11) into the second slice for the refinement's inverse row norms
+  double* Ri = Tar + hbox.T_off + (int64_t)n * n;
+  RSKX_BLAS(rocblas_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, k, X, n, Ri, n));
   RSKX_HIP(hipGetLastError());
   return RSKX_OK;
 }
@@ -694,6 +698,126 @@ int big_elim(bool lu, int b, int n, int k, int r, const LvlBox* boxes, const Box
   RSKX_HIP(hipGetLastError());
   return RSKX_OK;
 }
+
+// ------------------------------------------------ the sweeps' largest --
+// X(rows, :) (tree order, leading dimension ldx) <-> a dense column-major
+// buffer (leading dimension nrow); perm (optional) maps buffer row i to
+// rows[perm[i]] (gather) or back (scatter)
+template <typename S>
+__global__ void k_big_rows(S* __restrict__ X, int64_t ldx, const int* __restrict__ rows, int nrow,
+                           int ncol, S* __restrict__ buf, int to_buf) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)nrow * ncol;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % nrow), c = (int)(e / nrow);
+    S* x = X + rows[i] + (int64_t)c * ldx;
+    if (to_buf) buf[e] = *x;
+    else *x = buf[e];
+  }
+}
+
+// xr = xr(piv) (gather) or xr(piv) = xr (scatter), through tmp
+template <typename S>
+__global__ void k_big_perm(S* __restrict__ xr, S* __restrict__ tmp, const int* __restrict__ piv,
+                           int r, int ncol, int gather) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)r * ncol;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % r);
+    const int64_t c = e / r;
+    if (gather) tmp[e] = xr[piv[i] + c * r];
+    else tmp[piv[i] + c * r] = xr[e];
+  }
+}
+
+size_t big_solve_scratch(int k, int r, int ncol) {
+  return (size_t)(k + 2 * (size_t)r) * ncol;
+}
+
+// One factor's part of a solve sweep with library calls (the operations of
+// k_solve for programs 0 'n', 1 transposed 'n', 2 'p'): sv_nn.m, sv_nc.m,
+// sv_p.m.  The factor's rows of X are gathered into dense blocks, updated
+// by GEMMs and triangular solves, and scattered back.
+template <typename S>
+int big_solve(int prog, bool up, const FacRec& f, const S* fval, const int* fidx, S* X,
+              int64_t ldx, int ncol, S* scratch, hipStream_t st) {
+  const int k = f.k, r = f.r;
+  rocblas_handle h = big_handle(st);
+  if (!h) return (set_error("FLAM:rskx:blas: rocblas_create_handle failed"), RSKX_ERR_HIP);
+  S* xs = scratch;
+  S* xr = xs + (size_t)k * ncol;
+  S* tmp = xr + (size_t)r * ncol;
+  const int* sk = fidx + f.sk;
+  const int* rd = fidx + f.rd;
+  const int* piv = fidx + f.piv;
+  auto grid = [](int64_t e) { return (int)std::max<int64_t>(1, std::min<int64_t>((e + 255) / 256, 65536)); };
+  if (k > 0)
+    hipLaunchKernelGGL(k_big_rows<S>, dim3(grid((int64_t)k * ncol)), dim3(256), 0, st, X, ldx, sk, k,
+                       ncol, xs, 1);
+  hipLaunchKernelGGL(k_big_rows<S>, dim3(grid((int64_t)r * ncol)), dim3(256), 0, st, X, ldx, rd, r,
+                     ncol, xr, 1);
+  const S one = 1, mone = -1;
+  const rocblas_operation N_ = rocblas_operation_none, T_ = rocblas_operation_transpose;
+  const S* T = fval + f.T;     // k x r
+  const S* LU = fval + f.LU;   // r x r
+  const S* E = fval + f.E;     // k x r
+  const S* G = fval + f.G;     // r x k
+  auto perm = [&](int gather) {
+    hipLaunchKernelGGL(k_big_perm<S>, dim3(grid((int64_t)r * ncol)), dim3(256), 0, st, xr, tmp, piv,
+                       r, ncol, gather);
+    RSKX_HIP(hipMemcpyAsync(xr, tmp, (size_t)r * ncol * sizeof(S), hipMemcpyDeviceToDevice, st));
+    return RSKX_OK;
+  };
+  if (prog == 0) {
+    if (up) {
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, T_, N_, r, ncol, k, &mone, T, k, xs, k, &one, xr, r));
+      if (const int rc = perm(1)) return rc;
+      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_left, rocblas_fill_lower, N_, rocblas_diagonal_unit,
+                              r, ncol, &one, LU, r, xr, r));
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, N_, N_, k, ncol, r, &mone, E, k, xr, r, &one, xs, k));
+    } else {
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, N_, N_, r, ncol, k, &mone, G, r, xs, k, &one, xr, r));
+      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_left, rocblas_fill_upper, N_,
+                              rocblas_diagonal_non_unit, r, ncol, &one, LU, r, xr, r));
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, N_, N_, k, ncol, r, &mone, T, k, xr, r, &one, xs, k));
+    }
+  } else if (prog == 1) {
+    if (up) {
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, T_, N_, r, ncol, k, &mone, T, k, xs, k, &one, xr, r));
+      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_left, rocblas_fill_upper, T_,
+                              rocblas_diagonal_non_unit, r, ncol, &one, LU, r, xr, r));
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, T_, N_, k, ncol, r, &mone, G, r, xr, r, &one, xs, k));
+    } else {
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, T_, N_, r, ncol, k, &mone, E, k, xs, k, &one, xr, r));
+      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_left, rocblas_fill_lower, T_, rocblas_diagonal_unit,
+                              r, ncol, &one, LU, r, xr, r));
+      if (const int rc = perm(0)) return rc;
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, N_, N_, k, ncol, r, &mone, T, k, xr, r, &one, xs, k));
+    }
+  } else {
+    if (up) {
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, T_, N_, r, ncol, k, &mone, T, k, xs, k, &one, xr, r));
+      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_left, rocblas_fill_lower, N_,
+                              rocblas_diagonal_non_unit, r, ncol, &one, LU, r, xr, r));
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, N_, N_, k, ncol, r, &mone, E, k, xr, r, &one, xs, k));
+    } else {
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, T_, N_, r, ncol, k, &mone, E, k, xs, k, &one, xr, r));
+      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_left, rocblas_fill_lower, T_,
+                              rocblas_diagonal_non_unit, r, ncol, &one, LU, r, xr, r));
+      if (k > 0) RSKX_BLAS(Blas<S>::gemm(h, N_, N_, k, ncol, r, &mone, T, k, xr, r, &one, xs, k));
+    }
+  }
+  if (k > 0)
+    hipLaunchKernelGGL(k_big_rows<S>, dim3(grid((int64_t)k * ncol)), dim3(256), 0, st, X, ldx, sk, k,
+                       ncol, xs, 0);
+  hipLaunchKernelGGL(k_big_rows<S>, dim3(grid((int64_t)r * ncol)), dim3(256), 0, st, X, ldx, rd, r,
+                     ncol, xr, 0);
+  RSKX_HIP(hipGetLastError());
+  return RSKX_OK;
+}
+
+template int big_solve<float>(int, bool, const FacRec&, const float*, const int*, float*, int64_t,
+                              int, float*, hipStream_t);
+template int big_solve<double>(int, bool, const FacRec&, const double*, const int*, double*,
+                               int64_t, int, double*, hipStream_t);
 
 template int big_elim<float>(bool, int, int, int, int, const LvlBox*, const BoxOut*, const Seg*,
                              int*, const int*, const double*, const float*, float*, float*, int*,

@@ -318,12 +318,12 @@ __device__ void trsv_upper(const double* R, int kk, int ldr, const double* b, do
 
 // Gu-Eisenstat refinement with R11 in R (upper, leading dimension ldt) and
 // T = R11 \ R12 (k x r), in O(k^2 + k r) per exchange like id.m:116-236
-// with c2 = 0: the inverse row norms r2 are formed once (when the first
-// test fails) and then updated by triangular solves (id.m:151-152,
-// 191-192, 218); vb holds six n-vectors
+// with c2 = 0: the inverse row norms r2 (id.m:124; in vb already when
+// have_r2, else formed when the first test fails) are updated by
+// triangular solves (id.m:151-152, 191-192, 218); vb holds six n-vectors
 __device__ void refine_large(int& k, int n, double* T, double* R, double* X, int ldt, int* perm,
                              const IdParams& prm, double tol_abs, double* vb, double* red,
-                             int& iter, bool& conv) {
+                             int& iter, bool& conv, bool have_r2) {
   const int tid = threadIdx.x, nt = blockDim.x;
   double* r2 = vb;
   double* u = vb + n;
@@ -333,7 +333,6 @@ __device__ void refine_large(int& k, int n, double* T, double* R, double* X, int
   double* sn = vb + 5 * n;
   const double f2 = prm.Tmax * prm.Tmax;
   const double cap = fmin(prm.rrqr_iter, 4.0 * n + 64.0);
-  bool have_r2 = false;
   conv = false;
   while ((double)iter < cap) {
     const int r = n - k;
@@ -611,6 +610,14 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
   int iter = 0;
   bool conv = true;
   if (MODE == 3 && isfinite(prm.Tmax) && prm.rrqr_iter > 0 && k > 0 && k < n) {
+    // inv(R11) is in R's slice (a library triangular inverse ran before
+    // this kernel): its squared row norms first, then R11 itself
+    double* r2 = vglob + blockIdx.x * vstride;
+    for (int l = tid; l < k; l += nt) {
+      double acc = 0;
+      for (int c = l; c < k; ++c) acc += R[l + (int64_t)c * ldt] * R[l + (int64_t)c * ldt];
+      r2[l] = acc;
+    }
     __syncthreads();
     for (int e = tid; e < k * k; e += nt) {
       int i = e % k, j = e / k;
@@ -618,7 +625,7 @@ __global__ void __launch_bounds__(256) k_id(const int* __restrict__ ids,
     }
     __syncthreads();
     refine_large(k, n, T, R, X, ldt, perm, prm, tol_abs, vglob + blockIdx.x * vstride, red, iter,
-                 conv);
+                 conv, true);
   } else if (isfinite(prm.Tmax) && prm.rrqr_iter > 0 && k > 0 && k < n) {
     // Gu-Eisenstat refinement in the fixed-precision form the reference
     // takes when R has no trailing block (id.m:116-236, c2 = 0).

@@ -556,7 +556,8 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(const FacRec* __restric
                                                          const S* __restrict__ fval,
                                                          const int* __restrict__ fidx, S* X,
                                                          int64_t ldx, int nrhs, int kmax,
-                                                         int rmax, S* gws, int64_t gstride) {
+                                                         int rmax, S* gws, int64_t gstride,
+                                                         int rlib) {
   extern __shared__ char smem_raw[];
   // shared memory, or (GWS, a compile-time choice that leaves the shared
   // variant its LDS accesses) this workgroup's slice of a global workspace
@@ -567,6 +568,7 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(const FacRec* __restric
   S* tmp = red + (size_t)kRhs * kSolveThreads;
   const FacRec f = recs[blockIdx.x];
   const int k = f.k, r = f.r;
+  if (r >= rlib && f.D < 0) return;      // solved with the whole GPU (library calls)
   const int* sk = fidx + f.sk;
   const int* rd = fidx + f.rd;
   const int* piv = fidx + f.piv;
@@ -654,14 +656,14 @@ size_t solve_shmem(int prog, int kmax, int rmax, size_t es) {
 template <typename S>
 void launch_solve(int prog, bool up, const FacRec* recs, int nrec, const S* fval, const int* fidx,
                   S* X, int64_t ldx, int nrhs, int kmax, int rmax, size_t shmem, S* gws,
-                  int64_t gstride, hipStream_t st) {
+                  int64_t gstride, int rlib, hipStream_t st) {
 #define RSKX_SOLVE(P)                                                                           \
   {                                                                                             \
     auto kern = gws ? (up ? k_solve<S, P, true, true> : k_solve<S, P, false, true>)            \
                     : (up ? k_solve<S, P, true, false> : k_solve<S, P, false, false>);          \
     ensure_dyn_lds((const void*)kern, shmem);                                                    \
     hipLaunchKernelGGL(kern, dim3(nrec), dim3(kSolveThreads), shmem, st, recs, fval, fidx, X,    \
-                       ldx, nrhs, kmax, rmax, gws, gstride);                                    \
+                       ldx, nrhs, kmax, rmax, gws, gstride, rlib);                              \
   }
   if (prog == 0) RSKX_SOLVE(0) else if (prog == 1) RSKX_SOLVE(1) else RSKX_SOLVE(2)
 #undef RSKX_SOLVE
@@ -957,7 +959,7 @@ void launch_gather_points(const double* x, const double* nu, const double* w, co
   template void launch_logdet<S>(const FacRec*, int, const S*, const int*, int, double*,      \
                                  hipStream_t);                                                \
   template void launch_solve<S>(int, bool, const FacRec*, int, const S*, const int*, S*,      \
-                                int64_t, int, int, int, size_t, S*, int64_t, hipStream_t);
+                                int64_t, int, int, int, size_t, S*, int64_t, int, hipStream_t);
 RSKX_SWEEP_INST(double)
 RSKX_SWEEP_INST(float)
 

@@ -118,3 +118,34 @@ def test_big_path_circle_cholesky(prec):
     la, lb = flam_amd.rskelf_logdet(Fa), flam_amd.rskelf_logdet(Fb)
     print(f"logdet: big {la}, default {lb}", flush=True)
     assert abs(complex(la) - complex(lb)) <= (1e-3 if prec == "fp32" else 1e-6) * max(1.0, abs(complex(lb)))
+
+
+@pytest.mark.parametrize("symm,prec", [("n", "fp64"), ("n", "fp32"), ("p", "fp64")])
+def test_library_solve_matches_kernel_solve(symm, prec):
+    """Factors with at least lib_solve_r redundant points are solved by
+    library triangular solves and products; forced down to 64, most
+    non-leaf factors take that path, and the solves (both transposes for
+    'n') must agree with the per-workgroup sweeps to rounding."""
+    import flam_amd
+    from flam_amd import geometry, tune
+    if symm == "n":
+        g = geometry.sphere(5120)
+        occ, tol = 128, 1e-6
+    else:
+        g = geometry.circle(1 << 14)
+        occ, tol = 64, 1e-8
+    print(f"library solve: {g.N} points symm={symm} {prec} lib_solve_r=64", flush=True)
+    F = flam_amd.rskelf(g.kernel, g.x, occ, tol, g.proxy, dict(symm=symm, precision=prec))
+    B = torch.from_numpy(np.random.default_rng(13).standard_normal((g.N, 5))).cuda()
+    rtol = 1e-4 if prec == "fp32" else 1e-10
+    for trans in (("n", "t") if symm == "n" else ("n",)):
+        tune.set_tuning("lib_solve_r", 0)
+        try:
+            Xk = flam_amd.rskelf_sv(F, B, trans)
+            tune.set_tuning("lib_solve_r", 64)
+            Xl = flam_amd.rskelf_sv(F, B, trans)
+        finally:
+            tune.set_tuning("lib_solve_r", 4096)
+        d = ((Xl - Xk).abs().max() / Xk.abs().max()).item()
+        print(f"trans {trans}: max relative difference {d:.2e}", flush=True)
+        assert d <= rtol
