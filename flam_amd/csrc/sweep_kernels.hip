@@ -893,6 +893,27 @@ __global__ void __launch_bounds__(256) k_matvec_sl3(const double* __restrict__ x
       }
       __syncthreads();
       const int jn = blk1 - j0 < 256 ? (int)(blk1 - j0) : 256;
+      if (j0 >= my0 && j0 + jn <= my1) {
+        // every source of the tile belongs to this row's problem: 1/r from
+        // the FP32 reciprocal square root and two Newton steps in FP64
+        // (relative error ~1e-16); the diagonal is added after the loop
+        for (int t = 0; t < jn; ++t) {
+          const double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          double y = (double)rsqrtf((float)r2);
+          y = y * (1.5 - 0.5 * r2 * y * y);
+          y = y * (1.5 - 0.5 * r2 * y * y);
+          const double a = r2 > 0 ? c0 * sw[t] * y : 0.0;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) if (c < cc) acc[c] += a * sX[c][t];
+        }
+        if (act && i >= j0 && i < j0 + jn) {
+          const double d = diag[i];
+#pragma unroll
+          for (int c = 0; c < 8; ++c) if (c < cc) acc[c] += d * sX[c][i - j0];
+        }
+        continue;
+      }
       for (int t = 0; t < jn; ++t) {
         int64_t jj = j0 + t;
         if (jj < my0 || jj >= my1) continue;   // another problem's source
