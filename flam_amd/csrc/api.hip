@@ -70,12 +70,12 @@ int gram_wide_chol(const int* ids, int nw, int nmax, const LvlBox* boxes, const 
                    hipStream_t st);
 int gram_wide_tsolve(const LvlBox* boxes, int b, const LvlBox& hbox, int k, double* Tar,
                      hipStream_t st);
-size_t big_elim_scratch(int n, int k, int r, size_t es);
+size_t big_elim_scratch(int nbx, int rM, int kM, size_t es);
 template <typename S>
-int big_elim(bool lu, int b, int n, int k, int r, const LvlBox* boxes, const BoxOut* outs,
-             const Seg* csegs, int* pool, const int* permar, const double* Tar, const S* Mprev,
-             S* Mcur, S* fval, int* fidx, const PointsDev& P, int* fail, void* scratch,
-             hipStream_t st);
+int big_elim(bool lu, const int* ids, int nbx, int rM, int kM, const LvlBox* boxes,
+             const BoxOut* outs, const Seg* csegs, int* pool, const int* permar, const double* Tar,
+             const S* Mprev, S* Mcur, S* fval, int* fidx, const PointsDev& P, int* fail,
+             void* scratch, hipStream_t st);
 constexpr int kMfmaShapes = 4;
 template <typename S>
 void launch_elim_mfma(int, size_t, const int*, int, const LvlBox*, const BoxOut*, const Seg*, int*,
@@ -1703,14 +1703,12 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     size_t shm_cls[3] = {0, 0, 0}, shm_reg[2] = {0, 0}, shm_mf[kMfmaShapes] = {0, 0, 0, 0};
     const int nreg8 = std::min(elim_reg_nmax(sizeof(S), 8), g_tune.elim8_nmax),
               nreg16 = elim_reg_nmax(sizeof(S), 16);
-    size_t big_need = 0;
     for (int64_t ib = 0; ib < nb; ++ib) {
       const BoxOut& O = ho[ib];
       const int n = hb[ib].n;
       // the largest boxes: dense LU / Cholesky with the whole GPU
       if (O.r > 0 && !ldl && n > g_tune.big_n) {
         cbig.push_back((int)ib);
-        big_need = std::max(big_need, big_elim_scratch(n, O.k, O.r, sizeof(S)));
         continue;
       }
       // Cholesky boxes go to the matrix-core elimination when its tiles and
@@ -1766,6 +1764,33 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     allids.insert(allids.end(), creg[1].begin(), creg[1].end());
     allids.insert(allids.end(), creg[0].begin(), creg[0].end());
     for (int c = 2; c >= 0; --c) allids.insert(allids.end(), cls[c].begin(), cls[c].end());
+    // the largest boxes in batches of similar sizes (n within 25% of the
+    // batch's largest, scratch within 8 GB), padded to the batch's largest
+    // r and k
+    struct BigBatch { int off, cnt, rM, kM; };
+    std::vector<BigBatch> bb;
+    size_t big_need = 0;
+    if (!cbig.empty()) {
+      std::sort(cbig.begin(), cbig.end(), [&](int a, int b) { return hb[a].n > hb[b].n; });
+      for (size_t i = 0; i < cbig.size();) {
+        BigBatch g{(int)(allids.size() + i), 0, 0, 0};
+        const int n0 = hb[cbig[i]].n;
+        while (i < cbig.size()) {
+          const int ib = cbig[i];
+          const int rM = std::max(g.rM, ho[ib].r), kM = std::max(g.kM, ho[ib].k);
+          if (g.cnt > 0 && (4 * hb[ib].n < 3 * n0 ||
+                            big_elim_scratch(g.cnt + 1, rM, kM, sizeof(S)) > (size_t(8) << 30)))
+            break;
+          g.rM = rM;
+          g.kM = kM;
+          ++g.cnt;
+          ++i;
+        }
+        big_need = std::max(big_need, big_elim_scratch(g.cnt, g.rM, g.kM, sizeof(S)));
+        bb.push_back(g);
+      }
+      allids.insert(allids.end(), cbig.begin(), cbig.end());
+    }
     RSKX_TRY(upload(ws.ids, allids));
     if (!cbig.empty()) RSKX_TRY(ws.big.ensure(big_need));
     prof.begin("factor", ev);
@@ -1810,9 +1835,9 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
         off += (int)cls[c].size();
       }
       if (conc) RSKX_TRY(side_join());
-      // the largest boxes one after another, each with the whole GPU
-      for (int ib : cbig)
-        RSKX_TRY(big_elim<S>(lu, ib, hb[ib].n, ho[ib].k, ho[ib].r, ws.boxes.as<LvlBox>(),
+      // the largest boxes batch after batch, each batch with the whole GPU
+      for (const BigBatch& g : bb)
+        RSKX_TRY(big_elim<S>(lu, ws.ids.as<int>() + g.off, g.cnt, g.rM, g.kM, ws.boxes.as<LvlBox>(),
                              ws.outs.as<BoxOut>(), ws.csegs.as<Seg>(), ws.pool.as<int>(),
                              ws.perm.as<int>(), ws.T.as<double>(), ws.M[mpar].as<S>(),
                              ws.M[mcur].as<S>(), fval, fidx, P, ws.flag.as<int>(), ws.big.p,

@@ -300,22 +300,39 @@ __global__ void k_gw_tcopy(const LvlBox* __restrict__ boxes, int b, int k, doubl
 }
 
 // ------------------------------------------------------ the elimination --
-// K = A(slf, slf) + M in the order [rd; sk] (n x n, leading dimension n),
-// one workgroup per column; with mo_only, only the sk x sk block and only
-// M there (the new skeleton block starts from M(sk, sk), rskelf.m:246)
+// A batch of boxes padded to common sizes rM >= r and kM >= k: box w's
+// matrix Z (nM x nM, nM = rM + kM, column-major) holds K = A(slf,slf) + M
+// in the order [rd, rd padding, sk, sk padding]; the rd padding is an
+// identity block (LU and Cholesky of [K(rd,rd) 0; 0 I] leave the real
+// block's factors, and no padded row is ever a pivot candidate with a
+// nonzero entry), everything else in the padding is zero, so every
+// product and solve below runs on the whole batch with uniform sizes.
+
+// padded index -> position in q (rd first, then sk), or -1 for padding
+__device__ inline int bigb_pos(int i, int r, int k, int rM) {
+  if (i < rM) return i < r ? k + i : -1;
+  const int a = i - rM;
+  return a < k ? a : -1;
+}
+
+// column blockIdx.x of box blockIdx.y's Z; with mo_only only the sk x sk
+// block and only M there (the new skeleton block starts from M(sk, sk),
+// rskelf.m:246)
 template <typename S>
-__global__ void __launch_bounds__(256) k_big_kassemble(const LvlBox* __restrict__ boxes, int b,
+__global__ void __launch_bounds__(256) k_bigb_assemble(const int* __restrict__ ids,
+                                                       const LvlBox* __restrict__ boxes,
                                                        const BoxOut* __restrict__ outs,
                                                        const Seg* __restrict__ csegs,
                                                        const int* __restrict__ pool,
                                                        const int* __restrict__ permar,
                                                        const S* __restrict__ Mprev,
-                                                       S* __restrict__ Z, PointsDev P,
-                                                       int mo_only) {
+                                                       S* __restrict__ Z, PointsDev P, int rM,
+                                                       int kM, int mo_only) {
   __shared__ ChildMap cm;
+  const int b = ids[blockIdx.y];
   const LvlBox B = boxes[b];
   const BoxOut O = outs[b];
-  const int n = B.n, k = O.k, r = O.r;
+  const int n = B.n, k = O.k, r = O.r, nM = rM + kM;
   if (threadIdx.x == 0) {
     const int nc = B.ncs > 8 ? 8 : B.ncs;
     cm.nc = nc;
@@ -332,18 +349,27 @@ __global__ void __launch_bounds__(256) k_big_kassemble(const LvlBox* __restrict_
   const bool leafbox = cm.nc == 1 && cm.moff[0] < 0;
   const int* slf = pool + B.slf_off;
   const int* q = permar + B.perm_off;           // q[0..k) skeletons, q[k..n) redundant
-  const int j = mo_only ? r + blockIdx.x : blockIdx.x;
-  const int sj = j < r ? q[k + j] : q[j - r];
+  const int j = mo_only ? rM + blockIdx.x : blockIdx.x;
+  S* Zj = Z + (int64_t)blockIdx.y * nM * nM + (int64_t)j * nM;
+  const int pj = bigb_pos(j, r, k, rM);
+  const int i0 = mo_only ? rM : 0;
+  if (pj < 0) {                                 // padded column
+    for (int i = i0 + threadIdx.x; i < nM; i += blockDim.x) Zj[i] = (S)(i == j && j < rM ? 1 : 0);
+    return;
+  }
+  const int sj = q[pj];
   double xj, yj, f2j, f3j, wj;
   point_fields(P, slf[sj], xj, yj, f2j, f3j, wj);
   const double dgj = P.diag[slf[sj]];
-  S* Zj = Z + (int64_t)j * n;
-  for (int i = (mo_only ? r : 0) + threadIdx.x; i < n; i += blockDim.x) {
-    const int si = i < r ? q[k + i] : q[i - r];
+  (void)n;
+  for (int i = i0 + threadIdx.x; i < nM; i += blockDim.x) {
+    const int pi = bigb_pos(i, r, k, rM);
+    if (pi < 0) { Zj[i] = (S)0; continue; }
+    const int si = q[pi];
     double v = 0;
     if (!mo_only) {
       if (P.kind == RSKX_KERNEL_CALLBACK) {
-        v = P.blk[B.A_off + si + (int64_t)sj * n];
+        v = P.blk[B.A_off + si + (int64_t)sj * B.n];
       } else if (si == sj) {
         v = dgj;
       } else {
@@ -363,34 +389,53 @@ __global__ void __launch_bounds__(256) k_big_kassemble(const LvlBox* __restrict_
   }
 }
 
-// T (k x r, FP64, leading dimension ldt) in the factor's precision,
-// leading dimension k
+// T of box blockIdx.y (k x r, FP64, leading dimension ldt) in the factor's
+// precision, padded to kM x rM with zeros
 template <typename S>
-__global__ void k_big_tconv(const LvlBox* __restrict__ boxes, int b, const BoxOut* __restrict__ outs,
-                            const double* __restrict__ Tar, S* __restrict__ Ts) {
+__global__ void k_bigb_tconv(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
+                             const BoxOut* __restrict__ outs, const double* __restrict__ Tar,
+                             S* __restrict__ Ts, int rM, int kM) {
+  const int b = ids[blockIdx.y];
   const LvlBox B = boxes[b];
   const BoxOut O = outs[b];
   const int k = O.k, r = O.r;
   const double* Tg = Tar + B.T_off;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)k * r;
+  S* Tw = Ts + (int64_t)blockIdx.y * kM * rM;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)kM * rM;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const int a = (int)(e % k), c = (int)(e / k);
-    Ts[e] = (S)Tg[a + (int64_t)c * B.ldt];
+    const int a = (int)(e % kM), c = (int)(e / kM);
+    Tw[e] = (a < k && c < r) ? (S)Tg[a + (int64_t)c * B.ldt] : (S)0;
   }
 }
 
-// the factor from the dense result (Z, leading dimension n): L and U (or
-// L), E, G, T, the new skeleton block, the pivot vector (from LAPACK's
-// interchanges), the point lists
+// LAPACK's row interchanges of the LU applied to Krs (rows 0..rM, columns
+// rM..nM): G = L \ K(rd(p), sk); one thread per column
 template <typename S>
-__global__ void k_big_out(const LvlBox* __restrict__ boxes, int b, const BoxOut* __restrict__ outs,
-                          int* pool, const int* __restrict__ permar, const S* __restrict__ Z,
-                          const S* __restrict__ Ts, const int* __restrict__ ipiv,
-                          S* __restrict__ fval, int* __restrict__ fidx, S* __restrict__ Mcur,
-                          int lu) {
+__global__ void k_bigb_laswp(S* __restrict__ Z, const int* __restrict__ ipiv, int rM, int kM) {
+  const int nM = rM + kM, c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= kM) return;
+  S* col = Z + (int64_t)blockIdx.y * nM * nM + (int64_t)(rM + c) * nM;
+  const int* pv = ipiv + (int64_t)blockIdx.y * rM;
+  for (int i = 0; i < rM; ++i) {
+    const int t = pv[i] - 1;
+    if (t != i) { const S a = col[i]; col[i] = col[t]; col[t] = a; }
+  }
+}
+
+// the factor from the batch's dense results: L and U (or L), E, G, T, the
+// new skeleton block, the point lists
+template <typename S>
+__global__ void k_bigb_out(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
+                           const BoxOut* __restrict__ outs, int* pool,
+                           const int* __restrict__ permar, const S* __restrict__ Z,
+                           const S* __restrict__ Ts, S* __restrict__ fval, int* __restrict__ fidx,
+                           S* __restrict__ Mcur, int lu, int rM, int kM) {
+  const int b = ids[blockIdx.y];
   const LvlBox B = boxes[b];
   const BoxOut O = outs[b];
-  const int n = B.n, k = O.k, r = O.r;
+  const int k = O.k, r = O.r, nM = rM + kM;
+  const S* Zw = Z + (int64_t)blockIdx.y * nM * nM;
+  const S* Tw = Ts + (int64_t)blockIdx.y * kM * rM;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t nt = (int64_t)gridDim.x * blockDim.x;
   S* fLU = fval + O.LU;
@@ -398,24 +443,24 @@ __global__ void k_big_out(const LvlBox* __restrict__ boxes, int b, const BoxOut*
   S* fT = fval + O.T;
   for (int64_t e = tid; e < (int64_t)r * r; e += nt) {
     const int i = (int)(e % r), j = (int)(e / r);
-    fLU[e] = (lu || i >= j) ? Z[i + (int64_t)j * n] : (S)0;
+    fLU[e] = (lu || i >= j) ? Zw[i + (int64_t)j * nM] : (S)0;
   }
   for (int64_t e = tid; e < (int64_t)k * r; e += nt) {
     const int a = (int)(e % k), j = (int)(e / k);
-    fE[e] = Z[r + a + (int64_t)j * n];
-    fT[e] = Ts[e];
+    fE[e] = Zw[rM + a + (int64_t)j * nM];
+    fT[e] = Tw[a + (int64_t)j * kM];
   }
   if (lu) {
     S* fG = fval + O.G;
     for (int64_t e = tid; e < (int64_t)r * k; e += nt) {
       const int i = (int)(e % r), a = (int)(e / r);
-      fG[e] = Z[i + (int64_t)(r + a) * n];
+      fG[e] = Zw[i + (int64_t)(rM + a) * nM];
     }
   }
   S* Mo = Mcur + O.M_off;
   for (int64_t e = tid; e < (int64_t)k * k; e += nt) {
     const int a = (int)(e % k), c = (int)(e / k);
-    Mo[e] = Z[r + a + (int64_t)(r + c) * n];
+    Mo[e] = Zw[rM + a + (int64_t)(rM + c) * nM];
   }
   const int* slf = pool + B.slf_off;
   const int* q = permar + B.perm_off;
@@ -431,17 +476,18 @@ __global__ void k_big_out(const LvlBox* __restrict__ boxes, int b, const BoxOut*
 
 // LAPACK's interchanges (1-based ipiv) as the permutation vector p with
 // L U = K(rd(p), rd), [L,U,p] = lu(...,'vector'); one thread walks them
-__global__ void k_big_piv(const BoxOut* __restrict__ outs, int b, const int* __restrict__ ipiv,
-                          int* __restrict__ fidx) {
-  const BoxOut O = outs[b];
+__global__ void k_bigb_piv(const int* __restrict__ ids, const BoxOut* __restrict__ outs,
+                           const int* __restrict__ ipiv, int* __restrict__ fidx, int rM) {
+  const BoxOut O = outs[ids[blockIdx.x]];
   const int r = O.r;
+  const int* pv = ipiv + (int64_t)blockIdx.x * rM;
   int* piv = fidx + O.piv;
   for (int i = threadIdx.x; i < r; i += blockDim.x) piv[i] = i;
   __syncthreads();
   if (threadIdx.x == 0)
     for (int i = 0; i < r; ++i) {
-      const int t = ipiv[i] - 1;
-      if (t != i) { const int a = piv[i]; piv[i] = piv[t]; piv[t] = a; }
+      const int t = pv[i] - 1;
+      if (t != i && t < r) { const int a = piv[i]; piv[i] = piv[t]; piv[t] = a; }
     }
 }
 
@@ -449,11 +495,13 @@ __global__ void k_big_piv(const BoxOut* __restrict__ outs, int b, const int* __r
 // per block: lane c forms column c of the block's inverse by substitution
 // from the factor just written
 template <typename S>
-__global__ void __launch_bounds__(64) k_big_inv(const BoxOut* __restrict__ outs, int b,
-                                                S* __restrict__ fval, int lu) {
-  const BoxOut O = outs[b];
+__global__ void __launch_bounds__(64) k_bigb_inv(const int* __restrict__ ids,
+                                                 const BoxOut* __restrict__ outs,
+                                                 S* __restrict__ fval, int lu) {
+  const BoxOut O = outs[ids[blockIdx.y]];
   const int r = O.r, nbk = (r + kDiagBlock - 1) / kDiagBlock;
   const int bb = blockIdx.x;
+  if (bb >= 2 * nbk) return;
   const bool upper = bb >= nbk;
   if (upper && !lu) return;
   const int blk = upper ? bb - nbk : bb;
@@ -494,8 +542,9 @@ __global__ void __launch_bounds__(64) k_big_inv(const BoxOut* __restrict__ outs,
 
 // a zero pivot (getrf) or a non-positive one (potrf) raises the level's
 // failure flag as the other elimination kernels do
-__global__ void k_big_info(const int* __restrict__ info, int lu, int* __restrict__ fail) {
-  if (*info > 0) atomicOr(fail, lu ? 2 : 1);
+__global__ void k_bigb_info(const int* __restrict__ info, int nbx, int lu, int* __restrict__ fail) {
+  for (int w = threadIdx.x; w < nbx; w += blockDim.x)
+    if (info[w] > 0) atomicOr(fail, lu ? 2 : 1);
 }
 
 // ------------------------------------------------------------ host side --
@@ -590,10 +639,6 @@ template void gram_wide_gram<double>(const int*, int, int, const LvlBox*, const 
                                      const double*, double*, hipStream_t);
 
 
-size_t big_elim_scratch(int n, int k, int r, size_t es) {
-  return ((size_t)n * n + (size_t)k * r + 64) * es + (size_t)(r + 16) * sizeof(int);
-}
-
 template <typename S> struct Blas;
 template <> struct Blas<float> {
   static rocblas_status gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
@@ -601,103 +646,143 @@ template <> struct Blas<float> {
                              int ldb, const float* be, float* C, int ldc) {
     return rocblas_sgemm(h, ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
   }
+  static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
+                                int n, int k, const float* al, const float* A, int lda, int64_t sa,
+                                const float* B, int ldb, int64_t sb, const float* be, float* C,
+                                int ldc, int64_t sc, int cnt) {
+    return rocblas_sgemm_strided_batched(h, ta, tb, m, n, k, al, A, lda, sa, B, ldb, sb, be, C, ldc,
+                                         sc, cnt);
+  }
   static rocblas_status trsm(rocblas_handle h, rocblas_side sd, rocblas_fill up, rocblas_operation ta,
                              rocblas_diagonal dg, int m, int n, const float* al, const float* A,
                              int lda, float* B, int ldb) {
     return rocblas_strsm(h, sd, up, ta, dg, m, n, al, A, lda, B, ldb);
   }
-  static rocblas_status getrf(rocblas_handle h, int m, int n, float* A, int lda, int* ipiv, int* info) {
-    return rocsolver_sgetrf(h, m, n, A, lda, ipiv, info);
+  static rocblas_status trsm_sb(rocblas_handle h, rocblas_side sd, rocblas_fill up,
+                                rocblas_operation ta, rocblas_diagonal dg, int m, int n,
+                                const float* al, const float* A, int lda, int64_t sa, float* B,
+                                int ldb, int64_t sb, int cnt) {
+    return rocblas_strsm_strided_batched(h, sd, up, ta, dg, m, n, al, A, lda, sa, B, ldb, sb, cnt);
   }
-  static rocblas_status potrf(rocblas_handle h, int n, float* A, int lda, int* info) {
-    return rocsolver_spotrf(h, rocblas_fill_lower, n, A, lda, info);
+  static rocblas_status getrf_sb(rocblas_handle h, int n, float* A, int lda, int64_t sa, int* ipiv,
+                                 int64_t sp, int* info, int cnt) {
+    return rocsolver_sgetrf_strided_batched(h, n, n, A, lda, sa, ipiv, sp, info, cnt);
   }
-  static rocblas_status laswp(rocblas_handle h, int n, float* A, int lda, int k1, int k2, const int* ipiv) {
-    return rocsolver_slaswp(h, n, A, lda, k1, k2, ipiv, 1);
+  static rocblas_status potrf_sb(rocblas_handle h, int n, float* A, int lda, int64_t sa, int* info,
+                                 int cnt) {
+    return rocsolver_spotrf_strided_batched(h, rocblas_fill_lower, n, A, lda, sa, info, cnt);
   }
 };
 template <> struct Blas<double> {
   static rocblas_status gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
-                             int n, int k, const double* al, const double* A, int lda, const double* B,
-                             int ldb, const double* be, double* C, int ldc) {
+                             int n, int k, const double* al, const double* A, int lda,
+                             const double* B, int ldb, const double* be, double* C, int ldc) {
     return rocblas_dgemm(h, ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+  }
+  static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
+                                int n, int k, const double* al, const double* A, int lda,
+                                int64_t sa, const double* B, int ldb, int64_t sb, const double* be,
+                                double* C, int ldc, int64_t sc, int cnt) {
+    return rocblas_dgemm_strided_batched(h, ta, tb, m, n, k, al, A, lda, sa, B, ldb, sb, be, C, ldc,
+                                         sc, cnt);
   }
   static rocblas_status trsm(rocblas_handle h, rocblas_side sd, rocblas_fill up, rocblas_operation ta,
                              rocblas_diagonal dg, int m, int n, const double* al, const double* A,
                              int lda, double* B, int ldb) {
     return rocblas_dtrsm(h, sd, up, ta, dg, m, n, al, A, lda, B, ldb);
   }
-  static rocblas_status getrf(rocblas_handle h, int m, int n, double* A, int lda, int* ipiv, int* info) {
-    return rocsolver_dgetrf(h, m, n, A, lda, ipiv, info);
+  static rocblas_status trsm_sb(rocblas_handle h, rocblas_side sd, rocblas_fill up,
+                                rocblas_operation ta, rocblas_diagonal dg, int m, int n,
+                                const double* al, const double* A, int lda, int64_t sa, double* B,
+                                int ldb, int64_t sb, int cnt) {
+    return rocblas_dtrsm_strided_batched(h, sd, up, ta, dg, m, n, al, A, lda, sa, B, ldb, sb, cnt);
   }
-  static rocblas_status potrf(rocblas_handle h, int n, double* A, int lda, int* info) {
-    return rocsolver_dpotrf(h, rocblas_fill_lower, n, A, lda, info);
+  static rocblas_status getrf_sb(rocblas_handle h, int n, double* A, int lda, int64_t sa, int* ipiv,
+                                 int64_t sp, int* info, int cnt) {
+    return rocsolver_dgetrf_strided_batched(h, n, n, A, lda, sa, ipiv, sp, info, cnt);
   }
-  static rocblas_status laswp(rocblas_handle h, int n, double* A, int lda, int k1, int k2, const int* ipiv) {
-    return rocsolver_dlaswp(h, n, A, lda, k1, k2, ipiv, 1);
+  static rocblas_status potrf_sb(rocblas_handle h, int n, double* A, int lda, int64_t sa, int* info,
+                                 int cnt) {
+    return rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, n, A, lda, sa, info, cnt);
   }
 };
 
-// one box's elimination (rskelf.m:218-246); lu: 'n' / 's' (real), else 'p'
+// scratch of one batch (bytes): Z, padded T, pivots, info
+size_t big_elim_scratch(int nbx, int rM, int kM, size_t es) {
+  const size_t nM = (size_t)rM + kM;
+  return ((size_t)nbx * (nM * nM + (size_t)kM * rM) + 64) * es +
+         ((size_t)nbx * (rM + 1) + 64) * sizeof(int);
+}
+
+// the elimination of a batch of boxes (rskelf.m:218-246) with the whole
+// GPU; lu: 'n' / 's' (real), else 'p'.  ids: the batch's box indices on the
+// device; rM, kM: the largest r and k in the batch
 template <typename S>
-int big_elim(bool lu, int b, int n, int k, int r, const LvlBox* boxes, const BoxOut* outs,
-             const Seg* csegs, int* pool, const int* permar, const double* Tar, const S* Mprev,
-             S* Mcur, S* fval, int* fidx, const PointsDev& P, int* fail, void* scratch,
-             hipStream_t st) {
+int big_elim(bool lu, const int* ids, int nbx, int rM, int kM, const LvlBox* boxes,
+             const BoxOut* outs, const Seg* csegs, int* pool, const int* permar, const double* Tar,
+             const S* Mprev, S* Mcur, S* fval, int* fidx, const PointsDev& P, int* fail,
+             void* scratch, hipStream_t st) {
+  const int nM = rM + kM;
+  const int64_t sZ = (int64_t)nM * nM, sT = (int64_t)kM * rM;
   S* Z = reinterpret_cast<S*>(scratch);
-  S* Ts = Z + (size_t)n * n;
-  int* ipiv = reinterpret_cast<int*>(Ts + (size_t)k * r + 64);
-  int* info = ipiv + r;
+  S* Ts = Z + (size_t)nbx * sZ;
+  int* ipiv = reinterpret_cast<int*>(Ts + (size_t)nbx * sT + 64);
+  int* info = ipiv + (size_t)nbx * rM;
   rocblas_handle h = big_handle(st);
   if (!h) return (set_error("FLAM:rskx:blas: rocblas_create_handle failed"), RSKX_ERR_HIP);
-  hipLaunchKernelGGL(k_big_kassemble<S>, dim3(n), dim3(256), 0, st, boxes, b, outs, csegs, pool,
-                     permar, Mprev, Z, P, 0);
+  hipLaunchKernelGGL(k_bigb_assemble<S>, dim3(nM, nbx), dim3(256), 0, st, ids, boxes, outs, csegs,
+                     pool, permar, Mprev, Z, P, rM, kM, 0);
   const S one = 1, mone = -1;
   const rocblas_operation N_ = rocblas_operation_none, T_ = rocblas_operation_transpose;
-  if (k > 0) {
-    hipLaunchKernelGGL(k_big_tconv<S>, dim3(((int64_t)k * r + 255) / 256 < 65536 ? ((int64_t)k * r + 255) / 256 : 65536),
-                       dim3(256), 0, st, boxes, b, outs, Tar, Ts);
+  auto grid1 = [](int64_t e) { return (int)std::max<int64_t>(1, std::min<int64_t>((e + 255) / 256, 4096)); };
+  if (kM > 0) {
+    hipLaunchKernelGGL(k_bigb_tconv<S>, dim3(grid1(sT), nbx), dim3(256), 0, st, ids, boxes, outs, Tar,
+                       Ts, rM, kM);
     // K(rd,:) -= T' K(sk,:), then K(:,rd) -= K(:,sk) T
-    RSKX_BLAS(Blas<S>::gemm(h, T_, N_, r, n, k, &mone, Ts, k, Z + r, n, &one, Z, n));
-    RSKX_BLAS(Blas<S>::gemm(h, N_, N_, n, r, k, &mone, Z + (size_t)r * n, n, Ts, k, &one, Z, n));
+    RSKX_BLAS(Blas<S>::gemm_sb(h, T_, N_, rM, nM, kM, &mone, Ts, kM, sT, Z + rM, nM, sZ, &one, Z, nM,
+                               sZ, nbx));
+    RSKX_BLAS(Blas<S>::gemm_sb(h, N_, N_, nM, rM, kM, &mone, Z + (size_t)rM * nM, nM, sZ, Ts, kM, sT,
+                               &one, Z, nM, sZ, nbx));
     // the new skeleton block starts from M(sk, sk)
-    hipLaunchKernelGGL(k_big_kassemble<S>, dim3(k), dim3(256), 0, st, boxes, b, outs, csegs, pool,
-                       permar, Mprev, Z, P, 1);
+    hipLaunchKernelGGL(k_bigb_assemble<S>, dim3(kM, nbx), dim3(256), 0, st, ids, boxes, outs, csegs,
+                       pool, permar, Mprev, Z, P, rM, kM, 1);
   }
   if (lu) {
-    RSKX_BLAS(Blas<S>::getrf(h, r, r, Z, n, ipiv, info));
-    if (k > 0) {
+    RSKX_BLAS(Blas<S>::getrf_sb(h, rM, Z, nM, sZ, ipiv, rM, info, nbx));
+    if (kM > 0) {
       // E = K(sk,rd) / U, G = L \ K(rd(p), sk), M(sk,sk) -= E G
-      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_right, rocblas_fill_upper, N_, rocblas_diagonal_non_unit,
-                              k, r, &one, Z, n, Z + r, n));
-      RSKX_BLAS(Blas<S>::laswp(h, k, Z + (size_t)r * n, n, 1, r, ipiv));
-      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_left, rocblas_fill_lower, N_, rocblas_diagonal_unit,
-                              r, k, &one, Z, n, Z + (size_t)r * n, n));
-      RSKX_BLAS(Blas<S>::gemm(h, N_, N_, k, k, r, &mone, Z + r, n, Z + (size_t)r * n, n, &one,
-                              Z + r + (size_t)r * n, n));
+      RSKX_BLAS(Blas<S>::trsm_sb(h, rocblas_side_right, rocblas_fill_upper, N_,
+                                 rocblas_diagonal_non_unit, kM, rM, &one, Z, nM, sZ, Z + rM, nM, sZ,
+                                 nbx));
+      hipLaunchKernelGGL(k_bigb_laswp<S>, dim3((kM + 255) / 256, nbx), dim3(256), 0, st, Z, ipiv, rM,
+                         kM);
+      RSKX_BLAS(Blas<S>::trsm_sb(h, rocblas_side_left, rocblas_fill_lower, N_, rocblas_diagonal_unit,
+                                 rM, kM, &one, Z, nM, sZ, Z + (size_t)rM * nM, nM, sZ, nbx));
+      RSKX_BLAS(Blas<S>::gemm_sb(h, N_, N_, kM, kM, rM, &mone, Z + rM, nM, sZ, Z + (size_t)rM * nM, nM,
+                                 sZ, &one, Z + rM + (size_t)rM * nM, nM, sZ, nbx));
     }
   } else {
-    RSKX_BLAS(Blas<S>::potrf(h, r, Z, n, info));
-    if (k > 0) {
+    RSKX_BLAS(Blas<S>::potrf_sb(h, rM, Z, nM, sZ, info, nbx));
+    if (kM > 0) {
       // E = K(sk,rd) / L', M(sk,sk) -= E E'
-      RSKX_BLAS(Blas<S>::trsm(h, rocblas_side_right, rocblas_fill_lower, T_, rocblas_diagonal_non_unit,
-                              k, r, &one, Z, n, Z + r, n));
-      RSKX_BLAS(Blas<S>::gemm(h, N_, T_, k, k, r, &mone, Z + r, n, Z + r, n, &one,
-                              Z + r + (size_t)r * n, n));
+      RSKX_BLAS(Blas<S>::trsm_sb(h, rocblas_side_right, rocblas_fill_lower, T_,
+                                 rocblas_diagonal_non_unit, kM, rM, &one, Z, nM, sZ, Z + rM, nM, sZ,
+                                 nbx));
+      RSKX_BLAS(Blas<S>::gemm_sb(h, N_, T_, kM, kM, rM, &mone, Z + rM, nM, sZ, Z + rM, nM, sZ, &one,
+                                 Z + rM + (size_t)rM * nM, nM, sZ, nbx));
     }
   }
-  hipLaunchKernelGGL(k_big_info, dim3(1), dim3(1), 0, st, info, lu ? 1 : 0, fail);
-  const int64_t tot = (int64_t)r * r + 2 * (int64_t)k * r;
-  int grid = (int)((tot + 255) / 256);
-  grid = grid < 1 ? 1 : grid > 65536 ? 65536 : grid;
-  hipLaunchKernelGGL(k_big_out<S>, dim3(grid), dim3(256), 0, st, boxes, b, outs, pool, permar, Z,
-                     Ts, ipiv, fval, fidx, Mcur, lu ? 1 : 0);
-  if (lu) hipLaunchKernelGGL(k_big_piv, dim3(1), dim3(256), 0, st, outs, b, ipiv, fidx);
-  const int nbk = (r + kDiagBlock - 1) / kDiagBlock;
-  hipLaunchKernelGGL(k_big_inv<S>, dim3(2 * nbk), dim3(64), 0, st, outs, b, fval, lu ? 1 : 0);
+  hipLaunchKernelGGL(k_bigb_info, dim3(1), dim3(256), 0, st, info, nbx, lu ? 1 : 0, fail);
+  hipLaunchKernelGGL(k_bigb_out<S>, dim3(grid1((int64_t)rM * rM + 2 * (int64_t)kM * rM), nbx),
+                     dim3(256), 0, st, ids, boxes, outs, pool, permar, Z, Ts, fval, fidx, Mcur,
+                     lu ? 1 : 0, rM, kM);
+  if (lu) hipLaunchKernelGGL(k_bigb_piv, dim3(nbx), dim3(256), 0, st, ids, outs, ipiv, fidx, rM);
+  const int nbk = (rM + kDiagBlock - 1) / kDiagBlock;
+  hipLaunchKernelGGL(k_bigb_inv<S>, dim3(2 * nbk, nbx), dim3(64), 0, st, ids, outs, fval, lu ? 1 : 0);
   RSKX_HIP(hipGetLastError());
   return RSKX_OK;
 }
+
 
 // ------------------------------------------------ the sweeps' largest --
 // X(rows, :) (tree order, leading dimension ldx) <-> a dense column-major
@@ -819,11 +904,11 @@ template int big_solve<float>(int, bool, const FacRec&, const float*, const int*
 template int big_solve<double>(int, bool, const FacRec&, const double*, const int*, double*,
                                int64_t, int, double*, hipStream_t);
 
-template int big_elim<float>(bool, int, int, int, int, const LvlBox*, const BoxOut*, const Seg*,
-                             int*, const int*, const double*, const float*, float*, float*, int*,
-                             const PointsDev&, int*, void*, hipStream_t);
-template int big_elim<double>(bool, int, int, int, int, const LvlBox*, const BoxOut*, const Seg*,
-                              int*, const int*, const double*, const double*, double*, double*,
-                              int*, const PointsDev&, int*, void*, hipStream_t);
+template int big_elim<float>(bool, const int*, int, int, int, const LvlBox*, const BoxOut*,
+                             const Seg*, int*, const int*, const double*, const float*, float*,
+                             float*, int*, const PointsDev&, int*, void*, hipStream_t);
+template int big_elim<double>(bool, const int*, int, int, int, const LvlBox*, const BoxOut*,
+                              const Seg*, int*, const int*, const double*, const double*, double*,
+                              double*, int*, const PointsDev&, int*, void*, hipStream_t);
 
 }  // namespace rskx
