@@ -284,7 +284,7 @@ def other_workload(args, log):
     # the two factorizations differ only in the precision of the rest
     route = dict(id_route="gram") if args.workload == "sphere" else {}
     res = {}
-    for prec in ("fp32", "fp64"):
+    for prec in (("fp32",) if args.no_plain else ("fp32", "fp64")):
         times, F = [], None
         for it in range(args.warmup + args.steps + 1):    # the last: profiled, untimed
             full = it == args.warmup + args.steps
@@ -325,8 +325,8 @@ def other_workload(args, log):
            "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
            "data": "synthetic", "config": {"workload": desc, "N": N},
-           "mixed": res["fp32"], "plain_fp64": res["fp64"],
-           "mixed_over_plain": res["fp32"]["value"] / res["fp64"]["value"]}
+           "mixed": res["fp32"], "plain_fp64": res.get("fp64"),
+           "mixed_over_plain": res["fp32"]["value"] / res["fp64"]["value"] if "fp64" in res else None}
     print(json.dumps(out))
 
 

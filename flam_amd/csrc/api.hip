@@ -173,7 +173,7 @@ struct Tuning {
   int elim_mfma = 1;               // matrix-core elimination for the Cholesky ('p') boxes
   int lib_solve_r = 4096;          // solve sweeps: factors with this many redundant points or
                                    // more take library triangular solves and products
-  int big_n = 1024;                // boxes with more points take the whole-GPU path (library
+  int big_n = 384;                 // boxes with more points take the whole-GPU path (library
                                    // GEMM / LU / Cholesky, grid-wide pivoted Cholesky of G)
   int gram_mfma = 0;               // matrix-core accumulation of G in the Gram ID (measured
                                    // slower than the vector FP64 one: 5.3 vs 4.9 ms)
@@ -1323,10 +1323,13 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       auto place = [&](int ib) { hb[ib].K_off = oK; oK += (int64_t)hb[ib].mcap * hb[ib].n; };
       for (int ib : idg) place(ib);
       for (int c = 0; c < 2; ++c) for (int ib : idr[c]) place(ib);
-      const int64_t kbase = oK, cap = kWideKBytes / (int64_t)sizeof(S);
+      // wide boxes: K and its leading dimension aligned to 4 elements (the
+      // Gram kernel reads 4-element vectors)
+      const int64_t kbase = (oK + 3) & ~int64_t(3), cap = kWideKBytes / (int64_t)sizeof(S);
       int64_t acc = 0, gmax = 0;
       for (size_t i = 0; i < idw.size(); ++i) {
         const int ib = idw[i];
+        hb[ib].mcap = (hb[ib].mcap + 3) & ~3;
         const int64_t sz = (int64_t)hb[ib].mcap * hb[ib].n;
         if (acc > 0 && acc + sz > cap) { wgrp.push_back((int)i); acc = 0; }
         hb[ib].K_off = kbase + acc;

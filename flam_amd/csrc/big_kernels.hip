@@ -60,14 +60,21 @@ __device__ inline void gw_tile(int t, int& ti, int& tj) {
 // G = K'K: workgroup (tile, box), four waves of 32 x 32.  MFMA D(j, i):
 // lane l holds i = i0 + (l & 15) and rows j = j0 + (l >> 4) + 4 v, so the
 // stores of G(i, j) (column-major, leading dimension n) are coalesced.  Per
-// 16 rows of K, lane l reads rows 4 (l >> 4) + [0, 4) of its columns and
-// MFMA c multiplies their c-th entries (both operands pair the same row).
+// 16 rows of K, lane l reads rows 4 (l >> 4) + [0, 4) of its columns (one
+// 4-element vector load: the host aligns K and its leading dimension to 4
+// elements) and MFMA c multiplies their c-th entries (both operands pair
+// the same row); the next 16 rows are loaded while these are multiplied.
+template <typename S> struct Vec4;
+template <> struct Vec4<float> { typedef float4 T; };
+template <> struct Vec4<double> { typedef double4 T; };
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_gw_gram(const int* __restrict__ ids,
                                                  const LvlBox* __restrict__ boxes,
                                                  const int* __restrict__ mrows,
                                                  const S* __restrict__ Kar,
                                                  double* __restrict__ Tar) {
+  typedef typename Vec4<S>::T V4;
   const int b = ids[blockIdx.y];
   const LvlBox B = boxes[b];
   const int n = B.n, nt = (n + 63) >> 6;
@@ -79,38 +86,54 @@ __global__ void __launch_bounds__(256) k_gw_gram(const int* __restrict__ ids,
   const int m = mrows[b], ld = B.mcap, rl = 4 * (lane >> 4);
   const S* K = Kar + B.K_off;
   // columns past n read column n - 1; their results are never stored
-  const S* pi[2];
-  const S* pj[2];
+  const S* pc[4];
 #pragma unroll
   for (int x = 0; x < 2; ++x) {
-    pi[x] = K + (int64_t)min(i0 + 16 * x + (lane & 15), n - 1) * ld + rl;
-    pj[x] = K + (int64_t)min(j0 + 16 * x + (lane & 15), n - 1) * ld + rl;
+    pc[x] = K + (int64_t)min(i0 + 16 * x + (lane & 15), n - 1) * ld + rl;
+    pc[2 + x] = K + (int64_t)min(j0 + 16 * x + (lane & 15), n - 1) * ld + rl;
   }
   mf_d4 acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) acc[x][y] = mf_d4{0, 0, 0, 0};
-  auto step = [&](int r0, bool tail) {
-    double ai[2][4], aj[2][4];
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const bool ok = !tail || r0 + rl + c < m;
-        ai[x][c] = ok ? (double)pi[x][r0 + c] : 0.0;
-        aj[x][c] = ok ? (double)pj[x][r0 + c] : 0.0;
-      }
+  auto mul = [&](const V4* a) {
+    // a[0..1]: i columns, a[2..3]: j columns
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int y = 0; y < 2; ++y)
 #pragma unroll
-        for (int x = 0; x < 2; ++x) acc[y][x] = Mf<double>::mac(aj[y][c], ai[x][c], acc[y][x]);
+        for (int x = 0; x < 2; ++x)
+          acc[y][x] = Mf<double>::mac((double)a[2 + y][c], (double)a[x][c], acc[y][x]);
   };
-  int r0 = 0;
-  for (; r0 + 16 <= m; r0 += 16) step(r0, false);
-  if (r0 < m) step(r0, true);
+  const int mfull = m & ~15;
+  V4 cur[4], nxt[4];
+  if (mfull > 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cur[q] = *reinterpret_cast<const V4*>(pc[q]);
+  }
+  for (int r0 = 0; r0 < mfull; r0 += 16) {
+    if (r0 + 16 < mfull) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) nxt[q] = *reinterpret_cast<const V4*>(pc[q] + r0 + 16);
+    }
+    mul(cur);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cur[q] = nxt[q];
+  }
+  if (mfull < m) {                       // the last rows, masked
+    V4 t[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const S* p = pc[q] + mfull;
+      t[q].x = mfull + rl + 0 < m ? p[0] : (S)0;
+      t[q].y = mfull + rl + 1 < m ? p[1] : (S)0;
+      t[q].z = mfull + rl + 2 < m ? p[2] : (S)0;
+      t[q].w = mfull + rl + 3 < m ? p[3] : (S)0;
+    }
+    mul(t);
+  }
   double* G = Tar + B.T_off;
 #pragma unroll
   for (int y = 0; y < 2; ++y)
