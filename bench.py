@@ -133,9 +133,11 @@ def roofline_of(F, F_full, precision, workload=None, name=None):
     avg_ms = st["ms"] / max(st["launches"], 1)
     if st["flops"] > 0:
         ach = st["flops"] / st["launches"] / (avg_ms * 1e-3)
-        peak = PEAK["fp64"] if name == "id" or precision == "fp64" else PEAK["fp32"]
+        # the wide boxes' G = K'K (gram) runs on the FP64 matrix cores in
+        # both precisions
+        peak = PEAK["fp64"] if name in ("id", "gram") or precision == "fp64" else PEAK["fp32"]
         pipe = ("valu-fp64" if name == "id" else
-                "mfma-fp64" if precision == "fp64" else "mfma-fp32")
+                "mfma-fp64" if name == "gram" or precision == "fp64" else "mfma-fp32")
         traffic, src = measured_traffic(name, workload) if workload else (None, None)
         return dict(kernel=name, bound="mfma", pipe=pipe, achieved=ach / 1e12, peak=peak / 1e12,
                     unit="TFLOP/s", frac=ach / peak, traffic=traffic, traffic_source=src,
@@ -148,10 +150,11 @@ def roofline_of(F, F_full, precision, workload=None, name=None):
 
 
 def class_rooflines(F, F_full, precision, workload):
-    """Rooflines of the three main kernel classes: elimination (factor),
-    Gram ID (id) and sweeps (sweep)."""
+    """Rooflines of the main kernel classes: elimination (factor), Gram ID
+    (id), the wide boxes' G = K'K (gram, 3D top levels only) and sweeps
+    (sweep)."""
     out = {}
-    for name in ("factor", "id", "sweep"):
+    for name in ("factor", "id", "gram", "sweep"):
         roof, _ = roofline_of(F, F_full, precision, workload, name)
         if roof is not None:
             out[name] = roof
@@ -310,13 +313,14 @@ def other_workload(args, log):
                 times.append((t2 - t0, 1e3 * (t1 - t0), 1e3 * (t2 - t1), info["iters"], rel,
                               1e3 * (t_op - t1)))
         roof, kt = roofline_of(F_timed, F, prec, None)
+        rooflines = class_rooflines(F_timed, F, prec, None)
         res[prec] = dict(value=N / float(np.mean([t[0] for t in times])),
                          ms_per_step=1e3 * float(np.mean([t[0] for t in times])),
                          factor_ms=float(np.mean([t[1] for t in times])),
                          refine_ms=float(np.mean([t[2] for t in times])),
                          operator_ms=float(np.mean([t[5] for t in times])),
                          iters=times[-1][3], relres_exact=max(t[4] for t in times),
-                         levels=F.level_table(), roofline=roof,
+                         levels=F.level_table(), roofline=roof, rooflines=rooflines,
                          kernels={k: dict(ms=v["ms"], launches=v["launches"]) for k, v in kt.items()})
         del F, F_timed
     dtype = "c64+c128" if args.workload == "helmholtz" else "f32+f64"

@@ -1204,6 +1204,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
                  ws.nbr.as<int>(), ws.mnbr.as<int>(), P, L, (int)nb, g_stream);
     prof.end("lists", ev);
     std::vector<int> hmn(nb);                  // neighbour rows the routing sized K for
+    std::vector<char> wide(nb, 0);             // boxes of the wide-box Gram ID
     const bool gram = !cb && !CPLX && prm.tol >= 1e-7 &&
                       (o.id_gram > 0 || (o.id_gram == 0 && sizeof(S) == 4));
     if constexpr (!CPLX) {
@@ -1290,6 +1291,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       }
       if (gram && n > 0 && mb > 0 && n <= kWideNmax) {
         idw.push_back((int)ib);
+        wide[ib] = 1;
         nmax_w = std::max(nmax_w, n);
         continue;
       }
@@ -1377,26 +1379,34 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
       nmax_a = std::max(nmax_a, hb[ib].n);
     }
-    for (int ib : idw) Kbytes += (int64_t)hb[ib].mcap * hb[ib].n;
-    prof.begin("assemble", ev);
-    if (ng && !cb)                         // callbacks uploaded K already
+    if (ng && !cb) {                       // callbacks uploaded K already
+      prof.begin("assemble", ev);
       launch_assemble<S>(dids, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
                          ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, ng, nmax_a,
                          g_stream);
+      prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
+    }
     // wide boxes: K by groups, each group's G = K'K before the next group's
     // K overwrites it
     for (size_t g = 0; g + 1 < wgrp.size(); ++g) {
       const int a = wgrp[g], e = wgrp[g + 1];
       if (e == a) continue;
       int nm = 0;
-      for (int i = a; i < e; ++i) nm = std::max(nm, hb[idw[i]].n);
+      int64_t kb = 0;
+      for (int i = a; i < e; ++i) {
+        nm = std::max(nm, hb[idw[i]].n);
+        kb += (int64_t)hb[idw[i]].mcap * hb[idw[i]].n;
+      }
+      prof.begin("assemble", ev);
       launch_assemble<S>(wids + a, ws.boxes.as<LvlBox>(), ws.pool.as<int>(), ws.nbr.as<int>(),
                          ws.mnbr.as<int>(), ws.mrows.as<int>(), ws.K.as<S>(), P, L, e - a, nm,
                          g_stream);
+      prof.end("assemble", ev, kb * (int64_t)sizeof(S));
+      prof.begin("gram", ev);             // flops added once the row counts are back
       gram_wide_gram<S>(wids + a, e - a, nm, ws.boxes.as<LvlBox>(), ws.mrows.as<int>(),
                         ws.K.as<S>(), ws.T.as<double>(), g_stream);
+      prof.end("gram", ev);
     }
-    prof.end("assemble", ev, Kbytes * (int64_t)sizeof(S));
     prof.begin("id", ev);
     if (ng) {
       size_t shg = (size_t)(2 * nmax_g + std::max(nmax_g, mcapmax_g) + 32) * sizeof(double);
@@ -1584,14 +1594,17 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     ho.assign(nb, BoxOut{});
     int64_t offM = 0, off_nl = 0, need_v = 0, need_i = 0;
     int kmx = 0, rmx = 0;
-    double id_flops = 0, el_flops = 0;
+    double id_flops = 0, el_flops = 0, gram_flops = 0;
     for (int64_t ib = 0; ib < nb; ++ib) {
       const LvlBox& B = hb[ib];
       BoxOut& O = ho[ib];
       O.k = hk[ib];
       O.r = B.n - O.k;
       const int64_t k = O.k, r = O.r, m = hm[ib], n = B.n;
-      if (gram && n > 0 && m > 0)   // G = K'K, pivoted Cholesky, T
+      if (gram && n > 0 && m > 0 && wide[ib]) {   // G = K'K in its own class, then the rest
+        gram_flops += (double)m * n * (n + 1);
+        id_flops += 2.0 * (double)n * n * k + (double)k * k * r;
+      } else if (gram && n > 0 && m > 0)   // G = K'K, pivoted Cholesky, T
         id_flops += (double)m * n * (n + 1) + 2.0 * (double)n * n * k + (double)k * k * r;
       else                                                  // truncated QRCP, T (x4 complex)
         id_flops += (CPLX ? 4.0 : 1.0) * (4.0 * m * n * k - 2.0 * k * k * (m + n) +
@@ -1646,6 +1659,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       }
     }
     if (prof.wants("id")) prof.st["id"].flops += id_flops;
+    if (prof.wants("gram") && gram_flops > 0) prof.st["gram"].flops += gram_flops;
     F->lval.emplace_back();
     F->lidx.emplace_back();
     RSKX_TRY(F->lval.back().ensure((need_v + 1) * sizeof(S)));

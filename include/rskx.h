@@ -269,8 +269,11 @@ void rskx_graph_free(rskx_graph* G);
  * the sweep's row / stage buffer budget ("sweep_stage"), the shared
  * bytes above which the QR ID keeps Householder vectors in global memory
  * ("id_lds"), the register QRCP classes on or off ("id_reg"), the
- * matrix-core elimination of symm 'p' boxes ("elim_mfma", on) and the
- * matrix-core accumulation of G in the Gram ID ("gram_mfma", off).  Values
+ * matrix-core elimination of symm 'p' boxes ("elim_mfma", on), the
+ * matrix-core accumulation of G in the Gram ID ("gram_mfma", off), the box
+ * size above which a box is eliminated by the batched whole-GPU library
+ * path ("big_n", 384) and the redundant count from which a factor's solve
+ * sweep uses library calls ("lib_solve_r", 4096; 0 turns it off).  Values
  * apply to later calls; results do not depend on them beyond rounding (only
  * which kernel shape runs). */
 int rskx_set_tuning(const char* name, double value);
