@@ -1207,6 +1207,10 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     std::vector<char> wide(nb, 0);             // boxes of the wide-box Gram ID
     const bool gram = !cb && !CPLX && prm.tol >= 1e-7 &&
                       (o.id_gram > 0 || (o.id_gram == 0 && sizeof(S) == 4));
+    // boxes beyond the register classes take the wide-box Gram ID on the
+    // automatic route in FP64 too (a QRCP over their K would run on one
+    // workgroup each; rocSOLVER has no column-pivoted QR)
+    const bool wide_ok = gram || (!cb && !CPLX && prm.tol >= 1e-7 && o.id_gram == 0);
     if constexpr (!CPLX) {
       if (cb) {
         RSKX_TRY(callback_level<S>(A, PX, d, t, b0, nb, use_proxy, two, pool_top, off_slf, off_nbr,
@@ -1289,7 +1293,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
           continue;
         }
       }
-      if (gram && n > 0 && mb > 0 && n <= kWideNmax) {
+      if (wide_ok && n > 0 && mb > 0 && n <= kWideNmax && (gram || gram_id_class(n) < 0)) {
         idw.push_back((int)ib);
         wide[ib] = 1;
         nmax_w = std::max(nmax_w, n);
@@ -1601,7 +1605,7 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
       O.k = hk[ib];
       O.r = B.n - O.k;
       const int64_t k = O.k, r = O.r, m = hm[ib], n = B.n;
-      if (gram && n > 0 && m > 0 && wide[ib]) {   // G = K'K in its own class, then the rest
+      if (wide[ib] && n > 0 && m > 0) {   // G = K'K in its own class, then the rest
         gram_flops += (double)m * n * (n + 1);
         id_flops += 2.0 * (double)n * n * k + (double)k * k * r;
       } else if (gram && n > 0 && m > 0)   // G = K'K, pivoted Cholesky, T

@@ -372,8 +372,10 @@ def rskelf(A, x, occ, rank_or_tol, pxyfun=None, opts=None):
     rrqr_iter (inf), symm ('n'), verb (0), stop (inf: a numeric stop
     factors that many levels, rskelf.m:128,172); plus precision ('fp64' or
     'fp32'), profile (0), id_route ('auto': the Gram-matrix ID for FP32
-    factors at tol >= 1e-7, column-pivoted QR otherwise; 'gram': the Gram
-    ID for FP64 too; 'qr': always QRCP), rank_model (a trained
+    factors at tol >= 1e-7 and, in FP64, for boxes beyond the register
+    Gram classes (n > 284, the top of 3D trees), column-pivoted QR
+    otherwise; 'gram': the Gram ID for every FP64 box too; 'qr': always
+    QRCP), rank_model (a trained
     flam_amd.rank.RankModel: the Gram ID's buffers sized by its predicted
     ranks, boxes beyond their cap redone with full buffers), rank_scale (1)
     and record_boxes (False: keep features and exact ranks of every box,

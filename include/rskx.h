@@ -171,7 +171,10 @@ typedef struct {
   int id_gram;          /* ID route at tol >= 1e-7: 0 the Gram-matrix ID for
                            FP32 factors and column-pivoted QR for FP64 (so
                            FP64 rank tables follow the reference's QRCP
-                           step by step); 1 the Gram ID for both; -1 QRCP */
+                           step by step) except boxes beyond the register
+                           Gram classes (n > 284), which take the
+                           whole-GPU Gram ID; 1 the Gram ID for both; -1
+                           QRCP */
   const rskx_rank_model* rank_model;
                         /* non-NULL: predict every box's rank before the
                            level loop and size the Gram ID's R / T work
