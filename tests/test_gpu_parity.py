@@ -561,16 +561,25 @@ def test_symm_h_bunch_kaufman(N):
         assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
 
 
-def test_large_root_global_paths():
-    """One box of 2600 points (occ = N): the root's dense LU runs in the
-    elimination's global-scratch class and both sweeps in their global
-    workspace slices (rows beyond shared memory); solve and apply against
-    the closed form and the exact product."""
+@pytest.mark.parametrize("route", ["library", "global"])
+def test_large_root_global_paths(route):
+    """One box of 2600 points (occ = N): the root's dense LU runs on the
+    whole-GPU library path (the default above 384 points) or, with big_n
+    raised, in the elimination's global-scratch class; both sweeps run in
+    their global workspace slices (rows beyond shared memory); solve and
+    apply against the closed form and the exact product."""
     flam_amd, geometry = _dev()
+    from flam_amd import tune
     N = 2600
-    print(f"large root: circle N={N} occ={N} symm n fp64", flush=True)
+    print(f"large root: circle N={N} occ={N} symm n fp64, {route} elimination", flush=True)
     g = geometry.circle(N)
-    F = flam_amd.rskelf(g.kernel, g.x, N, 1e-12, g.proxy, dict(symm="n"))
+    saved = tune.get_tuning("big_n")
+    try:
+        if route == "global":
+            tune.set_tuning("big_n", 1 << 30)
+        F = flam_amd.rskelf(g.kernel, g.x, N, 1e-12, g.proxy, dict(symm="n"))
+    finally:
+        tune.set_tuning("big_n", saved)
     assert F.level_table() == [(1, 1, N, 0)]
     pr = opr.CurveDL(N, "circle")
     B = np.random.default_rng(6).standard_normal((N, 3))
