@@ -859,8 +859,10 @@ __global__ void __launch_bounds__(256) k_matvec_sl3(const double* __restrict__ x
                                                     const double* __restrict__ X, int64_t ldx,
                                                     double* __restrict__ Y, int64_t ldy, int nrhs,
                                                     const int64_t* __restrict__ poff, int nprob) {
-  __shared__ double sx[256], sy[256], sz[256], sw[256];
-  __shared__ double sX[8][256];
+  // a staged source: its point and scale w_j / (4 pi), then its 8
+  // right-hand-side values together (two and four 16-byte LDS reads)
+  __shared__ double4 sp[256];
+  __shared__ double4 sX[256][2];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const bool act = i < N;
   const double xi = act ? x[3 * i] : 0, yi = act ? x[3 * i + 1] : 0, zi = act ? x[3 * i + 2] : 0;
@@ -885,46 +887,54 @@ __global__ void __launch_bounds__(256) k_matvec_sl3(const double* __restrict__ x
       __syncthreads();
       int64_t j = j0 + threadIdx.x;
       if (j < blk1) {
-        sx[threadIdx.x] = x[3 * j];
-        sy[threadIdx.x] = x[3 * j + 1];
-        sz[threadIdx.x] = x[3 * j + 2];
-        sw[threadIdx.x] = w[j];
-        for (int c = 0; c < cc; ++c) sX[c][threadIdx.x] = X[j + (c0r + c) * ldx];
+        sp[threadIdx.x] = double4{x[3 * j], x[3 * j + 1], x[3 * j + 2], c0 * w[j]};
+        double v[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = c < cc ? X[j + (c0r + c) * ldx] : 0.0;
+        sX[threadIdx.x][0] = double4{v[0], v[1], v[2], v[3]};
+        sX[threadIdx.x][1] = double4{v[4], v[5], v[6], v[7]};
       }
       __syncthreads();
       const int jn = blk1 - j0 < 256 ? (int)(blk1 - j0) : 256;
       if (j0 >= my0 && j0 + jn <= my1) {
         // every source of the tile belongs to this row's problem: 1/r from
-        // the FP32 reciprocal square root and two Newton steps in FP64
-        // (relative error ~1e-16); the diagonal is added after the loop
+        // the FP32 reciprocal square root and two Newton-Raphson steps in
+        // FP64 (relative error ~1e-16); the diagonal is added after the
+        // loop.  Columns past cc hold zeros and are not stored.
+#pragma unroll 2
         for (int t = 0; t < jn; ++t) {
-          const double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
+          const double4 q = sp[t];
+          const double dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
           const double r2 = dx * dx + dy * dy + dz * dz;
           double y = (double)rsqrtf((float)r2);
           y = y * (1.5 - 0.5 * r2 * y * y);
           y = y * (1.5 - 0.5 * r2 * y * y);
-          const double a = r2 > 0 ? c0 * sw[t] * y : 0.0;
-#pragma unroll
-          for (int c = 0; c < 8; ++c) if (c < cc) acc[c] += a * sX[c][t];
+          const double a = r2 > 0 ? q.w * y : 0.0;
+          const double4 u = sX[t][0], v = sX[t][1];
+          acc[0] += a * u.x; acc[1] += a * u.y; acc[2] += a * u.z; acc[3] += a * u.w;
+          acc[4] += a * v.x; acc[5] += a * v.y; acc[6] += a * v.z; acc[7] += a * v.w;
         }
         if (act && i >= j0 && i < j0 + jn) {
           const double d = diag[i];
-#pragma unroll
-          for (int c = 0; c < 8; ++c) if (c < cc) acc[c] += d * sX[c][i - j0];
+          const double4 u = sX[i - j0][0], v = sX[i - j0][1];
+          acc[0] += d * u.x; acc[1] += d * u.y; acc[2] += d * u.z; acc[3] += d * u.w;
+          acc[4] += d * v.x; acc[5] += d * v.y; acc[6] += d * v.z; acc[7] += d * v.w;
         }
         continue;
       }
       for (int t = 0; t < jn; ++t) {
         int64_t jj = j0 + t;
         if (jj < my0 || jj >= my1) continue;   // another problem's source
+        const double4 q = sp[t];
         double a;
         if (jj == i) a = diag[jj];
         else {
-          const double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
-          a = c0 * sw[t] / sqrt(dx * dx + dy * dy + dz * dz);
+          const double dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
+          a = q.w / sqrt(dx * dx + dy * dy + dz * dz);
         }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) if (c < cc) acc[c] += a * sX[c][t];
+        const double4 u = sX[t][0], v = sX[t][1];
+        acc[0] += a * u.x; acc[1] += a * u.y; acc[2] += a * u.z; acc[3] += a * u.w;
+        acc[4] += a * v.x; acc[5] += a * v.y; acc[6] += a * v.z; acc[7] += a * v.w;
       }
     }
     if (act)
