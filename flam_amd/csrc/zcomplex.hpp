@@ -92,13 +92,79 @@ constexpr HankelCoef hankel_coef() {
   return h;
 }
 
-// iz = 1 / z (callers that have it spare a division)
-__device__ inline void bessel01(double z, double iz, double& J0, double& Y0, double& J1,
-                                double& Y1) {
-  if (z < kHankelZ) {
-    J0 = j0(z); Y0 = y0(z); J1 = j1(z); Y1 = y1(z);
-    return;
+// sin z and cos z for 0 <= z < 2^20 (Cody-Waite reduction by pi/2 split in
+// three parts, the first two with 32 significant bits so q times them is
+// exact, then the Taylor series of sin and cos on |r| <= pi/4 + 1e-6 up to
+// r^17 and r^16, whose first omitted terms are below 2e-17): max error
+// 1.1e-16 on [0, 1e4] against the libm values
+__device__ inline void sincos_cw(double z, double& sz, double& cz) {
+  constexpr double kTwoOverPi = 0.6366197723675814;
+  constexpr double kP1 = 1.5707963267341256, kP2 = 6.077100506303966e-11,
+                   kP3 = 2.0222662487959506e-21;
+  const double q = rint(z * kTwoOverPi);
+  const double r = ((z - q * kP1) - q * kP2) - q * kP3;
+  const double r2 = r * r;
+  // Horner in r^2 with (-1)^k / (2k+1)! and (-1)^k / (2k)!
+  double sp = 1.0 / 355687428096000.0, cp = 1.0 / 20922789888000.0;   // 1/17!, 1/16!
+  sp = sp * r2 - 1.0 / 1307674368000.0;                               // 1/15!
+  cp = cp * r2 - 1.0 / 87178291200.0;                                 // 1/14!
+  sp = sp * r2 + 1.0 / 6227020800.0;                                  // 1/13!
+  cp = cp * r2 + 1.0 / 479001600.0;                                   // 1/12!
+  sp = sp * r2 - 1.0 / 39916800.0;                                    // 1/11!
+  cp = cp * r2 - 1.0 / 3628800.0;                                     // 1/10!
+  sp = sp * r2 + 1.0 / 362880.0;                                      // 1/9!
+  cp = cp * r2 + 1.0 / 40320.0;                                       // 1/8!
+  sp = sp * r2 - 1.0 / 5040.0;                                        // 1/7!
+  cp = cp * r2 - 1.0 / 720.0;                                         // 1/6!
+  sp = sp * r2 + 1.0 / 120.0;                                         // 1/5!
+  cp = cp * r2 + 1.0 / 24.0;                                          // 1/4!
+  sp = sp * r2 - 1.0 / 6.0;                                           // 1/3!
+  cp = cp * r2 - 0.5;                                                 // 1/2!
+  const double s = r + r * r2 * sp, c = 1.0 + r2 * cp;
+  const int qi = (int)q & 3;
+  sz = qi == 0 ? s : qi == 1 ? c : qi == 2 ? -s : -c;
+  cz = qi == 0 ? c : qi == 1 ? -s : qi == 2 ? -c : s;
+}
+
+// 1 / sqrt(a) for a > 0: the FP32 reciprocal square root refined by two
+// Newton-Raphson steps in FP64 (relative error ~3e-16)
+__device__ inline double rsqrt_nr(double a) {
+  double y = (double)rsqrtf((float)a);
+  y = y * (1.5 - 0.5 * a * y * y);
+  return y * (1.5 - 0.5 * a * y * y);
+}
+
+// the asymptotic expansion alone (z >= kHankelZ) with iz = 1 / z and
+// sq = sqrt(2 / (pi z)) given, sin and cos from sincos_cw
+__device__ inline void bessel01_far_sq(double z, double iz, double sq, double& J0, double& Y0,
+                                       double& J1, double& Y1) {
+  constexpr HankelCoef h = hankel_coef();
+  const double u = iz * iz;
+  double p0 = 0, q0 = 0, p1 = 0, q1 = 0;
+#pragma unroll
+  for (int k = kHankelTerms / 2 - 1; k >= 0; --k) {
+    const double sg = (k & 1) ? -1.0 : 1.0;
+    p0 = p0 * u + sg * h.a0[2 * k];
+    q0 = q0 * u + sg * h.a0[2 * k + 1];
+    p1 = p1 * u + sg * h.a1[2 * k];
+    q1 = q1 * u + sg * h.a1[2 * k + 1];
   }
+  q0 *= iz;
+  q1 *= iz;
+  double sz, cz;
+  sincos_cw(z, sz, cz);
+  const double rs2 = 0.70710678118654752440;     // 1 / sqrt 2
+  const double cw0 = (cz + sz) * rs2, sw0 = (sz - cz) * rs2;
+  const double cw1 = (sz - cz) * rs2, sw1 = -(sz + cz) * rs2;
+  J0 = sq * (p0 * cw0 - q0 * sw0);
+  Y0 = sq * (p0 * sw0 + q0 * cw0);
+  J1 = sq * (p1 * cw1 - q1 * sw1);
+  Y1 = sq * (p1 * sw1 + q1 * cw1);
+}
+
+// the asymptotic expansion alone (z >= kHankelZ); iz = 1 / z
+__device__ inline void bessel01_far(double z, double iz, double& J0, double& Y0, double& J1,
+                                    double& Y1) {
   constexpr HankelCoef h = hankel_coef();
   const double u = iz * iz;
   // P = sum_k (-1)^k a_2k u^k, Q = iz sum_k (-1)^k a_2k+1 u^k (Horner, highest first)
@@ -123,6 +189,16 @@ __device__ inline void bessel01(double z, double iz, double& J0, double& Y0, dou
   Y0 = s * (p0 * sw0 + q0 * cw0);
   J1 = s * (p1 * cw1 - q1 * sw1);
   Y1 = s * (p1 * sw1 + q1 * cw1);
+}
+
+// iz = 1 / z (callers that have it spare a division)
+__device__ inline void bessel01(double z, double iz, double& J0, double& Y0, double& J1,
+                                double& Y1) {
+  if (z < kHankelZ) {
+    J0 = j0(z); Y0 = y0(z); J1 = j1(z); Y1 = y1(z);
+    return;
+  }
+  bessel01_far(z, iz, J0, Y0, J1, Y1);
 }
 
 __device__ inline void bessel01(double z, double& J0, double& Y0, double& J1, double& Y1) {

@@ -220,37 +220,56 @@ __global__ void k_zfrom_tree(const S* __restrict__ Y, U* __restrict__ X, int64_t
 // Y = A X with the combined-field matrix of the points (x, nu, w, complex
 // diagonal), FP64, evaluated on the fly: a workgroup of 256 targets loops
 // over all sources, 256 at a time staged in shared memory with their
-// right-hand-side values; each thread accumulates NC columns.
+// right-hand-side values; each thread accumulates NC columns.  Two passes:
+// the far pass (NEAR false) takes the pairs with (k r)^2 >= kHankelZ^2, on
+// the Hankel asymptotic expansion alone (1 / r and 1 / sqrt(r) by
+// reciprocal square roots, sin and cos by sincos_cw), and writes Y; the
+// near pass adds the other pairs (HIP's Bessel functions, whose registers
+// would otherwise hold the far pass to one wave per SIMD) and the diagonal.
+// Both passes classify a pair by the same arithmetic, so each pair is
+// counted once.
 constexpr int kZMvTile = 256;
 constexpr int kZMvCols = 8;
 
 namespace {
-// ik = 1 / k: one division per pair (1 / r) serves c and 1 / z
-__device__ inline zd cfie_entry_mv(double k, double ik, double eta, double tx, double ty,
-                                   double sx, double sy, double nx, double ny, double w) {
-  const double dx = tx - sx, dy = ty - sy;
-  const double r = sqrt(dx * dx + dy * dy), ir = 1.0 / r;
+// near pairs: ik = 1 / k, one division per pair (1 / r) serves c and 1 / z
+__device__ inline zd cfie_entry_near(double k, double ik, double eta, double dx, double dy,
+                                     double r2, double nx, double ny, double w) {
+  const double r = sqrt(r2), ir = 1.0 / r;
   const double z = k * r, c = (dx * nx + dy * ny) * ir;
   double J0, Y0, J1, Y1;
   bessel01(z, ir * ik, J0, Y0, J1, Y1);
   const double q = 0.25 * w;
   return zd{q * (eta * J0 - k * c * Y1), q * (eta * Y0 + k * c * J1)};
 }
+// far pairs: 1 / r and sqrt(2 / (pi z)) = sqrt(2 / (pi k)) / sqrt(r) by
+// reciprocal square roots (sk = sqrt(2 / (pi k)))
+__device__ inline zd cfie_entry_far(double k, double ik, double sk, double eta, double dx,
+                                    double dy, double r2, double nx, double ny, double w) {
+  const double ir = rsqrt_nr(r2), r = r2 * ir;
+  const double z = k * r, c = (dx * nx + dy * ny) * ir;
+  double J0, Y0, J1, Y1;
+  bessel01_far_sq(z, ir * ik, sk * rsqrt_nr(r), J0, Y0, J1, Y1);
+  const double q = 0.25 * w;
+  return zd{q * (eta * J0 - k * c * Y1), q * (eta * Y0 + k * c * J1)};
+}
 }  // namespace
 
-__global__ void __launch_bounds__(kZMvTile) k_zmatvec(const double* __restrict__ x, int64_t N,
-                                                      const double* __restrict__ nu,
-                                                      const double* __restrict__ w,
-                                                      const double* __restrict__ diag, double k,
-                                                      double eta, const zd* __restrict__ X,
-                                                      int64_t ldx, zd* __restrict__ Y, int64_t ldy,
-                                                      int c0, int nc) {
-  __shared__ double sx[kZMvTile], sy[kZMvTile], snx[kZMvTile], sny[kZMvTile], sw[kZMvTile];
+template <bool NEAR>
+__device__ __forceinline__ void zmatvec_body(const double* __restrict__ x, int64_t N,
+                                             const double* __restrict__ nu,
+                                             const double* __restrict__ w,
+                                             const double* __restrict__ diag, double k, double eta,
+                                             const zd* __restrict__ X, int64_t ldx,
+                                             zd* __restrict__ Y, int64_t ldy, int c0, int nc) {
+  __shared__ double4 sp[kZMvTile];          // x, y, nx, ny
+  __shared__ double sw[kZMvTile];
   __shared__ zd sX[kZMvCols][kZMvTile];
   const int64_t i = blockIdx.x * (int64_t)kZMvTile + threadIdx.x;
   const bool act = i < N;
   const double tx = act ? x[2 * i] : 0.0, ty = act ? x[2 * i + 1] : 0.0;
-  const double ik = 1.0 / k;
+  const double ik = 1.0 / k, sk = sqrt(0.63661977236758134308 * ik);   // sqrt(2 / (pi k))
+  const double z2near = kHankelZ * kHankelZ * ik * ik;                  // k r < kHankelZ
   zd acc[kZMvCols];
 #pragma unroll
   for (int c = 0; c < kZMvCols; ++c) acc[c] = zd{0, 0};
@@ -258,10 +277,7 @@ __global__ void __launch_bounds__(kZMvTile) k_zmatvec(const double* __restrict__
     const int64_t j = j0s + threadIdx.x;
     __syncthreads();
     if (j < N) {
-      sx[threadIdx.x] = x[2 * j];
-      sy[threadIdx.x] = x[2 * j + 1];
-      snx[threadIdx.x] = nu[2 * j];
-      sny[threadIdx.x] = nu[2 * j + 1];
+      sp[threadIdx.x] = double4{x[2 * j], x[2 * j + 1], nu[2 * j], nu[2 * j + 1]};
       sw[threadIdx.x] = w[j];
 #pragma unroll
       for (int c = 0; c < kZMvCols; ++c)
@@ -272,14 +288,47 @@ __global__ void __launch_bounds__(kZMvTile) k_zmatvec(const double* __restrict__
     if (!act) continue;
     for (int t = 0; t < cnt; ++t) {
       const int64_t jj = j0s + t;
-      const zd a = jj == i ? zd{diag[2 * i], diag[2 * i + 1]}
-                           : cfie_entry_mv(k, ik, eta, tx, ty, sx[t], sy[t], snx[t], sny[t], sw[t]);
+      const double4 q = sp[t];
+      const double dx = tx - q.x, dy = ty - q.y;
+      const double r2 = dx * dx + dy * dy;
+      const bool near = r2 < z2near;             // the same test in both passes
+      zd a;
+      if (jj == i) {
+        if (!NEAR) continue;
+        a = zd{diag[2 * i], diag[2 * i + 1]};
+      } else if (near != NEAR) {
+        continue;
+      } else {
+        a = NEAR ? cfie_entry_near(k, ik, eta, dx, dy, r2, q.z, q.w, sw[t])
+                 : cfie_entry_far(k, ik, sk, eta, dx, dy, r2, q.z, q.w, sw[t]);
+      }
 #pragma unroll
       for (int c = 0; c < kZMvCols; ++c) acc[c] += a * sX[c][t];
     }
   }
   if (act)
-    for (int c = 0; c < nc; ++c) Y[i + (c0 + c) * ldy] = acc[c];
+    for (int c = 0; c < nc; ++c) {
+      zd* y = Y + i + (c0 + c) * ldy;
+      *y = NEAR ? *y + acc[c] : acc[c];
+    }
+}
+
+// the far pass at four waves per SIMD: the series and Taylor constants are
+// rematerialized instead of held in registers across the source loop
+__global__ void __launch_bounds__(kZMvTile) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_zmatvec_far(const double* __restrict__ x, int64_t N, const double* __restrict__ nu,
+              const double* __restrict__ w, const double* __restrict__ diag, double k, double eta,
+              const zd* __restrict__ X, int64_t ldx, zd* __restrict__ Y, int64_t ldy, int c0,
+              int nc) {
+  zmatvec_body<false>(x, N, nu, w, diag, k, eta, X, ldx, Y, ldy, c0, nc);
+}
+
+__global__ void __launch_bounds__(kZMvTile)
+k_zmatvec_near(const double* __restrict__ x, int64_t N, const double* __restrict__ nu,
+               const double* __restrict__ w, const double* __restrict__ diag, double k, double eta,
+               const zd* __restrict__ X, int64_t ldx, zd* __restrict__ Y, int64_t ldy, int c0,
+               int nc) {
+  zmatvec_body<true>(x, N, nu, w, diag, k, eta, X, ldx, Y, ldy, c0, nc);
 }
 
 // ------------------------------------------------------------ launchers --
@@ -327,8 +376,12 @@ void launch_zmatvec(const double* x, int64_t N, const double* nu, const double* 
   const int grid = (int)((N + kZMvTile - 1) / kZMvTile);
   for (int c0 = 0; c0 < nrhs; c0 += kZMvCols) {
     const int nc = nrhs - c0 < kZMvCols ? nrhs - c0 : kZMvCols;
-    hipLaunchKernelGGL(k_zmatvec, dim3(grid), dim3(kZMvTile), 0, st, x, N, nu, w, diag, k, eta,
-                       reinterpret_cast<const zd*>(X), ldx, reinterpret_cast<zd*>(Y), ldy, c0, nc);
+    hipLaunchKernelGGL(k_zmatvec_far, dim3(grid), dim3(kZMvTile), 0, st, x, N, nu, w, diag, k,
+                       eta, reinterpret_cast<const zd*>(X), ldx, reinterpret_cast<zd*>(Y), ldy, c0,
+                       nc);
+    hipLaunchKernelGGL(k_zmatvec_near, dim3(grid), dim3(kZMvTile), 0, st, x, N, nu, w, diag, k,
+                       eta, reinterpret_cast<const zd*>(X), ldx, reinterpret_cast<zd*>(Y), ldy, c0,
+                       nc);
   }
 }
 
