@@ -59,14 +59,20 @@ __device__ inline void gw_tile(int t, int& ti, int& tj) {
 
 // G = K'K: workgroup (tile, box), four waves of 32 x 32.  MFMA D(j, i):
 // lane l holds i = i0 + (l & 15) and rows j = j0 + (l >> 4) + 4 v, so the
-// stores of G(i, j) (column-major, leading dimension n) are coalesced.  Per
-// 16 rows of K, lane l reads rows 4 (l >> 4) + [0, 4) of its columns (one
-// 4-element vector load: the host aligns K and its leading dimension to 4
-// elements) and MFMA c multiplies their c-th entries (both operands pair
-// the same row); the next 16 rows are loaded while these are multiplied.
+// stores of G(i, j) (column-major, leading dimension n) are coalesced.
+// Per 16 rows of K the workgroup stages the tile's 64 i and 64 j columns
+// in LDS once (one 4-element vector load per thread and half: the host
+// aligns K and its leading dimension to 4 elements), double-buffered with
+// the next 16 rows loaded while these are multiplied; lane l then reads
+// rows 4 (l >> 4) + [0, 4) of its columns as one LDS vector and MFMA c
+// multiplies their c-th entries (both operands pair the same row).  Each
+// wave reading its own columns from L1 instead had the four waves load
+// every column twice and held the kernel at the L1 bandwidth.
 template <typename S> struct Vec4;
 template <> struct Vec4<float> { typedef float4 T; };
 template <> struct Vec4<double> { typedef double4 T; };
+
+constexpr int kGwLd = 20;          // LDS row pitch of a staged column (16 rows + 4 pad)
 
 template <typename S>
 __global__ void __launch_bounds__(256) k_gw_gram(const int* __restrict__ ids,
@@ -75,30 +81,73 @@ __global__ void __launch_bounds__(256) k_gw_gram(const int* __restrict__ ids,
                                                  const S* __restrict__ Kar,
                                                  double* __restrict__ Tar) {
   typedef typename Vec4<S>::T V4;
+  __shared__ __align__(32) S stg[2][128 * kGwLd];
   const int b = ids[blockIdx.y];
   const LvlBox B = boxes[b];
   const int n = B.n, nt = (n + 63) >> 6;
   if ((int)blockIdx.x >= nt * (nt + 1) / 2) return;
   int ti, tj;
   gw_tile(blockIdx.x, ti, tj);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int i0 = 64 * ti + 32 * (wid >> 1), j0 = 64 * tj + 32 * (wid & 1);
-  const int m = mrows[b], ld = B.mcap, rl = 4 * (lane >> 4);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int m = mrows[b], ld = B.mcap;
   const S* K = Kar + B.K_off;
-  // columns past n read column n - 1; their results are never stored
-  const S* pc[4];
+  // staging: thread tid moves column q = (tid + 256 h) >> 2, rows 4 ((tid + 256 h) & 3) + [0, 4);
+  // staged columns 0..63 are the tile's i columns, 64..127 its j columns
+  // (columns past n read column n - 1; their results are never stored)
+  const S* src[2];
+  int dst[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = tid + 256 * h, q = e >> 2, part = e & 3;
+    const int col = min((q < 64 ? 64 * ti + q : 64 * tj + q - 64), n - 1);
+    src[h] = K + (int64_t)col * ld + 4 * part;
+    dst[h] = q * kGwLd + 4 * part;
+  }
+  auto fetch = [&](int r0, V4* v) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int part = ((tid + 256 * h) & 3) * 4;
+      if (r0 + 16 <= m) {
+        v[h] = *reinterpret_cast<const V4*>(src[h] + r0);
+      } else {                             // the last rows, masked
+        const S* p = src[h] + r0;
+        v[h].x = r0 + part + 0 < m ? p[0] : (S)0;
+        v[h].y = r0 + part + 1 < m ? p[1] : (S)0;
+        v[h].z = r0 + part + 2 < m ? p[2] : (S)0;
+        v[h].w = r0 + part + 3 < m ? p[3] : (S)0;
+      }
+    }
+  };
+  auto stash = [&](int sb, const V4* v) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) *reinterpret_cast<V4*>(&stg[sb][dst[h]]) = v[h];
+  };
+  // this lane's operand columns in the staged block
+  int qc[4];
 #pragma unroll
   for (int x = 0; x < 2; ++x) {
-    pc[x] = K + (int64_t)min(i0 + 16 * x + (lane & 15), n - 1) * ld + rl;
-    pc[2 + x] = K + (int64_t)min(j0 + 16 * x + (lane & 15), n - 1) * ld + rl;
+    qc[x] = 32 * (wid >> 1) + 16 * x + (lane & 15);
+    qc[2 + x] = 64 + 32 * (wid & 1) + 16 * x + (lane & 15);
   }
+  const int rl = 4 * (lane >> 4);
   mf_d4 acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) acc[x][y] = mf_d4{0, 0, 0, 0};
-  auto mul = [&](const V4* a) {
-    // a[0..1]: i columns, a[2..3]: j columns
+  V4 nx[2];
+  if (m > 0) {
+    fetch(0, nx);
+    stash(0, nx);
+  }
+  __syncthreads();
+  int sb = 0;
+  for (int r0 = 0; r0 < m; r0 += 16) {
+    const bool more = r0 + 16 < m;
+    if (more) fetch(r0 + 16, nx);          // in flight while this block is multiplied
+    V4 a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = *reinterpret_cast<const V4*>(&stg[sb][qc[q] * kGwLd + rl]);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -106,34 +155,11 @@ __global__ void __launch_bounds__(256) k_gw_gram(const int* __restrict__ ids,
 #pragma unroll
         for (int x = 0; x < 2; ++x)
           acc[y][x] = Mf<double>::mac((double)a[2 + y][c], (double)a[x][c], acc[y][x]);
-  };
-  const int mfull = m & ~15;
-  V4 cur[4], nxt[4];
-  if (mfull > 0) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) cur[q] = *reinterpret_cast<const V4*>(pc[q]);
+    if (more) stash(sb ^ 1, nx);
+    __syncthreads();
+    sb ^= 1;
   }
-  for (int r0 = 0; r0 < mfull; r0 += 16) {
-    if (r0 + 16 < mfull) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) nxt[q] = *reinterpret_cast<const V4*>(pc[q] + r0 + 16);
-    }
-    mul(cur);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) cur[q] = nxt[q];
-  }
-  if (mfull < m) {                       // the last rows, masked
-    V4 t[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const S* p = pc[q] + mfull;
-      t[q].x = mfull + rl + 0 < m ? p[0] : (S)0;
-      t[q].y = mfull + rl + 1 < m ? p[1] : (S)0;
-      t[q].z = mfull + rl + 2 < m ? p[2] : (S)0;
-      t[q].w = mfull + rl + 3 < m ? p[3] : (S)0;
-    }
-    mul(t);
-  }
+  const int i0 = 64 * ti + 32 * (wid >> 1), j0 = 64 * tj + 32 * (wid & 1);
   double* G = Tar + B.T_off;
 #pragma unroll
   for (int y = 0; y < 2; ++y)
