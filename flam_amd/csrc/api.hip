@@ -1278,10 +1278,10 @@ int factor_impl(const double* x, int d, int64_t N, int nprob, const int64_t* pof
     int nmax_g = 0, mcapmax_g = 0, nmax_w = 0;
     // a box whose point data would not fit a class's shared memory goes to
     // the next route: Gram -> register QRCP -> QRCP on K in global memory;
-    // on the Gram route, boxes beyond the register-resident Gram classes
-    // take the batched wide-box Gram ID (K in global memory, G by MFMA
-    // tiles, blocked pivoted Cholesky) while the tail's shared memory
-    // (3 n + 32 doubles) holds them
+    // boxes beyond the register-resident Gram classes (every such box on
+    // the Gram route, and on FP64's automatic route too) take the batched
+    // wide-box Gram ID (K in global memory, G by MFMA tiles, blocked
+    // pivoted Cholesky) up to kWideNmax points
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int n = hb[ib].n, mb = hmn[ib] * (two ? 2 : 1) + L.p;
       if (gram && n > 0 && mb > 0 && gram_id_class(n) >= 0) {
