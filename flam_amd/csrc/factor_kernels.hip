@@ -1415,8 +1415,9 @@ k_id_reg(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
 // rows of K staged in shared memory per accumulation step of k_gram_id
 constexpr int kGramRows = 16;
 
+// (classes with two tiles per thread held to 128 VGPRs: four waves per SIMD)
 template <int NT, int TPT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TPT <= 2 ? 4 : 1, 8)))
 k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
           const int* __restrict__ pool, const int* __restrict__ nbrar,
           const int* __restrict__ mnbr, int* __restrict__ mrows, double* __restrict__ Tar,
@@ -1496,10 +1497,15 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
   // (rskelf.m:201-212), entries as in k_assemble
   const double c0 = P.scale / kTwoPi;
   const bool sl3 = P.kind == RSKX_KERNEL_LAPLACE3D_SL;
+  // a thread's entries of a chunk: the first at (tid / np, tid % np), the
+  // next ones by steps of NT = sR rows + sJ columns through the row-major
+  // chunk (one division per chunk instead of one per entry)
+  const int sR = NT / np, sJ = NT - sR * np;
   for (int r0 = 0; r0 < m; r0 += RC) {
     __syncthreads();
-    for (int e = tid; e < RC * np; e += NT) {
-      const int rr = e / np, j = e - rr * np, i = r0 + rr;
+    int rr = tid / np, j = tid - rr * np;
+    while (rr < RC) {
+      const int i = r0 + rr;
       double val = 0;
       if (i < m && j < n) {
         const double xj = sx[j], yj = sy[j];
@@ -1531,6 +1537,9 @@ k_gram_id(const int* __restrict__ ids, const LvlBox* __restrict__ boxes,
         }
       }
       chunk[rr * np + j] = val;
+      j += sJ;
+      rr += sR;
+      if (j >= np) { j -= np; ++rr; }
     }
     __syncthreads();
 #pragma unroll
